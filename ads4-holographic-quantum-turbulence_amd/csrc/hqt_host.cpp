@@ -1,0 +1,755 @@
+// C ABI of the AdS4 stepper (include/hqt.h): context set-up, device residency, the launch
+// sequence of one RK4 step, and the stand-alone operators.
+//
+// One step (tevol_driven_cluster.m:120-142) is queued as:
+//   zprep                          d_z A_t of the frozen A_t (A_t is held fixed in all 4 stages)
+//   4 x { line_x, line_y, zstage } RHS F1..F4 at the stage state, RK bookkeeping (:120-128)
+//   line_x_cons, line_y_cons       d_x A_x + d_y A_y of the updated state (:129)
+//   zpost                          A_t = Lzr(Lzr(cons) - rhoi) + mu, boundary reset, O (:129-136)
+//   winding                        removal of the global winding along y (:136-139)
+//   line_x_filter, line_y_filter   ff on Re Psi, Im Psi, A_t, A_x, A_y (:142), with the NaN guard (:182-185)
+#include "hqt.h"
+#include "hqt_kernels.h"
+#include "hqt_operators.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace hqt;
+
+namespace {
+thread_local std::string g_err;
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+}  // namespace
+
+#define HIPC(call)                                                                          \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) return fail(HQT_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+enum KName {
+    KN_ZPREP, KN_LX_STAGE, KN_LY_STAGE, KN_ZSTAGE, KN_LX_CONS, KN_LY_CONS, KN_ZPOST, KN_WIND,
+    KN_LX_FILT, KN_LY_FILT, KN_NAN, KN_LX_RHS, KN_LY_RHS, KN_ZRHS, KN_COUNT
+};
+static const char* kNames[KN_COUNT] = {
+    "zprep", "line_x_stage", "line_y_stage", "zstage", "line_x_cons", "line_y_cons", "zpost", "winding",
+    "line_x_filter", "line_y_filter", "nancheck", "line_x_rhs", "line_y_rhs", "zrhs"};
+
+enum Op { OP_D1X, OP_D2X, OP_ND1X, OP_PRX, OP_PIX, OP_NPIX, OP_D1Y, OP_D2Y, OP_ND1Y, OP_QRY, OP_QIY, OP_NQIY, OP_COUNT };
+
+enum Pass { PX1, PX2, PX3, PX4, PY1, PY2, PY3, PY4, PXC, PYC, PXF, PYF, PXR, PYR, NPASS };
+
+struct hqt_ctx {
+    hqt_params p{};
+    int nx = 0, ny = 0, nz = 0;
+    int64_t nxy = 0, npts = 0;
+    int fp32 = 0;
+    int tdt = DT_F64;
+    size_t es = 8;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+
+    void *ZR, *ZI, *ZAx, *ZAy;
+    void *SA[4], *SB[4], *AC[4];
+    void *dxR, *dxI, *dxxR, *dxxI, *dxAx, *dxAy, *dxxAy;
+    void *dyR, *dyI, *lapR, *lapI, *divA, *curl3, *curl4;
+    void *dxAt, *dyAt;
+    void *fa[5], *fb[5];
+    double *At, *rho, *dzAt, *arho, *O, *part, *W, *axb, *ayb, *yph, *zops, *AtR;
+    double *F1R = nullptr, *F1I = nullptr, *F2 = nullptr, *F3 = nullptr, *F4 = nullptr;
+    int32_t* flags = nullptr;
+    void* ops[OP_COUNT];
+    LPass hp[NPASS];
+    LPass* dp = nullptr;
+    ZOps zo{};
+    ZStage zs[4];
+    ZStage zr;
+    ZPost zp;
+    ZPrep zpre, zpreR;
+    Wind wind;
+    std::vector<double> Lz_host;     // nz*nz row-major
+    bool have_state = false;
+
+    bool prof = false;
+    struct Rec { int name; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+    int64_t counts[KN_COUNT] = {};
+    double totals[KN_COUNT] = {};
+};
+
+namespace {
+
+int dalloc(hqt_ctx* c, size_t bytes, void** out) {
+    void* p = nullptr;
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail(HQT_ERR_OOM, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    e = hipMemset(p, 0, bytes);
+    if (e != hipSuccess) { (void)hipFree(p); return fail(HQT_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e)); }
+    c->allocs.push_back(p);
+    *out = p;
+    return HQT_OK;
+}
+
+int upload_as(hqt_ctx* c, void* dst, const double* src, size_t n, int dt) {
+    if (dt == DT_F64) {
+        HIPC(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tmp(n);
+        for (size_t i = 0; i < n; ++i) tmp[i] = (float)src[i];
+        HIPC(hipMemcpy(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    }
+    (void)c;
+    return HQT_OK;
+}
+
+int download_as(double* dst, const void* src, size_t n, int dt) {
+    if (dt == DT_F64) {
+        HIPC(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tmp(n);
+        HIPC(hipMemcpy(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) dst[i] = tmp[i];
+    }
+    return HQT_OK;
+}
+
+// ---- line-pass descriptor helpers --------------------------------------------------------
+LPass base_pass(int N, int nx, int ny, int nz, int dir, void* const* ops, int32_t* flags) {
+    LPass P;
+    std::memset(&P, 0, sizeof(P));
+    P.N = N;
+    P.Np = 16 * ((N + 15) / 16);
+    P.S = P.Np + 1;
+    P.nx = nx; P.ny = ny; P.nz = nz;
+    P.nxc = (nx + 15) / 16;
+    if (dir == 0) P.ntiles = (int)(((int64_t)ny * nz + 15) / 16);
+    else P.ntiles = P.nxc * nz;
+    for (int i = 0; i < OP_COUNT; ++i) P.ops[i] = ops[i];
+    P.flags = flags;
+    return P;
+}
+
+LGroup& add_group(LPass& P, const void* s0, int d0, const void* s1 = nullptr, int d1 = DT_F64) {
+    LGroup& g = P.g[P.ngroups++];
+    std::memset(&g, 0, sizeof(g));
+    g.src[0] = s0; g.src_dt[0] = d0;
+    g.ntiles = 1;
+    if (s1) { g.src[1] = s1; g.src_dt[1] = d1; g.ntiles = 2; }
+    return g;
+}
+
+int slot_for(LGroup& g, int op) {
+    for (int s = 0; s < g.nslots; ++s)
+        if (g.slot_op[s] == op) return s;
+    g.slot_op[g.nslots] = op;
+    return g.nslots++;
+}
+
+void add_out(LGroup& g, void* out, int odt, int tile0, int op0, int tile1 = -1, int op1 = -1,
+             const void* addend = nullptr, int adt = DT_F64, int nan_check = 0) {
+    LOut& o = g.out[g.nout++];
+    std::memset(&o, 0, sizeof(o));
+    o.out = out; o.out_dt = odt; o.addend = addend; o.add_dt = adt; o.nan_check = nan_check;
+    o.term[o.nterms++] = LTerm{tile0, slot_for(g, op0)};
+    if (tile1 >= 0) o.term[o.nterms++] = LTerm{tile1, slot_for(g, op1)};
+}
+
+// Toeplitz tables of the x/y operators in the context precision.
+int build_tables(hqt_ctx* c, int N, double L, const uint8_t* mask, int base) {
+    std::vector<double> col, row;
+    fourdif_colrow(N, 1, col, row);
+    const double px = 2.0 * M_PI / L;                  // px = 2.*pi/L0 (:38)
+    const int Np = 16 * ((N + 15) / 16);
+    std::vector<std::vector<double>> tabs(6);
+    toeplitz_table(col, row, Np, px, tabs[0]);         // Lx = px*Dx (:47)
+    std::vector<double> cc(N), c2;
+    for (int m = 0; m < N; ++m) cc[m] = px * col[m];
+    circulant_square(cc, c2);                          // Lx*Lx (func_Lx applied twice)
+    std::vector<double> r2(N);
+    for (int m = 0; m < N; ++m) r2[m] = c2[(N - m) % N];
+    toeplitz_table(c2, r2, Np, 1.0, tabs[1]);
+    toeplitz_table(col, row, Np, -px, tabs[2]);
+    std::vector<double> pr, pi, prr(N), pir(N);
+    filter_tables(N, mask, pr, pi);
+    for (int m = 0; m < N; ++m) { prr[m] = pr[(N - m) % N]; pir[m] = pi[(N - m) % N]; }
+    toeplitz_table(pr, prr, Np, 1.0, tabs[3]);
+    toeplitz_table(pi, pir, Np, 1.0, tabs[4]);
+    toeplitz_table(pi, pir, Np, -1.0, tabs[5]);
+    for (int t = 0; t < 6; ++t) {
+        int rc = dalloc(c, tabs[t].size() * c->es, &c->ops[base + t]);
+        if (rc) return rc;
+        rc = upload_as(c, c->ops[base + t], tabs[t].data(), tabs[t].size(), c->tdt);
+        if (rc) return rc;
+    }
+    return HQT_OK;
+}
+
+template <class F>
+int run(hqt_ctx* c, int name, F&& f) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (c->prof) {
+        HIPC(hipEventCreate(&a));
+        HIPC(hipEventCreate(&b));
+        HIPC(hipEventRecord(a, c->stream));
+    }
+    hipError_t e = f();
+    if (e != hipSuccess) return fail(HQT_ERR_HIP, std::string("launch ") + kNames[name] + ": " + hipGetErrorString(e));
+    if (c->prof) {
+        HIPC(hipEventRecord(b, c->stream));
+        c->recs.push_back({name, a, b});
+    }
+    return HQT_OK;
+}
+
+int line(hqt_ctx* c, int pass, int dir, int name) {
+    return run(c, name, [&] { return launch_line_pass_dev(c->hp[pass], c->dp + pass, dir, c->fp32, c->stream); });
+}
+
+int enqueue_step(hqt_ctx* c) {
+    int rc;
+    if ((rc = run(c, KN_ZPREP, [&] { return launch_zprep(c->zpre, c->stream); }))) return rc;
+    for (int s = 0; s < 4; ++s) {
+        if ((rc = line(c, PX1 + s, 0, KN_LX_STAGE))) return rc;
+        if ((rc = line(c, PY1 + s, 1, KN_LY_STAGE))) return rc;
+        if ((rc = run(c, KN_ZSTAGE, [&] { return launch_zstage(c->zs[s], c->fp32, c->stream); }))) return rc;
+    }
+    if ((rc = line(c, PXC, 0, KN_LX_CONS))) return rc;
+    if ((rc = line(c, PYC, 1, KN_LY_CONS))) return rc;
+    if ((rc = run(c, KN_ZPOST, [&] { return launch_zpost(c->zp, c->fp32, c->stream); }))) return rc;
+    if (c->p.winding)
+        if ((rc = run(c, KN_WIND, [&] { return launch_winding(c->wind, c->fp32, c->stream); }))) return rc;
+    if (c->p.filter) {
+        if ((rc = line(c, PXF, 0, KN_LX_FILT))) return rc;
+        if ((rc = line(c, PYF, 1, KN_LY_FILT))) return rc;
+    } else {
+        if ((rc = run(c, KN_NAN, [&] {
+                 return launch_nancheck(c->ZR, c->ZI, c->ZAx, c->ZAy, c->At, c->npts, c->fp32, c->flags, c->stream);
+             })))
+            return rc;
+    }
+    return HQT_OK;
+}
+
+int resolve_profile(hqt_ctx* c) {
+    for (auto& r : c->recs) {
+        float ms = 0;
+        HIPC(hipEventElapsedTime(&ms, r.a, r.b));
+        c->counts[r.name] += 1;
+        c->totals[r.name] += ms;
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    c->recs.clear();
+    return HQT_OK;
+}
+
+int build_context(hqt_ctx* c) {
+    const int nx = c->nx, ny = c->ny, nz = c->nz;
+    const size_t es = c->es, np = (size_t)c->npts, nxy = (size_t)c->nxy;
+    int rc;
+#define ALLOC(ptr, bytes) do { void* q_; if ((rc = dalloc(c, (bytes), &q_))) return rc; ptr = (decltype(ptr))q_; } while (0)
+    ALLOC(c->ZR, np * es); ALLOC(c->ZI, np * es); ALLOC(c->ZAx, np * es); ALLOC(c->ZAy, np * es);
+    for (int i = 0; i < 4; ++i) { ALLOC(c->SA[i], np * es); ALLOC(c->SB[i], np * es); ALLOC(c->AC[i], np * es); }
+    ALLOC(c->dxR, np * es); ALLOC(c->dxI, np * es); ALLOC(c->dxxR, np * es); ALLOC(c->dxxI, np * es);
+    ALLOC(c->dxAx, np * es); ALLOC(c->dxAy, np * es); ALLOC(c->dxxAy, np * es);
+    ALLOC(c->dyR, np * es); ALLOC(c->dyI, np * es); ALLOC(c->lapR, np * es); ALLOC(c->lapI, np * es);
+    ALLOC(c->divA, np * es); ALLOC(c->curl3, np * es); ALLOC(c->curl4, np * es);
+    ALLOC(c->dxAt, np * es); ALLOC(c->dyAt, np * es);
+    for (int i = 0; i < 5; ++i) { ALLOC(c->fa[i], np * es); ALLOC(c->fb[i], np * es); }
+    ALLOC(c->At, np * 8); ALLOC(c->rho, np * 8); ALLOC(c->dzAt, np * 8); ALLOC(c->AtR, np * 8);
+    ALLOC(c->arho, nxy * 8); ALLOC(c->O, 2 * nxy * 8); ALLOC(c->part, (size_t)ny * 8); ALLOC(c->W, 8);
+    ALLOC(c->axb, nxy * 8); ALLOC(c->ayb, nxy * 8); ALLOC(c->yph, (size_t)ny * 8);
+    ALLOC(c->flags, 4 * sizeof(int32_t));
+    ALLOC(c->dp, NPASS * sizeof(LPass));
+
+    // ---- radial operators (tevol_driven_cluster.m:15,36-52,70-71) ----
+    std::vector<double> zc, DM;
+    chebdif(nz, 1, zc, DM);
+    const double small = 0.0, D = 0.0, zi = D, zf = 1 - small, dlz = zf - zi, pz = 2 / dlz, zh = 1.0;
+    std::vector<double> Lz((size_t)nz * nz), zz(nz);
+    for (int i = 0; i < nz; ++i)
+        for (int j = 0; j < nz; ++j) Lz[(size_t)i * nz + j] = pz * DM[i + (size_t)nz * j];
+    for (int k = 0; k < nz; ++k) zz[k] = (zc[k] - 1) * dlz / 2 + zf * 1.0;
+    const int m1 = nz - 1;
+    std::vector<double> A((size_t)m1 * m1), Lz1((size_t)m1 * m1, 0.0);
+    for (int i = 0; i < m1; ++i)
+        for (int j = 0; j < m1; ++j) A[(size_t)i * m1 + j] = Lz[(size_t)i * nz + j];
+    for (int i = 0; i < m1; ++i) Lz1[(size_t)i * m1 + i] = 1.0;
+    if (!lu_solve(m1, A, Lz1.data(), m1)) return fail(HQT_ERR_ARG, "Lz(1:nz-1,1:nz-1) is singular");
+    std::vector<double> zbuf;
+    zbuf.insert(zbuf.end(), Lz.begin(), Lz.end());
+    zbuf.insert(zbuf.end(), Lz1.begin(), Lz1.end());
+    const size_t off_hf = zbuf.size();
+    for (int k = 0; k < nz; ++k) zbuf.push_back(0.5 * (1 - (zz[k] / zh) * (zz[k] / zh) * (zz[k] / zh)));
+    const size_t off_cd = zbuf.size();
+    for (int k = 0; k < nz; ++k) zbuf.push_back(-0.5 * 3 * (zz[k] * zz[k]) / (zh * zh * zh));
+    const size_t off_cp = zbuf.size();
+    for (int k = 0; k < nz; ++k) zbuf.push_back(-0.5 * zz[k]);
+    const size_t off_f1 = zbuf.size();
+    for (int k = 0; k < nz; ++k) zbuf.push_back(1 - (zz[k] / zh) * (zz[k] / zh) * (zz[k] / zh));
+    const size_t off_f2 = zbuf.size();
+    for (int k = 0; k < nz; ++k) zbuf.push_back(1 - zz[k] * zz[k] * zz[k]);
+    ALLOC(c->zops, zbuf.size() * 8);
+    HIPC(hipMemcpy(c->zops, zbuf.data(), zbuf.size() * 8, hipMemcpyHostToDevice));
+    c->Lz_host = Lz;
+    c->zo.Lz = c->zops;
+    c->zo.Lz1 = c->zops + (size_t)nz * nz;
+    c->zo.hf = c->zops + off_hf;
+    c->zo.cd = c->zops + off_cd;
+    c->zo.cp = c->zops + off_cp;
+    c->zo.f1 = c->zops + off_f1;
+    c->zo.f2 = c->zops + off_f2;
+
+    // ---- boundary arrays and the winding phase (:43, :133, :139) ----
+    std::vector<double> bx(nxy), by(nxy), yph(ny);
+    for (size_t i = 0; i < nxy; ++i) {
+        bx[i] = c->p.ax_b ? c->p.ax_b[i] : c->p.ax_b_const;
+        by[i] = c->p.ay_b ? c->p.ay_b[i] : c->p.ay_b_const;
+    }
+    for (int j = 0; j < ny; ++j) {
+        const double ycheb = 2 * M_PI * j / ny;
+        const double yy = (ycheb * c->p.Ly) / 2 / M_PI;
+        yph[j] = 2 * M_PI * yy / c->p.Ly;
+    }
+    HIPC(hipMemcpy(c->axb, bx.data(), nxy * 8, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->ayb, by.data(), nxy * 8, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->yph, yph.data(), (size_t)ny * 8, hipMemcpyHostToDevice));
+
+    // ---- x/y operator tables (:13-14, :27-31, :38-39, :47-48) ----
+    std::vector<uint8_t> fx, fy;
+    matlab_masks(nx, fx);
+    matlab_masks(ny, fy);
+    if ((rc = build_tables(c, nx, c->p.Lx, fx.data(), OP_D1X))) return rc;
+    if ((rc = build_tables(c, ny, c->p.Ly, fy.data(), OP_D1Y))) return rc;
+
+    // ---- line-pass descriptors ----
+    const int T = c->tdt;
+    void* const* ops = c->ops;
+    void* Sin[4][4] = {{c->ZR, c->ZI, c->ZAx, c->ZAy},
+                       {c->SA[0], c->SA[1], c->SA[2], c->SA[3]},
+                       {c->SB[0], c->SB[1], c->SB[2], c->SB[3]},
+                       {c->SA[0], c->SA[1], c->SA[2], c->SA[3]}};
+    auto stage_x = [&](LPass& P, void* const* s, const double* at) {
+        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags);
+        LGroup* g = &add_group(P, s[0], T);
+        add_out(*g, c->dxR, T, 0, OP_D1X); add_out(*g, c->dxxR, T, 0, OP_D2X);
+        g = &add_group(P, s[1], T);
+        add_out(*g, c->dxI, T, 0, OP_D1X); add_out(*g, c->dxxI, T, 0, OP_D2X);
+        g = &add_group(P, s[2], T);
+        add_out(*g, c->dxAx, T, 0, OP_D1X);
+        g = &add_group(P, s[3], T);
+        add_out(*g, c->dxAy, T, 0, OP_D1X); add_out(*g, c->dxxAy, T, 0, OP_D2X);
+        if (at) { g = &add_group(P, at, DT_F64); add_out(*g, c->dxAt, T, 0, OP_D1X); }
+    };
+    auto stage_y = [&](LPass& P, void* const* s, const double* at) {
+        P = base_pass(ny, nx, ny, nz, 1, ops, c->flags);
+        LGroup* g = &add_group(P, s[0], T);
+        add_out(*g, c->dyR, T, 0, OP_D1Y); add_out(*g, c->lapR, T, 0, OP_D2Y, -1, -1, c->dxxR, T);
+        g = &add_group(P, s[1], T);
+        add_out(*g, c->dyI, T, 0, OP_D1Y); add_out(*g, c->lapI, T, 0, OP_D2Y, -1, -1, c->dxxI, T);
+        g = &add_group(P, s[3], T);
+        add_out(*g, c->divA, T, 0, OP_D1Y, -1, -1, c->dxAx, T);              // dxAx + dyAy
+        g = &add_group(P, s[2], T, c->dxAy, T);
+        add_out(*g, c->curl3, T, 0, OP_D2Y, 1, OP_ND1Y);                     // dyy Ax - dy(dx Ay)
+        g = &add_group(P, c->dxAx, T);
+        add_out(*g, c->curl4, T, 0, OP_ND1Y, -1, -1, c->dxxAy, T);           // dxx Ay - dy(dx Ax)
+        if (at) { g = &add_group(P, at, DT_F64); add_out(*g, c->dyAt, T, 0, OP_D1Y); }
+    };
+    for (int s = 0; s < 4; ++s) {
+        stage_x(c->hp[PX1 + s], Sin[s], s == 0 ? c->At : nullptr);
+        stage_y(c->hp[PY1 + s], Sin[s], s == 0 ? c->At : nullptr);
+    }
+    void* SR_[4] = {c->SB[0], c->SB[1], c->SB[2], c->SB[3]};
+    stage_x(c->hp[PXR], SR_, c->AtR);
+    stage_y(c->hp[PYR], SR_, c->AtR);
+    {
+        LPass& P = c->hp[PXC];
+        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags);
+        LGroup& g = add_group(P, c->ZAx, T);
+        add_out(g, c->dxAx, T, 0, OP_D1X);
+        LPass& Q = c->hp[PYC];
+        Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags);
+        LGroup& h = add_group(Q, c->ZAy, T);
+        add_out(h, c->divA, T, 0, OP_D1Y, -1, -1, c->dxAx, T);
+    }
+    {
+        void* f[5] = {c->ZR, c->ZI, c->At, c->ZAx, c->ZAy};
+        int fdt[5] = {T, T, DT_F64, T, T};
+        LPass& P = c->hp[PXF];
+        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags);
+        LPass& Q = c->hp[PYF];
+        Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags);
+        for (int i = 0; i < 5; ++i) {
+            LGroup& g = add_group(P, f[i], fdt[i]);
+            add_out(g, c->fa[i], T, 0, OP_PRX);
+            add_out(g, c->fb[i], T, 0, OP_PIX);
+            LGroup& h = add_group(Q, c->fa[i], T, c->fb[i], T);
+            add_out(h, f[i], fdt[i], 0, OP_QRY, 1, OP_NQIY, nullptr, DT_F64, 1);
+        }
+    }
+    HIPC(hipMemcpy(c->dp, c->hp, sizeof(c->hp), hipMemcpyHostToDevice));
+
+    // ---- radial kernel descriptors ----
+    for (int s = 0; s < 4; ++s) {
+        ZStage& Z = c->zs[s];
+        std::memset(&Z, 0, sizeof(Z));
+        Z.nxy = c->nxy; Z.nz = nz; Z.stage = s + 1; Z.mode = 0; Z.h = c->p.dt; Z.zo = c->zo;
+        Z.sR = Sin[s][0]; Z.sI = Sin[s][1]; Z.sAx = Sin[s][2]; Z.sAy = Sin[s][3];
+        Z.At = c->At; Z.dzAt = c->dzAt; Z.dxAt = c->dxAt; Z.dyAt = c->dyAt;
+        Z.dxR = c->dxR; Z.dxI = c->dxI; Z.dyR = c->dyR; Z.dyI = c->dyI; Z.lapR = c->lapR; Z.lapI = c->lapI;
+        Z.divA = c->divA; Z.curl3 = c->curl3; Z.curl4 = c->curl4;
+        Z.zR = c->ZR; Z.zI = c->ZI; Z.zAx = c->ZAx; Z.zAy = c->ZAy;
+        Z.aR = c->AC[0]; Z.aI = c->AC[1]; Z.aAx = c->AC[2]; Z.aAy = c->AC[3];
+        Z.rho = c->rho; Z.arho = c->arho;
+        void* const* nxt = (s == 0 || s == 2) ? c->SA : c->SB;
+        Z.nR = nxt[0]; Z.nI = nxt[1]; Z.nAx = nxt[2]; Z.nAy = nxt[3];
+        Z.flags = c->flags;
+    }
+    c->zr = c->zs[0];
+    c->zr.mode = 1;
+    c->zr.sR = c->SB[0]; c->zr.sI = c->SB[1]; c->zr.sAx = c->SB[2]; c->zr.sAy = c->SB[3];
+    c->zr.At = c->AtR;
+    std::memset(&c->zp, 0, sizeof(c->zp));
+    c->zp.nxy = c->nxy; c->zp.nz = nz; c->zp.mu = c->p.mu; c->zp.zo = c->zo;
+    c->zp.R = c->ZR; c->zp.I = c->ZI; c->zp.Ax = c->ZAx; c->zp.Ay = c->ZAy; c->zp.divA = c->divA;
+    c->zp.rho = c->rho; c->zp.At = c->At; c->zp.axb = c->axb; c->zp.ayb = c->ayb; c->zp.O = c->O;
+    c->zp.flags = c->flags;
+    c->zpre.nxy = c->nxy; c->zpre.nz = nz; c->zpre.Lz = c->zo.Lz; c->zpre.At = c->At; c->zpre.dzAt = c->dzAt;
+    c->zpre.flags = c->flags; c->zpre.bump = 1;
+    c->zpreR = c->zpre; c->zpreR.At = c->AtR; c->zpreR.bump = 0;
+    std::memset(&c->wind, 0, sizeof(c->wind));
+    c->wind.nx = nx; c->wind.ny = ny; c->wind.nz = nz; c->wind.nxy = c->nxy;
+    c->wind.O = c->O; c->wind.part = c->part; c->wind.R = c->ZR; c->wind.I = c->ZI;
+    c->wind.yph = c->yph; c->wind.W = c->W; c->wind.flags = c->flags;
+#undef ALLOC
+    return HQT_OK;
+}
+
+int check_params(const hqt_params* p) {
+    if (!p) return fail(HQT_ERR_ARG, "params is NULL");
+    if (p->nx < 2 || p->ny < 2 || p->nx > 512 || p->ny > 512)
+        return fail(HQT_ERR_ARG, "nx, ny must lie in [2, 512]");
+    if (!zkernels_support_nz(p->nz)) return fail(HQT_ERR_ARG, "nz must lie in [4, 32]");
+    if (!(p->Lx > 0) || !(p->Ly > 0)) return fail(HQT_ERR_ARG, "Lx, Ly must be positive");
+    if (p->precision != HQT_FP64 && p->precision != HQT_MIXED) return fail(HQT_ERR_ARG, "unknown precision");
+    return HQT_OK;
+}
+
+int read_flags(hqt_ctx* c, int32_t* nan_step) {
+    int32_t f[2];
+    HIPC(hipMemcpy(f, c->flags, sizeof(f), hipMemcpyDeviceToHost));
+    if (nan_step) *nan_step = f[0];
+    if (f[0] != 0) return fail(HQT_ERR_NAN, "NaN in the state at step " + std::to_string(f[0]));
+    return HQT_OK;
+}
+
+void split_complex(const double* Z, size_t n, std::vector<double>& re, std::vector<double>& im) {
+    re.resize(n); im.resize(n);
+    for (size_t i = 0; i < n; ++i) { re[i] = Z[2 * i]; im[i] = Z[2 * i + 1]; }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hqt_last_error(void) { return g_err.c_str(); }
+const char* hqt_version(void) { return "hqt 0.1 (gfx950)"; }
+
+int hqt_ctx_create(const hqt_params* p, hqt_ctx** out) {
+    if (!out) return fail(HQT_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int rc = check_params(p);
+    if (rc) return rc;
+    hqt_ctx* c = new (std::nothrow) hqt_ctx();
+    if (!c) return fail(HQT_ERR_OOM, "host allocation failed");
+    c->p = *p;
+    c->p.ax_b = nullptr;   // copied to the device below; the caller's arrays are borrowed
+    c->p.ay_b = nullptr;
+    c->nx = p->nx; c->ny = p->ny; c->nz = p->nz;
+    c->nxy = (int64_t)p->nx * p->ny;
+    c->npts = c->nxy * p->nz;
+    c->fp32 = p->precision == HQT_MIXED;
+    c->tdt = c->fp32 ? DT_F32 : DT_F64;
+    c->es = c->fp32 ? 4 : 8;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return fail(HQT_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+    hqt_params tmp = *p;
+    c->p = tmp;     // keep the caller's boundary pointers only during build_context
+    rc = build_context(c);
+    c->p.ax_b = nullptr;
+    c->p.ay_b = nullptr;
+    if (rc) { hqt_ctx_destroy(c); return rc; }
+    *out = c;
+    return HQT_OK;
+}
+
+int hqt_ctx_destroy(hqt_ctx* c) {
+    if (!c) return HQT_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto& r : c->recs) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+    for (void* q : c->allocs) hipFree(q);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return HQT_OK;
+}
+
+int hqt_set_state(hqt_ctx* c, const double* Z1, const double* Z2, const double* Z3, const double* Z4,
+                  const double* rhoi) {
+    if (!c || !Z1 || !Z2 || !Z3 || !Z4 || !rhoi) return fail(HQT_ERR_ARG, "NULL argument");
+    HIPC(hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)c->npts;
+    std::vector<double> re, im;
+    split_complex(Z1, n, re, im);
+    int rc;
+    if ((rc = upload_as(c, c->ZR, re.data(), n, c->tdt))) return rc;
+    if ((rc = upload_as(c, c->ZI, im.data(), n, c->tdt))) return rc;
+    if ((rc = upload_as(c, c->ZAx, Z3, n, c->tdt))) return rc;
+    if ((rc = upload_as(c, c->ZAy, Z4, n, c->tdt))) return rc;
+    HIPC(hipMemcpy(c->At, Z2, n * 8, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->rho, rhoi, n * 8, hipMemcpyHostToDevice));
+    HIPC(hipMemset(c->flags, 0, 4 * sizeof(int32_t)));
+    c->have_state = true;
+    return HQT_OK;
+}
+
+int hqt_get_state(hqt_ctx* c, double* Z1, double* Z2, double* Z3, double* Z4, double* rhoi) {
+    if (!c) return fail(HQT_ERR_ARG, "NULL context");
+    if (!c->have_state) return fail(HQT_ERR_STATE, "no state uploaded");
+    HIPC(hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)c->npts;
+    int rc;
+    if (Z1) {
+        std::vector<double> re(n), im(n);
+        if ((rc = download_as(re.data(), c->ZR, n, c->tdt))) return rc;
+        if ((rc = download_as(im.data(), c->ZI, n, c->tdt))) return rc;
+        for (size_t i = 0; i < n; ++i) { Z1[2 * i] = re[i]; Z1[2 * i + 1] = im[i]; }
+    }
+    if (Z2) HIPC(hipMemcpy(Z2, c->At, n * 8, hipMemcpyDeviceToHost));
+    if (Z3 && (rc = download_as(Z3, c->ZAx, n, c->tdt))) return rc;
+    if (Z4 && (rc = download_as(Z4, c->ZAy, n, c->tdt))) return rc;
+    if (rhoi) HIPC(hipMemcpy(rhoi, c->rho, n * 8, hipMemcpyDeviceToHost));
+    return read_flags(c, nullptr);
+}
+
+int hqt_step_n(hqt_ctx* c, int32_t nsteps) {
+    if (!c) return fail(HQT_ERR_ARG, "NULL context");
+    if (!c->have_state) return fail(HQT_ERR_STATE, "hqt_step_n before hqt_set_state");
+    if (nsteps < 0) return fail(HQT_ERR_ARG, "nsteps < 0");
+    for (int i = 0; i < nsteps; ++i) {
+        int rc = enqueue_step(c);
+        if (rc) return rc;
+    }
+    return HQT_OK;
+}
+
+int hqt_sync(hqt_ctx* c, int32_t* nan_step) {
+    if (!c) return fail(HQT_ERR_ARG, "NULL context");
+    HIPC(hipStreamSynchronize(c->stream));
+    int rc = resolve_profile(c);
+    if (rc) return rc;
+    return read_flags(c, nan_step);
+}
+
+int hqt_last_winding(hqt_ctx* c, double* w) {
+    if (!c || !w) return fail(HQT_ERR_ARG, "NULL argument");
+    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(hipMemcpy(w, c->W, 8, hipMemcpyDeviceToHost));
+    return HQT_OK;
+}
+
+int hqt_rhs(hqt_ctx* c, const double* Z1, const double* Z2, const double* Z3, const double* Z4, double* F1,
+            double* F2, double* F3, double* F4) {
+    if (!c || !Z1 || !Z2 || !Z3 || !Z4) return fail(HQT_ERR_ARG, "NULL argument");
+    const size_t n = (size_t)c->npts;
+    int rc;
+    if (!c->F1R) {
+#define ALLOC(ptr, bytes) do { void* q_; if ((rc = dalloc(c, (bytes), &q_))) return rc; ptr = (decltype(ptr))q_; } while (0)
+        ALLOC(c->F1R, n * 8); ALLOC(c->F1I, n * 8); ALLOC(c->F2, n * 8); ALLOC(c->F3, n * 8); ALLOC(c->F4, n * 8);
+#undef ALLOC
+        c->zr.F1R = c->F1R; c->zr.F1I = c->F1I; c->zr.F2 = c->F2; c->zr.F3 = c->F3; c->zr.F4 = c->F4;
+    }
+    HIPC(hipStreamSynchronize(c->stream));
+    std::vector<double> re, im;
+    split_complex(Z1, n, re, im);
+    if ((rc = upload_as(c, c->SB[0], re.data(), n, c->tdt))) return rc;
+    if ((rc = upload_as(c, c->SB[1], im.data(), n, c->tdt))) return rc;
+    if ((rc = upload_as(c, c->SB[2], Z3, n, c->tdt))) return rc;
+    if ((rc = upload_as(c, c->SB[3], Z4, n, c->tdt))) return rc;
+    HIPC(hipMemcpy(c->AtR, Z2, n * 8, hipMemcpyHostToDevice));
+    int32_t saved[2];
+    HIPC(hipMemcpy(saved, c->flags, sizeof(saved), hipMemcpyDeviceToHost));
+    HIPC(hipMemset(c->flags, 0, sizeof(saved)));
+    if ((rc = run(c, KN_ZPREP, [&] { return launch_zprep(c->zpreR, c->stream); }))) return rc;
+    if ((rc = line(c, PXR, 0, KN_LX_RHS))) return rc;
+    if ((rc = line(c, PYR, 1, KN_LY_RHS))) return rc;
+    if ((rc = run(c, KN_ZRHS, [&] { return launch_zstage(c->zr, c->fp32, c->stream); }))) return rc;
+    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(hipMemcpy(c->flags, saved, sizeof(saved), hipMemcpyHostToDevice));
+    if (F1) {
+        std::vector<double> r(n), i(n);
+        HIPC(hipMemcpy(r.data(), c->F1R, n * 8, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(i.data(), c->F1I, n * 8, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < n; ++k) { F1[2 * k] = r[k]; F1[2 * k + 1] = i[k]; }
+    }
+    if (F2) HIPC(hipMemcpy(F2, c->F2, n * 8, hipMemcpyDeviceToHost));
+    if (F3) HIPC(hipMemcpy(F3, c->F3, n * 8, hipMemcpyDeviceToHost));
+    if (F4) HIPC(hipMemcpy(F4, c->F4, n * 8, hipMemcpyDeviceToHost));
+    return resolve_profile(c);
+}
+
+int hqt_order_parameter(hqt_ctx* c, double* O) {
+    if (!c || !O) return fail(HQT_ERR_ARG, "NULL argument");
+    if (!c->have_state) return fail(HQT_ERR_STATE, "no state uploaded");
+    HIPC(hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)c->npts, nxy = (size_t)c->nxy;
+    std::vector<double> re(n), im(n);
+    int rc;
+    if ((rc = download_as(re.data(), c->ZR, n, c->tdt))) return rc;
+    if ((rc = download_as(im.data(), c->ZI, n, c->tdt))) return rc;
+    const int nz = c->nz;
+    for (size_t col = 0; col < nxy; ++col) {
+        double a = 0, b = 0;
+        for (int m = 0; m < nz; ++m) {
+            a += c->Lz_host[(size_t)(nz - 1) * nz + m] * re[col + nxy * m];
+            b += c->Lz_host[(size_t)(nz - 1) * nz + m] * im[col + nxy * m];
+        }
+        O[2 * col] = a; O[2 * col + 1] = b;
+    }
+    return HQT_OK;
+}
+
+int hqt_ff(int32_t nx, int32_t ny, int32_t nslices, const double* x_re, const double* x_im, const uint8_t* fx,
+           const uint8_t* fy, double* w) {
+    if (!x_re || !fx || !fy || !w) return fail(HQT_ERR_ARG, "NULL argument");
+    if (nx < 1 || ny < 1 || nx > 512 || ny > 512 || nslices < 1) return fail(HQT_ERR_ARG, "bad shape");
+    hqt_ctx tmp;   // scratch holder for allocations only
+    tmp.es = 8; tmp.tdt = DT_F64;
+    const size_t n = (size_t)nx * ny * nslices;
+    int rc = HQT_OK;
+    void *u = nullptr, *v = nullptr, *a = nullptr, *b = nullptr, *out = nullptr;
+    int32_t* flags = nullptr;
+    LPass* dp = nullptr;
+    void* ops[OP_COUNT] = {};
+    hipStream_t s = nullptr;
+    do {
+        if ((rc = dalloc(&tmp, n * 8, &u))) break;
+        if (x_im && (rc = dalloc(&tmp, n * 8, &v))) break;
+        if ((rc = dalloc(&tmp, n * 8, &a))) break;
+        if ((rc = dalloc(&tmp, n * 8, &b))) break;
+        if ((rc = dalloc(&tmp, n * 8, &out))) break;
+        void* fp;
+        if ((rc = dalloc(&tmp, 16, &fp))) break;
+        flags = (int32_t*)fp;
+        void* dpp;
+        if ((rc = dalloc(&tmp, 2 * sizeof(LPass), &dpp))) break;
+        dp = (LPass*)dpp;
+        // filter tables only (D1/D2 unused by this entry point, the table slots stay NULL)
+        for (int dir = 0; dir < 2; ++dir) {
+            const int N = dir ? ny : nx;
+            const uint8_t* mk = dir ? fy : fx;
+            const int Np = 16 * ((N + 15) / 16);
+            std::vector<double> pr, pi, prr(N), pir(N), t0, t1, t2;
+            filter_tables(N, mk, pr, pi);
+            for (int m = 0; m < N; ++m) { prr[m] = pr[(N - m) % N]; pir[m] = pi[(N - m) % N]; }
+            toeplitz_table(pr, prr, Np, 1.0, t0);
+            toeplitz_table(pi, pir, Np, 1.0, t1);
+            toeplitz_table(pi, pir, Np, -1.0, t2);
+            const int base = dir ? OP_D1Y : OP_D1X;
+            std::vector<double>* tt[3] = {&t0, &t1, &t2};
+            for (int k = 0; k < 3; ++k) {
+                if ((rc = dalloc(&tmp, tt[k]->size() * 8, &ops[base + 3 + k]))) break;
+                if (hipMemcpy(ops[base + 3 + k], tt[k]->data(), tt[k]->size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+                    rc = fail(HQT_ERR_HIP, "hipMemcpy table");
+                    break;
+                }
+            }
+            if (rc) break;
+        }
+        if (rc) break;
+        if (hipMemcpy(u, x_re, n * 8, hipMemcpyHostToDevice) != hipSuccess) { rc = fail(HQT_ERR_HIP, "hipMemcpy x"); break; }
+        if (x_im && hipMemcpy(v, x_im, n * 8, hipMemcpyHostToDevice) != hipSuccess) { rc = fail(HQT_ERR_HIP, "hipMemcpy x_im"); break; }
+        LPass hp[2];
+        hp[0] = base_pass(nx, nx, ny, nslices, 0, ops, flags);
+        LGroup& g = add_group(hp[0], u, DT_F64, v, DT_F64);
+        if (x_im) {
+            // Px (u + i v) = (Pr u - Pi v) + i (Pi u + Pr v)
+            add_out(g, a, DT_F64, 0, OP_PRX, 1, OP_NPIX);
+            add_out(g, b, DT_F64, 0, OP_PIX, 1, OP_PRX);
+        } else {
+            g.ntiles = 1;
+            add_out(g, a, DT_F64, 0, OP_PRX);
+            add_out(g, b, DT_F64, 0, OP_PIX);
+        }
+        hp[1] = base_pass(ny, nx, ny, nslices, 1, ops, flags);
+        LGroup& h = add_group(hp[1], a, DT_F64, b, DT_F64);
+        add_out(h, out, DT_F64, 0, OP_QRY, 1, OP_NQIY);     // Re(Py (a + i b)) = Qr a - Qi b
+        if (hipMemcpy(dp, hp, sizeof(hp), hipMemcpyHostToDevice) != hipSuccess) { rc = fail(HQT_ERR_HIP, "hipMemcpy pass"); break; }
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) { rc = fail(HQT_ERR_HIP, "stream"); break; }
+        hipError_t e = launch_line_pass_dev(hp[0], dp, 0, 0, s);
+        if (e == hipSuccess) e = launch_line_pass_dev(hp[1], dp + 1, 1, 0, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) { rc = fail(HQT_ERR_HIP, std::string("hqt_ff: ") + hipGetErrorString(e)); break; }
+        if (hipMemcpy(w, out, n * 8, hipMemcpyDeviceToHost) != hipSuccess) { rc = fail(HQT_ERR_HIP, "hipMemcpy w"); break; }
+    } while (0);
+    if (s) hipStreamDestroy(s);
+    for (void* q : tmp.allocs) hipFree(q);
+    tmp.allocs.clear();
+    return rc;
+}
+
+int hqt_fourdif(int32_t N, int32_t m, double* x, double* DM) {
+    if (N < 1 || m < 0 || !x || !DM) return fail(HQT_ERR_ARG, "bad argument");
+    fourdif_dense(N, m, x, DM);
+    return HQT_OK;
+}
+
+int hqt_chebdif(int32_t N, int32_t M, double* x, double* DM) {
+    if (N < 2 || M < 1 || M > N - 1 || !x || !DM) return fail(HQT_ERR_ARG, "need N >= 2 and 0 < M <= N-1");
+    std::vector<double> xv, dm;
+    chebdif(N, M, xv, dm);
+    std::memcpy(x, xv.data(), N * sizeof(double));
+    std::memcpy(DM, dm.data(), dm.size() * sizeof(double));
+    return HQT_OK;
+}
+
+int hqt_newton_vy(int32_t nz, double zh, double small, double D, const double* Z1, const double* Z2,
+                  const double* vy1, double vy, double* z1, double* z2, double* VY1, int32_t* it, double* err) {
+    if (nz < 3 || !Z1 || !Z2 || !vy1 || !z1 || !z2 || !VY1 || !it || !err) return fail(HQT_ERR_ARG, "bad argument");
+    int iti = 0;
+    const int r = newton_vy(nz, zh, small, D, Z1, Z2, vy1, vy, z1, z2, VY1, &iti, err);
+    *it = iti;
+    if (r) return fail(HQT_ERR_NOCONV, "NewtonRaphson_vy: no convergence (it=" + std::to_string(iti) + ")");
+    return HQT_OK;
+}
+
+int hqt_profile(hqt_ctx* c, int32_t enable) {
+    if (!c) return fail(HQT_ERR_ARG, "NULL context");
+    HIPC(hipStreamSynchronize(c->stream));
+    int rc = resolve_profile(c);
+    if (rc) return rc;
+    c->prof = enable != 0;
+    for (int i = 0; i < KN_COUNT; ++i) { c->counts[i] = 0; c->totals[i] = 0; }
+    return HQT_OK;
+}
+
+int hqt_kernel_stats(hqt_ctx* c, int32_t idx, char* name, int32_t name_len, int64_t* count, double* total_ms) {
+    if (!c) return fail(HQT_ERR_ARG, "NULL context");
+    if (idx < 0 || idx >= KN_COUNT) return fail(HQT_ERR_ARG, "index past the last kernel");
+    HIPC(hipStreamSynchronize(c->stream));
+    int rc = resolve_profile(c);
+    if (rc) return rc;
+    if (name && name_len > 0) std::snprintf(name, name_len, "%s", kNames[idx]);
+    if (count) *count = c->counts[idx];
+    if (total_ms) *total_ms = c->totals[idx];
+    return HQT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+// Internal interface between the host library (hqt_host.cpp) and the gfx950 kernels
+// (hqt_kernels.hip).  Plain structs passed by value as kernel arguments.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hqt {
+
+enum Dtype : int32_t { DT_F64 = 0, DT_F32 = 1 };
+
+// ---------------------------------------------------------------------------------------
+// Line pass: apply Toeplitz/circulant line operators along x (dir 0) or y (dir 1).
+// One work-group owns a tile of 16 lines and one "group" of outputs.  An output is a sum
+// of up to two terms op(tile) plus an optional point-wise addend field.
+// ---------------------------------------------------------------------------------------
+constexpr int LP_MAXG = 8;      // groups per pass
+constexpr int LP_MAXOPS = 12;   // operator table entries per pass
+constexpr int LP_THREADS = 256;
+
+struct LTerm { int32_t tile; int32_t slot; };
+struct LOut {
+    void* out; const void* addend;
+    int32_t out_dt, add_dt;
+    int32_t nterms; LTerm term[2];
+    int32_t nan_check;
+};
+struct LGroup {
+    int32_t ntiles; int32_t src_dt[2]; const void* src[2];
+    int32_t nslots; int32_t slot_op[4];
+    int32_t nout; LOut out[2];
+};
+struct LPass {
+    int32_t N, Np, S;          // line length, padded length (multiple of 16), LDS row stride
+    int32_t nx, ny, nz;
+    int32_t ntiles;            // line tiles (grid.x)
+    int32_t nxc;               // dir 1: x chunks of 16
+    int32_t ngroups;           // grid.y
+    LGroup g[LP_MAXG];
+    const void* ops[LP_MAXOPS]; // Toeplitz tables, 2*Np entries of the pass precision
+    int32_t* flags;            // [0] = NaN step (0: none), [1] = step counter
+};
+
+// ---------------------------------------------------------------------------------------
+// Radial (z) kernels: one thread per boundary-plane column.
+// ---------------------------------------------------------------------------------------
+struct ZOps {
+    const double* Lz;     // nz*nz row-major (row k, column m)
+    const double* Lz1;    // (nz-1)*(nz-1) row-major
+    const double* hf;     // 0.5*f(z)           (F1 term 0.5*(1-(zz/zh)^3))
+    const double* cd;     // -0.5*3*zz^2/zh^3   (F1 coefficient of d_z Psi)
+    const double* cp;     // -0.5*zz            (F1 coefficient of Psi)
+    const double* f1;     // 1-(zz/zh)^3         (F3/F4)
+    const double* f2;     // 1-zz^3              (F2)
+};
+
+struct ZStage {
+    int64_t nxy; int32_t nz; int32_t stage; int32_t mode; // mode 0: RK stage, 1: write F1..F4
+    double h;
+    ZOps zo;
+    const void *sR, *sI, *sAx, *sAy;          // stage state (TS)
+    const double *At, *dzAt;                  // frozen A_t and d_z A_t
+    const void *dxAt, *dyAt;                  // frozen x/y derivatives of A_t (TS)
+    const void *dxR, *dxI, *dyR, *dyI, *lapR, *lapI, *divA, *curl3, *curl4;  // (TS)
+    void *zR, *zI, *zAx, *zAy;                // step-start state (TS), updated at stage 4
+    void *aR, *aI, *aAx, *aAy;                // RK accumulators (TS)
+    double *rho, *arho;                       // rho [nxy*nz], boundary-flux accumulator [nxy]
+    void *nR, *nI, *nAx, *nAy;                // next stage state (TS)
+    double *F1R, *F1I, *F2, *F3, *F4;         // mode 1 outputs
+    int32_t* flags;
+};
+
+struct ZPost {
+    int64_t nxy; int32_t nz; double mu;
+    ZOps zo;
+    void *R, *I, *Ax, *Ay;        // state (TS): read Psi, reset z=0 rows
+    const void* divA;             // d_x A_x + d_y A_y of the updated state (TS)
+    const double* rho;            // [nxy*nz]
+    double* At;                   // out
+    const double *axb, *ayb;      // [nxy] boundary values
+    double* O;                    // [2*nxy] boundary order parameter (re, im)
+    int32_t* flags;
+};
+
+struct ZPrep {
+    int64_t nxy; int32_t nz;
+    const double* Lz; const double* At; double* dzAt;
+    int32_t* flags; int32_t bump;   // bump: increment the step counter
+};
+
+struct Wind {
+    int32_t nx, ny, nz; int64_t nxy;
+    const double* O; double* part;   // [ny] partial sums
+    void *R, *I;                     // Psi (TS)
+    const double* yph;               // [ny] 2*pi*yy/L
+    double* W;                       // result
+    int32_t* flags;
+};
+
+// Launchers (hqt_kernels.hip).  All return the hipError_t of the launch.
+// H: host copy of the descriptor (grid sizes, LDS bytes); dP: the same descriptor in device memory.
+hipError_t launch_line_pass_dev(const LPass& H, const LPass* dP, int dir, int fp32, hipStream_t s);
+size_t line_pass_lds_bytes(const LPass& P, int fp32);
+hipError_t launch_zstage(const ZStage& A, int fp32, hipStream_t s);
+hipError_t launch_zpost(const ZPost& A, int fp32, hipStream_t s);
+hipError_t launch_zprep(const ZPrep& A, hipStream_t s);
+hipError_t launch_winding(const Wind& A, int fp32, hipStream_t s);
+// NaN guard over the state when the filter (which carries the guard) is off.
+hipError_t launch_nancheck(const void* R, const void* I, const void* Ax, const void* Ay, const double* At,
+                           int64_t n, int fp32, int32_t* flags, hipStream_t s);
+bool zkernels_support_nz(int nz);
+
+}  // namespace hqt

@@ -1,0 +1,212 @@
+// Line-operator passes for gfx950 (MI355X): the x/y spectral derivatives and the Fourier
+// filter of the AdS4 stepper, applied as Toeplitz matrix products on MFMA.
+//
+// Reference: tevol_driven_cluster.m:60-61 (func_Lx / func_Ly: dense fourdif matrix times
+// every line), src/numerics/ff.m:7-11 (filter).  A work-group owns 16 lines (the MFMA
+// N/M dimension) of one output group and stages those lines in LDS once; each wave then
+// sweeps its output position tiles over the full line with 16x16x4 MFMAs.  The operator
+// matrix is never stored: its 16x4 fragments are read from a 2*Np-entry Toeplitz table
+// (T[i][j] = tab[i - j + Np]) kept in LDS.
+//
+// Orientation: x-lines are contiguous in memory, so the output tile is (line x position)
+// and every 16 lanes write 16 consecutive x values; y-lines are strided by nx, so the tile
+// is (position x line) with lines = 16 consecutive x values, which again makes every 16
+// lanes write 128 contiguous bytes.  The MFMA operand order is the only difference.
+#include "hqt_kernels.h"
+
+namespace hqt {
+
+template <typename T> struct MF;
+template <> struct MF<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct MF<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ T ldv(const void* p, int dt, int64_t i) {
+    return dt == DT_F64 ? (T)(static_cast<const double*>(p)[i]) : (T)(static_cast<const float*>(p)[i]);
+}
+template <typename T>
+__device__ __forceinline__ void stv(void* p, int dt, int64_t i, T v) {
+    if (dt == DT_F64) static_cast<double*>(p)[i] = (double)v;
+    else static_cast<float*>(p)[i] = (float)v;
+}
+
+template <typename T, int DIR, int Q>
+__global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restrict__ Pp) {
+    const LPass& P = *Pp;
+    if (__builtin_nontemporal_load(&P.flags[0]) != 0) return;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    const int S = P.S, Np = P.Np, N = P.N;
+    T* V = reinterpret_cast<T*>(smem_raw);
+    T* C = V + 16 * S;
+    const LGroup& G = P.g[blockIdx.y];
+    const int tid = threadIdx.x;
+
+    for (int s = 0; s < G.nslots; ++s) {
+        const T* src = static_cast<const T*>(P.ops[G.slot_op[s]]);
+        for (int t = tid; t < 2 * Np; t += LP_THREADS) C[s * 2 * Np + t] = src[t];
+    }
+
+    int64_t line0 = 0, nl = 0;
+    int z = 0, x0 = 0;
+    if (DIR == 0) {
+        line0 = (int64_t)blockIdx.x * 16;
+        nl = (int64_t)P.ny * P.nz;
+    } else {
+        const int xc = blockIdx.x % P.nxc;
+        z = blockIdx.x / P.nxc;
+        x0 = xc * 16;
+    }
+
+    typedef typename MF<T>::v4 v4;
+    v4 acc[2][Q];
+#pragma unroll
+    for (int o = 0; o < 2; ++o)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) acc[o][q] = v4{0, 0, 0, 0};
+
+    const int lane = tid & 63, wave = tid >> 6, ln = lane & 15, lk = lane >> 4;
+
+    for (int t = 0; t < G.ntiles; ++t) {
+        __syncthreads();
+        const void* src = G.src[t];
+        const int dt = G.src_dt[t];
+        if (DIR == 0) {
+            for (int idx = tid; idx < 16 * Np; idx += LP_THREADS) {
+                const int m = idx / Np, j = idx - m * Np;
+                const int64_t line = line0 + m;
+                T v = 0;
+                if (line < nl && j < N) v = ldv<T>(src, dt, line * P.nx + j);
+                V[m * S + j] = v;
+            }
+        } else {
+            for (int idx = tid; idx < 16 * Np; idx += LP_THREADS) {
+                const int j = idx >> 4, m = idx & 15;
+                const int x = x0 + m;
+                T v = 0;
+                if (x < P.nx && j < N) v = ldv<T>(src, dt, (int64_t)x + (int64_t)P.nx * (j + (int64_t)P.ny * z));
+                V[m * S + j] = v;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int o = 0; o < 2; ++o) {
+            if (o >= G.nout) break;
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                if (e >= G.out[o].nterms || G.out[o].term[e].tile != t) continue;
+                const T* Cs = C + G.out[o].term[e].slot * 2 * Np + Np + ln - lk;
+                const T* Vr = V + ln * S + lk;
+#pragma unroll 4
+                for (int j0 = 0; j0 < Np; j0 += 4) {
+                    const T va = Vr[j0];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) {
+                        const int i0 = (wave + 4 * q) * 16;
+                        if (i0 < Np) {
+                            const T cb = Cs[i0 - j0];
+                            acc[o][q] = DIR == 0 ? MF<T>::mma(va, cb, acc[o][q]) : MF<T>::mma(cb, va, acc[o][q]);
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+        if (o >= G.nout) break;
+        const LOut& O = G.out[o];
+        bool bad = false;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int i0 = (wave + 4 * q) * 16;
+            if (i0 >= Np) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int64_t idx;
+                bool ok;
+                if (DIR == 0) {
+                    const int64_t line = line0 + 4 * lk + r;
+                    const int i = i0 + ln;
+                    ok = line < nl && i < N;
+                    idx = line * P.nx + i;
+                } else {
+                    const int x = x0 + ln;
+                    const int i = i0 + 4 * lk + r;
+                    ok = x < P.nx && i < N;
+                    idx = (int64_t)x + (int64_t)P.nx * (i + (int64_t)P.ny * z);
+                }
+                if (ok) {
+                    T v = acc[o][q][r];
+                    if (O.addend) v += ldv<T>(O.addend, O.add_dt, idx);
+                    stv<T>(O.out, O.out_dt, idx, v);
+                    if (v != v) bad = true;
+                }
+            }
+        }
+        if (O.nan_check) {
+            if (__any(bad) && lane == 0) {
+                const int step = __atomic_load_n(&P.flags[1], __ATOMIC_RELAXED);
+                atomicCAS(&P.flags[0], 0, step > 0 ? step : 1);
+            }
+        }
+    }
+}
+
+template <typename T, int DIR>
+static hipError_t launch_q(const LPass& H, const LPass* dP, int Q, size_t lds, hipStream_t s) {
+    dim3 grid(H.ntiles, H.ngroups), block(LP_THREADS);
+    switch (Q) {
+#define HQT_LP_CASE(q) \
+    case q: hipLaunchKernelGGL((k_line_pass<T, DIR, q>), grid, block, lds, s, dP); break;
+        HQT_LP_CASE(1) HQT_LP_CASE(2) HQT_LP_CASE(3) HQT_LP_CASE(4)
+        HQT_LP_CASE(5) HQT_LP_CASE(6) HQT_LP_CASE(7) HQT_LP_CASE(8)
+#undef HQT_LP_CASE
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+size_t line_pass_lds_bytes(const LPass& P, int fp32) {
+    int maxslots = 0;
+    for (int g = 0; g < P.ngroups; ++g) maxslots = P.g[g].nslots > maxslots ? P.g[g].nslots : maxslots;
+    const size_t es = fp32 ? sizeof(float) : sizeof(double);
+    return es * (16 * (size_t)P.S + (size_t)maxslots * 2 * P.Np);
+}
+
+template <typename T, int DIR, int Q>
+static void set_lds_attr(size_t bytes) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_line_pass<T, DIR, Q>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+// H: host copy of the descriptor (for grid sizes), dP: the same descriptor in device memory.
+hipError_t launch_line_pass_dev(const LPass& H, const LPass* dP, int dir, int fp32, hipStream_t s) {
+    const int Q = (H.Np / 16 + 3) / 4;
+    const size_t lds = line_pass_lds_bytes(H, fp32);
+    if (lds > 64 * 1024) {
+        static bool done[2][2][9] = {};
+        if (!done[fp32][dir][Q]) {
+            done[fp32][dir][Q] = true;
+            const size_t cap = 160 * 1024;
+#define HQT_ATTR(q) if (Q == q) { if (fp32) { if (dir) set_lds_attr<float, 1, q>(cap); else set_lds_attr<float, 0, q>(cap); } \
+                                  else { if (dir) set_lds_attr<double, 1, q>(cap); else set_lds_attr<double, 0, q>(cap); } }
+            HQT_ATTR(1) HQT_ATTR(2) HQT_ATTR(3) HQT_ATTR(4) HQT_ATTR(5) HQT_ATTR(6) HQT_ATTR(7) HQT_ATTR(8)
+#undef HQT_ATTR
+        }
+    }
+    if (fp32) return dir ? launch_q<float, 1>(H, dP, Q, lds, s) : launch_q<float, 0>(H, dP, Q, lds, s);
+    return dir ? launch_q<double, 1>(H, dP, Q, lds, s) : launch_q<double, 0>(H, dP, Q, lds, s);
+}
+
+}  // namespace hqt

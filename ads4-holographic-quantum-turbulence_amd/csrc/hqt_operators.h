@@ -1,0 +1,30 @@
+// Host-side operator construction and the host Newton solver (hqt_operators.cpp).
+#pragma once
+#include <stdint.h>
+#include <vector>
+
+namespace hqt {
+
+// First column / first row of fourdif(N, m) (dmsuite fourdif.m:30-71).
+void fourdif_colrow(int N, int m, std::vector<double>& col, std::vector<double>& row);
+// x (N) and DM (N*N, column-major) of fourdif(N, m).
+void fourdif_dense(int N, int m, double* x, double* DM);
+// x (N) and DM (N*N*M, column-major) of chebdif(N, M) (dmsuite chebdif.m:25-53).
+void chebdif(int N, int M, std::vector<double>& x, std::vector<double>& DM);
+// Solve A X = B in place (A row-major n*n by value, B row-major n*nrhs); false if singular.
+bool lu_solve(int n, std::vector<double> A, double* B, int nrhs);
+// 0/1 mask fx(round(round(n/3)):round(n/3*2)) = 1 (tevol_driven_cluster.m:27-30).
+void matlab_masks(int n, std::vector<uint8_t>& f);
+// First column (pr + i pi) of the complex circulant F^-1 diag(1-mask) F.
+void filter_tables(int N, const uint8_t* mask, std::vector<double>& pr, std::vector<double>& pi);
+// First column of C*C for the circulant C with first column c (long-double accumulation).
+void circulant_square(const std::vector<double>& c, std::vector<double>& c2);
+// Toeplitz lookup table tab[t] = scale * T[i][j] with t = i - j + Np, 2*Np entries.
+void toeplitz_table(const std::vector<double>& col, const std::vector<double>& row, int Np, double scale,
+                    std::vector<double>& tab);
+// NewtonRaphson_vy; returns 0 when the residual test passed.
+int newton_vy(int nz, double zh, double small, double D, const double* Z1, const double* Z2, const double* vy1,
+              double vy, double* z1, double* z2, double* VY1, int* it, double* err, int it_max = 1000,
+              double tol = 1e-6);
+
+}  // namespace hqt

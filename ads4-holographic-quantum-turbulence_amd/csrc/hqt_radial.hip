@@ -1,0 +1,150 @@
+// Radial kernels: nz dispatch and the small kernels that need no compile-time nz (d_z A_t,
+// winding removal, NaN guard).  The per-nz templates live in hqt_radial_core.h and are
+// instantiated by the hqt_radial_nz_*.hip translation units so the build runs in parallel.
+#include "hqt_radial_core.h"
+
+namespace hqt {
+
+// ---- d_z A_t of the frozen A_t; also advances the device step counter ----------------------
+__global__ __launch_bounds__(256) void k_zprep(const ZPrep A) {
+    if (stopped(A.flags)) return;
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (A.bump && c == 0) A.flags[1] = A.flags[1] + 1;
+    if (c >= A.nxy) return;
+    const int64_t n = A.nxy;
+    const int nz = A.nz;
+    for (int k = 0; k < nz; ++k) {
+        double a = 0.0;
+        for (int m = 0; m < nz; ++m) a = fma(A.Lz[k * nz + m], A.At[c + n * m], a);
+        A.dzAt[c + n * k] = a;
+    }
+}
+
+// ---- winding: round(mean(sum(angle(circshift(O,-1,2)./O),1))/(2*pi)) (:137) --------------
+__device__ __forceinline__ double block_sum_256(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    for (int off = 128; off > 0; off >>= 1) {
+        __syncthreads();
+        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    }
+    __syncthreads();
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// angle(a/b): atan2 of a*conj(b) (same angle, b != 0); b == 0 gives NaN (0/0 in the reference).
+__device__ __forceinline__ double angle_ratio(double ar, double ai, double br, double bi) {
+    if (br == 0.0 && bi == 0.0) return __builtin_nan("");
+    if (ar == 0.0 && ai == 0.0) return 0.0;
+    return atan2(ai * br - ar * bi, ar * br + ai * bi);
+}
+
+__global__ __launch_bounds__(256) void k_wind_partial(const Wind A) {
+    __shared__ double sh[256];
+    if (stopped(A.flags)) return;
+    const int j = blockIdx.x, jn = (j + 1) % A.ny;
+    double s = 0.0;
+    for (int x = threadIdx.x; x < A.nx; x += 256) {
+        const int64_t a = (int64_t)x + (int64_t)A.nx * jn, b = (int64_t)x + (int64_t)A.nx * j;
+        s += angle_ratio(A.O[2 * a], A.O[2 * a + 1], A.O[2 * b], A.O[2 * b + 1]);
+    }
+    const double t = block_sum_256(s, sh);
+    if (threadIdx.x == 0) A.part[j] = t;
+}
+
+template <typename TS>
+__global__ __launch_bounds__(256) void k_wind_apply(const Wind A) {
+    __shared__ double sh[256];
+    if (stopped(A.flags)) return;
+    double s = 0.0;
+    for (int j = threadIdx.x; j < A.ny; j += 256) s += A.part[j];
+    const double tot = block_sum_256(s, sh);
+    const double W = round((tot / A.ny) / (2.0 * M_PI));
+    if (blockIdx.x == 0 && threadIdx.x == 0) A.W[0] = W;
+    if (W == 0.0) return;
+    const int64_t total = A.nxy * A.nz;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+        const int64_t c = e % A.nxy;
+        const int y = (int)(c / A.nx);
+        double sn, cs;
+        sincos(-A.yph[y] * W, &sn, &cs);
+        const double r = ld<TS>(A.R, e), i = ld<TS>(A.I, e);
+        st<TS>(A.R, e, r * cs - i * sn);
+        st<TS>(A.I, e, r * sn + i * cs);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+#define HQT_NZ_LIST(X) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) \
+    X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31) X(32)
+
+bool zkernels_support_nz(int nz) { return nz >= 4 && nz <= 32; }
+
+#define HQT_DECL(n) extern template hipError_t zstage_launch_nz<n>(const ZStage&, int, hipStream_t); \
+                    extern template hipError_t zpost_launch_nz<n>(const ZPost&, int, hipStream_t);
+HQT_NZ_LIST(HQT_DECL)
+#undef HQT_DECL
+
+hipError_t launch_zstage(const ZStage& A, int fp32, hipStream_t s) {
+    switch (A.nz) {
+#define HQT_ZS(n) case n: return zstage_launch_nz<n>(A, fp32, s);
+        HQT_NZ_LIST(HQT_ZS)
+#undef HQT_ZS
+        default: return hipErrorInvalidValue;
+    }
+}
+hipError_t launch_zpost(const ZPost& A, int fp32, hipStream_t s) {
+    switch (A.nz) {
+#define HQT_ZP(n) case n: return zpost_launch_nz<n>(A, fp32, s);
+        HQT_NZ_LIST(HQT_ZP)
+#undef HQT_ZP
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_zprep(const ZPrep& A, hipStream_t s) {
+    dim3 grid((unsigned)((A.nxy + 255) / 256));
+    hipLaunchKernelGGL(k_zprep, grid, dim3(256), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_winding(const Wind& A, int fp32, hipStream_t s) {
+    hipLaunchKernelGGL(k_wind_partial, dim3(A.ny), dim3(256), 0, s, A);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    int64_t total = A.nxy * A.nz;
+    unsigned blocks = (unsigned)((total + 255) / 256);
+    if (blocks > 2048) blocks = 2048;
+    if (fp32) hipLaunchKernelGGL(k_wind_apply<float>, dim3(blocks), dim3(256), 0, s, A);
+    else hipLaunchKernelGGL(k_wind_apply<double>, dim3(blocks), dim3(256), 0, s, A);
+    return hipGetLastError();
+}
+
+}  // namespace hqt
+
+namespace hqt {
+template <typename TS>
+__global__ __launch_bounds__(256) void k_nancheck(const void* R, const void* I, const void* Ax, const void* Ay,
+                                                  const double* At, int64_t n, int32_t* flags) {
+    if (stopped(flags)) return;
+    bool bad = false;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+        const double a = ld<TS>(R, e), b = ld<TS>(I, e), c = ld<TS>(Ax, e), d = ld<TS>(Ay, e), f = At[e];
+        if (a != a || b != b || c != c || d != d || f != f) bad = true;
+    }
+    if (__any(bad) && (threadIdx.x & 63) == 0) {
+        const int step = __atomic_load_n(&flags[1], __ATOMIC_RELAXED);
+        atomicCAS(&flags[0], 0, step > 0 ? step : 1);
+    }
+}
+
+hipError_t launch_nancheck(const void* R, const void* I, const void* Ax, const void* Ay, const double* At,
+                           int64_t n, int fp32, int32_t* flags, hipStream_t s) {
+    unsigned blocks = (unsigned)((n + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    if (fp32) hipLaunchKernelGGL(k_nancheck<float>, dim3(blocks), dim3(256), 0, s, R, I, Ax, Ay, At, n, flags);
+    else hipLaunchKernelGGL(k_nancheck<double>, dim3(blocks), dim3(256), 0, s, R, I, Ax, Ay, At, n, flags);
+    return hipGetLastError();
+}
+}  // namespace hqt

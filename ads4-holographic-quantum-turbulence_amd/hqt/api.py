@@ -1,0 +1,257 @@
+"""Python mirror of the reference's interface for the stepper path, over the C ABI.
+
+Names, argument meaning and error behaviour follow the reference's MATLAB routines:
+
+* :func:`fourdif`, :func:`chebdif`  -- ``third_party/dmsuite/fourdif.m:1``, ``chebdif.m:1``
+* :func:`ff`                        -- ``src/numerics/ff.m:1`` (computed on the GPU)
+* :func:`NewtonRaphson_vy`          -- ``src/solvers/NewtonRaphson_vy.m:1``
+* :class:`Stepper`                  -- the closures ``F1..F4`` (``tevol_driven_cluster.m:72-76``)
+  and the loop body ``:120-142`` with its NaN guard ``:182-185``
+
+Fields are NumPy arrays of shape ``(nx, ny, nz)`` (MATLAB ``[nx,ny,nz]``); they are handed
+to the library in column-major order (x fastest), the reference's memory layout.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (HQT_ERR_ARG, HQT_ERR_NAN, HQT_FP64, HQT_MIXED, HQTError, Params, check, dptr, lib)
+
+__all__ = ["fourdif", "chebdif", "ff", "NewtonRaphson_vy", "Stepper", "HQTError", "mask_factors"]
+
+
+def fourdif(N: int, m: int):
+    """``[x, DM] = fourdif(N, m)``: Fourier differentiation matrix on N equispaced points."""
+    x = np.empty(N)
+    DM = np.empty((N, N), order="F")
+    check(lib().hqt_fourdif(int(N), int(m), dptr(x), dptr(DM)))
+    return x, DM
+
+
+def chebdif(N: int, M: int):
+    """``[x, DM] = chebdif(N, M)``: Chebyshev nodes (descending) and DM of shape (N, N, M)."""
+    x = np.empty(N)
+    DM = np.empty((N, N, M), order="F")
+    check(lib().hqt_chebdif(int(N), int(M), dptr(x), dptr(DM)))
+    return x, DM
+
+
+def mask_factors(mask: np.ndarray):
+    """Split a boolean ``[nx, ny(, nz)]`` mask of the form ``fx | fy`` (every z slice equal,
+    ``tevol_driven_cluster.m:27-31``) into the 1-D factors ``(fx, fy)``."""
+    m = np.asarray(mask, dtype=bool)
+    if m.ndim == 3:
+        if not (m == m[:, :, :1]).all():
+            raise HQTError(HQT_ERR_ARG, "ff: mask slices differ; only masks fx|fy repeated over z are supported")
+        m = m[:, :, 0]
+    if m.ndim != 2:
+        raise HQTError(HQT_ERR_ARG, "ff: mask must be [nx, ny] or [nx, ny, nz]")
+    fx = m.all(axis=1)
+    fy = m.all(axis=0)
+    if not np.array_equal(m, fx[:, None] | fy[None, :]):
+        raise HQTError(HQT_ERR_ARG, "ff: mask is not of the form fx | fy")
+    return fx, fy
+
+
+def ff(x, y):
+    """``w = ff(x, y)``: zero the masked 2-D Fourier modes of every slice and return the real
+    part (``src/numerics/ff.m:7-11``).  ``x`` is ``[nx, ny]`` or ``[nx, ny, nslices]``, real or
+    complex; ``y`` is a logical mask of the form ``fx | fy``."""
+    x = np.asarray(x)
+    shape = x.shape
+    if x.ndim == 2:
+        x = x[:, :, None]
+    if x.ndim != 3:
+        raise HQTError(HQT_ERR_ARG, "ff: x must be 2-D or 3-D")
+    nx, ny, ns = x.shape
+    fx, fy = mask_factors(y)
+    if fx.shape[0] != nx or fy.shape[0] != ny:
+        raise HQTError(HQT_ERR_ARG, "ff: mask shape does not match x")
+    xr = np.asfortranarray(np.real(x), dtype=np.float64)
+    xi = np.asfortranarray(np.imag(x), dtype=np.float64) if np.iscomplexobj(x) else None
+    w = np.empty((nx, ny, ns), order="F")
+    fxb = np.ascontiguousarray(fx, dtype=np.uint8)
+    fyb = np.ascontiguousarray(fy, dtype=np.uint8)
+    u8 = C.POINTER(C.c_uint8)
+    check(lib().hqt_ff(nx, ny, ns, dptr(xr), dptr(xi), fxb.ctypes.data_as(u8), fyb.ctypes.data_as(u8), dptr(w)))
+    return w.reshape(shape)
+
+
+def NewtonRaphson_vy(nz, zh, small, D, Z1, Z2, vy1, vy, raise_on_failure=False):
+    """``[z1,z2,VY1,it,error] = NewtonRaphson_vy(nz,zh,small,D,Z1,Z2,vy1,vy)``.
+
+    Like the reference, a run that stops at ``it_max`` or on a NaN update returns normally
+    (``it`` then reports the count); pass ``raise_on_failure=True`` to raise instead."""
+    nz = int(nz)
+    a = np.ascontiguousarray(np.ravel(Z1), dtype=np.float64)
+    b = np.ascontiguousarray(np.ravel(Z2), dtype=np.float64)
+    v = np.ascontiguousarray(np.ravel(vy1), dtype=np.float64)
+    if a.size != nz or b.size != nz or v.size != nz:
+        raise HQTError(HQT_ERR_ARG, "NewtonRaphson_vy: profiles must have nz entries")
+    z1, z2, VY1 = np.empty(nz), np.empty(nz), np.empty(nz)
+    it = C.c_int32(0)
+    err = C.c_double(0.0)
+    rc = lib().hqt_newton_vy(nz, float(zh), float(small), float(D), dptr(a), dptr(b), dptr(v), float(vy),
+                             dptr(z1), dptr(z2), dptr(VY1), C.byref(it), C.byref(err))
+    if rc != 0 and raise_on_failure:
+        check(rc)
+    return z1, z2, VY1, int(it.value), float(err.value)
+
+
+def _f64F(a, shape):
+    a = np.asarray(a, dtype=np.float64)
+    if a.shape != shape:
+        a = np.broadcast_to(a, shape)
+    return np.asfortranarray(a)
+
+
+def _c128F(a, shape):
+    a = np.asarray(a, dtype=np.complex128)
+    if a.shape != shape:
+        a = np.broadcast_to(a, shape)
+    return np.asfortranarray(a)
+
+
+class Stepper:
+    """A device-resident AdS4 evolution (one ``hqt_ctx``).
+
+    Parameters mirror the workspace of ``tevol_driven_cluster.m``: grid ``nx, ny, nz``, box
+    sides ``Lx`` (L0) and ``Ly`` (L), ``mu``, time step ``dt`` (h), winding removal on/off
+    (on in the driven scripts, off in ``tevol_lattice.m``), the filter on/off, the z=0
+    values ``ax_b`` / ``ay_b`` of A_x / A_y restored after every step (scalars or [nx, ny]
+    arrays; the reference uses 0 and -vy), and ``precision`` ``"fp64"`` or ``"mixed"``."""
+
+    def __init__(self, nx, ny, nz, Lx, Ly, mu=6.0, dt=0.02, winding=True, filter=True,
+                 precision="fp64", ax_b=0.0, ay_b=0.0):
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.shape = (self.nx, self.ny, self.nz)
+        self.precision = precision
+        p = Params()
+        p.nx, p.ny, p.nz = self.nx, self.ny, self.nz
+        p.Lx, p.Ly, p.mu, p.dt = float(Lx), float(Ly), float(mu), float(dt)
+        p.winding = 1 if winding else 0
+        p.filter = 1 if filter else 0
+        if precision not in ("fp64", "mixed"):
+            raise HQTError(HQT_ERR_ARG, "precision must be 'fp64' or 'mixed'")
+        p.precision = HQT_FP64 if precision == "fp64" else HQT_MIXED
+        keep = []
+        for name in ("ax_b", "ay_b"):
+            val = ax_b if name == "ax_b" else ay_b
+            if np.ndim(val) == 0:
+                setattr(p, name + "_const", float(val))
+                setattr(p, name, None)
+            else:
+                arr = np.asfortranarray(np.broadcast_to(np.asarray(val, dtype=np.float64), (self.nx, self.ny)))
+                keep.append(arr)
+                setattr(p, name, dptr(arr))
+        self._h = C.c_void_p()
+        check(lib().hqt_ctx_create(C.byref(p), C.byref(self._h)))
+
+    # -- life cycle ---------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().hqt_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- state --------------------------------------------------------------------------
+    def set_state(self, Z1, Z2, Z3, Z4, rhoi):
+        s = self.shape
+        z1 = _c128F(Z1, s)
+        check(lib().hqt_set_state(self._h, dptr(z1), dptr(_f64F(Z2, s)), dptr(_f64F(Z3, s)),
+                                  dptr(_f64F(Z4, s)), dptr(_f64F(rhoi, s))))
+
+    def get_state(self, check_nan=True):
+        """Return ``(Z1, Z2, Z3, Z4, rhoi)``; raises HQTError(HQT_ERR_NAN) if the guard tripped
+        (unless ``check_nan=False``)."""
+        s = self.shape
+        Z1 = np.empty(s, dtype=np.complex128, order="F")
+        Z2, Z3, Z4, rh = (np.empty(s, order="F") for _ in range(4))
+        rc = lib().hqt_get_state(self._h, dptr(Z1), dptr(Z2), dptr(Z3), dptr(Z4), dptr(rh))
+        if rc != 0 and not (rc == HQT_ERR_NAN and not check_nan):
+            check(rc)
+        return Z1, Z2, Z3, Z4, rh
+
+    # -- stepping -----------------------------------------------------------------------
+    def step_async(self, nsteps=1):
+        check(lib().hqt_step_n(self._h, int(nsteps)))
+
+    def sync(self):
+        """Wait for queued steps; raise HQTError(HQT_ERR_NAN) if the NaN guard tripped."""
+        ns = C.c_int32(0)
+        check(lib().hqt_sync(self._h, C.byref(ns)))
+
+    def nan_step(self):
+        ns = C.c_int32(0)
+        lib().hqt_sync(self._h, C.byref(ns))
+        return int(ns.value)
+
+    def step(self, nsteps=1):
+        self.step_async(nsteps)
+        self.sync()
+
+    @property
+    def last_winding(self):
+        w = C.c_double(0.0)
+        check(lib().hqt_last_winding(self._h, C.byref(w)))
+        return w.value
+
+    # -- closures F1..F4 ------------------------------------------------------------------
+    def rhs(self, Z1, Z2, Z3, Z4):
+        """``(F1, F2, F3, F4)`` at the given fields (``tevol_driven_cluster.m:72-76``)."""
+        s = self.shape
+        z1 = _c128F(Z1, s)
+        F1 = np.empty(s, dtype=np.complex128, order="F")
+        F2, F3, F4 = (np.empty(s, order="F") for _ in range(3))
+        check(lib().hqt_rhs(self._h, dptr(z1), dptr(_f64F(Z2, s)), dptr(_f64F(Z3, s)),
+                            dptr(_f64F(Z4, s)), dptr(F1), dptr(F2), dptr(F3), dptr(F4)))
+        return F1, F2, F3, F4
+
+    def F1(self, y1, y2, y3, y4):
+        return self.rhs(y1, y2, y3, y4)[0]
+
+    def F2(self, y1, y2, y3, y4):
+        return self.rhs(y1, y2, y3, y4)[1]
+
+    def F3(self, y1, y2, y3, y4):
+        return self.rhs(y1, y2, y3, y4)[2]
+
+    def F4(self, y1, y2, y3, y4):
+        return self.rhs(y1, y2, y3, y4)[3]
+
+    def order_parameter(self):
+        """O = (func_Lz(Z1))(:,:,end) of the current state, complex ``[nx, ny]``."""
+        O = np.empty((self.nx, self.ny), dtype=np.complex128, order="F")
+        check(lib().hqt_order_parameter(self._h, dptr(O)))
+        return O
+
+    # -- measurement --------------------------------------------------------------------
+    def profile(self, enable=True):
+        check(lib().hqt_profile(self._h, 1 if enable else 0))
+
+    def kernel_stats(self):
+        """``{name: (launches, total_ms)}`` for kernels launched since ``profile(True)``."""
+        out = {}
+        buf = C.create_string_buffer(64)
+        cnt = C.c_int64(0)
+        tot = C.c_double(0.0)
+        i = 0
+        while lib().hqt_kernel_stats(self._h, i, buf, 64, C.byref(cnt), C.byref(tot)) == 0:
+            if cnt.value:
+                out[buf.value.decode()] = (int(cnt.value), float(tot.value))
+            i += 1
+        return out

@@ -1,0 +1,123 @@
+/*
+ * hqt.h -- C ABI of the MI355X-native AdS4 holographic-superfluid stepper.
+ *
+ * The reference (Eureka4399/AdS4-holographic-quantum-turbulence) is pure MATLAB; its hot
+ * path has no function API: the right-hand sides are anonymous closures and the RK4
+ * step is an inline loop body.  This header is the drop-in boundary a MATLAB MEX
+ * gateway (csrc/hqt_mex.c) or any FFI binds instead of those closures.  Each entry
+ * point below names the reference code it replaces.
+ *
+ * Conventions (docs/DATA_DICTIONARY.md:7-8 of the reference):
+ *   - every field is a MATLAB column-major array [nx, ny, nz], linear index
+ *     i + nx*(j + ny*k), x fastest; k = nz-1 is the AdS boundary z=0, k = 0 the horizon;
+ *   - the complex scalar Psi (Z1) is interleaved (re, im) pairs (MATLAB R2018a layout);
+ *   - Z2 = A_t, Z3 = A_x, Z4 = A_y are real; rhoi is [nx, ny, nz] (replicated along z
+ *     by the reference, tevol_driven_cluster.m:98), stored in full here.
+ * All host pointers are borrowed for the duration of the call.  Every function returns
+ * an int status (HQT_OK = 0); on failure hqt_last_error() returns a thread-local
+ * message.  No C++ exception crosses this boundary.
+ */
+#ifndef HQT_H
+#define HQT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HQT_OK          0
+#define HQT_ERR_ARG     1   /* bad shape / argument */
+#define HQT_ERR_NAN     2   /* NaN in the state: the reference's guard, tevol_driven_cluster.m:182-185 */
+#define HQT_ERR_HIP     3   /* HIP runtime error */
+#define HQT_ERR_OOM     4   /* device allocation failed */
+#define HQT_ERR_STATE   5   /* call out of order (e.g. step before set_state) */
+#define HQT_ERR_NOCONV  6   /* Newton did not converge within it_max */
+
+#define HQT_FP64   0        /* plain form: FP64 end to end */
+#define HQT_MIXED  1        /* x/y operators, filter, stage arithmetic FP32; z operators FP64 */
+
+typedef struct hqt_ctx hqt_ctx;
+
+/* Run parameters.  Replaces the hard-coded workspace of tevol_driven_cluster.m:11,23-24,79,86,110. */
+typedef struct hqt_params {
+    int32_t nx, ny, nz;     /* grid (nx=ny=301, nz=20 at :11) */
+    double  Lx, Ly;         /* periodic box sides L0, L (:23-24,85) */
+    double  mu;             /* chemical potential: A_t(z=0) (:79,131,133) */
+    double  dt;             /* RK4 step h (:110) */
+    int32_t winding;        /* 1: winding removal after each step (:136-139); 0: off (tevol_lattice.m:168-170) */
+    int32_t filter;         /* 1: Fourier filter after each step (:142); 0: off */
+    int32_t precision;      /* HQT_FP64 or HQT_MIXED */
+    int32_t reserved;
+    const double* ax_b;     /* [nx*ny] A_x(z=0) restored each step, or NULL for ax_b_const (:133: 0) */
+    const double* ay_b;     /* [nx*ny] A_y(z=0) restored each step, or NULL for ay_b_const (:133: -vy) */
+    double  ax_b_const, ay_b_const;
+} hqt_params;
+
+/* ---- context life cycle: tevol_driven_cluster.m:11-77 operator set-up ------------------ */
+int hqt_ctx_create(const hqt_params* p, hqt_ctx** out);
+int hqt_ctx_destroy(hqt_ctx* ctx);
+
+/* Upload / download the state (Z1 interleaved complex, Z2, Z3, Z4, rhoi; each [nx,ny,nz]).
+ * Replaces the gpuArray residency of :81-102 (upload) and the checkpoint save of :178-181
+ * (download).  The state stays device-resident between calls.  hqt_get_state waits for
+ * all queued work and returns HQT_ERR_NAN if the NaN guard tripped. */
+int hqt_set_state(hqt_ctx* ctx, const double* Z1, const double* Z2, const double* Z3,
+                  const double* Z4, const double* rhoi);
+int hqt_get_state(hqt_ctx* ctx, double* Z1, double* Z2, double* Z3, double* Z4, double* rhoi);
+
+/* Queue nsteps RK4 steps: the loop body tevol_driven_cluster.m:120-142 (+ the NaN guard of
+ * :182-185) iterated.  Returns after queueing; hqt_sync waits. */
+int hqt_step_n(hqt_ctx* ctx, int32_t nsteps);
+/* Wait for queued work.  HQT_ERR_NAN if the guard tripped; *nan_step (nullable) is the
+ * 1-based step (since hqt_set_state) whose state first held a NaN, else 0. */
+int hqt_sync(hqt_ctx* ctx, int32_t* nan_step);
+/* Winding number removed by the most recent step (:137); NaN when O had exact zeros. */
+int hqt_last_winding(hqt_ctx* ctx, double* w);
+
+/* Right-hand sides at a given state: the closures F1, F2, F3, F4 of
+ * tevol_driven_cluster.m:72-76 evaluated over the full volume.  F1 interleaved complex.
+ * Inputs and outputs are host arrays [nx,ny,nz]; overwrites the context's stage buffers. */
+int hqt_rhs(hqt_ctx* ctx, const double* Z1, const double* Z2, const double* Z3, const double* Z4,
+            double* F1, double* F2, double* F3, double* F4);
+
+/* Boundary order parameter O = (d_z Psi)(z=0) of the current state, [nx,ny] interleaved
+ * complex (tevol_driven_cluster.m:136,148; scripts/analysis/movie_show.m:91). */
+int hqt_order_parameter(hqt_ctx* ctx, double* O);
+
+/* ---- stand-alone operators ---------------------------------------------------------- */
+/* w = ff(x, fx|fy): src/numerics/ff.m:7-11 with the mask of tevol_driven_cluster.m:27-31.
+ * x is [nx, ny, nslices], real (x_im == NULL) or complex (x_im = imaginary part);
+ * fx [nx], fy [ny] are 0/1 bytes; the 2-D mask is fx(i) | fy(j).  Computed on the GPU. */
+int hqt_ff(int32_t nx, int32_t ny, int32_t nslices, const double* x_re, const double* x_im,
+           const uint8_t* fx, const uint8_t* fy, double* w);
+
+/* [x, DM] = fourdif(N, m): third_party/dmsuite/fourdif.m:30-71.  DM column-major N*N. */
+int hqt_fourdif(int32_t N, int32_t m, double* x, double* DM);
+/* [x, DM] = chebdif(N, M): third_party/dmsuite/chebdif.m:25-53.  DM column-major N*N*M. */
+int hqt_chebdif(int32_t N, int32_t M, double* x, double* DM);
+
+/* [z1,z2,VY1,it,error] = NewtonRaphson_vy(nz,zh,small,D,Z1,Z2,vy1,vy):
+ * src/solvers/NewtonRaphson_vy.m:1-105 (plots and prints dropped), host FP64.
+ * Inputs/outputs are nz-vectors; returns HQT_OK when the residual test (:92) passed,
+ * HQT_ERR_NOCONV when it_max (:8) was reached or the update was NaN (:88). */
+int hqt_newton_vy(int32_t nz, double zh, double small, double D, const double* Z1,
+                  const double* Z2, const double* vy1, double vy, double* z1, double* z2,
+                  double* VY1, int32_t* it, double* err);
+
+/* ---- measurement ---------------------------------------------------------------------- */
+/* Enable (1) / disable (0) per-kernel HIP-event timing on the context's stream. */
+int hqt_profile(hqt_ctx* ctx, int32_t enable);
+/* Per-kernel totals since profiling was enabled: entry idx (0-based) -> name, launches,
+ * summed duration in ms.  Returns HQT_ERR_ARG when idx is past the last entry. */
+int hqt_kernel_stats(hqt_ctx* ctx, int32_t idx, char* name, int32_t name_len, int64_t* count,
+                     double* total_ms);
+
+const char* hqt_last_error(void);
+const char* hqt_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HQT_H */

@@ -1,0 +1,128 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle on the same seeded inputs.
+
+Tolerances (FP64 plain form): one RHS evaluation and one RK4 step agree to 1e-12 relative
+(max-norm over the field, relative to the field's max); 20 steps of a vortex-laden
+evolution to 1e-10 relative.  The differences come only from summation order (MFMA tile
+accumulation vs. BLAS, FFT vs. Toeplitz filter).
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+L = 50.0
+
+
+def _hqt():
+    import hqt
+    return hqt
+
+
+def _oracle():
+    from oracle import hqt_oracle as O
+    return O
+
+
+def _state(nx, ny, nz, seed, nv=2):
+    from hqt import ic
+    return ic.initial_state(nx, ny, nz, L, L, mu=6.0, seed=seed, n_vortices=nv)
+
+
+def _perturbed(nx, ny, nz, seed):
+    """A generic state: the stationary profile plus smooth random perturbations of every field."""
+    z1, z2, z3, z4, rh = _state(nx, ny, nz, seed)
+    rng = np.random.default_rng(seed + 100)
+    shp = (nx, ny, nz)
+    z1 = z1 * (1 + 0.05 * rng.standard_normal(shp)) + 0.01j * rng.standard_normal(shp)
+    z2 = z2 + 0.05 * rng.standard_normal(shp)
+    z3 = z3 + 0.05 * rng.standard_normal(shp)
+    z4 = z4 + 0.05 * rng.standard_normal(shp)
+    return z1, z2, z3, z4, rh
+
+
+@pytest.mark.parametrize("nx,ny,nz", [(16, 16, 8), (33, 17, 10), (64, 48, 12)])
+def test_rhs_matches_oracle(nx, ny, nz):
+    print(f"rhs parity {nx}x{ny}x{nz}", flush=True)
+    hqt, O = _hqt(), _oracle()
+    z1, z2, z3, z4, _ = _perturbed(nx, ny, nz, seed=nx + nz)
+    with hqt.Stepper(nx, ny, nz, L, L) as st:
+        F = st.rhs(z1, z2, z3, z4)
+    ops = O.Operators(nx, ny, nz, L, L)
+    R = O.rhs(ops, z1, z2, z3, z4)
+    for name, a, b in zip(("F1", "F2", "F3", "F4"), F, R):
+        e = rel_err(a, b)
+        print(f"  {name}: {e:.3e}", flush=True)
+        assert e < 1e-12, (name, e)
+
+
+@pytest.mark.parametrize("nx,ny,nz,winding", [(16, 16, 8, True), (33, 17, 10, True), (32, 32, 12, False)])
+def test_one_step_matches_oracle(nx, ny, nz, winding):
+    print(f"step parity {nx}x{ny}x{nz} winding={winding}", flush=True)
+    hqt, O = _hqt(), _oracle()
+    from hqt import ic
+    z1, z2, z3, z4, rh = _state(nx, ny, nz, seed=7, nv=4)
+    axb = ic.sinusoidal_drive(nx, ny, L, kk=3, amp=0.1)
+    with hqt.Stepper(nx, ny, nz, L, L, mu=6.0, dt=0.02, winding=winding, ax_b=axb, ay_b=0.0) as st:
+        st.set_state(z1, z2, z3, z4, rh)
+        st.step(1)
+        G = st.get_state()
+        W = st.last_winding if winding else 0.0
+    ops = O.Operators(nx, ny, nz, L, L)
+    *R, Wo = O.step(ops, O.Params(mu=6.0, h=0.02, ax_b=axb, ay_b=0.0, winding=winding), z1, z2, z3, z4, rh)
+    assert W == Wo
+    for name, a, b in zip(("Z1", "Z2", "Z3", "Z4", "rhoi"), G, R):
+        e = rel_err(a, b)
+        print(f"  {name}: {e:.3e}", flush=True)
+        assert e < 1e-12, (name, e)
+
+
+def test_twenty_steps_match_oracle():
+    print("20-step parity 32x32x12", flush=True)
+    hqt, O = _hqt(), _oracle()
+    nx = ny = 32
+    nz = 12
+    z1, z2, z3, z4, rh = _state(nx, ny, nz, seed=11, nv=4)
+    prm = O.Params(mu=6.0, h=0.02, ax_b=0.0, ay_b=0.3, winding=True)
+    with hqt.Stepper(nx, ny, nz, L, L, mu=6.0, dt=0.02, ax_b=0.0, ay_b=prm.ay_b) as st:
+        st.set_state(z1, z2, z3, z4, rh)
+        st.step(20)
+        G = st.get_state()
+    ops = O.Operators(nx, ny, nz, L, L)
+    S = [z1, z2, z3, z4, rh]
+    for _ in range(20):
+        *S, _w = O.step(ops, prm, *S)
+    for name, a, b in zip(("Z1", "Z2", "Z3", "Z4", "rhoi"), G, S):
+        e = rel_err(a, b)
+        print(f"  {name}: {e:.3e}", flush=True)
+        assert e < 1e-10, (name, e)
+
+
+@pytest.mark.parametrize("nx,ny,ns,cplx", [(17, 19, 3, True), (64, 64, 2, False), (301, 301, 2, False), (33, 16, 4, True)])
+def test_ff_matches_oracle(nx, ny, ns, cplx):
+    print(f"ff parity {nx}x{ny}x{ns} complex={cplx}", flush=True)
+    hqt, O = _hqt(), _oracle()
+    rng = np.random.default_rng(nx * ny)
+    x = rng.standard_normal((nx, ny, ns))
+    if cplx:
+        x = x + 1j * rng.standard_normal((nx, ny, ns))
+    fx, fy = O.filter_masks(nx, ny)
+    mask = fx[:, None] | fy[None, :]
+    w = hqt.ff(x, mask)
+    wo = O.ff(x, mask)
+    e = rel_err(w, wo)
+    print(f"  err {e:.3e}", flush=True)
+    assert e < 1e-12
+
+
+def test_ff_empty_mask_is_real_part():
+    """The reference's known answer (tests/hqt_smoke_test.m:139-141): ff(field, empty) == real(field)."""
+    hqt = _hqt()
+    rng = np.random.default_rng(5)
+    field = rng.standard_normal((17, 19)) + 1j * rng.standard_normal((17, 19))
+    w = hqt.ff(field, np.zeros((17, 19), dtype=bool))
+    assert np.max(np.abs(w - np.real(field))) < 1e-12

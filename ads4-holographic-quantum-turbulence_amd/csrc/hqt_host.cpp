@@ -730,6 +730,23 @@ int hqt_newton_vy(int32_t nz, double zh, double small, double D, const double* Z
     return HQT_OK;
 }
 
+int hqt_debug_field(hqt_ctx* c, const char* name, double* out) {
+    if (!c || !name || !out) return fail(HQT_ERR_ARG, "NULL argument");
+    struct { const char* n; const void* p; int dt; } tab[] = {
+        {"dxR", c->dxR, c->tdt}, {"dxI", c->dxI, c->tdt}, {"dxxR", c->dxxR, c->tdt}, {"dxxI", c->dxxI, c->tdt},
+        {"dxAx", c->dxAx, c->tdt}, {"dxAy", c->dxAy, c->tdt}, {"dxxAy", c->dxxAy, c->tdt},
+        {"dyR", c->dyR, c->tdt}, {"dyI", c->dyI, c->tdt}, {"lapR", c->lapR, c->tdt}, {"lapI", c->lapI, c->tdt},
+        {"divA", c->divA, c->tdt}, {"curl3", c->curl3, c->tdt}, {"curl4", c->curl4, c->tdt},
+        {"dxAt", c->dxAt, c->tdt}, {"dyAt", c->dyAt, c->tdt}, {"dzAt", c->dzAt, DT_F64}};
+    for (auto& t : tab) {
+        if (std::strcmp(t.n, name) == 0) {
+            HIPC(hipStreamSynchronize(c->stream));
+            return download_as(out, t.p, (size_t)c->npts, t.dt);
+        }
+    }
+    return fail(HQT_ERR_ARG, std::string("unknown field ") + name);
+}
+
 int hqt_profile(hqt_ctx* c, int32_t enable) {
     if (!c) return fail(HQT_ERR_ARG, "NULL context");
     HIPC(hipStreamSynchronize(c->stream));

@@ -16,18 +16,23 @@
 
 namespace hqt {
 
+// 16x16x4 MFMA operand layouts (A[m][k] in lane m + 16k, B[k][n] in lane n + 16k).  The
+// accumulator row held by lane group g = lane/16 in register r differs between the two
+// instructions on gfx950 (measured): f64 rows are 4r + g, f32 rows are 4g + r.
 template <typename T> struct MF;
 template <> struct MF<double> {
     typedef double v4 __attribute__((ext_vector_type(4)));
     static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int g, int r) { return 4 * r + g; }
 };
 template <> struct MF<float> {
     typedef float v4 __attribute__((ext_vector_type(4)));
     static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
 };
 
 template <typename T>
@@ -136,13 +141,13 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restric
                 int64_t idx;
                 bool ok;
                 if (DIR == 0) {
-                    const int64_t line = line0 + 4 * lk + r;
+                    const int64_t line = line0 + MF<T>::row(lk, r);
                     const int i = i0 + ln;
                     ok = line < nl && i < N;
                     idx = line * P.nx + i;
                 } else {
                     const int x = x0 + ln;
-                    const int i = i0 + 4 * lk + r;
+                    const int i = i0 + MF<T>::row(lk, r);
                     ok = x < P.nx && i < N;
                     idx = (int64_t)x + (int64_t)P.nx * (i + (int64_t)P.ny * z);
                 }

@@ -239,6 +239,12 @@ class Stepper:
         check(lib().hqt_order_parameter(self._h, dptr(O)))
         return O
 
+    def debug_field(self, name):
+        """An internal device field of the last rhs/step (x/y/z derivative buffers)."""
+        out = np.empty(self.shape, order="F")
+        check(lib().hqt_debug_field(self._h, name.encode(), dptr(out)))
+        return out
+
     # -- measurement --------------------------------------------------------------------
     def profile(self, enable=True):
         check(lib().hqt_profile(self._h, 1 if enable else 0))

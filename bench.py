@@ -1,0 +1,214 @@
+"""Benchmark: RK4 steps/s of the AdS4 stepper on one MI355X.
+
+Workload (the reference's headline configuration): driven flow, nx=ny=301, nz=20, box side
+L=50, mu=6, dt=0.02, boundary forcing A_x(z=0) = 0.1*sin(2*pi*7*y/L), seeded zero-flow
+stationary profile with 1e-3 phase noise and 4 windings.  One "step" is one full RK4 step
+(tevol_driven_cluster.m:120-142: 4 RHS stages, A_t constraint, boundary reset, winding
+removal, filter, NaN guard) with the state resident in HBM.
+
+Prints ONE JSON line.  ``value`` is FP64 steps/s; the mixed-precision form is reported beside
+it with its per-field error against the FP64 run after the same steps.  ``roofline`` is the
+dominant kernel's algorithmic FLOP rate from HIP events recorded around its launches on the
+library's stream (a profiled repeat of the timed steps); ``cpu_baseline`` is the NumPy oracle
+(the reference's dense-matrix form) timed on a bounded sample on this host.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "ads4-holographic-quantum-turbulence_amd"), ROOT):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+PEAK_FP64_TFLOPS = 78.6      # MI355X dense FP64 (vector = matrix)
+PEAK_FP32_TFLOPS = 157.3     # MI355X dense FP32 matrix
+PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
+
+
+def line_pass_flops(name, nx, ny, nz, stage_mix=(1, 3)):
+    """Algorithmic FLOPs of one launch of a line pass in the reference's dense form
+    (each operator application = one dense N x N matrix times every line: 2*N flops/point)."""
+    S = nx * ny * nz
+    if name == "line_x_stage":      # dx, dxx of Re/Im Psi; dx Ax; dx, dxx Ay; (+ dx At in stage 1)
+        apps = (8 * stage_mix[0] + 7 * stage_mix[1]) / sum(stage_mix)
+        return apps * 2 * nx * S
+    if name == "line_y_stage":      # dy, dyy of Re/Im Psi; dy Ay; dyy Ax + dy(dxAy); dy(dxAx) (+ dy At)
+        apps = (9 * stage_mix[0] + 8 * stage_mix[1]) / sum(stage_mix)
+        return apps * 2 * ny * S
+    if name == "line_x_filter":     # Pr, Pi on 5 fields
+        return 10 * 2 * nx * S
+    if name == "line_y_filter":     # Qr, -Qi on 5 fields
+        return 10 * 2 * ny * S
+    if name == "line_x_cons":
+        return 1 * 2 * nx * S
+    if name == "line_y_cons":
+        return 1 * 2 * ny * S
+    return None
+
+
+def step_flops_dense(nx, ny, nz):
+    tot = 0.0
+    for k, n in (("line_x_stage", 4), ("line_y_stage", 4), ("line_x_filter", 1), ("line_y_filter", 1),
+                 ("line_x_cons", 1), ("line_y_cons", 1)):
+        tot += n * line_pass_flops(k, nx, ny, nz)
+    return tot
+
+
+def run_gpu(nx, ny, nz, L, precision, steps, warmup, state, axb, profile_steps):
+    import hqt
+    st = hqt.Stepper(nx, ny, nz, L, L, mu=6.0, dt=0.02, winding=True, filter=True, precision=precision,
+                     ax_b=axb, ay_b=0.0)
+    st.set_state(*state)
+    st.step(warmup)
+    st.sync()
+    t0 = time.perf_counter()
+    st.step_async(steps)
+    st.sync()
+    t1 = time.perf_counter()
+    final = st.get_state()
+    stats = {}
+    if profile_steps:
+        st.profile(True)
+        st.step(profile_steps)
+        stats = st.kernel_stats()
+        st.profile(False)
+    st.close()
+    return (t1 - t0), final, stats
+
+
+def cpu_baseline(nx, ny, nz, L, state, axb, budget_s=20.0):
+    """The oracle (reference dense form, OpenBLAS/NumPy) on this host: 1 warm-up step, then as
+    many steps as fit in ``budget_s`` (at least 2)."""
+    from oracle import hqt_oracle as O
+    ops = O.Operators(nx, ny, nz, L, L)
+    prm = O.Params(mu=6.0, h=0.02, ax_b=axb, ay_b=0.0, winding=True)
+    S = list(state)
+    *S, _w = O.step(ops, prm, *S)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        *S, _w = O.step(ops, prm, *S)
+        n += 1
+        if n >= 2 and time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return n / dt, n
+
+
+def field_errors(a, b):
+    names = ("Psi", "A_t", "A_x", "A_y", "rho")
+    out = {}
+    for nm, u, v in zip(names, a, b):
+        out[nm] = float(np.max(np.abs(u - v)) / max(np.max(np.abs(v)), 1e-300))
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=301)
+    ap.add_argument("--nz", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(independent evolutions shard one per GPU, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import hqt
+    from hqt import diagnostics, ic
+
+    nx = ny = args.n
+    nz = args.nz
+    L = 50.0
+    state = ic.initial_state(nx, ny, nz, L, L, mu=6.0, seed=1, n_vortices=4, phase_noise=1e-3)
+    axb = ic.sinusoidal_drive(nx, ny, L, kk=7, amp=0.1)
+    S = nx * ny * nz
+
+    prof_steps = max(5, min(args.steps, 50))
+    t64, fin64, stats = run_gpu(nx, ny, nz, L, "fp64", args.steps, args.warmup, state, axb, prof_steps)
+    sps = args.steps / t64
+
+    # dominant kernel roofline (HIP events on the library's stream)
+    kernels = {}
+    for k, (cnt, ms) in stats.items():
+        kernels[k] = {"launches": cnt, "avg_us": 1e3 * ms / cnt, "ms_per_step": ms / prof_steps}
+    dom = max(stats.items(), key=lambda kv: kv[1][1])[0] if stats else None
+    roof = None
+    if dom is not None:
+        cnt, ms = stats[dom]
+        fl = line_pass_flops(dom, nx, ny, nz)
+        avg_s = ms / cnt / 1e3
+        if fl is not None:
+            ach = fl / avg_s / 1e12
+            roof = {"kernel": dom, "bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                    "avg_launch_us": round(avg_s * 1e6, 2),
+                    "flops_per_launch": fl}
+
+    step_bytes = 368.0 * S          # compulsory FP64 bytes per step (4 stages + post-step)
+    out = {
+        "metric": "RK4 steps/sec @301^2x20 (FP64 vs mixed, stated err) + p50/p99 request latency",
+        "value": round(sps, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t64 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded stationary profile + phase noise + 4 windings)",
+        "config": {"workload": "driven flow, FP64, nx=ny=%d nz=%d L=50 mu=6 dt=0.02 kk=7 sinusoidal A_x drive" % (nx, nz),
+                   "nx": nx, "ny": ny, "nz": nz, "L": L, "mu": 6.0, "dt": 0.02, "kk": 7, "drive_amp": 0.1},
+        "point_updates_per_s": round(sps * S, 1),
+        "step_hbm": {"algorithmic_bytes_per_step": step_bytes,
+                     "achieved_GBps": round(step_bytes * sps / 1e9, 2),
+                     "frac_of_peak": round(step_bytes * sps / 1e9 / PEAK_HBM_GBS, 4)},
+        "step_flops_dense_form": step_flops_dense(nx, ny, nz),
+        "roofline": roof,
+        "kernels": kernels,
+    }
+    O64 = None
+    try:
+        # boundary order parameter and vortex count of the FP64 final state
+        g = ic.Grid(nx, ny, nz, L, L)
+        O64 = diagnostics.order_parameter(g, fin64[0])
+        out["fp64_final"] = {"vortices": diagnostics.countN(O64), "O_abs_mean": float(np.mean(np.abs(O64)))}
+    except Exception as e:  # diagnostics are reported, never fatal
+        out["fp64_final"] = {"error": repr(e)}
+
+    if not args.no_mixed:
+        tm, finm, _ = run_gpu(nx, ny, nz, L, "mixed", args.steps, args.warmup, state, axb, 0)
+        err = field_errors(finm, fin64)
+        mixed = {"value": round(args.steps / tm, 3), "unit": "steps/s", "speedup_vs_fp64": round(t64 / tm, 3),
+                 "err_vs_fp64_after_steps": err}
+        if O64 is not None:
+            Om = diagnostics.order_parameter(g, finm[0])
+            mixed["O_abs_rel_err"] = float(np.max(np.abs(np.abs(Om) - np.abs(O64))) / np.max(np.abs(O64)))
+            mixed["vortices"] = diagnostics.countN(Om)
+        out["mixed"] = mixed
+
+    if not args.no_cpu:
+        cps, ncs = cpu_baseline(nx, ny, nz, L, state, axb)
+        out["cpu_baseline"] = {"value": round(cps, 5), "unit": "steps/s", "cores": os.cpu_count(),
+                               "kind": "port",
+                               "sample": f"{ncs} steps of the same workload after 1 warm-up step, NumPy/OpenBLAS "
+                                         f"dense-matrix restatement (oracle), all host threads"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

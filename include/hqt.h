@@ -106,6 +106,12 @@ int hqt_newton_vy(int32_t nz, double zh, double small, double D, const double* Z
                   const double* Z2, const double* vy1, double vy, double* z1, double* z2,
                   double* VY1, int32_t* it, double* err);
 
+/* Copy one internal device field of the last hqt_rhs / hqt_step_n into out ([nx,ny,nz], FP64
+ * values).  Names: "dxR" "dxI" "dxxR" "dxxI" "dxAx" "dxAy" "dxxAy" (x-pass outputs), "dyR" "dyI"
+ * "lapR" "lapI" "divA" "curl3" "curl4" (y-pass outputs), "dxAt" "dyAt" "dzAt" (frozen A_t
+ * derivatives).  Diagnostic access for per-kernel parity tests; no counterpart in the reference. */
+int hqt_debug_field(hqt_ctx* ctx, const char* name, double* out);
+
 /* ---- measurement ---------------------------------------------------------------------- */
 /* Enable (1) / disable (0) per-kernel HIP-event timing on the context's stream. */
 int hqt_profile(hqt_ctx* ctx, int32_t enable);

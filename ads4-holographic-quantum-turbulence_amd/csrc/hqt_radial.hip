@@ -5,21 +5,6 @@
 
 namespace hqt {
 
-// ---- d_z A_t of the frozen A_t; also advances the device step counter ----------------------
-__global__ __launch_bounds__(256) void k_zprep(const ZPrep A) {
-    if (stopped(A.flags)) return;
-    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (A.bump && c == 0) A.flags[1] = A.flags[1] + 1;
-    if (c >= A.nxy) return;
-    const int64_t n = A.nxy;
-    const int nz = A.nz;
-    for (int k = 0; k < nz; ++k) {
-        double a = 0.0;
-        for (int m = 0; m < nz; ++m) a = fma(A.Lz[k * nz + m], A.At[c + n * m], a);
-        A.dzAt[c + n * k] = a;
-    }
-}
-
 // ---- winding: round(mean(sum(angle(circshift(O,-1,2)./O),1))/(2*pi)) (:137) --------------
 __device__ __forceinline__ double block_sum_256(double v, double* sh) {
     sh[threadIdx.x] = v;
@@ -82,7 +67,8 @@ __global__ __launch_bounds__(256) void k_wind_apply(const Wind A) {
 bool zkernels_support_nz(int nz) { return nz >= 4 && nz <= 32; }
 
 #define HQT_DECL(n) extern template hipError_t zstage_launch_nz<n>(const ZStage&, int, hipStream_t); \
-                    extern template hipError_t zpost_launch_nz<n>(const ZPost&, int, hipStream_t);
+                    extern template hipError_t zpost_launch_nz<n>(const ZPost&, int, hipStream_t); \
+                    extern template hipError_t zprep_launch_nz<n>(const ZPrep&, hipStream_t);
 HQT_NZ_LIST(HQT_DECL)
 #undef HQT_DECL
 
@@ -104,9 +90,12 @@ hipError_t launch_zpost(const ZPost& A, int fp32, hipStream_t s) {
 }
 
 hipError_t launch_zprep(const ZPrep& A, hipStream_t s) {
-    dim3 grid((unsigned)((A.nxy + 255) / 256));
-    hipLaunchKernelGGL(k_zprep, grid, dim3(256), 0, s, A);
-    return hipGetLastError();
+    switch (A.nz) {
+#define HQT_ZR(n) case n: return zprep_launch_nz<n>(A, s);
+        HQT_NZ_LIST(HQT_ZR)
+#undef HQT_ZR
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_winding(const Wind& A, int fp32, hipStream_t s) {

@@ -1,22 +1,31 @@
-// Radial (Chebyshev z) kernels for gfx950: the per-column part of the AdS4 RK4 stage,
-// the A_t constraint solve with the boundary reset, d_z A_t, and the winding removal.
+// Radial (Chebyshev z) kernels for gfx950: the per-column part of the AdS4 RK4 stage and
+// the A_t constraint solve with the boundary reset.
 //
-// One thread owns one boundary-plane column (x, y) and holds it in registers; the
-// nz x nz operators (Lz, Lz1) are wave-uniform, so every matrix entry is a scalar
-// operand (s_load) of a v_fma_f64.  All radial arithmetic is FP64 in both precisions;
-// the storage type TS of the x/y fields is double (plain form) or float (mixed form).
+// Every nz x nz radial operator (Lz = d_z, Lz1 = the integration of func_Lzr) is applied
+// with FP64 MFMA: one wave owns 16 boundary-plane columns, a column vector is held in
+// registers as "slots" (slot s, lane group g = lane/16 holds row 4s + g of column lane%16),
+// which is both the B-operand layout of v_mfma_f64_16x16x4 for k-block s and the layout of
+// its accumulator (gfx950: accumulator register r of row tile t holds row 16t + 4r + g).
+// A matvec is therefore RT x KS MFMAs with the operator tiles kept in registers, and
+// matvecs chain (Lz(Lz Psi), Lzr(E)) without leaving registers.  The point-wise parts of
+// the right-hand sides run on the same slots, with 16 consecutive columns per lane group,
+// so every global access is a 128-byte coalesced segment.
+//
+// All radial arithmetic is FP64 in both precisions; TS is the storage type of the x/y
+// fields (double: plain form, float: mixed form).
 //
 // Reference (scripts/simulation/tevol_driven_cluster.m):
 //   F1, F3, F4 point-wise parts + func_Lz / func_Lzr     :62-65, :72-75
 //   F2 (only the z=0 row is consumed, k2(:,:,nz) at :128) :76
 //   RK4 stage states / combine / update                  :120-128
-//   A_t constraint solve, boundary reset                 :129-133
-//   winding removal                                      :136-139
+//   A_t constraint solve, boundary reset, O              :129-136
 #pragma once
 #include "hqt_kernels.h"
 #include <math.h>
 
 namespace hqt {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
 
 template <typename TS> __device__ __forceinline__ double ld(const void* p, int64_t i) {
     return (double)static_cast<const TS*>(p)[i];
@@ -27,6 +36,49 @@ template <typename TS> __device__ __forceinline__ void st(void* p, int64_t i, do
 
 __device__ __forceinline__ bool stopped(const int32_t* flags) {
     return __builtin_nontemporal_load(&flags[0]) != 0;
+}
+
+template <int NZ> struct ZG {
+    static constexpr int KS = (NZ + 3) / 4;     // slots (k-blocks of 4 rows)
+    static constexpr int RT = (NZ + 15) / 16;   // 16-row output tiles
+};
+
+// Operator tiles: A[t][s] = M[16t + m][4s + kk] (m = lane%16, kk = lane/16), zero outside n x n.
+template <int NZ>
+__device__ __forceinline__ void load_op(const double* __restrict__ M, int ldm, int n, double (&A)[ZG<NZ>::RT][ZG<NZ>::KS],
+                                        int m, int kk) {
+#pragma unroll
+    for (int t = 0; t < ZG<NZ>::RT; ++t)
+#pragma unroll
+        for (int s = 0; s < ZG<NZ>::KS; ++s) {
+            const int i = 16 * t + m, j = 4 * s + kk;
+            A[t][s] = (i < n && j < n) ? M[i * ldm + j] : 0.0;
+        }
+}
+
+// out = A * in on the slot layout (rows >= n of the operator are zero, so padded slots stay 0).
+template <int NZ>
+__device__ __forceinline__ void mv(const double (&A)[ZG<NZ>::RT][ZG<NZ>::KS], const double (&in)[ZG<NZ>::KS],
+                                   double (&out)[ZG<NZ>::KS]) {
+#pragma unroll
+    for (int t = 0; t < ZG<NZ>::RT; ++t) {
+        v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < ZG<NZ>::KS; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(A[t][s], in[s], acc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            if (4 * t + r < ZG<NZ>::KS) out[4 * t + r] = acc[r];
+    }
+}
+
+// Load a [nxy, nz] field into slots (0 outside the grid).
+template <int NZ, typename TS>
+__device__ __forceinline__ void load_slots(const void* f, int64_t c, bool cok, int g, int64_t n, double (&v)[ZG<NZ>::KS]) {
+#pragma unroll
+    for (int s = 0; s < ZG<NZ>::KS; ++s) {
+        const int k = 4 * s + g;
+        v[s] = (cok && k < NZ) ? ld<TS>(f, c + n * k) : 0.0;
+    }
 }
 
 // RK4 bookkeeping for one value of one evolved field (:120-128).
@@ -51,244 +103,258 @@ __device__ __forceinline__ void rk_update(int stage, double kv, void* acc, void*
     }
 }
 
-// ---- Psi equation (F1, :72-73) ------------------------------------------------------------
+// ---- Psi equation (F1, :72-73) --------------------------------------------------------------
+//   E1 = 1i*At*dPsi + 0.5i*Psi*dAt + 0.5*f*d2Psi - 1.5*z^2*dPsi - 0.5*z*Psi
+//        - 0.5*(Ax^2 + Ay^2)*Psi + 0.5*(dxx+dyy)Psi - 0.5i*Psi*(dxAx+dyAy) - 1i*(Ax*dxPsi + Ay*dyPsi)
 template <int NZ, typename TS>
-__device__ void zstage_psi(const ZStage& A, int64_t c) {
+__device__ void zstage_psi(const ZStage& A, int64_t c, bool cok, int g, const double (&Lz)[ZG<NZ>::RT][ZG<NZ>::KS],
+                           const double (&L1)[ZG<NZ>::RT][ZG<NZ>::KS]) {
+    constexpr int KS = ZG<NZ>::KS;
     const int64_t n = A.nxy;
-    const double* __restrict__ Lz = A.zo.Lz;
-    double dR[NZ], dI[NZ];
+    double sR[KS], sI[KS], dR[KS], dI[KS], eR[KS], eI[KS];
+    load_slots<NZ, TS>(A.sR, c, cok, g, n, sR);
+    load_slots<NZ, TS>(A.sI, c, cok, g, n, sI);
+    mv<NZ>(Lz, sR, dR);
+    mv<NZ>(Lz, sI, dI);
     {
-        double sR[NZ], sI[NZ];
+        double d2R[KS], d2I[KS];
+        mv<NZ>(Lz, dR, d2R);
+        mv<NZ>(Lz, dI, d2I);
 #pragma unroll
-        for (int k = 0; k < NZ; ++k) { sR[k] = ld<TS>(A.sR, c + n * k); sI[k] = ld<TS>(A.sI, c + n * k); }
-#pragma unroll
-        for (int k = 0; k < NZ; ++k) {
-            double ar = 0.0, ai = 0.0;
-#pragma unroll
-            for (int m = 0; m < NZ; ++m) { ar = fma(Lz[k * NZ + m], sR[m], ar); ai = fma(Lz[k * NZ + m], sI[m], ai); }
-            dR[k] = ar; dI[k] = ai;
+        for (int s = 0; s < KS; ++s) {
+            const int k = 4 * s + g;
+            double er = 0.0, ei = 0.0;
+            if (cok && k < NZ) {
+                const int64_t p = c + n * k;
+                const double sr = sR[s], si = sI[s];
+                const double at = A.At[p], dat = A.dzAt[p];
+                const double ax = ld<TS>(A.sAx, p), ay = ld<TS>(A.sAy, p);
+                const double dxr = ld<TS>(A.dxR, p), dxi = ld<TS>(A.dxI, p);
+                const double dyr = ld<TS>(A.dyR, p), dyi = ld<TS>(A.dyI, p);
+                const double lpr = ld<TS>(A.lapR, p), lpi = ld<TS>(A.lapI, p);
+                const double dv = ld<TS>(A.divA, p);
+                const double hf = A.zo.hf[k], cd = A.zo.cd[k], cp = A.zo.cp[k];
+                er = -at * dI[s];            ei = at * dR[s];
+                er += (-0.5 * si) * dat;     ei += (0.5 * sr) * dat;
+                er += hf * d2R[s];           ei += hf * d2I[s];
+                er += cd * dR[s];            ei += cd * dI[s];
+                er += cp * sr;               ei += cp * si;
+                const double a2 = (-0.5 * ax) * ax, b2 = (-0.5 * ay) * ay;
+                er += a2 * sr;               ei += a2 * si;
+                er += b2 * sr;               ei += b2 * si;
+                er += 0.5 * lpr;             ei += 0.5 * lpi;
+                er += (0.5 * si) * dv;       ei += (-0.5 * sr) * dv;
+                const double qr = ax * dxr + ay * dyr, qi = ax * dxi + ay * dyi;
+                er += qi;                    ei -= qr;
+            }
+            eR[s] = er;
+            eI[s] = ei;
         }
     }
-    double eR[NZ], eI[NZ];
+    mv<NZ>(L1, eR, dR);     // F1 = Lzr(E1); the last row of Lzr is zero
+    mv<NZ>(L1, eI, dI);
 #pragma unroll
-    for (int k = 0; k < NZ; ++k) {
-        double d2r = 0.0, d2i = 0.0;
-#pragma unroll
-        for (int m = 0; m < NZ; ++m) { d2r = fma(Lz[k * NZ + m], dR[m], d2r); d2i = fma(Lz[k * NZ + m], dI[m], d2i); }
+    for (int s = 0; s < KS; ++s) {
+        const int k = 4 * s + g;
+        if (!cok || k >= NZ) continue;
         const int64_t p = c + n * k;
-        const double sr = ld<TS>(A.sR, p), si = ld<TS>(A.sI, p);
-        const double at = A.At[p], dat = A.dzAt[p];
-        const double ax = ld<TS>(A.sAx, p), ay = ld<TS>(A.sAy, p);
-        const double dxr = ld<TS>(A.dxR, p), dxi = ld<TS>(A.dxI, p);
-        const double dyr = ld<TS>(A.dyR, p), dyi = ld<TS>(A.dyI, p);
-        const double lpr = ld<TS>(A.lapR, p), lpi = ld<TS>(A.lapI, p);
-        const double dv = ld<TS>(A.divA, p);
-        // 1i*At*dPsi + 0.5i*Psi*dAt + 0.5*f*d2Psi - 1.5*z^2*dPsi - 0.5*z*Psi
-        double er = -at * dI[k], ei = at * dR[k];
-        er += (-0.5 * si) * dat; ei += (0.5 * sr) * dat;
-        er += A.zo.hf[k] * d2r;  ei += A.zo.hf[k] * d2i;
-        er += A.zo.cd[k] * dR[k]; ei += A.zo.cd[k] * dI[k];
-        er += A.zo.cp[k] * sr;   ei += A.zo.cp[k] * si;
-        // -0.5*Ax*Ax*Psi - 0.5*Ay*Ay*Psi
-        const double a2 = (-0.5 * ax) * ax, b2 = (-0.5 * ay) * ay;
-        er += a2 * sr; ei += a2 * si;
-        er += b2 * sr; ei += b2 * si;
-        // +0.5*(dxx+dyy)Psi - 0.5i*Psi*(dxAx+dyAy) - 1i*(Ax*dxPsi + Ay*dyPsi)
-        er += 0.5 * lpr; ei += 0.5 * lpi;
-        er += (0.5 * si) * dv; ei += (-0.5 * sr) * dv;
-        const double qr = ax * dxr + ay * dyr, qi = ax * dxi + ay * dyi;
-        er += qi; ei -= qr;
-        eR[k] = er; eI[k] = ei;
-    }
-    const double* __restrict__ Lz1 = A.zo.Lz1;
-#pragma unroll
-    for (int k = 0; k < NZ; ++k) {
-        double fr = 0.0, fi = 0.0;
-        if (k < NZ - 1) {
-#pragma unroll
-            for (int m = 0; m < NZ - 1; ++m) { fr = fma(Lz1[k * (NZ - 1) + m], eR[m], fr); fi = fma(Lz1[k * (NZ - 1) + m], eI[m], fi); }
-        }
-        const int64_t p = c + n * k;
-        if (A.mode == 1) { A.F1R[p] = fr; A.F1I[p] = fi; continue; }
-        rk_update<TS>(A.stage, A.h * fr, A.aR, A.zR, A.nR, p);
-        rk_update<TS>(A.stage, A.h * fi, A.aI, A.zI, A.nI, p);
+        if (A.mode == 1) { A.F1R[p] = dR[s]; A.F1I[p] = dI[s]; continue; }
+        rk_update<TS>(A.stage, A.h * dR[s], A.aR, A.zR, A.nR, p);
+        rk_update<TS>(A.stage, A.h * dI[s], A.aI, A.zI, A.nI, p);
     }
 }
 
 // ---- A_x / A_y equations (F3, F4, :74-75) -------------------------------------------------
-// dir 0: F3 = Re Lzr[0.5*Lz(dxAt + f*Lz Ax) + 0.5*curl3 + Im(conj(Psi)*dxPsi) - Ax*|Psi|^2]
-// dir 1: F4 = Re Lzr[0.5*Lz(dyAt + f*Lz Ay) + 0.5*curl4 + Im(conj(Psi)*dyPsi) - Ay*|Psi|^2]
+// DIR 0: F3 = Re Lzr[0.5*Lz(dxAt + f*Lz Ax) + 0.5*curl3 + Im(conj(Psi)*dxPsi) - Ax*|Psi|^2]
+// DIR 1: F4 = Re Lzr[0.5*Lz(dyAt + f*Lz Ay) + 0.5*curl4 + Im(conj(Psi)*dyPsi) - Ay*|Psi|^2]
+// (the reference's current term -0.5i*(conj(Psi)*dPsi - Psi*d conj(Psi)) equals Im(conj(Psi)*dPsi))
 template <int NZ, typename TS, int DIR>
-__device__ void zstage_a(const ZStage& A, int64_t c) {
+__device__ void zstage_a(const ZStage& A, int64_t c, bool cok, int g, const double (&Lz)[ZG<NZ>::RT][ZG<NZ>::KS],
+                         const double (&L1)[ZG<NZ>::RT][ZG<NZ>::KS]) {
+    constexpr int KS = ZG<NZ>::KS;
     const int64_t n = A.nxy;
-    const double* __restrict__ Lz = A.zo.Lz;
     const void* sA = DIR == 0 ? A.sAx : A.sAy;
     const void* dAt = DIR == 0 ? A.dxAt : A.dyAt;
     const void* curl = DIR == 0 ? A.curl3 : A.curl4;
     const void* dPR = DIR == 0 ? A.dxR : A.dyR;
     const void* dPI = DIR == 0 ? A.dxI : A.dyI;
-    double u[NZ];
-    {
-        double s[NZ];
+    double a[KS], u[KS], e[KS];
+    load_slots<NZ, TS>(sA, c, cok, g, n, a);
+    mv<NZ>(Lz, a, u);
 #pragma unroll
-        for (int k = 0; k < NZ; ++k) s[k] = ld<TS>(sA, c + n * k);
+    for (int s = 0; s < KS; ++s) {
+        const int k = 4 * s + g;
+        u[s] = (cok && k < NZ) ? ld<TS>(dAt, c + n * k) + A.zo.f1[k] * u[s] : 0.0;
+    }
+    mv<NZ>(Lz, u, e);
 #pragma unroll
-        for (int k = 0; k < NZ; ++k) {
-            double a = 0.0;
-#pragma unroll
-            for (int m = 0; m < NZ; ++m) a = fma(Lz[k * NZ + m], s[m], a);
-            u[k] = ld<TS>(dAt, c + n * k) + A.zo.f1[k] * a;
+    for (int s = 0; s < KS; ++s) {
+        const int k = 4 * s + g;
+        double ev = 0.0;
+        if (cok && k < NZ) {
+            const int64_t p = c + n * k;
+            const double sr = ld<TS>(A.sR, p), si = ld<TS>(A.sI, p);
+            const double av = a[s];
+            const double cur = sr * ld<TS>(dPI, p) - si * ld<TS>(dPR, p);
+            ev = 0.5 * e[s] + 0.5 * ld<TS>(curl, p) + cur - ((av * sr) * sr + (av * si) * si);
         }
+        u[s] = ev;
     }
-    double e[NZ];
-#pragma unroll
-    for (int k = 0; k < NZ; ++k) {
-        double v = 0.0;
-#pragma unroll
-        for (int m = 0; m < NZ; ++m) v = fma(Lz[k * NZ + m], u[m], v);
-        const int64_t p = c + n * k;
-        const double sr = ld<TS>(A.sR, p), si = ld<TS>(A.sI, p);
-        const double av = ld<TS>(sA, p);
-        const double cur = sr * ld<TS>(dPI, p) - si * ld<TS>(dPR, p);
-        e[k] = 0.5 * v + 0.5 * ld<TS>(curl, p) + cur - ((av * sr) * sr + (av * si) * si);
-    }
-    const double* __restrict__ Lz1 = A.zo.Lz1;
+    mv<NZ>(L1, u, e);
     void* acc = DIR == 0 ? A.aAx : A.aAy;
     void* zf = DIR == 0 ? A.zAx : A.zAy;
     void* nx = DIR == 0 ? A.nAx : A.nAy;
     double* Fo = DIR == 0 ? A.F3 : A.F4;
 #pragma unroll
-    for (int k = 0; k < NZ; ++k) {
-        double f = 0.0;
-        if (k < NZ - 1) {
-#pragma unroll
-            for (int m = 0; m < NZ - 1; ++m) f = fma(Lz1[k * (NZ - 1) + m], e[m], f);
-        }
+    for (int s = 0; s < KS; ++s) {
+        const int k = 4 * s + g;
+        if (!cok || k >= NZ) continue;
         const int64_t p = c + n * k;
-        if (A.mode == 1) { Fo[p] = f; continue; }
-        rk_update<TS>(A.stage, A.h * f, acc, zf, nx, p);
+        if (A.mode == 1) { Fo[p] = e[s]; continue; }
+        rk_update<TS>(A.stage, A.h * e[s], acc, zf, nx, p);
     }
 }
 
 // ---- F2 = (1-z^3) * Lz(dxAx + dyAy) (:76); the RK step consumes only the z=0 row (:128) ----
 template <int NZ, typename TS>
-__device__ void zstage_f2(const ZStage& A, int64_t c) {
+__device__ void zstage_f2(const ZStage& A, int64_t c, bool cok, int g, const double (&Lz)[ZG<NZ>::RT][ZG<NZ>::KS]) {
+    constexpr int KS = ZG<NZ>::KS;
     const int64_t n = A.nxy;
-    const double* __restrict__ Lz = A.zo.Lz;
-    double dv[NZ];
+    double dv[KS], w[KS];
+    load_slots<NZ, TS>(A.divA, c, cok, g, n, dv);
+    mv<NZ>(Lz, dv, w);
 #pragma unroll
-    for (int k = 0; k < NZ; ++k) dv[k] = ld<TS>(A.divA, c + n * k);
-    if (A.mode == 1) {
-#pragma unroll
-        for (int k = 0; k < NZ; ++k) {
-            double a = 0.0;
-#pragma unroll
-            for (int m = 0; m < NZ; ++m) a = fma(Lz[k * NZ + m], dv[m], a);
-            A.F2[c + n * k] = A.zo.f2[k] * a;
+    for (int s = 0; s < KS; ++s) {
+        const int k = 4 * s + g;
+        if (!cok || k >= NZ) continue;
+        if (A.mode == 1) { A.F2[c + n * k] = A.zo.f2[k] * w[s]; continue; }
+        if (k != NZ - 1) continue;
+        const double kv = A.h * (A.zo.f2[k] * w[s]);
+        double* ar = A.arho;
+        if (A.stage == 1) ar[c] = kv;
+        else if (A.stage < 4) ar[c] = ar[c] + 2.0 * kv;
+        else {
+            const double fin = (1.0 / 6.0) * (ar[c] + kv);
+            for (int q = 0; q < NZ; ++q) A.rho[c + n * q] = A.rho[c + n * q] - fin;
         }
-        return;
-    }
-    double b = 0.0;
-#pragma unroll
-    for (int m = 0; m < NZ; ++m) b = fma(Lz[(NZ - 1) * NZ + m], dv[m], b);
-    const double kv = A.h * (A.zo.f2[NZ - 1] * b);
-    double* ar = A.arho;
-    if (A.stage == 1) ar[c] = kv;
-    else if (A.stage < 4) ar[c] = ar[c] + 2.0 * kv;
-    else {
-        const double fin = (1.0 / 6.0) * (ar[c] + kv);
-#pragma unroll
-        for (int k = 0; k < NZ; ++k) A.rho[c + n * k] = A.rho[c + n * k] - fin;
     }
 }
 
+// grid: x = ceil(nxy/64) blocks of 4 waves (16 columns each), y = role
+// (0: Psi, 1: A_x, 2: A_y, 3: F2 / rho).
 template <int NZ, typename TS>
 __global__ __launch_bounds__(256) void k_zstage(const ZStage A) {
     if (stopped(A.flags)) return;
-    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (c >= A.nxy) return;
-    switch (blockIdx.y) {
-        case 0: zstage_psi<NZ, TS>(A, c); break;
-        case 1: zstage_a<NZ, TS, 0>(A, c); break;
-        case 2: zstage_a<NZ, TS, 1>(A, c); break;
-        default: zstage_f2<NZ, TS>(A, c); break;
-    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, g = lane >> 4;
+    const int64_t c = (int64_t)blockIdx.x * 64 + wave * 16 + m;
+    const bool cok = c < A.nxy;
+    double Lz[ZG<NZ>::RT][ZG<NZ>::KS];
+    load_op<NZ>(A.zo.Lz, NZ, NZ, Lz, m, g);
+    if (blockIdx.y == 3) { zstage_f2<NZ, TS>(A, c, cok, g, Lz); return; }
+    double L1[ZG<NZ>::RT][ZG<NZ>::KS];
+    load_op<NZ>(A.zo.Lz1, NZ - 1, NZ - 1, L1, m, g);
+    if (blockIdx.y == 0) zstage_psi<NZ, TS>(A, c, cok, g, Lz, L1);
+    else if (blockIdx.y == 1) zstage_a<NZ, TS, 0>(A, c, cok, g, Lz, L1);
+    else zstage_a<NZ, TS, 1>(A, c, cok, g, Lz, L1);
 }
 
 // ---- post-step: A_t constraint solve and boundary reset (:129-133), O for the winding (:136)
 template <int NZ, typename TS>
 __global__ __launch_bounds__(256) void k_zpost(const ZPost A) {
     if (stopped(A.flags)) return;
-    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (c >= A.nxy) return;
+    constexpr int KS = ZG<NZ>::KS;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, g = lane >> 4;
+    const int64_t c = (int64_t)blockIdx.x * 64 + wave * 16 + m;
+    const bool cok = c < A.nxy;
     const int64_t n = A.nxy;
-    const double* __restrict__ Lz = A.zo.Lz;
-    const double* __restrict__ Lz1 = A.zo.Lz1;
-    double R[NZ], I[NZ], w[NZ];
+    double Lz[ZG<NZ>::RT][KS], L1[ZG<NZ>::RT][KS];
+    load_op<NZ>(A.zo.Lz, NZ, NZ, Lz, m, g);
+    load_op<NZ>(A.zo.Lz1, NZ - 1, NZ - 1, L1, m, g);
+    double R[KS], I[KS], a[KS], b[KS], w[KS];
+    load_slots<NZ, TS>(A.R, c, cok, g, n, R);
+    load_slots<NZ, TS>(A.I, c, cok, g, n, I);
+    load_slots<NZ, TS>(A.divA, c, cok, g, n, a);
+    mv<NZ>(Lz, a, w);                // Lz(dxAx + dyAy)
+    mv<NZ>(Lz, R, a);                // Lz Re Psi
+    mv<NZ>(Lz, I, b);                // Lz Im Psi
+    // cons = real(Lz(divA) + 1i*(conj(Psi)*Lz(Psi) - Psi*Lz(conj(Psi)))) = Lz(divA) - 2*Im(conj(Psi)*Lz Psi)
 #pragma unroll
-    for (int k = 0; k < NZ; ++k) { R[k] = ld<TS>(A.R, c + n * k); I[k] = ld<TS>(A.I, c + n * k); }
-    {
-        double dv[NZ];
+    for (int s = 0; s < KS; ++s) {
+        const double pim = R[s] * b[s] - I[s] * a[s];
+        w[s] = w[s] - 2.0 * pim;
+    }
+    mv<NZ>(L1, w, a);                // Lzr(cons)
 #pragma unroll
-        for (int k = 0; k < NZ; ++k) dv[k] = ld<TS>(A.divA, c + n * k);
-        // cons = real(Lz(divA) + 1i*(conj(Psi)*Lz(Psi) - Psi*Lz(conj(Psi)))) = Lz(divA) - 2*Im(conj(Psi)*Lz Psi)
+    for (int s = 0; s < KS; ++s) {
+        const int k = 4 * s + g;
+        w[s] = (cok && k < NZ) ? a[s] - A.rho[c + n * k] : 0.0;
+    }
+    mv<NZ>(L1, w, a);                // Lzr(Lzr(cons) - rhoi)
 #pragma unroll
-        for (int k = 0; k < NZ; ++k) {
-            double a = 0.0, dr = 0.0, di = 0.0;
-#pragma unroll
-            for (int m = 0; m < NZ; ++m) {
-                a = fma(Lz[k * NZ + m], dv[m], a);
-                dr = fma(Lz[k * NZ + m], R[m], dr);
-                di = fma(Lz[k * NZ + m], I[m], di);
-            }
-            const double pim = R[k] * di - I[k] * dr;
-            w[k] = a - 2.0 * pim;
+    for (int s = 0; s < KS; ++s) {
+        const int k = 4 * s + g;
+        if (!cok || k >= NZ) continue;
+        A.At[c + n * k] = (k < NZ - 1) ? a[s] + A.mu : A.mu;
+        if (k == NZ - 1) {
+            // boundary reset: Z1(:,:,nz)=0; Z3(:,:,nz)=ax_b; Z4(:,:,nz)=ay_b  (Z2 row set above)
+            const int64_t pb = c + n * k;
+            st<TS>(A.R, pb, 0.0); st<TS>(A.I, pb, 0.0);
+            st<TS>(A.Ax, pb, A.axb[c]); st<TS>(A.Ay, pb, A.ayb[c]);
+            R[s] = 0.0;
+            I[s] = 0.0;
         }
     }
-    // Z2 = Lzr(Lzr(cons) - rhoi) + mu
-    double w2[NZ];
+    // O1 = func_Lz(Z1) at z=0 after the reset
+    mv<NZ>(Lz, R, a);
+    mv<NZ>(Lz, I, b);
 #pragma unroll
-    for (int k = 0; k < NZ; ++k) {
-        double a = 0.0;
-        if (k < NZ - 1) {
-#pragma unroll
-            for (int m = 0; m < NZ - 1; ++m) a = fma(Lz1[k * (NZ - 1) + m], w[m], a);
-        }
-        w2[k] = a - A.rho[c + n * k];
+    for (int s = 0; s < KS; ++s) {
+        const int k = 4 * s + g;
+        if (cok && k == NZ - 1) { A.O[2 * c] = a[s]; A.O[2 * c + 1] = b[s]; }
     }
+}
+
+// ---- d_z A_t of the frozen A_t; also advances the device step counter ----------------------
+template <int NZ>
+__global__ __launch_bounds__(256) void k_zprep(const ZPrep A) {
+    if (stopped(A.flags)) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, g = lane >> 4;
+    const int64_t c = (int64_t)blockIdx.x * 64 + wave * 16 + m;
+    if (A.bump && blockIdx.x == 0 && threadIdx.x == 0) A.flags[1] = A.flags[1] + 1;
+    const bool cok = c < A.nxy;
+    double Lz[ZG<NZ>::RT][ZG<NZ>::KS], a[ZG<NZ>::KS], d[ZG<NZ>::KS];
+    load_op<NZ>(A.Lz, NZ, NZ, Lz, m, g);
+    load_slots<NZ, double>(A.At, c, cok, g, A.nxy, a);
+    mv<NZ>(Lz, a, d);
 #pragma unroll
-    for (int k = 0; k < NZ; ++k) {
-        double a = 0.0;
-        if (k < NZ - 1) {
-#pragma unroll
-            for (int m = 0; m < NZ - 1; ++m) a = fma(Lz1[k * (NZ - 1) + m], w2[m], a);
-        }
-        A.At[c + n * k] = (k < NZ - 1) ? a + A.mu : A.mu;
+    for (int s = 0; s < ZG<NZ>::KS; ++s) {
+        const int k = 4 * s + g;
+        if (cok && k < NZ) A.dzAt[c + A.nxy * k] = d[s];
     }
-    // boundary reset: Z1(:,:,nz)=0; Z3(:,:,nz)=ax_b; Z4(:,:,nz)=ay_b  (Z2 row set above)
-    const int64_t pb = c + n * (NZ - 1);
-    st<TS>(A.R, pb, 0.0); st<TS>(A.I, pb, 0.0);
-    st<TS>(A.Ax, pb, A.axb[c]); st<TS>(A.Ay, pb, A.ayb[c]);
-    // O1 = func_Lz(Z1) at z=0, after the reset
-    double orr = 0.0, oii = 0.0;
-#pragma unroll
-    for (int m = 0; m < NZ - 1; ++m) { orr = fma(Lz[(NZ - 1) * NZ + m], R[m], orr); oii = fma(Lz[(NZ - 1) * NZ + m], I[m], oii); }
-    A.O[2 * c] = orr; A.O[2 * c + 1] = oii;
 }
 
 // Per-nz launchers; each hqt_radial_nz_*.hip translation unit instantiates a range of nz.
 template <int NZ>
 hipError_t zstage_launch_nz(const ZStage& A, int fp32, hipStream_t s) {
-    dim3 grid((unsigned)((A.nxy + 255) / 256), 4);
+    dim3 grid((unsigned)((A.nxy + 63) / 64), 4);
     if (fp32) hipLaunchKernelGGL((k_zstage<NZ, float>), grid, dim3(256), 0, s, A);
     else hipLaunchKernelGGL((k_zstage<NZ, double>), grid, dim3(256), 0, s, A);
     return hipGetLastError();
 }
 template <int NZ>
 hipError_t zpost_launch_nz(const ZPost& A, int fp32, hipStream_t s) {
-    dim3 grid((unsigned)((A.nxy + 255) / 256));
+    dim3 grid((unsigned)((A.nxy + 63) / 64));
     if (fp32) hipLaunchKernelGGL((k_zpost<NZ, float>), grid, dim3(256), 0, s, A);
     else hipLaunchKernelGGL((k_zpost<NZ, double>), grid, dim3(256), 0, s, A);
+    return hipGetLastError();
+}
+template <int NZ>
+hipError_t zprep_launch_nz(const ZPrep& A, hipStream_t s) {
+    dim3 grid((unsigned)((A.nxy + 63) / 64));
+    hipLaunchKernelGGL((k_zprep<NZ>), grid, dim3(256), 0, s, A);
     return hipGetLastError();
 }
 

@@ -4,20 +4,29 @@
 namespace hqt {
 template hipError_t zstage_launch_nz<4>(const ZStage&, int, hipStream_t);
 template hipError_t zpost_launch_nz<4>(const ZPost&, int, hipStream_t);
+template hipError_t zprep_launch_nz<4>(const ZPrep&, hipStream_t);
 template hipError_t zstage_launch_nz<5>(const ZStage&, int, hipStream_t);
 template hipError_t zpost_launch_nz<5>(const ZPost&, int, hipStream_t);
+template hipError_t zprep_launch_nz<5>(const ZPrep&, hipStream_t);
 template hipError_t zstage_launch_nz<6>(const ZStage&, int, hipStream_t);
 template hipError_t zpost_launch_nz<6>(const ZPost&, int, hipStream_t);
+template hipError_t zprep_launch_nz<6>(const ZPrep&, hipStream_t);
 template hipError_t zstage_launch_nz<7>(const ZStage&, int, hipStream_t);
 template hipError_t zpost_launch_nz<7>(const ZPost&, int, hipStream_t);
+template hipError_t zprep_launch_nz<7>(const ZPrep&, hipStream_t);
 template hipError_t zstage_launch_nz<8>(const ZStage&, int, hipStream_t);
 template hipError_t zpost_launch_nz<8>(const ZPost&, int, hipStream_t);
+template hipError_t zprep_launch_nz<8>(const ZPrep&, hipStream_t);
 template hipError_t zstage_launch_nz<9>(const ZStage&, int, hipStream_t);
 template hipError_t zpost_launch_nz<9>(const ZPost&, int, hipStream_t);
+template hipError_t zprep_launch_nz<9>(const ZPrep&, hipStream_t);
 template hipError_t zstage_launch_nz<10>(const ZStage&, int, hipStream_t);
 template hipError_t zpost_launch_nz<10>(const ZPost&, int, hipStream_t);
+template hipError_t zprep_launch_nz<10>(const ZPrep&, hipStream_t);
 template hipError_t zstage_launch_nz<11>(const ZStage&, int, hipStream_t);
 template hipError_t zpost_launch_nz<11>(const ZPost&, int, hipStream_t);
+template hipError_t zprep_launch_nz<11>(const ZPrep&, hipStream_t);
 template hipError_t zstage_launch_nz<12>(const ZStage&, int, hipStream_t);
 template hipError_t zpost_launch_nz<12>(const ZPost&, int, hipStream_t);
+template hipError_t zprep_launch_nz<12>(const ZPrep&, hipStream_t);
 }  // namespace hqt

@@ -4,12 +4,17 @@
 namespace hqt {
 template hipError_t zstage_launch_nz<13>(const ZStage&, int, hipStream_t);
 template hipError_t zpost_launch_nz<13>(const ZPost&, int, hipStream_t);
+template hipError_t zprep_launch_nz<13>(const ZPrep&, hipStream_t);
 template hipError_t zstage_launch_nz<14>(const ZStage&, int, hipStream_t);
 template hipError_t zpost_launch_nz<14>(const ZPost&, int, hipStream_t);
+template hipError_t zprep_launch_nz<14>(const ZPrep&, hipStream_t);
 template hipError_t zstage_launch_nz<15>(const ZStage&, int, hipStream_t);
 template hipError_t zpost_launch_nz<15>(const ZPost&, int, hipStream_t);
+template hipError_t zprep_launch_nz<15>(const ZPrep&, hipStream_t);
 template hipError_t zstage_launch_nz<16>(const ZStage&, int, hipStream_t);
 template hipError_t zpost_launch_nz<16>(const ZPost&, int, hipStream_t);
+template hipError_t zprep_launch_nz<16>(const ZPrep&, hipStream_t);
 template hipError_t zstage_launch_nz<17>(const ZStage&, int, hipStream_t);
 template hipError_t zpost_launch_nz<17>(const ZPost&, int, hipStream_t);
+template hipError_t zprep_launch_nz<17>(const ZPrep&, hipStream_t);
 }  // namespace hqt

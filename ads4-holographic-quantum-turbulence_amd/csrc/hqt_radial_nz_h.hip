@@ -4,6 +4,8 @@
 namespace hqt {
 template hipError_t zstage_launch_nz<31>(const ZStage&, int, hipStream_t);
 template hipError_t zpost_launch_nz<31>(const ZPost&, int, hipStream_t);
+template hipError_t zprep_launch_nz<31>(const ZPrep&, hipStream_t);
 template hipError_t zstage_launch_nz<32>(const ZStage&, int, hipStream_t);
 template hipError_t zpost_launch_nz<32>(const ZPost&, int, hipStream_t);
+template hipError_t zprep_launch_nz<32>(const ZPrep&, hipStream_t);
 }  // namespace hqt

@@ -125,8 +125,11 @@ int download_as(double* dst, const void* src, size_t n, int dt) {
 LPass base_pass(int N, int nx, int ny, int nz, int dir, void* const* ops, int32_t* flags) {
     LPass P;
     std::memset(&P, 0, sizeof(P));
+    const LineGeom lg = line_geom(N);
     P.N = N;
-    P.Np = 16 * ((N + 15) / 16);
+    P.Np = lg.Np;
+    P.Mp = lg.Mp;
+    P.TL = lg.TL;
     P.S = P.Np + 1;
     P.nx = nx; P.ny = ny; P.nz = nz;
     P.nxc = (nx + 15) / 16;
@@ -167,7 +170,7 @@ int build_tables(hqt_ctx* c, int N, double L, const uint8_t* mask, int base) {
     std::vector<double> col, row;
     fourdif_colrow(N, 1, col, row);
     const double px = 2.0 * M_PI / L;                  // px = 2.*pi/L0 (:38)
-    const int Np = 16 * ((N + 15) / 16);
+    const LineGeom Np = line_geom(N);
     std::vector<std::vector<double>> tabs(6);
     toeplitz_table(col, row, Np, px, tabs[0]);         // Lx = px*Dx (:47)
     std::vector<double> cc(N), c2;
@@ -210,7 +213,8 @@ int run(hqt_ctx* c, int name, F&& f) {
 }
 
 int line(hqt_ctx* c, int pass, int dir, int name) {
-    return run(c, name, [&] { return launch_line_pass_dev(c->hp[pass], c->dp + pass, dir, c->fp32, c->stream); });
+    const int kind = (name == KN_LX_FILT || name == KN_LY_FILT) ? 2 : (name == KN_LX_CONS || name == KN_LY_CONS) ? 1 : 0;
+    return run(c, name, [&] { return launch_line_pass_dev(c->hp[pass], c->dp + pass, dir, c->fp32, kind, c->stream); });
 }
 
 int enqueue_step(hqt_ctx* c) {
@@ -655,7 +659,7 @@ int hqt_ff(int32_t nx, int32_t ny, int32_t nslices, const double* x_re, const do
         for (int dir = 0; dir < 2; ++dir) {
             const int N = dir ? ny : nx;
             const uint8_t* mk = dir ? fy : fx;
-            const int Np = 16 * ((N + 15) / 16);
+            const LineGeom Np = line_geom(N);
             std::vector<double> pr, pi, prr(N), pir(N), t0, t1, t2;
             filter_tables(N, mk, pr, pi);
             for (int m = 0; m < N; ++m) { prr[m] = pr[(N - m) % N]; pir[m] = pi[(N - m) % N]; }
@@ -693,8 +697,8 @@ int hqt_ff(int32_t nx, int32_t ny, int32_t nslices, const double* x_re, const do
         add_out(h, out, DT_F64, 0, OP_QRY, 1, OP_NQIY);     // Re(Py (a + i b)) = Qr a - Qi b
         if (hipMemcpy(dp, hp, sizeof(hp), hipMemcpyHostToDevice) != hipSuccess) { rc = fail(HQT_ERR_HIP, "hipMemcpy pass"); break; }
         if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) { rc = fail(HQT_ERR_HIP, "stream"); break; }
-        hipError_t e = launch_line_pass_dev(hp[0], dp, 0, 0, s);
-        if (e == hipSuccess) e = launch_line_pass_dev(hp[1], dp + 1, 1, 0, s);
+        hipError_t e = launch_line_pass_dev(hp[0], dp, 0, 0, 2, s);
+        if (e == hipSuccess) e = launch_line_pass_dev(hp[1], dp + 1, 1, 0, 2, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) { rc = fail(HQT_ERR_HIP, std::string("hqt_ff: ") + hipGetErrorString(e)); break; }
         if (hipMemcpy(w, out, n * 8, hipMemcpyDeviceToHost) != hipSuccess) { rc = fail(HQT_ERR_HIP, "hipMemcpy w"); break; }

@@ -31,12 +31,13 @@ struct LGroup {
 };
 struct LPass {
     int32_t N, Np, S;          // line length, padded length (multiple of 16), LDS row stride
+    int32_t Mp, TL;            // output positions padded to 64 (4 waves x Q tiles), Toeplitz table length
     int32_t nx, ny, nz;
     int32_t ntiles;            // line tiles (grid.x)
     int32_t nxc;               // dir 1: x chunks of 16
     int32_t ngroups;           // grid.y
     LGroup g[LP_MAXG];
-    const void* ops[LP_MAXOPS]; // Toeplitz tables, 2*Np entries of the pass precision
+    const void* ops[LP_MAXOPS]; // Toeplitz tables: tab[i - j + Np + 4] = T[i][j], TL entries
     int32_t* flags;            // [0] = NaN step (0: none), [1] = step counter
 };
 
@@ -98,7 +99,8 @@ struct Wind {
 
 // Launchers (hqt_kernels.hip).  All return the hipError_t of the launch.
 // H: host copy of the descriptor (grid sizes, LDS bytes); dP: the same descriptor in device memory.
-hipError_t launch_line_pass_dev(const LPass& H, const LPass* dP, int dir, int fp32, hipStream_t s);
+// kind: 0 RHS stage, 1 constraint, 2 filter (separate instantiations for profiling).
+hipError_t launch_line_pass_dev(const LPass& H, const LPass* dP, int dir, int fp32, int kind, hipStream_t s);
 size_t line_pass_lds_bytes(const LPass& P, int fp32);
 hipError_t launch_zstage(const ZStage& A, int fp32, hipStream_t s);
 hipError_t launch_zpost(const ZPost& A, int fp32, hipStream_t s);

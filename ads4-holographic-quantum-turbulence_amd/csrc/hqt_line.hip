@@ -45,12 +45,14 @@ __device__ __forceinline__ void stv(void* p, int dt, int64_t i, T v) {
     else static_cast<float*>(p)[i] = (float)v;
 }
 
-template <typename T, int DIR, int Q>
+// KIND only tags the instantiation (0: RHS stage, 1: constraint, 2: filter) so that
+// profilers report the three uses separately; the code is identical.
+template <typename T, int DIR, int Q, int KIND>
 __global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restrict__ Pp) {
     const LPass& P = *Pp;
     if (__builtin_nontemporal_load(&P.flags[0]) != 0) return;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-    const int S = P.S, Np = P.Np, N = P.N;
+    const int S = P.S, Np = P.Np, N = P.N, TL = P.TL;
     T* V = reinterpret_cast<T*>(smem_raw);
     T* C = V + 16 * S;
     const LGroup& G = P.g[blockIdx.y];
@@ -58,7 +60,7 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restric
 
     for (int s = 0; s < G.nslots; ++s) {
         const T* src = static_cast<const T*>(P.ops[G.slot_op[s]]);
-        for (int t = tid; t < 2 * Np; t += LP_THREADS) C[s * 2 * Np + t] = src[t];
+        for (int t = tid; t < TL; t += LP_THREADS) C[s * TL + t] = src[t];
     }
 
     int64_t line0 = 0, nl = 0;
@@ -79,7 +81,10 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restric
 #pragma unroll
         for (int q = 0; q < Q; ++q) acc[o][q] = v4{0, 0, 0, 0};
 
-    const int lane = tid & 63, wave = tid >> 6, ln = lane & 15, lk = lane >> 4;
+    const int lane = tid & 63, ln = lane & 15, lk = lane >> 4;
+    // wave index and its tile count are wave-uniform: keep them in scalar registers so the
+    // per-tile guards below are scalar branches, not exec-masked ones
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
     for (int t = 0; t < G.ntiles; ++t) {
         __syncthreads();
@@ -109,19 +114,29 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restric
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
                 if (e >= G.out[o].nterms || G.out[o].term[e].tile != t) continue;
-                const T* Cs = C + G.out[o].term[e].slot * 2 * Np + Np + ln - lk;
+                // table entry of T[i][j] is tab[i - j + Np + 4]; lane (ln, lk) reads i = i0 + ln, j = j0 + lk
+                const T* Cs = C + G.out[o].term[e].slot * TL + Np + 4 + ln - lk + wave * 16;
                 const T* Vr = V + ln * S + lk;
-#pragma unroll 4
-                for (int j0 = 0; j0 < Np; j0 += 4) {
-                    const T va = Vr[j0];
+                // two k-steps per iteration with ping-pong operand registers: the loads of
+                // one k-step are in flight while the other's MFMAs issue.  The last loads run
+                // one k-step past the tile (values unused; the table is padded for them).
+                T va0 = Vr[0], va1;
+                T cb0[Q], cb1[Q];
 #pragma unroll
-                    for (int q = 0; q < Q; ++q) {
-                        const int i0 = (wave + 4 * q) * 16;
-                        if (i0 < Np) {
-                            const T cb = Cs[i0 - j0];
-                            acc[o][q] = DIR == 0 ? MF<T>::mma(va, cb, acc[o][q]) : MF<T>::mma(cb, va, acc[o][q]);
-                        }
-                    }
+                for (int q = 0; q < Q; ++q) cb0[q] = Cs[64 * q];
+                for (int j0 = 0; j0 < Np; j0 += 8) {
+                    va1 = Vr[j0 + 4];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) cb1[q] = Cs[64 * q - j0 - 4];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q)
+                        acc[o][q] = DIR == 0 ? MF<T>::mma(va0, cb0[q], acc[o][q]) : MF<T>::mma(cb0[q], va0, acc[o][q]);
+                    va0 = Vr[j0 + 8];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) cb0[q] = Cs[64 * q - j0 - 8];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q)
+                        acc[o][q] = DIR == 0 ? MF<T>::mma(va1, cb1[q], acc[o][q]) : MF<T>::mma(cb1[q], va1, acc[o][q]);
                 }
             }
         }
@@ -135,7 +150,6 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restric
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const int i0 = (wave + 4 * q) * 16;
-            if (i0 >= Np) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 int64_t idx;
@@ -168,12 +182,12 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restric
     }
 }
 
-template <typename T, int DIR>
+template <typename T, int DIR, int KIND>
 static hipError_t launch_q(const LPass& H, const LPass* dP, int Q, size_t lds, hipStream_t s) {
     dim3 grid(H.ntiles, H.ngroups), block(LP_THREADS);
     switch (Q) {
 #define HQT_LP_CASE(q) \
-    case q: hipLaunchKernelGGL((k_line_pass<T, DIR, q>), grid, block, lds, s, dP); break;
+    case q: hipLaunchKernelGGL((k_line_pass<T, DIR, q, KIND>), grid, block, lds, s, dP); break;
         HQT_LP_CASE(1) HQT_LP_CASE(2) HQT_LP_CASE(3) HQT_LP_CASE(4)
         HQT_LP_CASE(5) HQT_LP_CASE(6) HQT_LP_CASE(7) HQT_LP_CASE(8)
 #undef HQT_LP_CASE
@@ -186,18 +200,22 @@ size_t line_pass_lds_bytes(const LPass& P, int fp32) {
     int maxslots = 0;
     for (int g = 0; g < P.ngroups; ++g) maxslots = P.g[g].nslots > maxslots ? P.g[g].nslots : maxslots;
     const size_t es = fp32 ? sizeof(float) : sizeof(double);
-    return es * (16 * (size_t)P.S + (size_t)maxslots * 2 * P.Np);
+    return es * (16 * (size_t)P.S + (size_t)maxslots * P.TL);
 }
 
 template <typename T, int DIR, int Q>
 static void set_lds_attr(size_t bytes) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_line_pass<T, DIR, Q>),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_line_pass<T, DIR, Q, 0>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_line_pass<T, DIR, Q, 1>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_line_pass<T, DIR, Q, 2>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 // H: host copy of the descriptor (for grid sizes), dP: the same descriptor in device memory.
-hipError_t launch_line_pass_dev(const LPass& H, const LPass* dP, int dir, int fp32, hipStream_t s) {
-    const int Q = (H.Np / 16 + 3) / 4;
+hipError_t launch_line_pass_dev(const LPass& H, const LPass* dP, int dir, int fp32, int kind, hipStream_t s) {
+    const int Q = H.Mp / 64;
     const size_t lds = line_pass_lds_bytes(H, fp32);
     if (lds > 64 * 1024) {
         static bool done[2][2][9] = {};
@@ -210,8 +228,11 @@ hipError_t launch_line_pass_dev(const LPass& H, const LPass* dP, int dir, int fp
 #undef HQT_ATTR
         }
     }
-    if (fp32) return dir ? launch_q<float, 1>(H, dP, Q, lds, s) : launch_q<float, 0>(H, dP, Q, lds, s);
-    return dir ? launch_q<double, 1>(H, dP, Q, lds, s) : launch_q<double, 0>(H, dP, Q, lds, s);
+#define HQT_K(T_, D_) (kind == 0 ? launch_q<T_, D_, 0>(H, dP, Q, lds, s) : kind == 1 ? launch_q<T_, D_, 1>(H, dP, Q, lds, s) \
+                                                                          : launch_q<T_, D_, 2>(H, dP, Q, lds, s))
+    if (fp32) return dir ? HQT_K(float, 1) : HQT_K(float, 0);
+    return dir ? HQT_K(double, 1) : HQT_K(double, 0);
+#undef HQT_K
 }
 
 }  // namespace hqt

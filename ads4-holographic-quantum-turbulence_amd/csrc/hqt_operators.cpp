@@ -203,12 +203,21 @@ void circulant_square(const std::vector<double>& c, std::vector<double>& c2) {
     }
 }
 
-void toeplitz_table(const std::vector<double>& col, const std::vector<double>& row, int Np, double scale,
+LineGeom line_geom(int N) {
+    LineGeom g;
+    g.N = N;
+    g.Np = 16 * ((N + 15) / 16);
+    g.Mp = 64 * ((N + 63) / 64);
+    g.TL = 4 * ((g.Np + g.Mp + 4 + 3) / 4);
+    return g;
+}
+
+void toeplitz_table(const std::vector<double>& col, const std::vector<double>& row, const LineGeom& g, double scale,
                     std::vector<double>& tab) {
     const int N = (int)col.size();
-    tab.assign(2 * (size_t)Np, 0.0);
-    for (int t = 0; t < 2 * Np; ++t) {
-        const int d = t - Np;
+    tab.assign((size_t)g.TL, 0.0);
+    for (int t = 0; t < g.TL; ++t) {
+        const int d = t - g.Np - 4;
         if (d >= 0 && d < N) tab[t] = scale * col[d];
         else if (d < 0 && -d < N) tab[t] = scale * row[-d];
     }

@@ -19,8 +19,11 @@ void matlab_masks(int n, std::vector<uint8_t>& f);
 void filter_tables(int N, const uint8_t* mask, std::vector<double>& pr, std::vector<double>& pi);
 // First column of C*C for the circulant C with first column c (long-double accumulation).
 void circulant_square(const std::vector<double>& c, std::vector<double>& c2);
-// Toeplitz lookup table tab[t] = scale * T[i][j] with t = i - j + Np, 2*Np entries.
-void toeplitz_table(const std::vector<double>& col, const std::vector<double>& row, int Np, double scale,
+// Line-pass geometry: K padded to 16 (Np), outputs padded to 64 (Mp), table length TL.
+struct LineGeom { int N, Np, Mp, TL; };
+LineGeom line_geom(int N);
+// Toeplitz lookup table tab[u] = scale * T[i][j] with u = i - j + Np + 4 (zero outside |i-j| < N).
+void toeplitz_table(const std::vector<double>& col, const std::vector<double>& row, const LineGeom& g, double scale,
                     std::vector<double>& tab);
 // NewtonRaphson_vy; returns 0 when the residual test passed.
 int newton_vy(int nz, double zh, double small, double D, const double* Z1, const double* Z2, const double* vy1,

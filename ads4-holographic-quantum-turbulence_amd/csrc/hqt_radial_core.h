@@ -103,81 +103,25 @@ __device__ __forceinline__ void rk_update(int stage, double kv, void* acc, void*
     }
 }
 
-// ---- Psi equation (F1, :72-73) --------------------------------------------------------------
+// ---- one RK stage for 16 columns per wave (F1, F3, F4, F2 of :72-76 + RK bookkeeping) ----
 //   E1 = 1i*At*dPsi + 0.5i*Psi*dAt + 0.5*f*d2Psi - 1.5*z^2*dPsi - 0.5*z*Psi
 //        - 0.5*(Ax^2 + Ay^2)*Psi + 0.5*(dxx+dyy)Psi - 0.5i*Psi*(dxAx+dyAy) - 1i*(Ax*dxPsi + Ay*dyPsi)
-template <int NZ, typename TS>
-__device__ void zstage_psi(const ZStage& A, int64_t c, bool cok, int g, const double (&Lz)[ZG<NZ>::RT][ZG<NZ>::KS],
-                           const double (&L1)[ZG<NZ>::RT][ZG<NZ>::KS]) {
-    constexpr int KS = ZG<NZ>::KS;
-    const int64_t n = A.nxy;
-    double sR[KS], sI[KS], dR[KS], dI[KS], eR[KS], eI[KS];
-    load_slots<NZ, TS>(A.sR, c, cok, g, n, sR);
-    load_slots<NZ, TS>(A.sI, c, cok, g, n, sI);
-    mv<NZ>(Lz, sR, dR);
-    mv<NZ>(Lz, sI, dI);
-    {
-        double d2R[KS], d2I[KS];
-        mv<NZ>(Lz, dR, d2R);
-        mv<NZ>(Lz, dI, d2I);
-#pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            const int k = 4 * s + g;
-            double er = 0.0, ei = 0.0;
-            if (cok && k < NZ) {
-                const int64_t p = c + n * k;
-                const double sr = sR[s], si = sI[s];
-                const double at = A.At[p], dat = A.dzAt[p];
-                const double ax = ld<TS>(A.sAx, p), ay = ld<TS>(A.sAy, p);
-                const double dxr = ld<TS>(A.dxR, p), dxi = ld<TS>(A.dxI, p);
-                const double dyr = ld<TS>(A.dyR, p), dyi = ld<TS>(A.dyI, p);
-                const double lpr = ld<TS>(A.lapR, p), lpi = ld<TS>(A.lapI, p);
-                const double dv = ld<TS>(A.divA, p);
-                const double hf = A.zo.hf[k], cd = A.zo.cd[k], cp = A.zo.cp[k];
-                er = -at * dI[s];            ei = at * dR[s];
-                er += (-0.5 * si) * dat;     ei += (0.5 * sr) * dat;
-                er += hf * d2R[s];           ei += hf * d2I[s];
-                er += cd * dR[s];            ei += cd * dI[s];
-                er += cp * sr;               ei += cp * si;
-                const double a2 = (-0.5 * ax) * ax, b2 = (-0.5 * ay) * ay;
-                er += a2 * sr;               ei += a2 * si;
-                er += b2 * sr;               ei += b2 * si;
-                er += 0.5 * lpr;             ei += 0.5 * lpi;
-                er += (0.5 * si) * dv;       ei += (-0.5 * sr) * dv;
-                const double qr = ax * dxr + ay * dyr, qi = ax * dxi + ay * dyi;
-                er += qi;                    ei -= qr;
-            }
-            eR[s] = er;
-            eI[s] = ei;
-        }
-    }
-    mv<NZ>(L1, eR, dR);     // F1 = Lzr(E1); the last row of Lzr is zero
-    mv<NZ>(L1, eI, dI);
-#pragma unroll
-    for (int s = 0; s < KS; ++s) {
-        const int k = 4 * s + g;
-        if (!cok || k >= NZ) continue;
-        const int64_t p = c + n * k;
-        if (A.mode == 1) { A.F1R[p] = dR[s]; A.F1I[p] = dI[s]; continue; }
-        rk_update<TS>(A.stage, A.h * dR[s], A.aR, A.zR, A.nR, p);
-        rk_update<TS>(A.stage, A.h * dI[s], A.aI, A.zI, A.nI, p);
-    }
-}
-
-// ---- A_x / A_y equations (F3, F4, :74-75) -------------------------------------------------
-// DIR 0: F3 = Re Lzr[0.5*Lz(dxAt + f*Lz Ax) + 0.5*curl3 + Im(conj(Psi)*dxPsi) - Ax*|Psi|^2]
-// DIR 1: F4 = Re Lzr[0.5*Lz(dyAt + f*Lz Ay) + 0.5*curl4 + Im(conj(Psi)*dyPsi) - Ay*|Psi|^2]
-// (the reference's current term -0.5i*(conj(Psi)*dPsi - Psi*d conj(Psi)) equals Im(conj(Psi)*dPsi))
+//   E3 = 0.5*Lz(dxAt + f*Lz Ax) + 0.5*curl3 + Im(conj(Psi)*dxPsi) - Ax*|Psi|^2
+//   E4 = 0.5*Lz(dyAt + f*Lz Ay) + 0.5*curl4 + Im(conj(Psi)*dyPsi) - Ay*|Psi|^2
+//   F1 = Lzr(E1), F3 = Lzr(E3), F4 = Lzr(E4), F2 = (1-z^3)*Lz(dxAx + dyAy)
+// (the reference's current term -0.5i*(conj(Psi)*dPsi - Psi*d conj(Psi)) equals Im(conj(Psi)*dPsi)).
+// Every input field is read once; the stage state Psi and the x/y derivatives of Psi stay in
+// registers across the three equations.
 template <int NZ, typename TS, int DIR>
-__device__ void zstage_a(const ZStage& A, int64_t c, bool cok, int g, const double (&Lz)[ZG<NZ>::RT][ZG<NZ>::KS],
-                         const double (&L1)[ZG<NZ>::RT][ZG<NZ>::KS]) {
+__device__ __forceinline__ void zstage_a(const ZStage& A, int64_t c, bool cok, int g, const double (&Lz)[ZG<NZ>::RT][ZG<NZ>::KS],
+                                         const double (&L1)[ZG<NZ>::RT][ZG<NZ>::KS], const double (&sR)[ZG<NZ>::KS],
+                                         const double (&sI)[ZG<NZ>::KS], const double (&dR)[ZG<NZ>::KS],
+                                         const double (&dI)[ZG<NZ>::KS]) {
     constexpr int KS = ZG<NZ>::KS;
     const int64_t n = A.nxy;
     const void* sA = DIR == 0 ? A.sAx : A.sAy;
     const void* dAt = DIR == 0 ? A.dxAt : A.dyAt;
     const void* curl = DIR == 0 ? A.curl3 : A.curl4;
-    const void* dPR = DIR == 0 ? A.dxR : A.dyR;
-    const void* dPI = DIR == 0 ? A.dxI : A.dyI;
     double a[KS], u[KS], e[KS];
     load_slots<NZ, TS>(sA, c, cok, g, n, a);
     mv<NZ>(Lz, a, u);
@@ -193,9 +137,8 @@ __device__ void zstage_a(const ZStage& A, int64_t c, bool cok, int g, const doub
         double ev = 0.0;
         if (cok && k < NZ) {
             const int64_t p = c + n * k;
-            const double sr = ld<TS>(A.sR, p), si = ld<TS>(A.sI, p);
-            const double av = a[s];
-            const double cur = sr * ld<TS>(dPI, p) - si * ld<TS>(dPR, p);
+            const double sr = sR[s], si = sI[s], av = a[s];
+            const double cur = sr * dI[s] - si * dR[s];
             ev = 0.5 * e[s] + 0.5 * ld<TS>(curl, p) + cur - ((av * sr) * sr + (av * si) * si);
         }
         u[s] = ev;
@@ -215,48 +158,99 @@ __device__ void zstage_a(const ZStage& A, int64_t c, bool cok, int g, const doub
     }
 }
 
-// ---- F2 = (1-z^3) * Lz(dxAx + dyAy) (:76); the RK step consumes only the z=0 row (:128) ----
+// grid: x = ceil(nxy/64) blocks of 4 waves, 16 columns per wave.
 template <int NZ, typename TS>
-__device__ void zstage_f2(const ZStage& A, int64_t c, bool cok, int g, const double (&Lz)[ZG<NZ>::RT][ZG<NZ>::KS]) {
-    constexpr int KS = ZG<NZ>::KS;
-    const int64_t n = A.nxy;
-    double dv[KS], w[KS];
-    load_slots<NZ, TS>(A.divA, c, cok, g, n, dv);
-    mv<NZ>(Lz, dv, w);
-#pragma unroll
-    for (int s = 0; s < KS; ++s) {
-        const int k = 4 * s + g;
-        if (!cok || k >= NZ) continue;
-        if (A.mode == 1) { A.F2[c + n * k] = A.zo.f2[k] * w[s]; continue; }
-        if (k != NZ - 1) continue;
-        const double kv = A.h * (A.zo.f2[k] * w[s]);
-        double* ar = A.arho;
-        if (A.stage == 1) ar[c] = kv;
-        else if (A.stage < 4) ar[c] = ar[c] + 2.0 * kv;
-        else {
-            const double fin = (1.0 / 6.0) * (ar[c] + kv);
-            for (int q = 0; q < NZ; ++q) A.rho[c + n * q] = A.rho[c + n * q] - fin;
-        }
-    }
-}
-
-// grid: x = ceil(nxy/64) blocks of 4 waves (16 columns each), y = role
-// (0: Psi, 1: A_x, 2: A_y, 3: F2 / rho).
-template <int NZ, typename TS>
-__global__ __launch_bounds__(256) void k_zstage(const ZStage A) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_zstage(const ZStage A) {
     if (stopped(A.flags)) return;
+    constexpr int KS = ZG<NZ>::KS;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int m = lane & 15, g = lane >> 4;
     const int64_t c = (int64_t)blockIdx.x * 64 + wave * 16 + m;
     const bool cok = c < A.nxy;
-    double Lz[ZG<NZ>::RT][ZG<NZ>::KS];
+    const int64_t n = A.nxy;
+    double Lz[ZG<NZ>::RT][KS], L1[ZG<NZ>::RT][KS];
     load_op<NZ>(A.zo.Lz, NZ, NZ, Lz, m, g);
-    if (blockIdx.y == 3) { zstage_f2<NZ, TS>(A, c, cok, g, Lz); return; }
-    double L1[ZG<NZ>::RT][ZG<NZ>::KS];
     load_op<NZ>(A.zo.Lz1, NZ - 1, NZ - 1, L1, m, g);
-    if (blockIdx.y == 0) zstage_psi<NZ, TS>(A, c, cok, g, Lz, L1);
-    else if (blockIdx.y == 1) zstage_a<NZ, TS, 0>(A, c, cok, g, Lz, L1);
-    else zstage_a<NZ, TS, 1>(A, c, cok, g, Lz, L1);
+    double sR[KS], sI[KS], xR[KS], xI[KS], yR[KS], yI[KS], dv[KS];
+    load_slots<NZ, TS>(A.sR, c, cok, g, n, sR);
+    load_slots<NZ, TS>(A.sI, c, cok, g, n, sI);
+    load_slots<NZ, TS>(A.dxR, c, cok, g, n, xR);
+    load_slots<NZ, TS>(A.dxI, c, cok, g, n, xI);
+    load_slots<NZ, TS>(A.dyR, c, cok, g, n, yR);
+    load_slots<NZ, TS>(A.dyI, c, cok, g, n, yI);
+    load_slots<NZ, TS>(A.divA, c, cok, g, n, dv);
+    // ---- F1 ----
+    {
+        double dR[KS], dI[KS], eR[KS], eI[KS];
+        mv<NZ>(Lz, sR, dR);
+        mv<NZ>(Lz, sI, dI);
+        {
+            double d2R[KS], d2I[KS];
+            mv<NZ>(Lz, dR, d2R);
+            mv<NZ>(Lz, dI, d2I);
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const int k = 4 * s + g;
+                double er = 0.0, ei = 0.0;
+                if (cok && k < NZ) {
+                    const int64_t p = c + n * k;
+                    const double sr = sR[s], si = sI[s];
+                    const double at = A.At[p], dat = A.dzAt[p];
+                    const double ax = ld<TS>(A.sAx, p), ay = ld<TS>(A.sAy, p);
+                    const double lpr = ld<TS>(A.lapR, p), lpi = ld<TS>(A.lapI, p);
+                    const double hf = A.zo.hf[k], cd = A.zo.cd[k], cp = A.zo.cp[k];
+                    er = -at * dI[s];            ei = at * dR[s];
+                    er += (-0.5 * si) * dat;     ei += (0.5 * sr) * dat;
+                    er += hf * d2R[s];           ei += hf * d2I[s];
+                    er += cd * dR[s];            ei += cd * dI[s];
+                    er += cp * sr;               ei += cp * si;
+                    const double a2 = (-0.5 * ax) * ax, b2 = (-0.5 * ay) * ay;
+                    er += a2 * sr;               ei += a2 * si;
+                    er += b2 * sr;               ei += b2 * si;
+                    er += 0.5 * lpr;             ei += 0.5 * lpi;
+                    er += (0.5 * si) * dv[s];    ei += (-0.5 * sr) * dv[s];
+                    const double qr = ax * xR[s] + ay * yR[s], qi = ax * xI[s] + ay * yI[s];
+                    er += qi;                    ei -= qr;
+                }
+                eR[s] = er;
+                eI[s] = ei;
+            }
+        }
+        mv<NZ>(L1, eR, dR);     // F1 = Lzr(E1); the last row of Lzr is zero
+        mv<NZ>(L1, eI, dI);
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int k = 4 * s + g;
+            if (!cok || k >= NZ) continue;
+            const int64_t p = c + n * k;
+            if (A.mode == 1) { A.F1R[p] = dR[s]; A.F1I[p] = dI[s]; continue; }
+            rk_update<TS>(A.stage, A.h * dR[s], A.aR, A.zR, A.nR, p);
+            rk_update<TS>(A.stage, A.h * dI[s], A.aI, A.zI, A.nI, p);
+        }
+    }
+    // ---- F3, F4 ----
+    zstage_a<NZ, TS, 0>(A, c, cok, g, Lz, L1, sR, sI, xR, xI);
+    zstage_a<NZ, TS, 1>(A, c, cok, g, Lz, L1, sR, sI, yR, yI);
+    // ---- F2 (z=0 row feeds the charge update at :128) ----
+    {
+        double w[KS];
+        mv<NZ>(Lz, dv, w);
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int k = 4 * s + g;
+            if (!cok || k >= NZ) continue;
+            if (A.mode == 1) { A.F2[c + n * k] = A.zo.f2[k] * w[s]; continue; }
+            if (k != NZ - 1) continue;
+            const double kv = A.h * (A.zo.f2[k] * w[s]);
+            double* ar = A.arho;
+            if (A.stage == 1) ar[c] = kv;
+            else if (A.stage < 4) ar[c] = ar[c] + 2.0 * kv;
+            else {
+                const double fin = (1.0 / 6.0) * (ar[c] + kv);
+                for (int q = 0; q < NZ; ++q) A.rho[c + n * q] = A.rho[c + n * q] - fin;
+            }
+        }
+    }
 }
 
 // ---- post-step: A_t constraint solve and boundary reset (:129-133), O for the winding (:136)
@@ -339,7 +333,7 @@ __global__ __launch_bounds__(256) void k_zprep(const ZPrep A) {
 // Per-nz launchers; each hqt_radial_nz_*.hip translation unit instantiates a range of nz.
 template <int NZ>
 hipError_t zstage_launch_nz(const ZStage& A, int fp32, hipStream_t s) {
-    dim3 grid((unsigned)((A.nxy + 63) / 64), 4);
+    dim3 grid((unsigned)((A.nxy + 63) / 64));
     if (fp32) hipLaunchKernelGGL((k_zstage<NZ, float>), grid, dim3(256), 0, s, A);
     else hipLaunchKernelGGL((k_zstage<NZ, double>), grid, dim3(256), 0, s, A);
     return hipGetLastError();

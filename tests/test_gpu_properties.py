@@ -1,0 +1,219 @@
+"""GPU tests beyond element-wise parity: the committed golden trajectory, the NaN guard,
+fixed points, size-independent invariants at the full 301 x 301 x 20 size, determinism,
+the nz range, and the error bounds of the mixed-precision form.
+
+Tolerances: FP64 against the oracle/golden 1e-12 (one step); full-size invariants 1e-11
+relative; mixed precision against FP64 as stated per test (FP32 x/y operators and storage).
+"""
+
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, rel_err
+
+pytestmark = pytest.mark.gpu
+L = 50.0
+
+
+def _ic(nx, ny, nz, seed=3, nv=4):
+    from hqt import ic
+    return ic.initial_state(nx, ny, nz, L, L, mu=6.0, seed=seed, n_vortices=nv)
+
+
+def test_golden_fixture():
+    import hqt
+    g = np.load(os.path.join(GOLDEN, "golden_step_12x10x6.npz"))
+    print("golden 12x10x6", flush=True)
+    with hqt.Stepper(12, 10, 6, L, L, mu=6.0, dt=0.02, ax_b=g["ax_b"], ay_b=0.0) as st:
+        F = st.rhs(g["in_Z1"], g["in_Z2"], g["in_Z3"], g["in_Z4"])
+        for name, f in zip(("F1", "F2", "F3", "F4"), F):
+            if np.max(np.abs(g[name])) < 1e-10:      # F2 vanishes at A_x = A_y = 0: absolute check
+                assert np.max(np.abs(f - g[name])) < 1e-12, name
+            else:
+                assert rel_err(f, g[name]) < 1e-12, name
+        st.set_state(g["in_Z1"], g["in_Z2"], g["in_Z3"], g["in_Z4"], g["in_rhoi"])
+        st.step(1)
+        S1 = st.get_state()
+        assert st.last_winding == float(g["step1_W"])
+        st.step(2)
+        S3 = st.get_state()
+    for name, a, b in zip(("Z1", "Z2", "Z3", "Z4", "rhoi"), S1, [g["step1_" + k] for k in ("Z1", "Z2", "Z3", "Z4", "rhoi")]):
+        assert rel_err(a, b) < 1e-12, name
+    for name, a, b in zip(("Z1", "Z2", "Z3", "Z4", "rhoi"), S3, [g["step3_" + k] for k in ("Z1", "Z2", "Z3", "Z4", "rhoi")]):
+        assert rel_err(a, b) < 1e-11, name
+
+
+def test_nan_guard_normal_state_with_winding():
+    """O == 0 everywhere: angle(0/0) = NaN, the reference's guard stops the run at step 1
+    (tevol_driven_cluster.m:137,182-185); later steps leave the state untouched."""
+    import hqt
+    nx, ny, nz, mu = 16, 16, 8, 5.0
+    print("nan guard", flush=True)
+    zc, _ = hqt.chebdif(nz, 1)
+    z = (zc - 1) / 2 + 1
+    shp = (nx, ny, nz)
+    S = [np.zeros(shp, complex), np.broadcast_to(mu * (1 - z), shp), np.zeros(shp), np.zeros(shp), np.full(shp, mu)]
+    with hqt.Stepper(nx, ny, nz, L, L, mu=mu) as st:
+        st.set_state(*S)
+        st.step_async(5)
+        assert st.nan_step() == 1
+        with pytest.raises(hqt.HQTError) as ei:
+            st.sync()
+        assert ei.value.code == hqt.HQT_ERR_NAN
+        a = st.get_state(check_nan=False)
+        assert np.isnan(a[0]).all()
+        st.step_async(3)                      # frozen: nothing runs after the guard tripped
+        b = st.get_state(check_nan=False)
+        for u, v in zip(a, b):
+            assert np.array_equal(np.isnan(u), np.isnan(v)) and np.array_equal(np.nan_to_num(u), np.nan_to_num(v))
+
+
+def test_nan_in_initial_state_trips_guard():
+    import hqt
+    S = list(_ic(16, 16, 8))
+    S[2] = S[2].copy()
+    S[2][3, 4, 2] = np.nan
+    with hqt.Stepper(16, 16, 8, L, L, winding=False) as st:
+        st.set_state(*S)
+        st.step_async(2)
+        assert st.nan_step() == 1
+
+
+def test_normal_state_is_fixed_point():
+    import hqt
+    nx, ny, nz, mu = 24, 20, 10, 5.0
+    zc, _ = hqt.chebdif(nz, 1)
+    z = (zc - 1) / 2 + 1
+    shp = (nx, ny, nz)
+    S = [np.zeros(shp, complex), np.broadcast_to(mu * (1 - z), shp), np.zeros(shp), np.full(shp, 0.7), np.full(shp, mu)]
+    with hqt.Stepper(nx, ny, nz, L, L, mu=mu, winding=False, ay_b=0.7) as st:
+        st.set_state(*S)
+        st.step(10)
+        out = st.get_state()
+    for a, b in zip(out, S):
+        assert np.max(np.abs(a - b)) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64"])
+def test_full_size_translation_covariance(precision):
+    """All operators are circulant in x and y: step(shift(s)) == shift(step(s)) at 301x301x20."""
+    import hqt
+    from hqt import ic
+    n, nz = 301, 20
+    print("translation covariance 301x301x20", flush=True)
+    S = _ic(n, n, nz, seed=9, nv=4)
+    axb = ic.sinusoidal_drive(n, n, L, kk=7, amp=0.1)
+    sh = lambda a: np.roll(np.roll(a, 37, 0), 101, 1)
+    with hqt.Stepper(n, n, nz, L, L, ax_b=axb, precision=precision, winding=False) as st:
+        st.set_state(*S)
+        st.step(3)
+        A = st.get_state()
+    with hqt.Stepper(n, n, nz, L, L, ax_b=sh(axb), precision=precision, winding=False) as st:
+        st.set_state(*[sh(a) for a in S])
+        st.step(3)
+        B = st.get_state()
+    for a, b in zip(A, B):
+        assert rel_err(sh(a), b) < 1e-11
+
+
+def test_full_size_rhs_matches_oracle():
+    import hqt
+    from oracle import hqt_oracle as O
+    n, nz = 301, 20
+    print("rhs 301x301x20", flush=True)
+    S = _ic(n, n, nz, seed=2)
+    rng = np.random.default_rng(0)
+    z3 = S[2] + 0.02 * rng.standard_normal(S[2].shape)
+    with hqt.Stepper(n, n, nz, L, L) as st:
+        F = st.rhs(S[0], S[1], z3, S[3])
+    R = O.rhs(O.Operators(n, n, nz, L, L), S[0], S[1], z3, S[3])
+    for name, a, b in zip(("F1", "F2", "F3", "F4"), F, R):
+        e = rel_err(a, b)
+        print(f"  {name} {e:.2e}", flush=True)
+        assert e < 1e-11, name
+
+
+def test_full_size_charge_conservation():
+    """sum_xy rho is conserved by the boundary flux F2 (a divergence)."""
+    import hqt
+    from hqt import ic
+    n, nz = 301, 20
+    S = _ic(n, n, nz, seed=4)
+    with hqt.Stepper(n, n, nz, L, L, ax_b=ic.sinusoidal_drive(n, n, L, 7, 0.1)) as st:
+        st.set_state(*S)
+        st.step(20)
+        out = st.get_state()
+    q0 = S[4][:, :, -1].sum()
+    assert abs(out[4][:, :, -1].sum() - q0) < 1e-11 * abs(q0)
+
+
+def test_determinism():
+    import hqt
+    S = _ic(64, 48, 12)
+    outs = []
+    for _ in range(2):
+        with hqt.Stepper(64, 48, 12, L, L) as st:
+            st.set_state(*S)
+            st.step(7)
+            outs.append(st.get_state())
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("nz", [4, 5, 17, 24, 32])
+def test_nz_range_one_step(nz):
+    import hqt
+    from oracle import hqt_oracle as O
+    nx, ny = 20, 18
+    S = _ic(nx, ny, nz, seed=nz)
+    with hqt.Stepper(nx, ny, nz, L, L) as st:
+        st.set_state(*S)
+        st.step(1)
+        G = st.get_state()
+    *R, _w = O.step(O.Operators(nx, ny, nz, L, L), O.Params(mu=6.0, h=0.02), *S)
+    for a, b in zip(G, R):
+        assert rel_err(a, b) < 1e-11
+
+
+def test_mixed_rhs_error_bound():
+    """Mixed form: x/y operators and storage FP32, radial operators FP64.  F1..F4 within 1e-4
+    relative (max norm) of the FP64 oracle on a perturbed stationary state."""
+    import hqt
+    from oracle import hqt_oracle as O
+    nx, ny, nz = 64, 64, 12
+    S = list(_ic(nx, ny, nz, seed=6))
+    rng = np.random.default_rng(1)
+    S[2] = S[2] + 0.02 * rng.standard_normal(S[2].shape)
+    with hqt.Stepper(nx, ny, nz, L, L, precision="mixed") as st:
+        F = st.rhs(*S[:4])
+    R = O.rhs(O.Operators(nx, ny, nz, L, L), *S[:4])
+    for name, a, b in zip(("F1", "F2", "F3", "F4"), F, R):
+        e = rel_err(a, b)
+        print(f"  mixed {name} {e:.2e}", flush=True)
+        assert e < 1e-4, name
+
+
+def test_mixed_vs_fp64_trajectory():
+    """500 steps (t = 10) on a 64x64x12 lattice of 16 windings: the mixed form keeps the
+    vortex count and the condensate |O| within 1e-3 relative of the FP64 run."""
+    import hqt
+    from hqt import diagnostics, ic
+    nx = ny = 64
+    nz = 12
+    S = ic.initial_state(nx, ny, nz, L, L, mu=6.0, seed=8, lattice=(4, 0.5))
+    g = ic.Grid(nx, ny, nz, L, L)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        with hqt.Stepper(nx, ny, nz, L, L, precision=prec, winding=False) as st:
+            st.set_state(*S)
+            st.step(500)
+            res[prec] = st.get_state()
+    O64 = diagnostics.order_parameter(g, res["fp64"][0])
+    Om = diagnostics.order_parameter(g, res["mixed"][0])
+    e = float(np.max(np.abs(np.abs(Om) - np.abs(O64))) / np.max(np.abs(O64)))
+    print(f"  |O| rel err {e:.2e}, vortices {diagnostics.countN(O64)} {diagnostics.countN(Om)}", flush=True)
+    assert diagnostics.countN(Om) == diagnostics.countN(O64)
+    assert e < 1e-3

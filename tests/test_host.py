@@ -1,0 +1,206 @@
+"""CPU-side tests (no GPU): the C ABI library loads and exports every declared symbol, its
+host-only entry points (operator construction, Newton) agree with the oracle, argument
+errors are reported through the status codes, and the host-side set-up is pinned."""
+
+import math
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, rel_err
+
+HEADER = os.path.join(ROOT, "include", "hqt.h")
+
+
+def _declared_symbols():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(hqt_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    import hqt
+    from hqt import _lib
+    L = hqt.lib()
+    declared = _declared_symbols()
+    assert len(declared) >= 17
+    for name in declared:
+        assert hasattr(L, name), name
+    assert set(declared) == set(_lib.EXPORTS)
+    out = subprocess.run(["nm", "-D", "--defined-only", hqt.LIB_PATH], capture_output=True, text=True).stdout
+    for name in declared:
+        assert re.search(rf"\bT {name}\b", out), name
+
+
+def test_version_and_error_string():
+    import hqt
+    assert b"hqt" in hqt.lib().hqt_version()
+    assert isinstance(hqt.lib().hqt_last_error(), bytes)
+
+
+@pytest.mark.parametrize("N", [9, 16, 33, 64, 301])
+def test_fourdif_matches_oracle(N):
+    import hqt
+    from oracle import hqt_oracle as O
+    for m in (0, 1, 2, 3):
+        x, D = hqt.fourdif(N, m)
+        xo, Do = O.fourdif(N, m)
+        assert np.array_equal(x, xo)
+        # m <= 1 bit-exact; m = 2 differs by one rounding of csc^2; m = 3 is built by a DFT
+        tol = {0: 0.0, 1: 0.0, 2: 1e-15 * np.abs(Do).max()}.get(m, 1e-9 * max(1.0, np.abs(Do).max()))
+        assert np.max(np.abs(D - Do)) <= tol, (N, m)
+
+
+def test_fourdif_known_answer():
+    """tests/hqt_smoke_test.m:129-130 through the C ABI."""
+    import hqt
+    x, D = hqt.fourdif(33, 1)
+    assert np.max(np.abs(D @ np.sin(3 * x) - 3 * np.cos(3 * x))) < 1e-10
+
+
+@pytest.mark.parametrize("N", [5, 12, 20, 30])
+def test_chebdif_matches_oracle(N):
+    import hqt
+    from oracle import hqt_oracle as O
+    x, D = hqt.chebdif(N, 2)
+    xo, Do = O.chebdif(N, 2)
+    assert np.array_equal(x, xo)
+    for ell in range(2):
+        assert rel_err(D[:, :, ell], Do[ell]) < 1e-14
+
+
+def test_chebdif_known_answer():
+    """tests/hqt_smoke_test.m:131-133 through the C ABI."""
+    import hqt
+    z, D = hqt.chebdif(20, 2)
+    assert np.max(np.abs(D[:, :, 0] @ z ** 4 - 4 * z ** 3)) < 1e-10
+    assert np.max(np.abs(D[:, :, 1] @ z ** 4 - 12 * z ** 2)) < 1e-8
+
+
+def test_newton_matches_oracle(golden_profile):
+    """src/solvers/NewtonRaphson_vy.m through the C ABI vs the oracle restatement, on the
+    reference's own solver test cases (tests/hqt_smoke_test.m:180-183)."""
+    import hqt
+    from oracle import hqt_oracle as O
+    p = golden_profile
+    for vy in (-2 * np.pi * 7 / 20, 0.0, -1.0):
+        a = hqt.NewtonRaphson_vy(30, 1, 0, 0, p["Z1"], p["Z2"], p["Vy1"], vy)
+        b = O.NewtonRaphson_vy(30, 1, 0, 0, p["Z1"], p["Z2"], p["Vy1"], vy)
+        assert a[3] == b[3] and a[3] < 1000
+        for u, v in zip(a[:3], b[:3]):
+            assert np.max(np.abs(u - v)) < 1e-9
+        assert np.isfinite(a[4])
+
+
+def test_newton_normal_phase_exact():
+    import hqt
+    nz = 20
+    zc, _ = hqt.chebdif(nz, 1)
+    z = (zc - 1) / 2 + 1
+    z1, z2, v, it, err = hqt.NewtonRaphson_vy(nz, 1, 0, 0, 0 * z, 3.5 * (1 - z), 0 * z, 0.0)
+    assert it == 0 and np.max(np.abs(z1)) == 0 and np.max(np.abs(z2 - 3.5 * (1 - z))) < 1e-13
+
+
+def test_newton_nan_guard():
+    """A NaN initial guess stops the iteration (NewtonRaphson_vy.m:88-90)."""
+    import hqt
+    nz = 12
+    g = np.full(nz, np.nan)
+    z1, z2, v, it, err = hqt.NewtonRaphson_vy(nz, 1, 0, 0, g, g, g, 0.0)
+    assert it == 0 and math.isnan(err)
+    with pytest.raises(hqt.HQTError) as ei:
+        hqt.NewtonRaphson_vy(nz, 1, 0, 0, g, g, g, 0.0, raise_on_failure=True)
+    assert ei.value.code == hqt.HQT_ERR_NOCONV
+
+
+@pytest.mark.parametrize("kw", [dict(nx=1), dict(nz=3), dict(nz=33), dict(nx=600), dict(Lx=0.0), dict(Ly=-1.0)])
+def test_ctx_create_rejects_bad_arguments(kw):
+    """Argument checks happen before any device call, so they run without a GPU."""
+    import hqt
+    a = dict(nx=16, ny=16, nz=8, Lx=50.0, Ly=50.0)
+    a.update(kw)
+    with pytest.raises(hqt.HQTError) as ei:
+        hqt.Stepper(a["nx"], a["ny"], a["nz"], a["Lx"], a["Ly"])
+    assert ei.value.code == hqt.HQT_ERR_ARG
+
+
+def test_ff_mask_factorisation():
+    import hqt
+    from oracle import hqt_oracle as O
+    fx, fy = O.filter_masks(31, 20)
+    m = fx[:, None] | fy[None, :]
+    a, b = hqt.mask_factors(m)
+    assert np.array_equal(a, fx) and np.array_equal(b, fy)
+    a, b = hqt.mask_factors(np.repeat(m[:, :, None], 3, axis=2))
+    assert np.array_equal(a, fx)
+    bad = np.zeros((5, 5), bool)
+    bad[1, 2] = True
+    with pytest.raises(hqt.HQTError):
+        hqt.mask_factors(bad)
+
+
+def test_spline_is_not_a_knot():
+    """interp1(...,'spline') (tevol_driven_cluster.m:84) reproduces cubics exactly."""
+    from hqt import ic
+    z0, _ = __import__("hqt").chebdif(30, 1)
+    Z = (z0 - 1) / 2 + 1
+    zi = np.linspace(0, 1, 17)
+    f = lambda t: 2 - t + 3 * t ** 2 - 0.7 * t ** 3
+    assert np.max(np.abs(ic.spline_interp(Z, f(Z), zi) - f(zi))) < 1e-12
+
+
+def test_initial_fields_match_oracle_restatement():
+    """The product's set-up of tevol_driven_cluster.m:82-98 vs the oracle's restatement."""
+    from hqt import ic
+    from oracle import hqt_oracle as O
+    prof = ic.zero_flow_profile(6.0)
+    ops = O.Operators(8, 6, 12, 50.0, 50.0)
+    a = O.stationary_fields(ops, prof)
+    b = ic.stationary_fields(ic.Grid(8, 6, 12, 50.0, 50.0), prof)
+    for u, v in zip(a, b):
+        assert np.max(np.abs(u - v)) < 1e-13
+
+
+def test_zero_flow_profile_is_superfluid_and_stationary():
+    from hqt import ic
+    from oracle import hqt_oracle as O
+    prof = ic.zero_flow_profile(6.0)
+    assert prof["Z1"].max() > 1.5                       # condensed, not the normal branch
+    assert np.max(np.abs(prof["Vy1"])) < 1e-8            # V = vy = 0 up to the Newton tolerance
+    ops = O.Operators(4, 4, 30, 50.0, 50.0)
+    F = O.rhs(ops, *O.stationary_fields(ops, prof)[:4])
+    assert max(float(np.max(np.abs(f))) for f in F) < 1e-5
+
+
+def test_golden_fixture_reproduced_by_oracle():
+    """The committed golden trajectory is what the oracle computes (pins oracle drift)."""
+    from oracle import hqt_oracle as O
+    g = np.load(os.path.join(GOLDEN, "golden_step_12x10x6.npz"))
+    ops = O.Operators(12, 10, 6, 50.0, 50.0)
+    S = [g["in_Z1"], g["in_Z2"], g["in_Z3"], g["in_Z4"], g["in_rhoi"]]
+    F = O.rhs(ops, *S[:4])
+    for name, f in zip(("F1", "F2", "F3", "F4"), F):
+        assert rel_err(f, g[name]) < 1e-13
+    prm = O.Params(mu=6.0, h=0.02, ax_b=g["ax_b"], ay_b=0.0, winding=True)
+    *S, W = O.step(ops, prm, *S)
+    for name, v in zip(("Z1", "Z2", "Z3", "Z4", "rhoi"), S):
+        assert rel_err(v, g["step1_" + name]) < 1e-13
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_diagnostics_countN_matches_reference_case():
+    """countN known answer (tests/hqt_smoke_test.m:147-151) for the product's diagnostics."""
+    from hqt import diagnostics
+    x, y = np.meshgrid(np.arange(33) * 2 * np.pi / 33, np.arange(33) * 2 * np.pi / 33, indexing="ij")
+    phi = np.sin(x + 0.271) + 1j * np.sin(y + 0.613)
+    assert diagnostics.countN(phi) == (4, 2, 2)
+    assert len(diagnostics.vortex_positions(phi)) == 4

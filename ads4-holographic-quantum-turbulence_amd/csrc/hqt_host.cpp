@@ -751,6 +751,32 @@ int hqt_debug_field(hqt_ctx* c, const char* name, double* out) {
     return fail(HQT_ERR_ARG, std::string("unknown field ") + name);
 }
 
+int hqt_calibration_copy(int64_t bytes, double* ms) {
+    if (bytes < 32 || !ms) return fail(HQT_ERR_ARG, "bytes must be >= 32");
+    bytes -= bytes % 32;
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = HQT_OK;
+    hipError_t e = hipMalloc(&a, (size_t)bytes);
+    if (e == hipSuccess) e = hipMalloc(&b, (size_t)bytes);
+    if (e == hipSuccess) e = hipMemset(a, 0, (size_t)bytes);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    if (e == hipSuccess) e = launch_copy_calib(a, b, bytes, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (e != hipSuccess) rc = fail(HQT_ERR_HIP, std::string("hqt_calibration_copy: ") + hipGetErrorString(e));
+    *ms = t;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}
+
 int hqt_profile(hqt_ctx* c, int32_t enable) {
     if (!c) return fail(HQT_ERR_ARG, "NULL context");
     HIPC(hipStreamSynchronize(c->stream));

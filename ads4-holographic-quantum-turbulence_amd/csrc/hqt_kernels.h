@@ -110,5 +110,7 @@ hipError_t launch_winding(const Wind& A, int fp32, hipStream_t s);
 hipError_t launch_nancheck(const void* R, const void* I, const void* Ax, const void* Ay, const double* At,
                            int64_t n, int fp32, int32_t* flags, hipStream_t s);
 bool zkernels_support_nz(int nz);
+// Plain streaming copy of `bytes` (multiple of 32) used to calibrate traffic counters.
+hipError_t launch_copy_calib(const void* src, void* dst, int64_t bytes, hipStream_t s);
 
 }  // namespace hqt

@@ -137,3 +137,14 @@ hipError_t launch_nancheck(const void* R, const void* I, const void* Ax, const v
     return hipGetLastError();
 }
 }  // namespace hqt
+
+namespace hqt {
+__global__ __launch_bounds__(256) void k_copy_calib(const double4* __restrict__ src, double4* __restrict__ dst, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) dst[i] = src[i];
+}
+hipError_t launch_copy_calib(const void* src, void* dst, int64_t bytes, hipStream_t s) {
+    const int64_t n = bytes / (int64_t)sizeof(double4);
+    hipLaunchKernelGGL(k_copy_calib, dim3(4096), dim3(256), 0, s, (const double4*)src, (double4*)dst, n);
+    return hipGetLastError();
+}
+}  // namespace hqt

@@ -20,7 +20,8 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "csrc", "libhqt.so")
 EXPORTS = (
     "hqt_ctx_create", "hqt_ctx_destroy", "hqt_set_state", "hqt_get_state", "hqt_step_n", "hqt_sync",
     "hqt_last_winding", "hqt_rhs", "hqt_order_parameter", "hqt_ff", "hqt_fourdif", "hqt_chebdif",
-    "hqt_newton_vy", "hqt_debug_field", "hqt_profile", "hqt_kernel_stats", "hqt_last_error", "hqt_version",
+    "hqt_newton_vy", "hqt_debug_field", "hqt_profile", "hqt_kernel_stats", "hqt_calibration_copy",
+    "hqt_last_error", "hqt_version",
 )
 
 
@@ -75,6 +76,7 @@ def lib():
                                 C.POINTER(i32), dp]
     L.hqt_debug_field.argtypes = [vp, C.c_char_p, dp]
     L.hqt_profile.argtypes = [vp, i32]
+    L.hqt_calibration_copy.argtypes = [C.c_int64, dp]
     L.hqt_kernel_stats.argtypes = [vp, i32, C.c_char_p, i32, C.POINTER(C.c_int64), dp]
     for name in EXPORTS:
         if name not in ("hqt_last_error", "hqt_version"):

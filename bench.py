@@ -54,6 +54,19 @@ def line_pass_flops(name, nx, ny, nz, stage_mix=(1, 3)):
     return None
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (calibrated against a plain copy kernel), or None when no summary covers it."""
+    import glob
+    import re
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for line in open(f):
+            m = re.match(r"^traffic_bytes_per_launch\s+(\S+)\s+(\S+)", line)
+            if m and m.group(1) == kernel:
+                return float(m.group(2))
+    return None
+
+
 def step_flops_dense(nx, ny, nz):
     tot = 0.0
     for k, n in (("line_x_stage", 4), ("line_y_stage", 4), ("line_x_filter", 1), ("line_y_filter", 1),
@@ -120,14 +133,22 @@ def main():
     ap.add_argument("--nz", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
+                    help="also run the plain copy kernel on MB megabytes (traffic-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(independent evolutions shard one per GPU, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    import ctypes
+
     import hqt
     from hqt import diagnostics, ic
+
+    if args.calibrate_copy:
+        ms = ctypes.c_double(0.0)
+        hqt._lib.check(hqt.lib().hqt_calibration_copy(int(args.calibrate_copy) << 20, ctypes.byref(ms)))
 
     nx = ny = args.n
     nz = args.nz
@@ -152,8 +173,9 @@ def main():
         avg_s = ms / cnt / 1e3
         if fl is not None:
             ach = fl / avg_s / 1e12
+            tr = profiled_traffic(dom) if (nx, ny, nz) == (301, 301, 20) else None
             roof = {"kernel": dom, "bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS,
-                    "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                    "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": tr,
                     "avg_launch_us": round(avg_s * 1e6, 2),
                     "flops_per_launch": fl}
 

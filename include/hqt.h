@@ -120,6 +120,11 @@ int hqt_profile(hqt_ctx* ctx, int32_t enable);
 int hqt_kernel_stats(hqt_ctx* ctx, int32_t idx, char* name, int32_t name_len, int64_t* count,
                      double* total_ms);
 
+/* Copy `bytes` between two fresh device buffers with a plain streaming kernel (k_copy_calib)
+ * and return its duration.  Measurement helper with no counterpart in the reference: its
+ * FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic figures of the other kernels. */
+int hqt_calibration_copy(int64_t bytes, double* ms);
+
 const char* hqt_last_error(void);
 const char* hqt_version(void);
 

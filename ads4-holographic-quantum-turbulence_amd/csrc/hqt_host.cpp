@@ -77,6 +77,8 @@ struct hqt_ctx {
     std::vector<double> Lz_host;     // nz*nz row-major
     bool have_state = false;
 
+    hipGraphExec_t graph = nullptr;  // one captured step (built on first use)
+
     bool prof = false;
     struct Rec { int name; hipEvent_t a, b; };
     std::vector<Rec> recs;
@@ -497,8 +499,9 @@ int hqt_ctx_create(const hqt_params* p, hqt_ctx** out) {
 
 int hqt_ctx_destroy(hqt_ctx* c) {
     if (!c) return HQT_OK;
-    if (c->stream) hipStreamSynchronize(c->stream);
-    for (auto& r : c->recs) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    for (auto& r : c->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (void* q : c->allocs) hipFree(q);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -543,13 +546,37 @@ int hqt_get_state(hqt_ctx* c, double* Z1, double* Z2, double* Z3, double* Z4, do
     return read_flags(c, nullptr);
 }
 
+// Capture one step's launches on the context stream into a graph (stream capture; the step
+// has no memset or copy nodes, only kernels, and all kernel arguments are fixed per context).
+static int build_graph(hqt_ctx* c) {
+    hipGraph_t g = nullptr;
+    HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(c);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(HQT_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) { c->graph = nullptr; return fail(HQT_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)); }
+    return HQT_OK;
+}
+
 int hqt_step_n(hqt_ctx* c, int32_t nsteps) {
     if (!c) return fail(HQT_ERR_ARG, "NULL context");
     if (!c->have_state) return fail(HQT_ERR_STATE, "hqt_step_n before hqt_set_state");
     if (nsteps < 0) return fail(HQT_ERR_ARG, "nsteps < 0");
-    for (int i = 0; i < nsteps; ++i) {
-        int rc = enqueue_step(c);
+    const bool use_graph = !(c->p.flags & HQT_NO_GRAPH) && !c->prof;
+    if (use_graph && nsteps > 0 && !c->graph) {
+        int rc = build_graph(c);
         if (rc) return rc;
+    }
+    for (int i = 0; i < nsteps; ++i) {
+        if (use_graph) {
+            HIPC(hipGraphLaunch(c->graph, c->stream));
+        } else {
+            int rc = enqueue_step(c);
+            if (rc) return rc;
+        }
     }
     return HQT_OK;
 }

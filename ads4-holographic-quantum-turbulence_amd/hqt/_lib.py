@@ -13,6 +13,7 @@ import numpy as np
 
 HQT_OK, HQT_ERR_ARG, HQT_ERR_NAN, HQT_ERR_HIP, HQT_ERR_OOM, HQT_ERR_STATE, HQT_ERR_NOCONV = range(7)
 HQT_FP64, HQT_MIXED = 0, 1
+HQT_NO_GRAPH = 1
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "csrc", "libhqt.so")
@@ -38,7 +39,7 @@ class Params(C.Structure):
     _fields_ = [
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
         ("Lx", C.c_double), ("Ly", C.c_double), ("mu", C.c_double), ("dt", C.c_double),
-        ("winding", C.c_int32), ("filter", C.c_int32), ("precision", C.c_int32), ("reserved", C.c_int32),
+        ("winding", C.c_int32), ("filter", C.c_int32), ("precision", C.c_int32), ("flags", C.c_int32),
         ("ax_b", C.POINTER(C.c_double)), ("ay_b", C.POINTER(C.c_double)),
         ("ax_b_const", C.c_double), ("ay_b_const", C.c_double),
     ]

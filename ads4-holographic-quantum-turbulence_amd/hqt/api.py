@@ -18,7 +18,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (HQT_ERR_ARG, HQT_ERR_NAN, HQT_FP64, HQT_MIXED, HQTError, Params, check, dptr, lib)
+from ._lib import (HQT_ERR_ARG, HQT_ERR_NAN, HQT_FP64, HQT_MIXED, HQT_NO_GRAPH, HQTError, Params, check, dptr, lib)
 
 __all__ = ["fourdif", "chebdif", "ff", "NewtonRaphson_vy", "Stepper", "HQTError", "mask_factors"]
 
@@ -122,10 +122,11 @@ class Stepper:
     sides ``Lx`` (L0) and ``Ly`` (L), ``mu``, time step ``dt`` (h), winding removal on/off
     (on in the driven scripts, off in ``tevol_lattice.m``), the filter on/off, the z=0
     values ``ax_b`` / ``ay_b`` of A_x / A_y restored after every step (scalars or [nx, ny]
-    arrays; the reference uses 0 and -vy), and ``precision`` ``"fp64"`` or ``"mixed"``."""
+    arrays; the reference uses 0 and -vy), and ``precision`` ``"fp64"`` or ``"mixed"``.
+    ``graph=False`` launches each kernel directly instead of replaying the captured step graph."""
 
     def __init__(self, nx, ny, nz, Lx, Ly, mu=6.0, dt=0.02, winding=True, filter=True,
-                 precision="fp64", ax_b=0.0, ay_b=0.0):
+                 precision="fp64", ax_b=0.0, ay_b=0.0, graph=True):
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.shape = (self.nx, self.ny, self.nz)
         self.precision = precision
@@ -137,6 +138,7 @@ class Stepper:
         if precision not in ("fp64", "mixed"):
             raise HQTError(HQT_ERR_ARG, "precision must be 'fp64' or 'mixed'")
         p.precision = HQT_FP64 if precision == "fp64" else HQT_MIXED
+        p.flags = 0 if graph else HQT_NO_GRAPH
         keep = []
         for name in ("ax_b", "ay_b"):
             val = ax_b if name == "ax_b" else ay_b

@@ -138,7 +138,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(independent evolutions shard one per GPU, see DESIGN.md)", file=sys.stderr)
+              "(independent evolutions would shard one per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import ctypes

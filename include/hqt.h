@@ -35,6 +35,8 @@ extern "C" {
 #define HQT_ERR_STATE   5   /* call out of order (e.g. step before set_state) */
 #define HQT_ERR_NOCONV  6   /* Newton did not converge within it_max */
 
+#define HQT_NO_GRAPH 1      /* hqt_params.flags */
+
 #define HQT_FP64   0        /* plain form: FP64 end to end */
 #define HQT_MIXED  1        /* x/y operators, filter, stage arithmetic FP32; z operators FP64 */
 
@@ -49,7 +51,8 @@ typedef struct hqt_params {
     int32_t winding;        /* 1: winding removal after each step (:136-139); 0: off (tevol_lattice.m:168-170) */
     int32_t filter;         /* 1: Fourier filter after each step (:142); 0: off */
     int32_t precision;      /* HQT_FP64 or HQT_MIXED */
-    int32_t reserved;
+    int32_t flags;          /* HQT_NO_GRAPH: launch every kernel of a step directly instead of replaying
+                               the step's captured hipGraph */
     const double* ax_b;     /* [nx*ny] A_x(z=0) restored each step, or NULL for ax_b_const (:133: 0) */
     const double* ay_b;     /* [nx*ny] A_y(z=0) restored each step, or NULL for ay_b_const (:133: -vy) */
     double  ax_b_const, ay_b_const;
@@ -68,7 +71,9 @@ int hqt_set_state(hqt_ctx* ctx, const double* Z1, const double* Z2, const double
 int hqt_get_state(hqt_ctx* ctx, double* Z1, double* Z2, double* Z3, double* Z4, double* rhoi);
 
 /* Queue nsteps RK4 steps: the loop body tevol_driven_cluster.m:120-142 (+ the NaN guard of
- * :182-185) iterated.  Returns after queueing; hqt_sync waits. */
+ * :182-185) iterated.  Returns after queueing; hqt_sync waits.  A step's 20 kernel launches are
+ * captured once into a hipGraph and replayed per step (unless HQT_NO_GRAPH, or while
+ * per-kernel profiling is on). */
 int hqt_step_n(hqt_ctx* ctx, int32_t nsteps);
 /* Wait for queued work.  HQT_ERR_NAN if the guard tripped; *nan_step (nullable) is the
  * 1-based step (since hqt_set_state) whose state first held a NaN, else 0. */

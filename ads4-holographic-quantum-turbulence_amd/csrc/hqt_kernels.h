@@ -97,6 +97,19 @@ struct Wind {
     int32_t* flags;
 };
 
+// One stationary Newton problem (NewtonRaphson_vy.m) of a batch; all pointers are device memory.
+struct NewtonJob {
+    int32_t nz, it_max;
+    double zh, tol, vy;
+    const double* Lz;    // [nz*nz] row-major pz*Dz
+    const double* Lz2;   // [nz*nz] row-major Lz*Lz
+    const double* zz;    // [nz] radial nodes
+    const double *Z1, *Z2, *vy1;   // [nz] initial guesses
+    double *z1, *z2, *VY1;         // [nz] results
+    double* work;                  // [(3nz)^2] Jacobian / LU factors when they do not fit in LDS
+    double* out;                   // [3]: it, err, converged
+};
+
 // Launchers (hqt_kernels.hip).  All return the hipError_t of the launch.
 // H: host copy of the descriptor (grid sizes, LDS bytes); dP: the same descriptor in device memory.
 // kind: 0 RHS stage, 1 constraint, 2 filter (separate instantiations for profiling).
@@ -112,5 +125,8 @@ hipError_t launch_nancheck(const void* R, const void* I, const void* Ax, const v
 bool zkernels_support_nz(int nz);
 // Plain streaming copy of `bytes` (multiple of 32) used to calibrate traffic counters.
 hipError_t launch_copy_calib(const void* src, void* dst, int64_t bytes, hipStream_t s);
+// Batched Newton (one work-group per problem); every job of one launch has the same nz <= 64.
+hipError_t launch_newton(const NewtonJob* jobs, int njobs, int nz, hipStream_t s);
+bool newton_lds_resident(int nz);   // Jacobian held in LDS (else in NewtonJob::work)
 
 }  // namespace hqt

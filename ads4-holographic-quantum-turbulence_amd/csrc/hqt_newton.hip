@@ -1,0 +1,307 @@
+// Batched stationary Newton solver on gfx950: NewtonRaphson_vy (src/solvers/NewtonRaphson_vy.m)
+// for many independent (vy, initial guess) problems of one radial size nz, one work-group each.
+//
+// A work-group follows the reference iteration step for step:
+//   - residual `fun` (:32-34) with the Dirichlet rows zeroed (:62,82);
+//   - analytic Jacobian `dfun` (:37-39) with the rows replaced by BCz (:50-52,63,66);
+//   - y = -dFun\Fun (:69), dense LU with partial pivoting and column-oriented triangular solves
+//     (the LAPACK getrf/getrs order);
+//   - the update and the new residual (:73-83);
+//   - the stop tests: NaN update norm (:88-90), max|Fun_next| < tol (:93).
+// The 3nz x 3nz Jacobian lives in LDS for nz <= 41 (the reference's nz = 30 takes 65 KB) and
+// in a per-problem global workspace above that.  The radial operators Lz, Lz^2 and all
+// vectors are always in LDS.
+#include "hqt_kernels.h"
+#include <math.h>
+
+namespace hqt {
+
+constexpr int NW_T = 512;          // threads per problem
+constexpr int NW_W = NW_T / 64;    // waves per problem
+constexpr int NW_MAXNZ = 64;
+constexpr size_t NW_LDS_LIMIT = 160 * 1024;
+
+struct NwVec {
+    double Z1[NW_MAXNZ], Z2[NW_MAXNZ], V[NW_MAXNZ], Z2f[NW_MAXNZ];
+    double T1[NW_MAXNZ], T2[NW_MAXNZ], TV[NW_MAXNZ], T2f[NW_MAXNZ];
+    double zz[NW_MAXNZ], f[NW_MAXNZ], df[NW_MAXNZ];
+    double F[3 * NW_MAXNZ], Fn[3 * NW_MAXNZ], y[3 * NW_MAXNZ], x[3 * NW_MAXNZ];
+    double pv[NW_W];
+    int pi[NW_W];
+    double ra[NW_W], rb[NW_W];
+};
+
+static size_t newton_dyn_bytes(int nz, bool mlds) {
+    return sizeof(double) * (size_t)nz * nz * (mlds ? 11 : 2);
+}
+
+bool newton_lds_resident(int nz) {
+    return newton_dyn_bytes(nz, true) + sizeof(NwVec) <= NW_LDS_LIMIT;
+}
+
+// max|v| over the block, skipping NaN like MATLAB's max (NaN only when every entry is NaN)
+__device__ __forceinline__ double block_max_skipnan(double v, double* red) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = red[0];
+    for (int t = 1; t < NW_W; ++t) r = fmax(r, red[t]);
+    return r;
+}
+
+// max|v| over the block with NaN propagation, like MATLAB's norm(y, inf)
+__device__ __forceinline__ double nanmax2(double a, double b) {
+    return (a != a || b != b) ? __builtin_nan("") : fmax(a, b);
+}
+__device__ __forceinline__ double block_max_nanprop(double v, double* red) {
+    for (int off = 32; off > 0; off >>= 1) v = nanmax2(v, __shfl_xor(v, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = red[0];
+    for (int t = 1; t < NW_W; ++t) r = nanmax2(r, red[t]);
+    return r;
+}
+
+// fun(Z1,Z2,Vy1,Z2_f) (:32-34) into out[3nz], one thread per (equation, node), Dirichlet rows
+// zeroed.  Caller syncs before and after.
+__device__ void nw_residual(int nz, const double* Lz, const double* L2, const NwVec& s, const double* a,
+                            const double* b, const double* v, const double* bf, double* out) {
+    const int r = threadIdx.x;
+    if (r >= 3 * nz) return;
+    const int eq = (r >= nz) + (r >= 2 * nz), i = r - eq * nz;
+    const double* l2 = L2 + i * nz;
+    const double* l1 = Lz + i * nz;
+    const double ai = a[i];
+    double o;
+    if (eq == 0) {
+        double a2 = 0, a1 = 0;
+        for (int j = 0; j < nz; ++j) { a2 += l2[j] * a[j]; a1 += l1[j] * a[j]; }
+        o = s.f[i] * a2 + s.df[i] * a1 - v[i] * v[i] * ai + b[i] * bf[i] * ai - s.zz[i] * ai;
+    } else if (eq == 1) {
+        double b2 = 0;
+        for (int j = 0; j < nz; ++j) b2 += l2[j] * b[j];
+        o = s.f[i] * b2 - 2 * ai * ai * b[i];
+    } else {
+        double v2 = 0, v1 = 0;
+        for (int j = 0; j < nz; ++j) { v2 += l2[j] * v[j]; v1 += l1[j] * v[j]; }
+        o = -s.f[i] * v2 - s.df[i] * v1 + 2 * v[i] * ai * ai;
+    }
+    const bool bc = (r == nz - 1) || (r == nz) || (r == 2 * nz - 1) || (r == 3 * nz - 1);
+    out[r] = bc ? 0.0 : o;
+}
+
+// Z2_f = Z2./f below the horizon, -Lz(1,:)*Z2/3 at the horizon node (:26-27, :76-78)
+__device__ void nw_z2f(int nz, const double* Lz, const NwVec& s, const double* b, double* bf) {
+    for (int i = threadIdx.x; i < nz; i += NW_T) {
+        if (i == 0) {
+            double p = 0;
+            for (int j = 0; j < nz; ++j) p += Lz[j] * b[j];
+            bf[0] = -p / 3;
+        } else {
+            bf[i] = b[i] / s.f[i];
+        }
+    }
+}
+
+template <bool MLDS>
+__global__ __launch_bounds__(NW_T) void k_newton(const NewtonJob* __restrict__ jobs) {
+    extern __shared__ double dyn[];
+    __shared__ NwVec s;
+    const NewtonJob J = jobs[blockIdx.x];
+    const int nz = J.nz, n = 3 * nz;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    double* Lz = dyn;
+    double* L2 = dyn + nz * nz;
+    double* M = MLDS ? dyn + 2 * nz * nz : J.work;   // row-major n x n
+
+    for (int e = tid; e < nz * nz; e += NW_T) { Lz[e] = J.Lz[e]; L2[e] = J.Lz2[e]; }
+    const double zh3 = J.zh * J.zh * J.zh;
+    for (int i = tid; i < nz; i += NW_T) {
+        const double zi = J.zz[i], q = zi / J.zh;
+        s.zz[i] = zi;
+        s.f[i] = 1 - q * q * q;
+        s.df[i] = -3 * zi * zi / zh3;
+        s.Z1[i] = J.Z1[i];
+        s.Z2[i] = J.Z2[i];
+        s.V[i] = J.vy + (J.vy1[i] - J.vy1[nz - 1]);   // :28-29
+    }
+    __syncthreads();
+    nw_z2f(nz, Lz, s, s.Z2, s.Z2f);
+    __syncthreads();
+    nw_residual(nz, Lz, L2, s, s.Z1, s.Z2, s.V, s.Z2f, s.F);
+    __syncthreads();
+
+    int K = 0, converged = 0;
+    double err = __builtin_nan("");
+    while (K < J.it_max) {
+        // ---- dFun (:37-39) with BCz rows; y <- Fun ----
+        for (int r = w; r < n; r += NW_W) {
+            const int br = (r >= nz) + (r >= 2 * nz), i = r - br * nz;
+            const bool bc = (r == nz - 1) || (r == nz) || (r == 2 * nz - 1) || (r == 3 * nz - 1);
+            for (int q = lane; q < n; q += 64) {
+                const int bq = (q >= nz) + (q >= 2 * nz), j = q - bq * nz;
+                double v = 0.0;
+                if (bc) {
+                    v = (q == r) ? 1.0 : 0.0;
+                } else if (br == bq) {
+                    const double l2 = L2[i * nz + j], l1 = Lz[i * nz + j];
+                    if (br == 0) v = s.f[i] * l2 + s.df[i] * l1;
+                    else if (br == 1) v = s.f[i] * l2;
+                    else v = -s.f[i] * l2 - s.df[i] * l1;
+                    if (i == j) {
+                        if (br == 0) v += s.Z2[i] * s.Z2f[i] - s.V[i] * s.V[i] - s.zz[i];
+                        else if (br == 1) v -= 2 * s.Z1[i] * s.Z1[i];
+                        else v += 2 * s.Z1[i] * s.Z1[i];
+                    }
+                } else if (i == j) {
+                    if (br == 0) v = (bq == 1) ? 2 * s.Z2f[i] * s.Z1[i] : -2 * s.V[i] * s.Z1[i];
+                    else if (br == 1) v = (bq == 0) ? -4 * s.Z1[i] * s.Z2[i] : 0.0;
+                    else v = (bq == 0) ? 4 * s.V[i] * s.Z1[i] : 0.0;
+                }
+                M[r * n + q] = v;
+            }
+        }
+        for (int i = tid; i < n; i += NW_T) s.y[i] = s.F[i];
+        __syncthreads();
+
+        // ---- LU with partial pivoting; forward elimination applied to y on the fly ----
+        bool singular = false;
+        for (int k = 0; k < n; ++k) {
+            double bv = -1.0;
+            int bi = n;
+            for (int i = k + tid; i < n; i += NW_T) {
+                const double a = fabs(M[i * n + k]);
+                if (a > bv) { bv = a; bi = i; }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ov = __shfl_xor(bv, off);
+                const int oi = __shfl_xor(bi, off);
+                if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+            }
+            if (lane == 0) { s.pv[w] = bv; s.pi[w] = bi; }
+            __syncthreads();
+            bv = s.pv[0];
+            bi = s.pi[0];
+            for (int t = 1; t < NW_W; ++t) {
+                const double ov = s.pv[t];
+                const int oi = s.pi[t];
+                if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+            }
+            if (!(bv > 0.0)) { singular = true; break; }   // same decision in every thread
+            if (bi != k) {
+                for (int q = k + tid; q < n; q += NW_T) {
+                    const double t = M[k * n + q];
+                    M[k * n + q] = M[bi * n + q];
+                    M[bi * n + q] = t;
+                }
+                if (tid == 0) { const double t = s.y[k]; s.y[k] = s.y[bi]; s.y[bi] = t; }
+                __syncthreads();
+            }
+            // rank-1 update of the trailing block.  Each wave takes rows i = k+1+w (mod NW_W), four
+            // rows per batch so that their LDS loads overlap; row k (the pivot row) is held in
+            // registers.  Multipliers use the reciprocal of the pivot, as LAPACK's getf2 does.
+            const double rdk = 1.0 / M[k * n + k];
+            const double yk = s.y[k];
+            const int q0 = k + 1 + lane;
+            double rk[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) rk[c] = (q0 + 64 * c < n) ? M[k * n + q0 + 64 * c] : 0.0;
+            int i = k + 1 + w;
+            for (; i + 3 * NW_W < n; i += 4 * NW_W) {
+                double l[4], m[4][3];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const double* row = M + (i + u * NW_W) * n;
+                    l[u] = row[k];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) m[u][c] = (q0 + 64 * c < n) ? row[q0 + 64 * c] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    double* row = M + (i + u * NW_W) * n;
+                    const double lu = l[u] * rdk;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        if (q0 + 64 * c < n) row[q0 + 64 * c] = m[u][c] - lu * rk[c];
+                    if (lane == 0) s.y[i + u * NW_W] -= lu * yk;
+                }
+            }
+            for (; i < n; i += NW_W) {
+                double* row = M + i * n;
+                const double lu = row[k] * rdk;
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    if (q0 + 64 * c < n) row[q0 + 64 * c] -= lu * rk[c];
+                if (lane == 0) s.y[i] -= lu * yk;
+            }
+            __syncthreads();
+        }
+        if (singular) { err = __builtin_nan(""); break; }
+        // back substitution, column-oriented; x = -U \ y is the Newton update
+        for (int k = n - 1; k >= 0; --k) {
+            const double xk = s.y[k] / M[k * n + k];
+            for (int i = tid; i < k; i += NW_T) s.y[i] -= M[i * n + k] * xk;
+            if (tid == 0) s.x[k] = -xk;
+            __syncthreads();
+        }
+
+        // ---- update, new residual, stop tests (:73-96) ----
+        for (int i = tid; i < nz; i += NW_T) {
+            s.T1[i] = s.Z1[i] + s.x[i];
+            s.T2[i] = s.Z2[i] + s.x[nz + i];
+            s.TV[i] = s.V[i] + s.x[2 * nz + i];
+        }
+        __syncthreads();
+        nw_z2f(nz, Lz, s, s.T2, s.T2f);
+        __syncthreads();
+        nw_residual(nz, Lz, L2, s, s.T1, s.T2, s.TV, s.T2f, s.Fn);
+        __syncthreads();
+        double la = __builtin_nan(""), lb = 0.0;
+        for (int i = tid; i < n; i += NW_T) {
+            la = fmax(la, fabs(s.Fn[i]));
+            lb = nanmax2(lb, fabs(s.x[i]));
+        }
+        const double err2 = block_max_skipnan(la, s.ra);
+        err = block_max_nanprop(lb, s.rb);
+        for (int i = tid; i < nz; i += NW_T) {
+            s.Z1[i] = s.T1[i];
+            s.Z2[i] = s.T2[i];
+            s.V[i] = s.TV[i];
+            s.Z2f[i] = s.T2f[i];
+        }
+        for (int i = tid; i < n; i += NW_T) s.F[i] = s.Fn[i];
+        __syncthreads();
+        if (err != err) break;
+        if (err2 < J.tol) { converged = 1; break; }
+        ++K;
+    }
+    for (int i = tid; i < nz; i += NW_T) {
+        J.z1[i] = s.Z1[i];
+        J.z2[i] = s.Z2[i];
+        J.VY1[i] = s.V[i];
+    }
+    if (tid == 0) {
+        J.out[0] = (double)K;
+        J.out[1] = err;
+        J.out[2] = converged ? 1.0 : 0.0;
+    }
+}
+
+hipError_t launch_newton(const NewtonJob* jobs, int njobs, int nz, hipStream_t st) {
+    if (nz < 2 || nz > NW_MAXNZ || njobs <= 0) return hipErrorInvalidValue;
+    const bool mlds = newton_lds_resident(nz);
+    const size_t dyn = newton_dyn_bytes(nz, mlds);
+    hipError_t e;
+    if (mlds) {
+        e = hipFuncSetAttribute((const void*)k_newton<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_newton<true>, dim3(njobs), dim3(NW_T), dyn, st, jobs);
+    } else {
+        e = hipFuncSetAttribute((const void*)k_newton<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_newton<false>, dim3(njobs), dim3(NW_T), dyn, st, jobs);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace hqt

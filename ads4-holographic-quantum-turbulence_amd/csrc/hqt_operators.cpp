@@ -316,9 +316,8 @@ int newton_vy(int nz, double zh, double small, double D, const double* Z1in, con
         z2f(b, bf);
         Fun_next = fun(a, b, v, bf);
         for (int r : kidx) Fun_next[r] = 0;
-        double err2 = 0;
-        for (double q : Fun_next) err2 = std::max(err2, std::fabs(q));
-        if (std::isnan(err2)) err2 = NAN;
+        double err2 = NAN;   // max(abs(Fun_next)) skipping NaN, like MATLAB's max
+        for (double q : Fun_next) err2 = std::fmax(err2, std::fabs(q));
         Z1 = a; Z2 = b; Vy1 = v; Z2_f = bf;
         err = 0;
         for (double q : y) {

@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "csrc", "libhqt.so")
 EXPORTS = (
     "hqt_ctx_create", "hqt_ctx_destroy", "hqt_set_state", "hqt_get_state", "hqt_step_n", "hqt_sync",
     "hqt_last_winding", "hqt_rhs", "hqt_order_parameter", "hqt_ff", "hqt_fourdif", "hqt_chebdif",
-    "hqt_newton_vy", "hqt_debug_field", "hqt_profile", "hqt_kernel_stats", "hqt_calibration_copy",
+    "hqt_newton_vy", "hqt_newton_vy_batch", "hqt_debug_field", "hqt_profile", "hqt_kernel_stats", "hqt_calibration_copy",
     "hqt_last_error", "hqt_version",
 )
 
@@ -75,6 +75,8 @@ def lib():
     L.hqt_chebdif.argtypes = [i32, i32, dp, dp]
     L.hqt_newton_vy.argtypes = [i32, C.c_double, C.c_double, C.c_double, dp, dp, dp, C.c_double, dp, dp, dp,
                                 C.POINTER(i32), dp]
+    L.hqt_newton_vy_batch.argtypes = [i32, C.c_double, C.c_double, C.c_double, i32, dp, dp, dp, dp, i32,
+                                      C.c_double, dp, dp, dp, C.POINTER(i32), dp, C.POINTER(C.c_uint8)]
     L.hqt_debug_field.argtypes = [vp, C.c_char_p, dp]
     L.hqt_profile.argtypes = [vp, i32]
     L.hqt_calibration_copy.argtypes = [C.c_int64, dp]

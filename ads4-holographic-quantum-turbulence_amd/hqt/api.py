@@ -18,9 +18,9 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (HQT_ERR_ARG, HQT_ERR_NAN, HQT_FP64, HQT_MIXED, HQT_NO_GRAPH, HQTError, Params, check, dptr, lib)
+from ._lib import (HQT_ERR_ARG, HQT_ERR_NAN, HQT_ERR_NOCONV, HQT_FP64, HQT_MIXED, HQT_NO_GRAPH, HQTError, Params, check, dptr, lib)
 
-__all__ = ["fourdif", "chebdif", "ff", "NewtonRaphson_vy", "Stepper", "HQTError", "mask_factors"]
+__all__ = ["fourdif", "chebdif", "ff", "NewtonRaphson_vy", "NewtonRaphson_vy_batch", "Stepper", "HQTError", "mask_factors"]
 
 
 def fourdif(N: int, m: int):
@@ -99,6 +99,39 @@ def NewtonRaphson_vy(nz, zh, small, D, Z1, Z2, vy1, vy, raise_on_failure=False):
     if rc != 0 and raise_on_failure:
         check(rc)
     return z1, z2, VY1, int(it.value), float(err.value)
+
+
+def NewtonRaphson_vy_batch(nz, zh, small, D, Z1, Z2, vy1, vy, it_max=1000, tol=1e-6, raise_on_failure=False):
+    """NewtonRaphson_vy over a batch on the GPU (one work-group per problem).
+
+    ``Z1, Z2, vy1`` are ``[nz, nb]`` (problem ``b`` in column ``b``; an nz-vector is broadcast
+    over the batch), ``vy`` is ``[nb]``.  Returns ``z1, z2, VY1`` (``[nz, nb]``), ``it``,
+    ``err`` and ``conv`` (``[nb]``), each problem iterated exactly as ``NewtonRaphson_vy``."""
+    nz = int(nz)
+    vy = np.atleast_1d(np.asarray(vy, dtype=np.float64))
+    nb = vy.size
+
+    def cols(a):
+        a = np.asarray(a, dtype=np.float64)
+        if a.ndim == 1:
+            a = np.repeat(a[:, None], nb, axis=1)
+        if a.shape != (nz, nb):
+            raise HQTError(HQT_ERR_ARG, f"NewtonRaphson_vy_batch: profiles must be [{nz}, {nb}]")
+        return np.asfortranarray(a)
+
+    a, b, v = cols(Z1), cols(Z2), cols(vy1)
+    vy = np.ascontiguousarray(vy)
+    z1, z2, VY1 = (np.empty((nz, nb), order="F") for _ in range(3))
+    it = np.zeros(nb, dtype=np.int32)
+    err = np.zeros(nb)
+    conv = np.zeros(nb, dtype=np.uint8)
+    rc = lib().hqt_newton_vy_batch(nz, float(zh), float(small), float(D), nb, dptr(a), dptr(b), dptr(v), dptr(vy),
+                                   int(it_max), float(tol), dptr(z1), dptr(z2), dptr(VY1),
+                                   it.ctypes.data_as(C.POINTER(C.c_int32)), dptr(err),
+                                   conv.ctypes.data_as(C.POINTER(C.c_uint8)))
+    if rc != 0 and (raise_on_failure or rc != HQT_ERR_NOCONV):
+        check(rc)
+    return z1, z2, VY1, it, err, conv.astype(bool)
 
 
 def _f64F(a, shape):

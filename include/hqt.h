@@ -111,6 +111,19 @@ int hqt_newton_vy(int32_t nz, double zh, double small, double D, const double* Z
                   const double* Z2, const double* vy1, double vy, double* z1, double* z2,
                   double* VY1, int32_t* it, double* err);
 
+/* NewtonRaphson_vy over a batch of nb independent problems of one radial size on the GPU,
+ * one work-group per problem: the iteration of src/solvers/NewtonRaphson_vy.m:1-105 (as
+ * hqt_newton_vy), e.g. a sweep over vy or over initial guesses in one launch.
+ * Z1, Z2, vy1 (inputs) and z1, z2, VY1 (outputs) are [nz, nb] column-major (problem b in
+ * column b); vy, it, err, conv are [nb].  err is norm(y, inf) of the last update (NaN when it
+ * was NaN, :88); conv[b] = 1 when the residual test (:92) passed.  it_max <= 0 and tol <= 0
+ * select the reference's 1000 and 1e-6 (:8-9).  3 <= nz <= 64.  Returns HQT_OK when every
+ * problem converged, HQT_ERR_NOCONV otherwise (all outputs are written either way). */
+int hqt_newton_vy_batch(int32_t nz, double zh, double small, double D, int32_t nb,
+                        const double* Z1, const double* Z2, const double* vy1, const double* vy,
+                        int32_t it_max, double tol, double* z1, double* z2, double* VY1,
+                        int32_t* it, double* err, uint8_t* conv);
+
 /* Copy one internal device field of the last hqt_rhs / hqt_step_n into out ([nx,ny,nz], FP64
  * values).  Names: "dxR" "dxI" "dxxR" "dxxI" "dxAx" "dxAy" "dxxAy" (x-pass outputs), "dyR" "dyI"
  * "lapR" "lapI" "divA" "curl3" "curl4" (y-pass outputs), "dxAt" "dyAt" "dzAt" (frozen A_t

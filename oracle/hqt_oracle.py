@@ -472,7 +472,7 @@ def NewtonRaphson_vy(nz, zh, small, D, Z1, Z2, vy1, vy, it_max=1000, tol=1e-6):
         Z2_f_temp[0] = -pAt[0] / 3
         Fun_next = fun(Z1_temp, Z2_temp, Vy1_temp, Z2_f_temp)
         Fun_next[kidx] = 0
-        err2 = np.max(np.abs(Fun_next))
+        err2 = np.fmax.reduce(np.abs(Fun_next))   # MATLAB max skips NaN
         Z1, Z2, Vy1, Z2_f = Z1_temp, Z2_temp, Vy1_temp, Z2_f_temp
         err = float(np.max(np.abs(y)))
         if math.isnan(err):

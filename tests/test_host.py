@@ -117,6 +117,22 @@ def test_newton_nan_guard():
     assert ei.value.code == hqt.HQT_ERR_NOCONV
 
 
+@pytest.mark.parametrize("nz,nb", [(2, 1), (65, 1), (20, 0)])
+def test_newton_batch_rejects_bad_arguments(nz, nb):
+    """hqt_newton_vy_batch validates nz in [3, 64] and nb >= 1 before touching the GPU."""
+    import ctypes as C
+    import hqt
+    L = hqt.lib()
+    buf = np.zeros(max(nz, 1) * max(nb, 1))
+    d = buf.ctypes.data_as(C.POINTER(C.c_double))
+    it = np.zeros(max(nb, 1), dtype=np.int32)
+    conv = np.zeros(max(nb, 1), dtype=np.uint8)
+    rc = L.hqt_newton_vy_batch(nz, 1.0, 0.0, 0.0, nb, d, d, d, d, 0, 0.0, d, d, d,
+                               it.ctypes.data_as(C.POINTER(C.c_int32)), d, conv.ctypes.data_as(C.POINTER(C.c_uint8)))
+    assert rc == hqt.HQT_ERR_ARG
+    assert "hqt_newton_vy_batch" in L.hqt_last_error().decode()
+
+
 @pytest.mark.parametrize("kw", [dict(nx=1), dict(nz=3), dict(nz=33), dict(nx=600), dict(Lx=0.0), dict(Ly=-1.0)])
 def test_ctx_create_rejects_bad_arguments(kw):
     """Argument checks happen before any device call, so they run without a GPU."""

@@ -74,7 +74,10 @@ struct hqt_ctx {
     ZPost zp;
     ZPrep zpre, zpreR;
     Wind wind;
-    std::vector<double> Lz_host;     // nz*nz row-major
+    double* Od = nullptr;            // [2*nxy] condensate of the current state (diagnostics)
+    int8_t* cmap = nullptr;          // [nxy] plaquette charges
+    int32_t* ccount = nullptr;       // [2] +1 / -1 plaquette counts
+    double* cpart = nullptr;         // [2*census_blocks] per-block max|O|, sum|O|
     bool have_state = false;
 
     hipGraphExec_t graph = nullptr;  // one captured step (built on first use)
@@ -275,6 +278,8 @@ int build_context(hqt_ctx* c) {
     ALLOC(c->axb, nxy * 8); ALLOC(c->ayb, nxy * 8); ALLOC(c->yph, (size_t)ny * 8);
     ALLOC(c->flags, 4 * sizeof(int32_t));
     ALLOC(c->dp, NPASS * sizeof(LPass));
+    ALLOC(c->Od, 2 * nxy * 8); ALLOC(c->cmap, nxy); ALLOC(c->ccount, 2 * sizeof(int32_t));
+    ALLOC(c->cpart, 2 * (size_t)census_blocks((int64_t)nxy) * 8);
 
     // ---- radial operators (tevol_driven_cluster.m:15,36-52,70-71) ----
     std::vector<double> zc, DM;
@@ -305,7 +310,6 @@ int build_context(hqt_ctx* c) {
     for (int k = 0; k < nz; ++k) zbuf.push_back(1 - zz[k] * zz[k] * zz[k]);
     ALLOC(c->zops, zbuf.size() * 8);
     HIPC(hipMemcpy(c->zops, zbuf.data(), zbuf.size() * 8, hipMemcpyHostToDevice));
-    c->Lz_host = Lz;
     c->zo.Lz = c->zops;
     c->zo.Lz1 = c->zops + (size_t)nz * nz;
     c->zo.hf = c->zops + off_hf;
@@ -636,23 +640,56 @@ int hqt_rhs(hqt_ctx* c, const double* Z1, const double* Z2, const double* Z3, co
     return resolve_profile(c);
 }
 
+namespace {
+// O of the current state into c->Od; with `census`, also the plaquette charges and counts.
+int enqueue_census(hqt_ctx* c, bool census) {
+    Census A{};
+    A.nxy = c->nxy; A.nx = c->nx; A.ny = c->ny; A.nz = c->nz;
+    A.R = c->ZR; A.I = c->ZI;
+    A.Lzb = c->zo.Lz + (size_t)(c->nz - 1) * c->nz;
+    A.O = c->Od; A.charge = c->cmap; A.counts = c->ccount; A.part = c->cpart;
+    HIPC(launch_order_param(A, c->tdt == DT_F32, c->stream));
+    if (census) {
+        HIPC(hipMemsetAsync(c->ccount, 0, 2 * sizeof(int32_t), c->stream));
+        HIPC(launch_census(A, c->stream));
+    }
+    return HQT_OK;
+}
+}  // namespace
+
 int hqt_order_parameter(hqt_ctx* c, double* O) {
     if (!c || !O) return fail(HQT_ERR_ARG, "NULL argument");
     if (!c->have_state) return fail(HQT_ERR_STATE, "no state uploaded");
-    HIPC(hipStreamSynchronize(c->stream));
-    const size_t n = (size_t)c->npts, nxy = (size_t)c->nxy;
-    std::vector<double> re(n), im(n);
     int rc;
-    if ((rc = download_as(re.data(), c->ZR, n, c->tdt))) return rc;
-    if ((rc = download_as(im.data(), c->ZI, n, c->tdt))) return rc;
-    const int nz = c->nz;
-    for (size_t col = 0; col < nxy; ++col) {
-        double a = 0, b = 0;
-        for (int m = 0; m < nz; ++m) {
-            a += c->Lz_host[(size_t)(nz - 1) * nz + m] * re[col + nxy * m];
-            b += c->Lz_host[(size_t)(nz - 1) * nz + m] * im[col + nxy * m];
+    if ((rc = enqueue_census(c, false))) return rc;
+    HIPC(hipMemcpyAsync(O, c->Od, 2 * (size_t)c->nxy * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    return HQT_OK;
+}
+
+int hqt_vortex_census(hqt_ctx* c, int32_t* counts, int8_t* charge, double* Oabs) {
+    if (!c || !counts) return fail(HQT_ERR_ARG, "NULL argument");
+    if (!c->have_state) return fail(HQT_ERR_STATE, "no state uploaded");
+    int rc;
+    if ((rc = enqueue_census(c, true))) return rc;
+    int32_t pm[2];
+    HIPC(hipMemcpyAsync(pm, c->ccount, sizeof(pm), hipMemcpyDeviceToHost, c->stream));
+    const int nb = census_blocks(c->nxy);
+    std::vector<double> part(2 * (size_t)nb);
+    if (Oabs) HIPC(hipMemcpyAsync(part.data(), c->cpart, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    if (charge) HIPC(hipMemcpyAsync(charge, c->cmap, (size_t)c->nxy, hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    counts[0] = pm[0] + pm[1];
+    counts[1] = pm[0];
+    counts[2] = pm[1];
+    if (Oabs) {
+        double mx = 0.0, sum = 0.0;
+        for (int b = 0; b < nb; ++b) {
+            mx = std::fmax(mx, part[2 * b]);
+            sum += part[2 * b + 1];
         }
-        O[2 * col] = a; O[2 * col + 1] = b;
+        Oabs[0] = mx;
+        Oabs[1] = sum / (double)c->nxy;
     }
     return HQT_OK;
 }

@@ -110,6 +110,17 @@ struct NewtonJob {
     double* out;                   // [3]: it, err, converged
 };
 
+// Boundary condensate and plaquette census of one state.
+struct Census {
+    int64_t nxy; int32_t nx, ny, nz;
+    const void *R, *I;        // Psi (TS), [nx*ny*nz]
+    const double* Lzb;        // [nz] last row of Lz (the z=0 row)
+    double* O;                // [2*nx*ny] interleaved complex
+    int8_t* charge;           // [nx*ny] plaquette charge, or NULL
+    int32_t* counts;          // [2]: number of +1 and -1 plaquettes (zeroed by the caller)
+    double* part;             // [2*census_blocks]: per-block max|O|, sum|O|
+};
+
 // Launchers (hqt_kernels.hip).  All return the hipError_t of the launch.
 // H: host copy of the descriptor (grid sizes, LDS bytes); dP: the same descriptor in device memory.
 // kind: 0 RHS stage, 1 constraint, 2 filter (separate instantiations for profiling).
@@ -128,5 +139,9 @@ hipError_t launch_copy_calib(const void* src, void* dst, int64_t bytes, hipStrea
 // Batched Newton (one work-group per problem); every job of one launch has the same nz <= 64.
 hipError_t launch_newton(const NewtonJob* jobs, int njobs, int nz, hipStream_t s);
 bool newton_lds_resident(int nz);   // Jacobian held in LDS (else in NewtonJob::work)
+// O from the state, then the census of O.
+hipError_t launch_order_param(const Census& A, int fp32, hipStream_t s);
+hipError_t launch_census(const Census& A, hipStream_t s);
+int census_blocks(int64_t nxy);
 
 }  // namespace hqt

@@ -2,6 +2,7 @@
 // winding removal, NaN guard).  The per-nz templates live in hqt_radial_core.h and are
 // instantiated by the hqt_radial_nz_*.hip translation units so the build runs in parallel.
 #include "hqt_radial_core.h"
+#include "hqt_angle.h"
 
 namespace hqt {
 
@@ -16,13 +17,6 @@ __device__ __forceinline__ double block_sum_256(double v, double* sh) {
     const double r = sh[0];
     __syncthreads();
     return r;
-}
-
-// angle(a/b): atan2 of a*conj(b) (same angle, b != 0); b == 0 gives NaN (0/0 in the reference).
-__device__ __forceinline__ double angle_ratio(double ar, double ai, double br, double bi) {
-    if (br == 0.0 && bi == 0.0) return __builtin_nan("");
-    if (ar == 0.0 && ai == 0.0) return 0.0;
-    return atan2(ai * br - ar * bi, ar * br + ai * bi);
 }
 
 __global__ __launch_bounds__(256) void k_wind_partial(const Wind A) {

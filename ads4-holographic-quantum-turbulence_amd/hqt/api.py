@@ -274,6 +274,19 @@ class Stepper:
         check(lib().hqt_order_parameter(self._h, dptr(O)))
         return O
 
+    def vortex_census(self, charges=False):
+        """``[n, np, nm] = countN(O)`` of the current state, computed on the GPU.
+
+        Returns ``(n, np, nm, Omax, Omean)``, plus the ``[nx, ny]`` int8 plaquette charge map
+        when ``charges`` is true."""
+        counts = np.zeros(3, dtype=np.int32)
+        oabs = np.zeros(2)
+        cmap = np.zeros((self.nx, self.ny), dtype=np.int8, order="F") if charges else None
+        check(lib().hqt_vortex_census(self._h, counts.ctypes.data_as(C.POINTER(C.c_int32)),
+                                      cmap.ctypes.data_as(C.POINTER(C.c_int8)) if charges else None, dptr(oabs)))
+        out = (int(counts[0]), int(counts[1]), int(counts[2]), float(oabs[0]), float(oabs[1]))
+        return out + (cmap,) if charges else out
+
     def debug_field(self, name):
         """An internal device field of the last rhs/step (x/y/z derivative buffers)."""
         out = np.empty(self.shape, order="F")

@@ -88,8 +88,18 @@ int hqt_rhs(hqt_ctx* ctx, const double* Z1, const double* Z2, const double* Z3, 
             double* F1, double* F2, double* F3, double* F4);
 
 /* Boundary order parameter O = (d_z Psi)(z=0) of the current state, [nx,ny] interleaved
- * complex (tevol_driven_cluster.m:136,148; scripts/analysis/movie_show.m:91). */
+ * complex (tevol_driven_cluster.m:136,148; scripts/analysis/movie_show.m:91).  Computed on
+ * the GPU (Lz(nz,:) applied to every column); only O is downloaded. */
 int hqt_order_parameter(hqt_ctx* ctx, double* O);
+
+/* Vortex census of the current state on the GPU: [n,np,nm] = countN(O)
+ * (src/diagnostics/countN.m:7-21: periodic plaquette winding w, counted as +1 when
+ * |w-1| <= 1e-4 and as -1 when |w+1| <= 1e-4).  counts[3] = {n, np, nm}.  charge (optional,
+ * [nx,ny] int8, column-major) = +1 / -1 / 0 for the plaquette with corners (i,j), (i+1,j),
+ * (i+1,j+1), (i,j+1), periodic (the +-1 entries of find_vortices.m:103-131's charge matrix).
+ * Oabs (optional, [2]) = {max|O|, mean|O|}.  Only counts, charges and per-block partial sums
+ * are downloaded. */
+int hqt_vortex_census(hqt_ctx* ctx, int32_t* counts, int8_t* charge, double* Oabs);
 
 /* ---- stand-alone operators ---------------------------------------------------------- */
 /* w = ff(x, fx|fy): src/numerics/ff.m:7-11 with the mask of tevol_driven_cluster.m:27-31.

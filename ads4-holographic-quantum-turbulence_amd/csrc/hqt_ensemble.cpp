@@ -1,0 +1,271 @@
+// Batched ensemble: many independent evolutions (contexts of uneven nx, ny, nz, mu, drive and
+// precision) advanced together.  Each kernel of the step sequence is launched once per kernel
+// class (same template instance: precision and Q = Mp/64 for the line passes, nz for the
+// radial kernels) over all members of that class; a member owns a contiguous block range of
+// the launch and reads its own descriptor (buffers, sizes, flags) from a device array.  The
+// whole step is captured once into a hipGraph and replayed per step.
+//
+// The per-member arithmetic is that of hqt_step_n on the same context (same kernels, same
+// operands), so results are identical to stepping each member alone.
+#include "hqt_internal.h"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <new>
+
+namespace {
+
+enum LKind { K_LINE, K_ZPREP, K_ZSTAGE, K_ZPOST, K_WPART, K_WAPPLY, K_NAN };
+
+struct Launch {
+    int kind = 0;
+    int dir = 0, lkind = 0, Q = 1, ngroups = 1, nz = 0, fp32 = 0, blocks = 0, nmem = 0;
+    size_t lds = 0;
+    size_t desc_off = 0, boff_off = 0;   // offsets into the device plan buffer
+};
+
+// Host image of the device plan: descriptor arrays and block offsets, 256-byte aligned.
+struct PlanImage {
+    std::vector<unsigned char> bytes;
+    size_t put(const void* p, size_t n) {
+        const size_t off = (bytes.size() + 255) & ~size_t(255);
+        bytes.resize(off + n);
+        std::memcpy(bytes.data() + off, p, n);
+        return off;
+    }
+};
+
+}  // namespace
+
+struct hqt_ensemble {
+    std::vector<hqt_ctx*> mem;       // in the caller's order
+    hipStream_t stream = nullptr;
+    std::vector<Launch> plan;
+    unsigned char* dplan = nullptr;
+    hipGraphExec_t graph = nullptr;
+};
+
+namespace {
+
+inline int efail(int code, const std::string& msg) { return hqt_fail(code, msg); }
+
+// Members grouped by key, largest first inside each group (large members' blocks are
+// dispatched first, which shortens the tail of a launch).
+template <class Key, class KeyFn, class Pred>
+std::map<Key, std::vector<hqt_ctx*>> classes(const std::vector<hqt_ctx*>& mem, KeyFn key, Pred use) {
+    std::map<Key, std::vector<hqt_ctx*>> out;
+    for (hqt_ctx* c : mem)
+        if (use(c)) out[key(c)].push_back(c);
+    for (auto& kv : out)
+        std::stable_sort(kv.second.begin(), kv.second.end(),
+                         [](const hqt_ctx* a, const hqt_ctx* b) { return a->npts > b->npts; });
+    return out;
+}
+
+template <class D, class DescFn, class BlocksFn>
+Launch make_launch(PlanImage& img, int kind, const std::vector<hqt_ctx*>& ms, DescFn desc, BlocksFn blocks) {
+    Launch L;
+    L.kind = kind;
+    L.nmem = (int)ms.size();
+    std::vector<D> d;
+    std::vector<int32_t> boff(1, 0);
+    for (hqt_ctx* c : ms) {
+        d.push_back(desc(c));
+        boff.push_back(boff.back() + blocks(c));
+    }
+    L.blocks = boff.back();
+    L.desc_off = img.put(d.data(), d.size() * sizeof(D));
+    L.boff_off = img.put(boff.data(), boff.size() * sizeof(int32_t));
+    return L;
+}
+
+void add_line(std::vector<Launch>& plan, PlanImage& img, const std::vector<hqt_ctx*>& mem, int pass, int dir, int name,
+              bool (*use)(const hqt_ctx*)) {
+    auto cls = classes<std::pair<int, int>>(
+        mem, [&](hqt_ctx* c) { return std::make_pair(c->fp32, c->hp[pass].Mp / 64); },
+        [&](hqt_ctx* c) { return use(c); });
+    for (auto& kv : cls) {
+        Launch L = make_launch<LPass>(img, K_LINE, kv.second, [&](hqt_ctx* c) { return c->hp[pass]; },
+                                      [&](hqt_ctx* c) { return c->hp[pass].ntiles; });
+        L.dir = dir;
+        L.fp32 = kv.first.first;
+        L.Q = kv.first.second;
+        L.lkind = (name == KN_LX_FILT || name == KN_LY_FILT) ? 2 : (name == KN_LX_CONS || name == KN_LY_CONS) ? 1 : 0;
+        for (hqt_ctx* c : kv.second) {
+            L.ngroups = std::max(L.ngroups, c->hp[pass].ngroups);
+            L.lds = std::max(L.lds, line_pass_lds_bytes(c->hp[pass], c->fp32));
+        }
+        plan.push_back(L);
+    }
+}
+
+bool all_members(const hqt_ctx*) { return true; }
+bool with_winding(const hqt_ctx* c) { return c->p.winding != 0; }
+bool with_filter(const hqt_ctx* c) { return c->p.filter != 0; }
+bool without_filter(const hqt_ctx* c) { return c->p.filter == 0; }
+
+template <class D, class DescFn, class BlocksFn>
+void add_z(std::vector<Launch>& plan, PlanImage& img, const std::vector<hqt_ctx*>& mem, int kind, bool by_nz,
+           bool (*use)(const hqt_ctx*), DescFn desc, BlocksFn blocks) {
+    auto cls = classes<std::pair<int, int>>(
+        mem, [&](hqt_ctx* c) { return std::make_pair(c->fp32, by_nz ? c->nz : 0); }, [&](hqt_ctx* c) { return use(c); });
+    for (auto& kv : cls) {
+        Launch L = make_launch<D>(img, kind, kv.second, desc, blocks);
+        L.fp32 = kv.first.first;
+        L.nz = kv.first.second;
+        plan.push_back(L);
+    }
+}
+
+// The step sequence of enqueue_step (hqt_host.cpp), class by class.
+void build_plan(hqt_ensemble* e, PlanImage& img) {
+    auto& P = e->plan;
+    const auto& M = e->mem;
+    auto zb = [](hqt_ctx* c) { return zblocks(c->nxy); };
+    add_z<ZPrep>(P, img, M, K_ZPREP, true, all_members, [](hqt_ctx* c) { return c->zpre; }, zb);
+    for (int s = 0; s < 4; ++s) {
+        add_line(P, img, M, PX1 + s, 0, KN_LX_STAGE, all_members);
+        add_line(P, img, M, PY1 + s, 1, KN_LY_STAGE, all_members);
+        add_z<ZStage>(P, img, M, K_ZSTAGE, true, all_members, [s](hqt_ctx* c) { return c->zs[s]; }, zb);
+    }
+    add_line(P, img, M, PXC, 0, KN_LX_CONS, all_members);
+    add_line(P, img, M, PYC, 1, KN_LY_CONS, all_members);
+    add_z<ZPost>(P, img, M, K_ZPOST, true, all_members, [](hqt_ctx* c) { return c->zp; }, zb);
+    add_z<Wind>(P, img, M, K_WPART, false, with_winding, [](hqt_ctx* c) { return c->wind; },
+                [](hqt_ctx* c) { return c->ny; });
+    add_z<Wind>(P, img, M, K_WAPPLY, false, with_winding, [](hqt_ctx* c) { return c->wind; },
+                [](hqt_ctx* c) { return wind_apply_blocks(c->nxy, c->nz); });
+    add_line(P, img, M, PXF, 0, KN_LX_FILT, with_filter);
+    add_line(P, img, M, PYF, 1, KN_LY_FILT, with_filter);
+    add_z<NanCheck>(P, img, M, K_NAN, false, without_filter, [](hqt_ctx* c) { return c->nan; },
+                    [](hqt_ctx* c) { return nancheck_blocks(c->npts); });
+}
+
+hipError_t issue(const hqt_ensemble* e, const Launch& L) {
+    const void* d = e->dplan + L.desc_off;
+    const Members M{reinterpret_cast<const int32_t*>(e->dplan + L.boff_off), L.nmem};
+    hipStream_t s = e->stream;
+    switch (L.kind) {
+        case K_LINE:
+            return launch_line_pass(static_cast<const LPass*>(d), M, L.blocks, L.ngroups, L.lds, L.Q, L.dir, L.fp32,
+                                    L.lkind, s);
+        case K_ZPREP: return launch_zprep(static_cast<const ZPrep*>(d), M, L.blocks, L.nz, s);
+        case K_ZSTAGE: return launch_zstage(static_cast<const ZStage*>(d), M, L.blocks, L.nz, L.fp32, s);
+        case K_ZPOST: return launch_zpost(static_cast<const ZPost*>(d), M, L.blocks, L.nz, L.fp32, s);
+        case K_WPART: return launch_wind_partial(static_cast<const Wind*>(d), M, L.blocks, s);
+        case K_WAPPLY: return launch_wind_apply(static_cast<const Wind*>(d), M, L.blocks, L.fp32, s);
+        case K_NAN: return launch_nancheck(static_cast<const NanCheck*>(d), M, L.blocks, L.fp32, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+int enqueue(const hqt_ensemble* e) {
+    for (const Launch& L : e->plan) {
+        const hipError_t r = issue(e, L);
+        if (r != hipSuccess) return efail(HQT_ERR_HIP, std::string("ensemble launch: ") + hipGetErrorString(r));
+    }
+    return HQT_OK;
+}
+
+int build_graph(hqt_ensemble* e) {
+    hipGraph_t g = nullptr;
+    HIPC(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue(e);
+    const hipError_t r = hipStreamEndCapture(e->stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (r != hipSuccess) return efail(HQT_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(r));
+    const hipError_t ri = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ri != hipSuccess) { e->graph = nullptr; return efail(HQT_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ri)); }
+    return HQT_OK;
+}
+
+void release(hqt_ensemble* e) {
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->graph) (void)hipGraphExecDestroy(e->graph);
+    if (e->dplan) (void)hipFree(e->dplan);
+    for (hqt_ctx* c : e->mem)
+        if (c && c->owner == e->stream) c->owner = nullptr;
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hqt_ensemble_create(hqt_ctx* const* members, int32_t n, int32_t flags, hqt_ensemble** out) {
+    if (!out || !members || n < 1) return efail(HQT_ERR_ARG, "hqt_ensemble_create: need n >= 1 members and out");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        hqt_ctx* c = members[i];
+        if (!c) return efail(HQT_ERR_ARG, "hqt_ensemble_create: NULL member " + std::to_string(i));
+        if (c->owner) return efail(HQT_ERR_STATE, "hqt_ensemble_create: member " + std::to_string(i) + " is in another ensemble");
+        if (!c->have_state) return efail(HQT_ERR_STATE, "hqt_ensemble_create: member " + std::to_string(i) + " has no state");
+        for (int j = 0; j < i; ++j)
+            if (members[j] == c) return efail(HQT_ERR_ARG, "hqt_ensemble_create: member listed twice");
+    }
+    hqt_ensemble* e = new (std::nothrow) hqt_ensemble();
+    if (!e) return efail(HQT_ERR_OOM, "host allocation failed");
+    e->mem.assign(members, members + n);
+    hipError_t r = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (r != hipSuccess) { e->stream = nullptr; release(e); return efail(HQT_ERR_HIP, "hipStreamCreate"); }
+    PlanImage img;
+    build_plan(e, img);
+    r = hipMalloc(&e->dplan, img.bytes.size());
+    if (r != hipSuccess) { e->dplan = nullptr; release(e); return efail(HQT_ERR_OOM, "hipMalloc(ensemble plan)"); }
+    r = hipMemcpy(e->dplan, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice);
+    if (r != hipSuccess) { release(e); return efail(HQT_ERR_HIP, "hipMemcpy(ensemble plan)"); }
+    for (hqt_ctx* c : e->mem) {
+        if (int rc = hqt_sync_streams(c)) { release(e); return rc; }
+        c->owner = e->stream;
+    }
+    if (!(flags & HQT_NO_GRAPH)) {
+        if (int rc = build_graph(e)) { release(e); return rc; }
+    }
+    *out = e;
+    return HQT_OK;
+}
+
+int hqt_ensemble_destroy(hqt_ensemble* e) {
+    if (e) release(e);
+    return HQT_OK;
+}
+
+int hqt_ensemble_launches(hqt_ensemble* e, int32_t* n) {
+    if (!e || !n) return efail(HQT_ERR_ARG, "NULL argument");
+    *n = (int32_t)e->plan.size();
+    return HQT_OK;
+}
+
+int hqt_ensemble_step_n(hqt_ensemble* e, int32_t nsteps) {
+    if (!e) return efail(HQT_ERR_ARG, "NULL ensemble");
+    if (nsteps < 0) return efail(HQT_ERR_ARG, "nsteps < 0");
+    for (hqt_ctx* c : e->mem) HIPC(hipStreamSynchronize(c->stream));   // members' own queued work first
+    for (int i = 0; i < nsteps; ++i) {
+        if (e->graph) {
+            HIPC(hipGraphLaunch(e->graph, e->stream));
+        } else if (int rc = enqueue(e)) {
+            return rc;
+        }
+    }
+    return HQT_OK;
+}
+
+int hqt_ensemble_sync(hqt_ensemble* e, int32_t* nan_steps) {
+    if (!e) return efail(HQT_ERR_ARG, "NULL ensemble");
+    HIPC(hipStreamSynchronize(e->stream));
+    int bad = 0, first = -1;
+    for (size_t i = 0; i < e->mem.size(); ++i) {
+        int32_t f[2];
+        HIPC(hipMemcpy(f, e->mem[i]->flags, sizeof(f), hipMemcpyDeviceToHost));
+        if (nan_steps) nan_steps[i] = f[0];
+        if (f[0] != 0) { ++bad; if (first < 0) first = (int)i; }
+    }
+    if (bad)
+        return efail(HQT_ERR_NAN, std::to_string(bad) + " ensemble member(s) hit NaN (first: member " + std::to_string(first) + ")");
+    return HQT_OK;
+}
+
+}  // extern "C"

@@ -8,8 +8,6 @@
 //   zpost                          A_t = Lzr(Lzr(cons) - rhoi) + mu, boundary reset, O (:129-136)
 //   winding                        removal of the global winding along y (:136-139)
 //   line_x_filter, line_y_filter   ff on Re Psi, Im Psi, A_t, A_x, A_y (:142), with the NaN guard (:182-185)
-#include "hqt.h"
-#include "hqt_kernels.h"
 #include "hqt_operators.h"
 
 #include <cmath>
@@ -18,76 +16,26 @@
 #include <string>
 #include <vector>
 
-using namespace hqt;
+#include "hqt_internal.h"
 
 namespace {
 thread_local std::string g_err;
-int fail(int code, const std::string& msg) {
+}  // namespace
+
+int hqt_fail(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
+
+namespace {
+inline int fail(int code, const std::string& msg) { return hqt_fail(code, msg); }
 }  // namespace
 
-#define HIPC(call)                                                                          \
-    do {                                                                                    \
-        hipError_t e_ = (call);                                                             \
-        if (e_ != hipSuccess) return fail(HQT_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-enum KName {
-    KN_ZPREP, KN_LX_STAGE, KN_LY_STAGE, KN_ZSTAGE, KN_LX_CONS, KN_LY_CONS, KN_ZPOST, KN_WIND,
-    KN_LX_FILT, KN_LY_FILT, KN_NAN, KN_LX_RHS, KN_LY_RHS, KN_ZRHS, KN_COUNT
-};
-static const char* kNames[KN_COUNT] = {
-    "zprep", "line_x_stage", "line_y_stage", "zstage", "line_x_cons", "line_y_cons", "zpost", "winding",
-    "line_x_filter", "line_y_filter", "nancheck", "line_x_rhs", "line_y_rhs", "zrhs"};
-
-enum Op { OP_D1X, OP_D2X, OP_ND1X, OP_PRX, OP_PIX, OP_NPIX, OP_D1Y, OP_D2Y, OP_ND1Y, OP_QRY, OP_QIY, OP_NQIY, OP_COUNT };
-
-enum Pass { PX1, PX2, PX3, PX4, PY1, PY2, PY3, PY4, PXC, PYC, PXF, PYF, PXR, PYR, NPASS };
-
-struct hqt_ctx {
-    hqt_params p{};
-    int nx = 0, ny = 0, nz = 0;
-    int64_t nxy = 0, npts = 0;
-    int fp32 = 0;
-    int tdt = DT_F64;
-    size_t es = 8;
-    hipStream_t stream = nullptr;
-    std::vector<void*> allocs;
-
-    void *ZR, *ZI, *ZAx, *ZAy;
-    void *SA[4], *SB[4], *AC[4];
-    void *dxR, *dxI, *dxxR, *dxxI, *dxAx, *dxAy, *dxxAy;
-    void *dyR, *dyI, *lapR, *lapI, *divA, *curl3, *curl4;
-    void *dxAt, *dyAt;
-    void *fa[5], *fb[5];
-    double *At, *rho, *dzAt, *arho, *O, *part, *W, *axb, *ayb, *yph, *zops, *AtR;
-    double *F1R = nullptr, *F1I = nullptr, *F2 = nullptr, *F3 = nullptr, *F4 = nullptr;
-    int32_t* flags = nullptr;
-    void* ops[OP_COUNT];
-    LPass hp[NPASS];
-    LPass* dp = nullptr;
-    ZOps zo{};
-    ZStage zs[4];
-    ZStage zr;
-    ZPost zp;
-    ZPrep zpre, zpreR;
-    Wind wind;
-    double* Od = nullptr;            // [2*nxy] condensate of the current state (diagnostics)
-    int8_t* cmap = nullptr;          // [nxy] plaquette charges
-    int32_t* ccount = nullptr;       // [2] +1 / -1 plaquette counts
-    double* cpart = nullptr;         // [2*census_blocks] per-block max|O|, sum|O|
-    bool have_state = false;
-
-    hipGraphExec_t graph = nullptr;  // one captured step (built on first use)
-
-    bool prof = false;
-    struct Rec { int name; hipEvent_t a, b; };
-    std::vector<Rec> recs;
-    int64_t counts[KN_COUNT] = {};
-    double totals[KN_COUNT] = {};
-};
+int hqt_sync_streams(hqt_ctx* c) {
+    if (c->owner) HIPC(hipStreamSynchronize(c->owner));
+    HIPC(hipStreamSynchronize(c->stream));
+    return HQT_OK;
+}
 
 namespace {
 
@@ -217,31 +165,58 @@ int run(hqt_ctx* c, int name, F&& f) {
     return HQT_OK;
 }
 
+const Members kOne{nullptr, 1};
+
+int line_kind(int name) {
+    return (name == KN_LX_FILT || name == KN_LY_FILT) ? 2 : (name == KN_LX_CONS || name == KN_LY_CONS) ? 1 : 0;
+}
+
 int line(hqt_ctx* c, int pass, int dir, int name) {
-    const int kind = (name == KN_LX_FILT || name == KN_LY_FILT) ? 2 : (name == KN_LX_CONS || name == KN_LY_CONS) ? 1 : 0;
-    return run(c, name, [&] { return launch_line_pass_dev(c->hp[pass], c->dp + pass, dir, c->fp32, kind, c->stream); });
+    const LPass& H = c->hp[pass];
+    return run(c, name, [&] {
+        return launch_line_pass(c->dp + pass, kOne, H.ntiles, H.ngroups, line_pass_lds_bytes(H, c->fp32), H.Mp / 64, dir,
+                                c->fp32, line_kind(name), c->stream);
+    });
+}
+
+int upload_desc(hqt_ctx* c) {
+    hqt_ctx::DevDesc h;
+    std::memset(&h, 0, sizeof(h));
+    for (int s = 0; s < 4; ++s) h.zs[s] = c->zs[s];
+    h.zs[4] = c->zr;
+    h.zp = c->zp;
+    h.zpre[0] = c->zpre;
+    h.zpre[1] = c->zpreR;
+    h.wind = c->wind;
+    h.nan = c->nan;
+    HIPC(hipMemcpy(c->dd, &h, sizeof(h), hipMemcpyHostToDevice));
+    return HQT_OK;
 }
 
 int enqueue_step(hqt_ctx* c) {
     int rc;
-    if ((rc = run(c, KN_ZPREP, [&] { return launch_zprep(c->zpre, c->stream); }))) return rc;
+    const int zb = zblocks(c->nxy), nz = c->nz, f32 = c->fp32;
+    hipStream_t st = c->stream;
+    if ((rc = run(c, KN_ZPREP, [&] { return launch_zprep(&c->dd->zpre[0], kOne, zb, nz, st); }))) return rc;
     for (int s = 0; s < 4; ++s) {
         if ((rc = line(c, PX1 + s, 0, KN_LX_STAGE))) return rc;
         if ((rc = line(c, PY1 + s, 1, KN_LY_STAGE))) return rc;
-        if ((rc = run(c, KN_ZSTAGE, [&] { return launch_zstage(c->zs[s], c->fp32, c->stream); }))) return rc;
+        if ((rc = run(c, KN_ZSTAGE, [&] { return launch_zstage(&c->dd->zs[s], kOne, zb, nz, f32, st); }))) return rc;
     }
     if ((rc = line(c, PXC, 0, KN_LX_CONS))) return rc;
     if ((rc = line(c, PYC, 1, KN_LY_CONS))) return rc;
-    if ((rc = run(c, KN_ZPOST, [&] { return launch_zpost(c->zp, c->fp32, c->stream); }))) return rc;
+    if ((rc = run(c, KN_ZPOST, [&] { return launch_zpost(&c->dd->zp, kOne, zb, nz, f32, st); }))) return rc;
     if (c->p.winding)
-        if ((rc = run(c, KN_WIND, [&] { return launch_winding(c->wind, c->fp32, c->stream); }))) return rc;
+        if ((rc = run(c, KN_WIND, [&] {
+                 hipError_t e = launch_wind_partial(&c->dd->wind, kOne, c->ny, st);
+                 return e != hipSuccess ? e : launch_wind_apply(&c->dd->wind, kOne, wind_apply_blocks(c->nxy, nz), f32, st);
+             })))
+            return rc;
     if (c->p.filter) {
         if ((rc = line(c, PXF, 0, KN_LX_FILT))) return rc;
         if ((rc = line(c, PYF, 1, KN_LY_FILT))) return rc;
     } else {
-        if ((rc = run(c, KN_NAN, [&] {
-                 return launch_nancheck(c->ZR, c->ZI, c->ZAx, c->ZAy, c->At, c->npts, c->fp32, c->flags, c->stream);
-             })))
+        if ((rc = run(c, KN_NAN, [&] { return launch_nancheck(&c->dd->nan, kOne, nancheck_blocks(c->npts), f32, st); })))
             return rc;
     }
     return HQT_OK;
@@ -278,6 +253,7 @@ int build_context(hqt_ctx* c) {
     ALLOC(c->axb, nxy * 8); ALLOC(c->ayb, nxy * 8); ALLOC(c->yph, (size_t)ny * 8);
     ALLOC(c->flags, 4 * sizeof(int32_t));
     ALLOC(c->dp, NPASS * sizeof(LPass));
+    ALLOC(c->dd, sizeof(hqt_ctx::DevDesc));
     ALLOC(c->Od, 2 * nxy * 8); ALLOC(c->cmap, nxy); ALLOC(c->ccount, 2 * sizeof(int32_t));
     ALLOC(c->cpart, 2 * (size_t)census_blocks((int64_t)nxy) * 8);
 
@@ -439,8 +415,10 @@ int build_context(hqt_ctx* c) {
     c->wind.nx = nx; c->wind.ny = ny; c->wind.nz = nz; c->wind.nxy = c->nxy;
     c->wind.O = c->O; c->wind.part = c->part; c->wind.R = c->ZR; c->wind.I = c->ZI;
     c->wind.yph = c->yph; c->wind.W = c->W; c->wind.flags = c->flags;
+    c->nan.R = c->ZR; c->nan.I = c->ZI; c->nan.Ax = c->ZAx; c->nan.Ay = c->ZAy; c->nan.At = c->At;
+    c->nan.n = (int64_t)np; c->nan.flags = c->flags;
 #undef ALLOC
-    return HQT_OK;
+    return upload_desc(c);
 }
 
 int check_params(const hqt_params* p) {
@@ -503,11 +481,12 @@ int hqt_ctx_create(const hqt_params* p, hqt_ctx** out) {
 
 int hqt_ctx_destroy(hqt_ctx* c) {
     if (!c) return HQT_OK;
+    if (c->owner) return fail(HQT_ERR_STATE, "context is an ensemble member; destroy the ensemble first");
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     for (auto& r : c->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
-    for (void* q : c->allocs) hipFree(q);
-    if (c->stream) hipStreamDestroy(c->stream);
+    for (void* q : c->allocs) (void)hipFree(q);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return HQT_OK;
 }
@@ -515,7 +494,7 @@ int hqt_ctx_destroy(hqt_ctx* c) {
 int hqt_set_state(hqt_ctx* c, const double* Z1, const double* Z2, const double* Z3, const double* Z4,
                   const double* rhoi) {
     if (!c || !Z1 || !Z2 || !Z3 || !Z4 || !rhoi) return fail(HQT_ERR_ARG, "NULL argument");
-    HIPC(hipStreamSynchronize(c->stream));
+    if (int rs_ = hqt_sync_streams(c)) return rs_;
     const size_t n = (size_t)c->npts;
     std::vector<double> re, im;
     split_complex(Z1, n, re, im);
@@ -534,7 +513,7 @@ int hqt_set_state(hqt_ctx* c, const double* Z1, const double* Z2, const double* 
 int hqt_get_state(hqt_ctx* c, double* Z1, double* Z2, double* Z3, double* Z4, double* rhoi) {
     if (!c) return fail(HQT_ERR_ARG, "NULL context");
     if (!c->have_state) return fail(HQT_ERR_STATE, "no state uploaded");
-    HIPC(hipStreamSynchronize(c->stream));
+    if (int rs_ = hqt_sync_streams(c)) return rs_;
     const size_t n = (size_t)c->npts;
     int rc;
     if (Z1) {
@@ -569,6 +548,7 @@ int hqt_step_n(hqt_ctx* c, int32_t nsteps) {
     if (!c) return fail(HQT_ERR_ARG, "NULL context");
     if (!c->have_state) return fail(HQT_ERR_STATE, "hqt_step_n before hqt_set_state");
     if (nsteps < 0) return fail(HQT_ERR_ARG, "nsteps < 0");
+    if (c->owner) HIPC(hipStreamSynchronize(c->owner));   // ensemble steps queued earlier come first
     const bool use_graph = !(c->p.flags & HQT_NO_GRAPH) && !c->prof;
     if (use_graph && nsteps > 0 && !c->graph) {
         int rc = build_graph(c);
@@ -587,7 +567,7 @@ int hqt_step_n(hqt_ctx* c, int32_t nsteps) {
 
 int hqt_sync(hqt_ctx* c, int32_t* nan_step) {
     if (!c) return fail(HQT_ERR_ARG, "NULL context");
-    HIPC(hipStreamSynchronize(c->stream));
+    if (int rs_ = hqt_sync_streams(c)) return rs_;
     int rc = resolve_profile(c);
     if (rc) return rc;
     return read_flags(c, nan_step);
@@ -595,7 +575,7 @@ int hqt_sync(hqt_ctx* c, int32_t* nan_step) {
 
 int hqt_last_winding(hqt_ctx* c, double* w) {
     if (!c || !w) return fail(HQT_ERR_ARG, "NULL argument");
-    HIPC(hipStreamSynchronize(c->stream));
+    if (int rs_ = hqt_sync_streams(c)) return rs_;
     HIPC(hipMemcpy(w, c->W, 8, hipMemcpyDeviceToHost));
     return HQT_OK;
 }
@@ -610,8 +590,9 @@ int hqt_rhs(hqt_ctx* c, const double* Z1, const double* Z2, const double* Z3, co
         ALLOC(c->F1R, n * 8); ALLOC(c->F1I, n * 8); ALLOC(c->F2, n * 8); ALLOC(c->F3, n * 8); ALLOC(c->F4, n * 8);
 #undef ALLOC
         c->zr.F1R = c->F1R; c->zr.F1I = c->F1I; c->zr.F2 = c->F2; c->zr.F3 = c->F3; c->zr.F4 = c->F4;
+        if ((rc = upload_desc(c))) return rc;
     }
-    HIPC(hipStreamSynchronize(c->stream));
+    if (int rs_ = hqt_sync_streams(c)) return rs_;
     std::vector<double> re, im;
     split_complex(Z1, n, re, im);
     if ((rc = upload_as(c, c->SB[0], re.data(), n, c->tdt))) return rc;
@@ -622,11 +603,12 @@ int hqt_rhs(hqt_ctx* c, const double* Z1, const double* Z2, const double* Z3, co
     int32_t saved[2];
     HIPC(hipMemcpy(saved, c->flags, sizeof(saved), hipMemcpyDeviceToHost));
     HIPC(hipMemset(c->flags, 0, sizeof(saved)));
-    if ((rc = run(c, KN_ZPREP, [&] { return launch_zprep(c->zpreR, c->stream); }))) return rc;
+    const int zb = zblocks(c->nxy);
+    if ((rc = run(c, KN_ZPREP, [&] { return launch_zprep(&c->dd->zpre[1], kOne, zb, c->nz, c->stream); }))) return rc;
     if ((rc = line(c, PXR, 0, KN_LX_RHS))) return rc;
     if ((rc = line(c, PYR, 1, KN_LY_RHS))) return rc;
-    if ((rc = run(c, KN_ZRHS, [&] { return launch_zstage(c->zr, c->fp32, c->stream); }))) return rc;
-    HIPC(hipStreamSynchronize(c->stream));
+    if ((rc = run(c, KN_ZRHS, [&] { return launch_zstage(&c->dd->zs[4], kOne, zb, c->nz, c->fp32, c->stream); }))) return rc;
+    if (int rs_ = hqt_sync_streams(c)) return rs_;
     HIPC(hipMemcpy(c->flags, saved, sizeof(saved), hipMemcpyHostToDevice));
     if (F1) {
         std::vector<double> r(n), i(n);
@@ -663,7 +645,7 @@ int hqt_order_parameter(hqt_ctx* c, double* O) {
     int rc;
     if ((rc = enqueue_census(c, false))) return rc;
     HIPC(hipMemcpyAsync(O, c->Od, 2 * (size_t)c->nxy * 8, hipMemcpyDeviceToHost, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
+    if (int rs_ = hqt_sync_streams(c)) return rs_;
     return HQT_OK;
 }
 
@@ -678,7 +660,7 @@ int hqt_vortex_census(hqt_ctx* c, int32_t* counts, int8_t* charge, double* Oabs)
     std::vector<double> part(2 * (size_t)nb);
     if (Oabs) HIPC(hipMemcpyAsync(part.data(), c->cpart, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
     if (charge) HIPC(hipMemcpyAsync(charge, c->cmap, (size_t)c->nxy, hipMemcpyDeviceToHost, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
+    if (int rs_ = hqt_sync_streams(c)) return rs_;
     counts[0] = pm[0] + pm[1];
     counts[1] = pm[0];
     counts[2] = pm[1];
@@ -761,14 +743,17 @@ int hqt_ff(int32_t nx, int32_t ny, int32_t nslices, const double* x_re, const do
         add_out(h, out, DT_F64, 0, OP_QRY, 1, OP_NQIY);     // Re(Py (a + i b)) = Qr a - Qi b
         if (hipMemcpy(dp, hp, sizeof(hp), hipMemcpyHostToDevice) != hipSuccess) { rc = fail(HQT_ERR_HIP, "hipMemcpy pass"); break; }
         if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) { rc = fail(HQT_ERR_HIP, "stream"); break; }
-        hipError_t e = launch_line_pass_dev(hp[0], dp, 0, 0, 2, s);
-        if (e == hipSuccess) e = launch_line_pass_dev(hp[1], dp + 1, 1, 0, 2, s);
+        hipError_t e = launch_line_pass(dp, kOne, hp[0].ntiles, hp[0].ngroups, line_pass_lds_bytes(hp[0], 0), hp[0].Mp / 64,
+                                        0, 0, 2, s);
+        if (e == hipSuccess)
+            e = launch_line_pass(dp + 1, kOne, hp[1].ntiles, hp[1].ngroups, line_pass_lds_bytes(hp[1], 0), hp[1].Mp / 64, 1,
+                                 0, 2, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) { rc = fail(HQT_ERR_HIP, std::string("hqt_ff: ") + hipGetErrorString(e)); break; }
         if (hipMemcpy(w, out, n * 8, hipMemcpyDeviceToHost) != hipSuccess) { rc = fail(HQT_ERR_HIP, "hipMemcpy w"); break; }
     } while (0);
-    if (s) hipStreamDestroy(s);
-    for (void* q : tmp.allocs) hipFree(q);
+    if (s) (void)hipStreamDestroy(s);
+    for (void* q : tmp.allocs) (void)hipFree(q);
     tmp.allocs.clear();
     return rc;
 }
@@ -910,7 +895,7 @@ int hqt_debug_field(hqt_ctx* c, const char* name, double* out) {
         {"dxAt", c->dxAt, c->tdt}, {"dyAt", c->dyAt, c->tdt}, {"dzAt", c->dzAt, DT_F64}};
     for (auto& t : tab) {
         if (std::strcmp(t.n, name) == 0) {
-            HIPC(hipStreamSynchronize(c->stream));
+            if (int rs_ = hqt_sync_streams(c)) return rs_;
             return download_as(out, t.p, (size_t)c->npts, t.dt);
         }
     }
@@ -945,7 +930,7 @@ int hqt_calibration_copy(int64_t bytes, double* ms) {
 
 int hqt_profile(hqt_ctx* c, int32_t enable) {
     if (!c) return fail(HQT_ERR_ARG, "NULL context");
-    HIPC(hipStreamSynchronize(c->stream));
+    if (int rs_ = hqt_sync_streams(c)) return rs_;
     int rc = resolve_profile(c);
     if (rc) return rc;
     c->prof = enable != 0;
@@ -956,7 +941,7 @@ int hqt_profile(hqt_ctx* c, int32_t enable) {
 int hqt_kernel_stats(hqt_ctx* c, int32_t idx, char* name, int32_t name_len, int64_t* count, double* total_ms) {
     if (!c) return fail(HQT_ERR_ARG, "NULL context");
     if (idx < 0 || idx >= KN_COUNT) return fail(HQT_ERR_ARG, "index past the last kernel");
-    HIPC(hipStreamSynchronize(c->stream));
+    if (int rs_ = hqt_sync_streams(c)) return rs_;
     int rc = resolve_profile(c);
     if (rc) return rc;
     if (name && name_len > 0) std::snprintf(name, name_len, "%s", kNames[idx]);

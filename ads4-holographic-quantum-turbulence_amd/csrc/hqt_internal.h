@@ -1,0 +1,82 @@
+// Internal definitions shared by the host-side translation units of libhqt: the context
+// structure, status helpers and the launch sequence of one step.
+#pragma once
+#include "hqt.h"
+#include "hqt_kernels.h"
+
+#include <string>
+#include <vector>
+
+using namespace hqt;
+
+// Sets the thread-local message returned by hqt_last_error() and returns code.
+int hqt_fail(int code, const std::string& msg);
+
+#define HIPC(call)                                                                          \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) return hqt_fail(HQT_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+enum KName {
+    KN_ZPREP, KN_LX_STAGE, KN_LY_STAGE, KN_ZSTAGE, KN_LX_CONS, KN_LY_CONS, KN_ZPOST, KN_WIND,
+    KN_LX_FILT, KN_LY_FILT, KN_NAN, KN_LX_RHS, KN_LY_RHS, KN_ZRHS, KN_COUNT
+};
+inline constexpr const char* kNames[KN_COUNT] = {
+    "zprep", "line_x_stage", "line_y_stage", "zstage", "line_x_cons", "line_y_cons", "zpost", "winding",
+    "line_x_filter", "line_y_filter", "nancheck", "line_x_rhs", "line_y_rhs", "zrhs"};
+
+enum Op { OP_D1X, OP_D2X, OP_ND1X, OP_PRX, OP_PIX, OP_NPIX, OP_D1Y, OP_D2Y, OP_ND1Y, OP_QRY, OP_QIY, OP_NQIY, OP_COUNT };
+
+enum Pass { PX1, PX2, PX3, PX4, PY1, PY2, PY3, PY4, PXC, PYC, PXF, PYF, PXR, PYR, NPASS };
+
+struct hqt_ctx {
+    hqt_params p{};
+    int nx = 0, ny = 0, nz = 0;
+    int64_t nxy = 0, npts = 0;
+    int fp32 = 0;
+    int tdt = DT_F64;
+    size_t es = 8;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+
+    void *ZR, *ZI, *ZAx, *ZAy;
+    void *SA[4], *SB[4], *AC[4];
+    void *dxR, *dxI, *dxxR, *dxxI, *dxAx, *dxAy, *dxxAy;
+    void *dyR, *dyI, *lapR, *lapI, *divA, *curl3, *curl4;
+    void *dxAt, *dyAt;
+    void *fa[5], *fb[5];
+    double *At, *rho, *dzAt, *arho, *O, *part, *W, *axb, *ayb, *yph, *zops, *AtR;
+    double *F1R = nullptr, *F1I = nullptr, *F2 = nullptr, *F3 = nullptr, *F4 = nullptr;
+    int32_t* flags = nullptr;
+    void* ops[OP_COUNT];
+    LPass hp[NPASS];
+    LPass* dp = nullptr;
+    ZOps zo{};
+    ZStage zs[4];
+    ZStage zr;
+    ZPost zp;
+    ZPrep zpre, zpreR;
+    Wind wind;
+    NanCheck nan{};
+    // device copies of the radial/winding/guard descriptors (kernels read them from memory)
+    struct DevDesc { ZStage zs[5]; ZPost zp; ZPrep zpre[2]; Wind wind; NanCheck nan; };
+    DevDesc* dd = nullptr;
+    hipStream_t owner = nullptr;     // stream of an ensemble that steps this context, if any
+    double* Od = nullptr;            // [2*nxy] condensate of the current state (diagnostics)
+    int8_t* cmap = nullptr;          // [nxy] plaquette charges
+    int32_t* ccount = nullptr;       // [2] +1 / -1 plaquette counts
+    double* cpart = nullptr;         // [2*census_blocks] per-block max|O|, sum|O|
+    bool have_state = false;
+
+    hipGraphExec_t graph = nullptr;  // one captured step (built on first use)
+
+    bool prof = false;
+    struct Rec { int name; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+    int64_t counts[KN_COUNT] = {};
+    double totals[KN_COUNT] = {};
+};
+
+// Waits for the context's own stream and, while it is an ensemble member, the ensemble's.
+int hqt_sync_streams(hqt_ctx* c);

@@ -9,6 +9,35 @@ namespace hqt {
 enum Dtype : int32_t { DT_F64 = 0, DT_F32 = 1 };
 
 // ---------------------------------------------------------------------------------------
+// Members of a launch.  A kernel launch serves one or more independent evolutions (members)
+// of one kernel class; its descriptors are an array in device memory, one per member, and
+// member m owns blocks [boff[m], boff[m+1]) of grid.x.  nmem = 1 (boff unused) is the
+// single-evolution case.
+// ---------------------------------------------------------------------------------------
+struct Members {
+    const int32_t* boff;   // [nmem + 1] block prefix offsets
+    int32_t nmem;
+};
+struct MemberBlock { int m, bx, nb; };   // member, block within the member, member's block count
+
+__device__ __forceinline__ MemberBlock member_block(const Members M) {
+    MemberBlock r{0, (int)blockIdx.x, (int)gridDim.x};
+    if (M.nmem > 1) {
+        int lo = 0, hi = M.nmem - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (M.boff[mid] <= r.bx) lo = mid;
+            else hi = mid - 1;
+        }
+        lo = __builtin_amdgcn_readfirstlane(lo);
+        r.m = lo;
+        r.bx -= M.boff[lo];
+        r.nb = M.boff[lo + 1] - M.boff[lo];
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------------------
 // Line pass: apply Toeplitz/circulant line operators along x (dir 0) or y (dir 1).
 // One work-group owns a tile of 16 lines and one "group" of outputs.  An output is a sum
 // of up to two terms op(tile) plus an optional point-wise addend field.
@@ -97,6 +126,14 @@ struct Wind {
     int32_t* flags;
 };
 
+// NaN guard over the state when the filter (which carries the guard) is off.
+struct NanCheck {
+    const void *R, *I, *Ax, *Ay;     // (TS)
+    const double* At;
+    int64_t n;
+    int32_t* flags;
+};
+
 // One stationary Newton problem (NewtonRaphson_vy.m) of a batch; all pointers are device memory.
 struct NewtonJob {
     int32_t nz, it_max;
@@ -121,18 +158,25 @@ struct Census {
     double* part;             // [2*census_blocks]: per-block max|O|, sum|O|
 };
 
-// Launchers (hqt_kernels.hip).  All return the hipError_t of the launch.
-// H: host copy of the descriptor (grid sizes, LDS bytes); dP: the same descriptor in device memory.
+// Launchers.  All return the hipError_t of the launch.  d: descriptor array in device memory
+// (one per member), M: members, blocks: total grid.x over the members.
+// Line passes: grid.y = ngroups (members with fewer groups skip the extra rows); lds bytes
+// must cover every member; Q = Mp/64 is shared by the members of one launch.
 // kind: 0 RHS stage, 1 constraint, 2 filter (separate instantiations for profiling).
-hipError_t launch_line_pass_dev(const LPass& H, const LPass* dP, int dir, int fp32, int kind, hipStream_t s);
+hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, size_t lds, int Q, int dir, int fp32,
+                            int kind, hipStream_t s);
 size_t line_pass_lds_bytes(const LPass& P, int fp32);
-hipError_t launch_zstage(const ZStage& A, int fp32, hipStream_t s);
-hipError_t launch_zpost(const ZPost& A, int fp32, hipStream_t s);
-hipError_t launch_zprep(const ZPrep& A, hipStream_t s);
-hipError_t launch_winding(const Wind& A, int fp32, hipStream_t s);
-// NaN guard over the state when the filter (which carries the guard) is off.
-hipError_t launch_nancheck(const void* R, const void* I, const void* Ax, const void* Ay, const double* At,
-                           int64_t n, int fp32, int32_t* flags, hipStream_t s);
+// Radial kernels: grid.x per member = zblocks(nxy); nz is shared by the members of one launch.
+inline int zblocks(int64_t nxy) { return (int)((nxy + 63) / 64); }
+hipError_t launch_zstage(const ZStage* d, Members M, int blocks, int nz, int fp32, hipStream_t s);
+hipError_t launch_zpost(const ZPost* d, Members M, int blocks, int nz, int fp32, hipStream_t s);
+hipError_t launch_zprep(const ZPrep* d, Members M, int blocks, int nz, hipStream_t s);
+// Winding: partial sums (grid.x per member = ny), then the removal (wind_apply_blocks per member).
+int wind_apply_blocks(int64_t nxy, int nz);
+hipError_t launch_wind_partial(const Wind* d, Members M, int blocks, hipStream_t s);
+hipError_t launch_wind_apply(const Wind* d, Members M, int blocks, int fp32, hipStream_t s);
+int nancheck_blocks(int64_t n);
+hipError_t launch_nancheck(const NanCheck* d, Members M, int blocks, int fp32, hipStream_t s);
 bool zkernels_support_nz(int nz);
 // Plain streaming copy of `bytes` (multiple of 32) used to calibrate traffic counters.
 hipError_t launch_copy_calib(const void* src, void* dst, int64_t bytes, hipStream_t s);

@@ -48,8 +48,10 @@ __device__ __forceinline__ void stv(void* p, int dt, int64_t i, T v) {
 // KIND only tags the instantiation (0: RHS stage, 1: constraint, 2: filter) so that
 // profilers report the three uses separately; the code is identical.
 template <typename T, int DIR, int Q, int KIND>
-__global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restrict__ Pp) {
-    const LPass& P = *Pp;
+__global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restrict__ Pp, const Members M) {
+    const MemberBlock mb = member_block(M);
+    const LPass& P = Pp[mb.m];
+    if ((int)blockIdx.y >= P.ngroups) return;
     if (__builtin_nontemporal_load(&P.flags[0]) != 0) return;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     const int S = P.S, Np = P.Np, N = P.N, TL = P.TL;
@@ -66,11 +68,11 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restric
     int64_t line0 = 0, nl = 0;
     int z = 0, x0 = 0;
     if (DIR == 0) {
-        line0 = (int64_t)blockIdx.x * 16;
+        line0 = (int64_t)mb.bx * 16;
         nl = (int64_t)P.ny * P.nz;
     } else {
-        const int xc = blockIdx.x % P.nxc;
-        z = blockIdx.x / P.nxc;
+        const int xc = mb.bx % P.nxc;
+        z = mb.bx / P.nxc;
         x0 = xc * 16;
     }
 
@@ -183,11 +185,11 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restric
 }
 
 template <typename T, int DIR, int KIND>
-static hipError_t launch_q(const LPass& H, const LPass* dP, int Q, size_t lds, hipStream_t s) {
-    dim3 grid(H.ntiles, H.ngroups), block(LP_THREADS);
+static hipError_t launch_q(const LPass* d, Members M, dim3 grid, int Q, size_t lds, hipStream_t s) {
+    const dim3 block(LP_THREADS);
     switch (Q) {
 #define HQT_LP_CASE(q) \
-    case q: hipLaunchKernelGGL((k_line_pass<T, DIR, q, KIND>), grid, block, lds, s, dP); break;
+    case q: hipLaunchKernelGGL((k_line_pass<T, DIR, q, KIND>), grid, block, lds, s, d, M); break;
         HQT_LP_CASE(1) HQT_LP_CASE(2) HQT_LP_CASE(3) HQT_LP_CASE(4)
         HQT_LP_CASE(5) HQT_LP_CASE(6) HQT_LP_CASE(7) HQT_LP_CASE(8)
 #undef HQT_LP_CASE
@@ -213,10 +215,9 @@ static void set_lds_attr(size_t bytes) {
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-// H: host copy of the descriptor (for grid sizes), dP: the same descriptor in device memory.
-hipError_t launch_line_pass_dev(const LPass& H, const LPass* dP, int dir, int fp32, int kind, hipStream_t s) {
-    const int Q = H.Mp / 64;
-    const size_t lds = line_pass_lds_bytes(H, fp32);
+hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, size_t lds, int Q, int dir, int fp32,
+                            int kind, hipStream_t s) {
+    if (Q < 1 || Q > 8 || blocks <= 0) return hipErrorInvalidValue;
     if (lds > 64 * 1024) {
         static bool done[2][2][9] = {};
         if (!done[fp32][dir][Q]) {
@@ -228,8 +229,9 @@ hipError_t launch_line_pass_dev(const LPass& H, const LPass* dP, int dir, int fp
 #undef HQT_ATTR
         }
     }
-#define HQT_K(T_, D_) (kind == 0 ? launch_q<T_, D_, 0>(H, dP, Q, lds, s) : kind == 1 ? launch_q<T_, D_, 1>(H, dP, Q, lds, s) \
-                                                                          : launch_q<T_, D_, 2>(H, dP, Q, lds, s))
+    const dim3 grid((unsigned)blocks, (unsigned)ngroups);
+#define HQT_K(T_, D_) (kind == 0 ? launch_q<T_, D_, 0>(d, M, grid, Q, lds, s) : kind == 1 ? launch_q<T_, D_, 1>(d, M, grid, Q, lds, s) \
+                                                                          : launch_q<T_, D_, 2>(d, M, grid, Q, lds, s))
     if (fp32) return dir ? HQT_K(float, 1) : HQT_K(float, 0);
     return dir ? HQT_K(double, 1) : HQT_K(double, 0);
 #undef HQT_K

@@ -19,10 +19,12 @@ __device__ __forceinline__ double block_sum_256(double v, double* sh) {
     return r;
 }
 
-__global__ __launch_bounds__(256) void k_wind_partial(const Wind A) {
+__global__ __launch_bounds__(256) void k_wind_partial(const Wind* __restrict__ Ap, const Members M) {
     __shared__ double sh[256];
+    const MemberBlock mb = member_block(M);
+    const Wind& A = Ap[mb.m];
     if (stopped(A.flags)) return;
-    const int j = blockIdx.x, jn = (j + 1) % A.ny;
+    const int j = mb.bx, jn = (j + 1) % A.ny;
     double s = 0.0;
     for (int x = threadIdx.x; x < A.nx; x += 256) {
         const int64_t a = (int64_t)x + (int64_t)A.nx * jn, b = (int64_t)x + (int64_t)A.nx * j;
@@ -33,17 +35,19 @@ __global__ __launch_bounds__(256) void k_wind_partial(const Wind A) {
 }
 
 template <typename TS>
-__global__ __launch_bounds__(256) void k_wind_apply(const Wind A) {
+__global__ __launch_bounds__(256) void k_wind_apply(const Wind* __restrict__ Ap, const Members M) {
     __shared__ double sh[256];
+    const MemberBlock mb = member_block(M);
+    const Wind& A = Ap[mb.m];
     if (stopped(A.flags)) return;
     double s = 0.0;
     for (int j = threadIdx.x; j < A.ny; j += 256) s += A.part[j];
     const double tot = block_sum_256(s, sh);
     const double W = round((tot / A.ny) / (2.0 * M_PI));
-    if (blockIdx.x == 0 && threadIdx.x == 0) A.W[0] = W;
+    if (mb.bx == 0 && threadIdx.x == 0) A.W[0] = W;
     if (W == 0.0) return;
     const int64_t total = A.nxy * A.nz;
-    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+    for (int64_t e = (int64_t)mb.bx * 256 + threadIdx.x; e < total; e += (int64_t)mb.nb * 256) {
         const int64_t c = e % A.nxy;
         const int y = (int)(c / A.nx);
         double sn, cs;
@@ -60,47 +64,51 @@ __global__ __launch_bounds__(256) void k_wind_apply(const Wind A) {
 
 bool zkernels_support_nz(int nz) { return nz >= 4 && nz <= 32; }
 
-#define HQT_DECL(n) extern template hipError_t zstage_launch_nz<n>(const ZStage&, int, hipStream_t); \
-                    extern template hipError_t zpost_launch_nz<n>(const ZPost&, int, hipStream_t); \
-                    extern template hipError_t zprep_launch_nz<n>(const ZPrep&, hipStream_t);
+#define HQT_DECL(n) extern template hipError_t zstage_launch_nz<n>(const ZStage*, Members, int, int, hipStream_t); \
+                    extern template hipError_t zpost_launch_nz<n>(const ZPost*, Members, int, int, hipStream_t); \
+                    extern template hipError_t zprep_launch_nz<n>(const ZPrep*, Members, int, hipStream_t);
 HQT_NZ_LIST(HQT_DECL)
 #undef HQT_DECL
 
-hipError_t launch_zstage(const ZStage& A, int fp32, hipStream_t s) {
-    switch (A.nz) {
-#define HQT_ZS(n) case n: return zstage_launch_nz<n>(A, fp32, s);
+hipError_t launch_zstage(const ZStage* d, Members M, int blocks, int nz, int fp32, hipStream_t s) {
+    switch (nz) {
+#define HQT_ZS(n) case n: return zstage_launch_nz<n>(d, M, blocks, fp32, s);
         HQT_NZ_LIST(HQT_ZS)
 #undef HQT_ZS
         default: return hipErrorInvalidValue;
     }
 }
-hipError_t launch_zpost(const ZPost& A, int fp32, hipStream_t s) {
-    switch (A.nz) {
-#define HQT_ZP(n) case n: return zpost_launch_nz<n>(A, fp32, s);
+hipError_t launch_zpost(const ZPost* d, Members M, int blocks, int nz, int fp32, hipStream_t s) {
+    switch (nz) {
+#define HQT_ZP(n) case n: return zpost_launch_nz<n>(d, M, blocks, fp32, s);
         HQT_NZ_LIST(HQT_ZP)
 #undef HQT_ZP
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_zprep(const ZPrep& A, hipStream_t s) {
-    switch (A.nz) {
-#define HQT_ZR(n) case n: return zprep_launch_nz<n>(A, s);
+hipError_t launch_zprep(const ZPrep* d, Members M, int blocks, int nz, hipStream_t s) {
+    switch (nz) {
+#define HQT_ZR(n) case n: return zprep_launch_nz<n>(d, M, blocks, s);
         HQT_NZ_LIST(HQT_ZR)
 #undef HQT_ZR
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_winding(const Wind& A, int fp32, hipStream_t s) {
-    hipLaunchKernelGGL(k_wind_partial, dim3(A.ny), dim3(256), 0, s, A);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    int64_t total = A.nxy * A.nz;
-    unsigned blocks = (unsigned)((total + 255) / 256);
-    if (blocks > 2048) blocks = 2048;
-    if (fp32) hipLaunchKernelGGL(k_wind_apply<float>, dim3(blocks), dim3(256), 0, s, A);
-    else hipLaunchKernelGGL(k_wind_apply<double>, dim3(blocks), dim3(256), 0, s, A);
+int wind_apply_blocks(int64_t nxy, int nz) {
+    const int64_t b = (nxy * nz + 255) / 256;
+    return (int)(b > 2048 ? 2048 : b);
+}
+
+hipError_t launch_wind_partial(const Wind* d, Members M, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_wind_partial, dim3(blocks), dim3(256), 0, s, d, M);
+    return hipGetLastError();
+}
+
+hipError_t launch_wind_apply(const Wind* d, Members M, int blocks, int fp32, hipStream_t s) {
+    if (fp32) hipLaunchKernelGGL(k_wind_apply<float>, dim3(blocks), dim3(256), 0, s, d, M);
+    else hipLaunchKernelGGL(k_wind_apply<double>, dim3(blocks), dim3(256), 0, s, d, M);
     return hipGetLastError();
 }
 
@@ -108,26 +116,29 @@ hipError_t launch_winding(const Wind& A, int fp32, hipStream_t s) {
 
 namespace hqt {
 template <typename TS>
-__global__ __launch_bounds__(256) void k_nancheck(const void* R, const void* I, const void* Ax, const void* Ay,
-                                                  const double* At, int64_t n, int32_t* flags) {
-    if (stopped(flags)) return;
+__global__ __launch_bounds__(256) void k_nancheck(const NanCheck* __restrict__ Ap, const Members M) {
+    const MemberBlock mb = member_block(M);
+    const NanCheck& A = Ap[mb.m];
+    if (stopped(A.flags)) return;
     bool bad = false;
-    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
-        const double a = ld<TS>(R, e), b = ld<TS>(I, e), c = ld<TS>(Ax, e), d = ld<TS>(Ay, e), f = At[e];
+    for (int64_t e = (int64_t)mb.bx * 256 + threadIdx.x; e < A.n; e += (int64_t)mb.nb * 256) {
+        const double a = ld<TS>(A.R, e), b = ld<TS>(A.I, e), c = ld<TS>(A.Ax, e), d = ld<TS>(A.Ay, e), f = A.At[e];
         if (a != a || b != b || c != c || d != d || f != f) bad = true;
     }
     if (__any(bad) && (threadIdx.x & 63) == 0) {
-        const int step = __atomic_load_n(&flags[1], __ATOMIC_RELAXED);
-        atomicCAS(&flags[0], 0, step > 0 ? step : 1);
+        const int step = __atomic_load_n(&A.flags[1], __ATOMIC_RELAXED);
+        atomicCAS(&A.flags[0], 0, step > 0 ? step : 1);
     }
 }
 
-hipError_t launch_nancheck(const void* R, const void* I, const void* Ax, const void* Ay, const double* At,
-                           int64_t n, int fp32, int32_t* flags, hipStream_t s) {
-    unsigned blocks = (unsigned)((n + 255) / 256);
-    if (blocks > 4096) blocks = 4096;
-    if (fp32) hipLaunchKernelGGL(k_nancheck<float>, dim3(blocks), dim3(256), 0, s, R, I, Ax, Ay, At, n, flags);
-    else hipLaunchKernelGGL(k_nancheck<double>, dim3(blocks), dim3(256), 0, s, R, I, Ax, Ay, At, n, flags);
+int nancheck_blocks(int64_t n) {
+    const int64_t b = (n + 255) / 256;
+    return (int)(b > 4096 ? 4096 : b);
+}
+
+hipError_t launch_nancheck(const NanCheck* d, Members M, int blocks, int fp32, hipStream_t s) {
+    if (fp32) hipLaunchKernelGGL(k_nancheck<float>, dim3(blocks), dim3(256), 0, s, d, M);
+    else hipLaunchKernelGGL(k_nancheck<double>, dim3(blocks), dim3(256), 0, s, d, M);
     return hipGetLastError();
 }
 }  // namespace hqt

@@ -160,12 +160,15 @@ __device__ __forceinline__ void zstage_a(const ZStage& A, int64_t c, bool cok, i
 
 // grid: x = ceil(nxy/64) blocks of 4 waves, 16 columns per wave.
 template <int NZ, typename TS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_zstage(const ZStage A) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_zstage(const ZStage* __restrict__ Ap,
+                                                                                          const Members M) {
+    const MemberBlock mb = member_block(M);
+    const ZStage& A = Ap[mb.m];
     if (stopped(A.flags)) return;
     constexpr int KS = ZG<NZ>::KS;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int m = lane & 15, g = lane >> 4;
-    const int64_t c = (int64_t)blockIdx.x * 64 + wave * 16 + m;
+    const int64_t c = (int64_t)mb.bx * 64 + wave * 16 + m;
     const bool cok = c < A.nxy;
     const int64_t n = A.nxy;
     double Lz[ZG<NZ>::RT][KS], L1[ZG<NZ>::RT][KS];
@@ -255,12 +258,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 
 // ---- post-step: A_t constraint solve and boundary reset (:129-133), O for the winding (:136)
 template <int NZ, typename TS>
-__global__ __launch_bounds__(256) void k_zpost(const ZPost A) {
+__global__ __launch_bounds__(256) void k_zpost(const ZPost* __restrict__ Ap, const Members M) {
+    const MemberBlock mb = member_block(M);
+    const ZPost& A = Ap[mb.m];
     if (stopped(A.flags)) return;
     constexpr int KS = ZG<NZ>::KS;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int m = lane & 15, g = lane >> 4;
-    const int64_t c = (int64_t)blockIdx.x * 64 + wave * 16 + m;
+    const int64_t c = (int64_t)mb.bx * 64 + wave * 16 + m;
     const bool cok = c < A.nxy;
     const int64_t n = A.nxy;
     double Lz[ZG<NZ>::RT][KS], L1[ZG<NZ>::RT][KS];
@@ -312,12 +317,14 @@ __global__ __launch_bounds__(256) void k_zpost(const ZPost A) {
 
 // ---- d_z A_t of the frozen A_t; also advances the device step counter ----------------------
 template <int NZ>
-__global__ __launch_bounds__(256) void k_zprep(const ZPrep A) {
+__global__ __launch_bounds__(256) void k_zprep(const ZPrep* __restrict__ Ap, const Members M) {
+    const MemberBlock mb = member_block(M);
+    const ZPrep& A = Ap[mb.m];
     if (stopped(A.flags)) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int m = lane & 15, g = lane >> 4;
-    const int64_t c = (int64_t)blockIdx.x * 64 + wave * 16 + m;
-    if (A.bump && blockIdx.x == 0 && threadIdx.x == 0) A.flags[1] = A.flags[1] + 1;
+    const int64_t c = (int64_t)mb.bx * 64 + wave * 16 + m;
+    if (A.bump && mb.bx == 0 && threadIdx.x == 0) A.flags[1] = A.flags[1] + 1;
     const bool cok = c < A.nxy;
     double Lz[ZG<NZ>::RT][ZG<NZ>::KS], a[ZG<NZ>::KS], d[ZG<NZ>::KS];
     load_op<NZ>(A.Lz, NZ, NZ, Lz, m, g);
@@ -332,23 +339,20 @@ __global__ __launch_bounds__(256) void k_zprep(const ZPrep A) {
 
 // Per-nz launchers; each hqt_radial_nz_*.hip translation unit instantiates a range of nz.
 template <int NZ>
-hipError_t zstage_launch_nz(const ZStage& A, int fp32, hipStream_t s) {
-    dim3 grid((unsigned)((A.nxy + 63) / 64));
-    if (fp32) hipLaunchKernelGGL((k_zstage<NZ, float>), grid, dim3(256), 0, s, A);
-    else hipLaunchKernelGGL((k_zstage<NZ, double>), grid, dim3(256), 0, s, A);
+hipError_t zstage_launch_nz(const ZStage* d, Members M, int blocks, int fp32, hipStream_t s) {
+    if (fp32) hipLaunchKernelGGL((k_zstage<NZ, float>), dim3(blocks), dim3(256), 0, s, d, M);
+    else hipLaunchKernelGGL((k_zstage<NZ, double>), dim3(blocks), dim3(256), 0, s, d, M);
     return hipGetLastError();
 }
 template <int NZ>
-hipError_t zpost_launch_nz(const ZPost& A, int fp32, hipStream_t s) {
-    dim3 grid((unsigned)((A.nxy + 63) / 64));
-    if (fp32) hipLaunchKernelGGL((k_zpost<NZ, float>), grid, dim3(256), 0, s, A);
-    else hipLaunchKernelGGL((k_zpost<NZ, double>), grid, dim3(256), 0, s, A);
+hipError_t zpost_launch_nz(const ZPost* d, Members M, int blocks, int fp32, hipStream_t s) {
+    if (fp32) hipLaunchKernelGGL((k_zpost<NZ, float>), dim3(blocks), dim3(256), 0, s, d, M);
+    else hipLaunchKernelGGL((k_zpost<NZ, double>), dim3(blocks), dim3(256), 0, s, d, M);
     return hipGetLastError();
 }
 template <int NZ>
-hipError_t zprep_launch_nz(const ZPrep& A, hipStream_t s) {
-    dim3 grid((unsigned)((A.nxy + 63) / 64));
-    hipLaunchKernelGGL((k_zprep<NZ>), grid, dim3(256), 0, s, A);
+hipError_t zprep_launch_nz(const ZPrep* d, Members M, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL((k_zprep<NZ>), dim3(blocks), dim3(256), 0, s, d, M);
     return hipGetLastError();
 }
 

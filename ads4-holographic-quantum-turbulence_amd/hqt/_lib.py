@@ -21,7 +21,8 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "csrc", "libhqt.so")
 EXPORTS = (
     "hqt_ctx_create", "hqt_ctx_destroy", "hqt_set_state", "hqt_get_state", "hqt_step_n", "hqt_sync",
     "hqt_last_winding", "hqt_rhs", "hqt_order_parameter", "hqt_ff", "hqt_fourdif", "hqt_chebdif",
-    "hqt_newton_vy", "hqt_newton_vy_batch", "hqt_vortex_census", "hqt_debug_field", "hqt_profile", "hqt_kernel_stats", "hqt_calibration_copy",
+    "hqt_newton_vy", "hqt_newton_vy_batch", "hqt_vortex_census", "hqt_ensemble_create", "hqt_ensemble_step_n", "hqt_ensemble_sync",
+    "hqt_ensemble_launches", "hqt_ensemble_destroy", "hqt_debug_field", "hqt_profile", "hqt_kernel_stats", "hqt_calibration_copy",
     "hqt_last_error", "hqt_version",
 )
 
@@ -70,6 +71,11 @@ def lib():
     L.hqt_last_winding.argtypes = [vp, dp]
     L.hqt_rhs.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp]
     L.hqt_order_parameter.argtypes = [vp, dp]
+    L.hqt_ensemble_create.argtypes = [C.POINTER(vp), i32, i32, C.POINTER(vp)]
+    L.hqt_ensemble_step_n.argtypes = [vp, i32]
+    L.hqt_ensemble_sync.argtypes = [vp, C.POINTER(i32)]
+    L.hqt_ensemble_launches.argtypes = [vp, C.POINTER(i32)]
+    L.hqt_ensemble_destroy.argtypes = [vp]
     L.hqt_vortex_census.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_int8), dp]
     L.hqt_ff.argtypes = [i32, i32, i32, dp, dp, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), dp]
     L.hqt_fourdif.argtypes = [i32, i32, dp, dp]

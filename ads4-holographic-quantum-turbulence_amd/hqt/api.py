@@ -20,7 +20,7 @@ import numpy as np
 
 from ._lib import (HQT_ERR_ARG, HQT_ERR_NAN, HQT_ERR_NOCONV, HQT_FP64, HQT_MIXED, HQT_NO_GRAPH, HQTError, Params, check, dptr, lib)
 
-__all__ = ["fourdif", "chebdif", "ff", "NewtonRaphson_vy", "NewtonRaphson_vy_batch", "Stepper", "HQTError", "mask_factors"]
+__all__ = ["fourdif", "chebdif", "ff", "NewtonRaphson_vy", "NewtonRaphson_vy_batch", "Stepper", "Ensemble", "HQTError", "mask_factors"]
 
 
 def fourdif(N: int, m: int):
@@ -187,6 +187,9 @@ class Stepper:
 
     # -- life cycle ---------------------------------------------------------------------
     def close(self):
+        ens = getattr(self, "_ensemble", None)
+        if ens is not None and ens() is not None:
+            ens().close()                       # release the context from its ensemble first
         if getattr(self, "_h", None) and self._h.value:
             lib().hqt_ctx_destroy(self._h)
             self._h = C.c_void_p()
@@ -309,3 +312,65 @@ class Stepper:
                 out[buf.value.decode()] = (int(cnt.value), float(tot.value))
             i += 1
         return out
+
+
+class Ensemble:
+    """Independent evolutions stepped together (``hqt_ensemble``): one launch per kernel class
+    per phase over all members, the whole step replayed from one captured hipGraph.
+
+    ``steppers`` are :class:`Stepper` objects with their state set; they may differ in grid,
+    nz, mu, drive, switches and precision.  Each member's result equals stepping it alone.
+    While the ensemble lives, the members' own calls wait for its queued steps."""
+
+    def __init__(self, steppers, graph=True):
+        import weakref
+        self.members = list(steppers)
+        arr = (C.c_void_p * len(self.members))(*[m._h.value for m in self.members])
+        self._h = C.c_void_p()
+        check(lib().hqt_ensemble_create(arr, len(self.members), 0 if graph else HQT_NO_GRAPH, C.byref(self._h)))
+        for m in self.members:
+            m._ensemble = weakref.ref(self)
+
+    @property
+    def launches_per_step(self):
+        n = C.c_int32(0)
+        check(lib().hqt_ensemble_launches(self._h, C.byref(n)))
+        return int(n.value)
+
+    def step_async(self, nsteps=1):
+        check(lib().hqt_ensemble_step_n(self._h, int(nsteps)))
+
+    def nan_steps(self):
+        """Wait; per-member NaN step (0 = none), without raising."""
+        ns = np.zeros(len(self.members), dtype=np.int32)
+        lib().hqt_ensemble_sync(self._h, ns.ctypes.data_as(C.POINTER(C.c_int32)))
+        return ns
+
+    def sync(self):
+        """Wait for queued steps; raise HQTError(HQT_ERR_NAN) if any member's guard tripped."""
+        ns = np.zeros(len(self.members), dtype=np.int32)
+        check(lib().hqt_ensemble_sync(self._h, ns.ctypes.data_as(C.POINTER(C.c_int32))))
+
+    def step(self, nsteps=1):
+        self.step_async(nsteps)
+        self.sync()
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().hqt_ensemble_destroy(self._h)
+            self._h = C.c_void_p()
+            for m in self.members:
+                m._ensemble = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+

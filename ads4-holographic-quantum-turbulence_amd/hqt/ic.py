@@ -44,9 +44,42 @@ def continue_profile(prof: dict, vy_from: float, vy_to: float, nsteps: int = 10,
     return {"Z1": Z1, "Z2": Z2, "Vy1": V, "rhoi": rhoi, "mu": mu if mu is not None else prof["mu"]}
 
 
+def continue_mu(prof: dict, mu_to: float, vy: float = 0.0, dmu: float = 0.1) -> dict:
+    """Newton continuation in ``mu`` at fixed boundary flow ``vy``.  A_t(z=0) = mu is the
+    Dirichlet value the initial guess carries (``NewtonRaphson_vy.m:50-52``), so each step
+    rescales Z2 by mu_new/mu_old (``scripts/legacy/Statinary_state_with_v.m:50``) and solves
+    again; steps of at most ``dmu`` keep the iteration on the superfluid branch down to the
+    critical mu (about 4.07 at vy = 0)."""
+    nz0 = len(prof["Z1"])
+    mu0 = float(prof["mu"])
+    n = max(1, int(math.ceil(abs(mu_to - mu0) / dmu - 1e-12)))
+    Z1, Z2, V = prof["Z1"].copy(), prof["Z2"].copy(), prof["Vy1"].copy()
+    prev = mu0
+    for s in range(1, n + 1):
+        m = mu0 + (mu_to - mu0) * s / n
+        Z2 = Z2 * (m / prev)
+        prev = m
+        Z1, Z2, V, it, err = NewtonRaphson_vy(nz0, 1, 0, 0, Z1, Z2, V, vy)
+        if it >= 1000 or not np.all(np.isfinite(Z1)):
+            raise RuntimeError(f"continue_mu: Newton failed at mu={m:.4f}")
+    _, Dz = chebdif(nz0, 1)
+    rhoi = -(2 * Dz[:, :, 0]) @ Z2
+    return {"Z1": Z1, "Z2": Z2, "Vy1": V, "rhoi": rhoi, "mu": float(mu_to)}
+
+
+_ZERO_FLOW_MU6 = None
+
+
 def zero_flow_profile(mu: float = 6.0) -> dict:
-    """Superfluid stationary profile at zero flow, continued from the kk=7 fixture."""
-    return continue_profile(load_profile(), -2 * math.pi * 7 / 20, 0.0, 10, mu=mu)
+    """Superfluid stationary profile at zero flow: the kk=7 fixture (mu = 6) continued to
+    vy = 0 in 10 steps, then, for another mu, continued in mu (:func:`continue_mu`)."""
+    global _ZERO_FLOW_MU6
+    if _ZERO_FLOW_MU6 is None:
+        _ZERO_FLOW_MU6 = continue_profile(load_profile(), -2 * math.pi * 7 / 20, 0.0, 10, mu=6.0)
+    base = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in _ZERO_FLOW_MU6.items()}
+    if float(mu) == 6.0:
+        return base
+    return continue_mu(base, float(mu))
 
 
 def spline_interp(X, Y, Xi):

@@ -101,6 +101,28 @@ int hqt_order_parameter(hqt_ctx* ctx, double* O);
  * are downloaded. */
 int hqt_vortex_census(hqt_ctx* ctx, int32_t* counts, int8_t* charge, double* Oabs);
 
+/* ---- batched ensemble ------------------------------------------------------------------ */
+/* Many independent evolutions stepped together: the contexts may differ in nx, ny, nz, mu,
+ * boundary drive, winding/filter switches and precision.  Every kernel of the step runs once
+ * per kernel class (precision and padded line length for the x/y passes, nz for the radial
+ * kernels) over all members of that class, and the whole step is one captured hipGraph
+ * (unless flags has HQT_NO_GRAPH).  Each member's result equals hqt_step_n on that context
+ * alone.  A member that hits NaN stops (its later kernels exit at once) while the others go
+ * on.  Members keep their state and accessors; while in an ensemble, their calls first wait
+ * for the ensemble's queued steps.  Serves the batched evolutions of the reference's
+ * parameter scans (scripts/simulation/tevol_driven_cluster.m run per (mu, kk) point); no
+ * single reference function corresponds. */
+typedef struct hqt_ensemble hqt_ensemble;
+int hqt_ensemble_create(hqt_ctx* const* members, int32_t n, int32_t flags, hqt_ensemble** out);
+/* Queue nsteps steps of every member (returns after queueing). */
+int hqt_ensemble_step_n(hqt_ensemble* ens, int32_t nsteps);
+/* Wait; nan_steps (nullable, [n]) = per member NaN step as hqt_sync.  HQT_ERR_NAN if any. */
+int hqt_ensemble_sync(hqt_ensemble* ens, int32_t* nan_steps);
+/* Number of kernel launches per step (the batched plan length). */
+int hqt_ensemble_launches(hqt_ensemble* ens, int32_t* n);
+/* Release the ensemble (members stay valid and are released from it). */
+int hqt_ensemble_destroy(hqt_ensemble* ens);
+
 /* ---- stand-alone operators ---------------------------------------------------------- */
 /* w = ff(x, fx|fy): src/numerics/ff.m:7-11 with the mask of tevol_driven_cluster.m:27-31.
  * x is [nx, ny, nslices], real (x_im == NULL) or complex (x_im = imaginary part);

@@ -220,3 +220,21 @@ def test_diagnostics_countN_matches_reference_case():
     phi = np.sin(x + 0.271) + 1j * np.sin(y + 0.613)
     assert diagnostics.countN(phi) == (4, 2, 2)
     assert len(diagnostics.vortex_positions(phi)) == 4
+
+
+def test_continue_mu_stays_on_superfluid_branch():
+    """Zero-flow profiles continued in mu: each is a Newton fixed point (re-solving takes zero
+    iterations), carries A_t(z=0) = mu, and the condensate grows with mu above the critical
+    value (about 4.07)."""
+    import hqt
+    from hqt import ic
+    prev = 0.0
+    for mu in (4.2, 5.0, 6.0, 8.0):
+        p = ic.zero_flow_profile(mu)
+        assert p["Z2"][-1] == pytest.approx(mu, abs=1e-12)
+        amp = float(np.max(np.abs(p["Z1"])))
+        assert amp > prev + 0.1
+        prev = amp
+        z1, z2, v, it, err = hqt.NewtonRaphson_vy(30, 1, 0, 0, p["Z1"], p["Z2"], p["Vy1"], 0.0)
+        assert it == 0
+

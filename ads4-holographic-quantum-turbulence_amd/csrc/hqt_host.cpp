@@ -31,6 +31,22 @@ namespace {
 inline int fail(int code, const std::string& msg) { return hqt_fail(code, msg); }
 }  // namespace
 
+int64_t ctx_footprint(const hqt_params& p) {
+    const int64_t nxy = (int64_t)p.nx * p.ny, np = nxy * p.nz, nz = p.nz;
+    const int64_t es = p.precision == HQT_MIXED ? 4 : 8;
+    auto r8 = [](int64_t b) { return b == 0 ? 8 : b; };
+    int64_t b = 0;
+    b += 42 * np * es;                                   // state, stage, derivative and filter buffers (TS)
+    b += 4 * np * 8;                                     // At, rho, dzAt, AtR
+    b += nxy * 8 + 2 * nxy * 8 + r8(p.ny * 8) + 8;       // arho, O, part, W
+    b += 2 * nxy * 8 + r8(p.ny * 8) + 16;                // axb, ayb, yph, flags
+    b += (int64_t)NPASS * (int64_t)sizeof(LPass) + (int64_t)sizeof(hqt_ctx::DevDesc);
+    b += 2 * nxy * 8 + nxy + 8 + 2 * (int64_t)census_blocks(nxy) * 8;   // diagnostics
+    b += (nz * nz + (nz - 1) * (nz - 1) + 5 * nz) * 8;   // radial operators
+    b += 6 * ((int64_t)line_geom(p.nx).TL + (int64_t)line_geom(p.ny).TL) * es;   // x/y operator tables
+    return b;
+}
+
 int hqt_sync_streams(hqt_ctx* c) {
     if (c->owner) HIPC(hipStreamSynchronize(c->owner));
     HIPC(hipStreamSynchronize(c->stream));
@@ -47,6 +63,7 @@ int dalloc(hqt_ctx* c, size_t bytes, void** out) {
     e = hipMemset(p, 0, bytes);
     if (e != hipSuccess) { (void)hipFree(p); return fail(HQT_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e)); }
     c->allocs.push_back(p);
+    c->bytes += (int64_t)bytes;
     *out = p;
     return HQT_OK;
 }
@@ -780,108 +797,6 @@ int hqt_newton_vy(int32_t nz, double zh, double small, double D, const double* Z
     const int r = newton_vy(nz, zh, small, D, Z1, Z2, vy1, vy, z1, z2, VY1, &iti, err);
     *it = iti;
     if (r) return fail(HQT_ERR_NOCONV, "NewtonRaphson_vy: no convergence (it=" + std::to_string(iti) + ")");
-    return HQT_OK;
-}
-
-namespace {
-// Device buffers of one hqt_newton_vy_batch call, freed on every return path.
-struct DevScope {
-    std::vector<void*> ptrs;
-    ~DevScope() {
-        for (void* p : ptrs) (void)hipFree(p);
-    }
-    int alloc(size_t bytes, void** out) {
-        void* p = nullptr;
-        const hipError_t e = hipMalloc(&p, bytes);
-        if (e != hipSuccess) return fail(HQT_ERR_OOM, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
-        ptrs.push_back(p);
-        *out = p;
-        return HQT_OK;
-    }
-};
-}  // namespace
-
-int hqt_newton_vy_batch(int32_t nz, double zh, double small, double D, int32_t nb, const double* Z1,
-                        const double* Z2, const double* vy1, const double* vy, int32_t it_max, double tol,
-                        double* z1, double* z2, double* VY1, int32_t* it, double* err, uint8_t* conv) {
-    if (nz < 3 || nz > 64) return fail(HQT_ERR_ARG, "hqt_newton_vy_batch: nz must lie in [3, 64]");
-    if (nb < 1) return fail(HQT_ERR_ARG, "hqt_newton_vy_batch: nb must be positive");
-    if (!Z1 || !Z2 || !vy1 || !vy || !z1 || !z2 || !VY1 || !it || !err || !conv)
-        return fail(HQT_ERR_ARG, "hqt_newton_vy_batch: NULL argument");
-    if (it_max <= 0) it_max = 1000;
-    if (!(tol > 0)) tol = 1e-6;
-    // radial grid and operators (:12-21), as in the host solver
-    std::vector<double> zc, DM;
-    chebdif(nz, 1, zc, DM);
-    const double zf = 1 - small, dlz = zf - D, pz = 2 / dlz;
-    const size_t nn = (size_t)nz * nz, n3 = 3 * (size_t)nz;
-    std::vector<double> host(2 * nn + nz);
-    double* Lz = host.data();
-    double* Lz2 = Lz + nn;
-    double* zz = Lz2 + nn;
-    for (int i = 0; i < nz; ++i) zz[i] = (zc[i] - 1) * dlz / 2 + zf;
-    for (int i = 0; i < nz; ++i)
-        for (int j = 0; j < nz; ++j) Lz[(size_t)i * nz + j] = pz * DM[i + (size_t)nz * j];
-    for (int i = 0; i < nz; ++i)
-        for (int j = 0; j < nz; ++j) {
-            double s = 0;
-            for (int k = 0; k < nz; ++k) s += Lz[(size_t)i * nz + k] * Lz[(size_t)k * nz + j];
-            Lz2[(size_t)i * nz + j] = s;
-        }
-    const bool lds = newton_lds_resident(nz);
-    const size_t vec = (size_t)nz * nb;
-    DevScope ds;
-    double *dops, *din, *dres, *dout, *dwork = nullptr;
-    NewtonJob* djobs;
-    int rc;
-    if ((rc = ds.alloc(host.size() * 8, (void**)&dops))) return rc;
-    if ((rc = ds.alloc(3 * vec * 8, (void**)&din))) return rc;
-    if ((rc = ds.alloc(3 * vec * 8, (void**)&dres))) return rc;
-    if ((rc = ds.alloc(3 * (size_t)nb * 8, (void**)&dout))) return rc;
-    if ((rc = ds.alloc((size_t)nb * sizeof(NewtonJob), (void**)&djobs))) return rc;
-    if (!lds && (rc = ds.alloc((size_t)nb * n3 * n3 * 8, (void**)&dwork))) return rc;
-    HIPC(hipMemcpy(dops, host.data(), host.size() * 8, hipMemcpyHostToDevice));
-    HIPC(hipMemcpy(din, Z1, vec * 8, hipMemcpyHostToDevice));
-    HIPC(hipMemcpy(din + vec, Z2, vec * 8, hipMemcpyHostToDevice));
-    HIPC(hipMemcpy(din + 2 * vec, vy1, vec * 8, hipMemcpyHostToDevice));
-    std::vector<NewtonJob> jobs(nb);
-    for (int b = 0; b < nb; ++b) {
-        NewtonJob& J = jobs[b];
-        J.nz = nz;
-        J.it_max = it_max;
-        J.zh = zh;
-        J.tol = tol;
-        J.vy = vy[b];
-        J.Lz = dops;
-        J.Lz2 = dops + nn;
-        J.zz = dops + 2 * nn;
-        J.Z1 = din + (size_t)b * nz;
-        J.Z2 = din + vec + (size_t)b * nz;
-        J.vy1 = din + 2 * vec + (size_t)b * nz;
-        J.z1 = dres + (size_t)b * nz;
-        J.z2 = dres + vec + (size_t)b * nz;
-        J.VY1 = dres + 2 * vec + (size_t)b * nz;
-        J.work = lds ? nullptr : dwork + (size_t)b * n3 * n3;
-        J.out = dout + 3 * (size_t)b;
-    }
-    HIPC(hipMemcpy(djobs, jobs.data(), jobs.size() * sizeof(NewtonJob), hipMemcpyHostToDevice));
-    HIPC(launch_newton(djobs, nb, nz, nullptr));
-    HIPC(hipStreamSynchronize(nullptr));
-    HIPC(hipMemcpy(z1, dres, vec * 8, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(z2, dres + vec, vec * 8, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(VY1, dres + 2 * vec, vec * 8, hipMemcpyDeviceToHost));
-    std::vector<double> o(3 * (size_t)nb);
-    HIPC(hipMemcpy(o.data(), dout, o.size() * 8, hipMemcpyDeviceToHost));
-    int nconv = 0;
-    for (int b = 0; b < nb; ++b) {
-        it[b] = (int32_t)o[3 * b];
-        err[b] = o[3 * b + 1];
-        conv[b] = o[3 * b + 2] != 0 ? 1 : 0;
-        nconv += conv[b];
-    }
-    if (nconv != nb)
-        return fail(HQT_ERR_NOCONV, "hqt_newton_vy_batch: " + std::to_string(nb - nconv) + " of " + std::to_string(nb) +
-                                        " problems did not converge");
     return HQT_OK;
 }
 

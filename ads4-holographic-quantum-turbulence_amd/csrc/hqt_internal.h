@@ -39,6 +39,7 @@ struct hqt_ctx {
     size_t es = 8;
     hipStream_t stream = nullptr;
     std::vector<void*> allocs;
+    int64_t bytes = 0;               // device bytes allocated by this context
 
     void *ZR, *ZI, *ZAx, *ZAy;
     void *SA[4], *SB[4], *AC[4];
@@ -80,3 +81,21 @@ struct hqt_ctx {
 
 // Waits for the context's own stream and, while it is an ensemble member, the ensemble's.
 int hqt_sync_streams(hqt_ctx* c);
+
+// Batched stationary Newton solves on a stream, with a device workspace that grows on demand
+// and is reused across calls (no allocation once it is large enough).  Inputs and outputs are
+// host arrays as in hqt_newton_vy_batch; returns HQT_OK or an error status (non-convergence is
+// reported per problem in conv, not as a status).
+struct NewtonWork {
+    unsigned char* buf = nullptr;
+    size_t cap = 0;
+    ~NewtonWork();
+};
+int newton_batch_run(hipStream_t s, NewtonWork& w, int nz, double zh, double small, double D, int nb,
+                     const double* Z1, const double* Z2, const double* vy1, const double* vy, int it_max, double tol,
+                     double* z1, double* z2, double* VY1, int32_t* it, double* err, uint8_t* conv);
+
+// Device bytes a context with these parameters allocates (state, stage buffers, operator
+// tables, descriptors), as hqt_ctx_create would; used for admission by HBM budget.
+int64_t ctx_footprint(const hqt_params& p);
+

@@ -5,10 +5,10 @@ Python mirror of the reference's MATLAB interface (``fourdif``, ``chebdif``, ``f
 ``NewtonRaphson_vy``, the ``F1..F4`` closures and the RK4 loop body as :class:`Stepper`).
 """
 
-from ._lib import HQT_ERR_ARG, HQT_ERR_HIP, HQT_ERR_NAN, HQT_ERR_NOCONV, HQT_ERR_OOM, HQT_ERR_STATE, HQTError, LIB_PATH, lib
-from .api import Ensemble, NewtonRaphson_vy, NewtonRaphson_vy_batch, Stepper, chebdif, ff, fourdif, mask_factors
+from ._lib import HQT_ERR_ARG, HQT_ERR_HIP, HQT_ERR_NAN, HQT_ERR_NOCONV, HQT_ERR_OOM, HQT_ERR_TIMEOUT, HQT_ERR_STATE, HQTError, LIB_PATH, lib
+from .api import Ensemble, NewtonRaphson_vy, NewtonRaphson_vy_batch, Server, Stepper, footprint, chebdif, ff, fourdif, mask_factors
 
 __all__ = [
-    "Stepper", "Ensemble", "fourdif", "chebdif", "ff", "NewtonRaphson_vy", "NewtonRaphson_vy_batch", "mask_factors", "HQTError", "lib", "LIB_PATH",
+    "Stepper", "Ensemble", "Server", "footprint", "fourdif", "chebdif", "ff", "NewtonRaphson_vy", "NewtonRaphson_vy_batch", "mask_factors", "HQTError", "lib", "LIB_PATH",
     "HQT_ERR_ARG", "HQT_ERR_NAN", "HQT_ERR_HIP", "HQT_ERR_OOM", "HQT_ERR_STATE", "HQT_ERR_NOCONV",
 ]

@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-HQT_OK, HQT_ERR_ARG, HQT_ERR_NAN, HQT_ERR_HIP, HQT_ERR_OOM, HQT_ERR_STATE, HQT_ERR_NOCONV = range(7)
+HQT_OK, HQT_ERR_ARG, HQT_ERR_NAN, HQT_ERR_HIP, HQT_ERR_OOM, HQT_ERR_STATE, HQT_ERR_NOCONV, HQT_ERR_TIMEOUT = range(8)
 HQT_FP64, HQT_MIXED = 0, 1
 HQT_NO_GRAPH = 1
 
@@ -22,7 +22,9 @@ EXPORTS = (
     "hqt_ctx_create", "hqt_ctx_destroy", "hqt_set_state", "hqt_get_state", "hqt_step_n", "hqt_sync",
     "hqt_last_winding", "hqt_rhs", "hqt_order_parameter", "hqt_ff", "hqt_fourdif", "hqt_chebdif",
     "hqt_newton_vy", "hqt_newton_vy_batch", "hqt_vortex_census", "hqt_ensemble_create", "hqt_ensemble_step_n", "hqt_ensemble_sync",
-    "hqt_ensemble_launches", "hqt_ensemble_destroy", "hqt_debug_field", "hqt_profile", "hqt_kernel_stats", "hqt_calibration_copy",
+    "hqt_ensemble_launches", "hqt_ensemble_destroy", "hqt_server_create", "hqt_server_destroy",
+    "hqt_server_submit_evolution", "hqt_server_submit_newton", "hqt_server_wait", "hqt_server_evolution_result",
+    "hqt_server_newton_result", "hqt_server_stats", "hqt_server_counters", "hqt_ctx_footprint", "hqt_ctx_device_bytes", "hqt_debug_field", "hqt_profile", "hqt_kernel_stats", "hqt_calibration_copy",
     "hqt_last_error", "hqt_version",
 )
 
@@ -76,6 +78,18 @@ def lib():
     L.hqt_ensemble_sync.argtypes = [vp, C.POINTER(i32)]
     L.hqt_ensemble_launches.argtypes = [vp, C.POINTER(i32)]
     L.hqt_ensemble_destroy.argtypes = [vp]
+    L.hqt_server_create.argtypes = [C.c_int64, i32, C.POINTER(vp)]
+    L.hqt_server_destroy.argtypes = [vp]
+    L.hqt_server_submit_evolution.argtypes = [vp, C.POINTER(Params), dp, dp, dp, dp, dp, i32, C.POINTER(C.c_int64)]
+    L.hqt_server_submit_newton.argtypes = [vp, i32, C.c_double, C.c_double, C.c_double, dp, dp, dp, C.c_double,
+                                           C.POINTER(C.c_int64)]
+    L.hqt_server_wait.argtypes = [vp, C.c_int64, C.c_double, dp]
+    L.hqt_server_evolution_result.argtypes = [vp, C.c_int64, dp, dp, dp, dp, dp, C.POINTER(i32)]
+    L.hqt_server_newton_result.argtypes = [vp, C.c_int64, dp, dp, dp, C.POINTER(i32), dp]
+    L.hqt_server_stats.argtypes = [vp, i32, dp, dp, C.POINTER(C.c_int64)]
+    L.hqt_server_counters.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.hqt_ctx_footprint.argtypes = [C.POINTER(Params), C.POINTER(C.c_int64)]
+    L.hqt_ctx_device_bytes.argtypes = [vp, C.POINTER(C.c_int64)]
     L.hqt_vortex_census.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_int8), dp]
     L.hqt_ff.argtypes = [i32, i32, i32, dp, dp, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), dp]
     L.hqt_fourdif.argtypes = [i32, i32, dp, dp]

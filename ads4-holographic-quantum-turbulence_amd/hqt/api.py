@@ -18,9 +18,9 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (HQT_ERR_ARG, HQT_ERR_NAN, HQT_ERR_NOCONV, HQT_FP64, HQT_MIXED, HQT_NO_GRAPH, HQTError, Params, check, dptr, lib)
+from ._lib import (HQT_ERR_ARG, HQT_ERR_NAN, HQT_ERR_NOCONV, HQT_ERR_TIMEOUT, HQT_FP64, HQT_MIXED, HQT_NO_GRAPH, HQTError, Params, check, dptr, lib)
 
-__all__ = ["fourdif", "chebdif", "ff", "NewtonRaphson_vy", "NewtonRaphson_vy_batch", "Stepper", "Ensemble", "HQTError", "mask_factors"]
+__all__ = ["fourdif", "chebdif", "ff", "NewtonRaphson_vy", "NewtonRaphson_vy_batch", "Stepper", "Ensemble", "Server", "footprint", "make_params", "HQTError", "mask_factors"]
 
 
 def fourdif(N: int, m: int):
@@ -148,6 +148,39 @@ def _c128F(a, shape):
     return np.asfortranarray(a)
 
 
+def make_params(nx, ny, nz, Lx, Ly, mu=6.0, dt=0.02, winding=True, filter=True, precision="fp64", ax_b=0.0,
+                ay_b=0.0, graph=True):
+    """``hqt_params`` for a run, plus the arrays its boundary pointers borrow (keep them alive
+    while the params are in use)."""
+    p = Params()
+    p.nx, p.ny, p.nz = int(nx), int(ny), int(nz)
+    p.Lx, p.Ly, p.mu, p.dt = float(Lx), float(Ly), float(mu), float(dt)
+    p.winding = 1 if winding else 0
+    p.filter = 1 if filter else 0
+    if precision not in ("fp64", "mixed"):
+        raise HQTError(HQT_ERR_ARG, "precision must be 'fp64' or 'mixed'")
+    p.precision = HQT_FP64 if precision == "fp64" else HQT_MIXED
+    p.flags = 0 if graph else HQT_NO_GRAPH
+    keep = []
+    for name, val in (("ax_b", ax_b), ("ay_b", ay_b)):
+        if np.ndim(val) == 0:
+            setattr(p, name + "_const", float(val))
+            setattr(p, name, None)
+        else:
+            arr = np.asfortranarray(np.broadcast_to(np.asarray(val, dtype=np.float64), (p.nx, p.ny)))
+            keep.append(arr)
+            setattr(p, name, dptr(arr))
+    return p, keep
+
+
+def footprint(nx, ny, nz, precision="fp64"):
+    """Device bytes a context of this size allocates (``hqt_ctx_footprint``)."""
+    p, _ = make_params(nx, ny, nz, 50.0, 50.0, precision=precision)
+    b = C.c_int64(0)
+    check(lib().hqt_ctx_footprint(C.byref(p), C.byref(b)))
+    return int(b.value)
+
+
 class Stepper:
     """A device-resident AdS4 evolution (one ``hqt_ctx``).
 
@@ -163,25 +196,7 @@ class Stepper:
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.shape = (self.nx, self.ny, self.nz)
         self.precision = precision
-        p = Params()
-        p.nx, p.ny, p.nz = self.nx, self.ny, self.nz
-        p.Lx, p.Ly, p.mu, p.dt = float(Lx), float(Ly), float(mu), float(dt)
-        p.winding = 1 if winding else 0
-        p.filter = 1 if filter else 0
-        if precision not in ("fp64", "mixed"):
-            raise HQTError(HQT_ERR_ARG, "precision must be 'fp64' or 'mixed'")
-        p.precision = HQT_FP64 if precision == "fp64" else HQT_MIXED
-        p.flags = 0 if graph else HQT_NO_GRAPH
-        keep = []
-        for name in ("ax_b", "ay_b"):
-            val = ax_b if name == "ax_b" else ay_b
-            if np.ndim(val) == 0:
-                setattr(p, name + "_const", float(val))
-                setattr(p, name, None)
-            else:
-                arr = np.asfortranarray(np.broadcast_to(np.asarray(val, dtype=np.float64), (self.nx, self.ny)))
-                keep.append(arr)
-                setattr(p, name, dptr(arr))
+        p, keep = make_params(nx, ny, nz, Lx, Ly, mu, dt, winding, filter, precision, ax_b, ay_b, graph)
         self._h = C.c_void_p()
         check(lib().hqt_ctx_create(C.byref(p), C.byref(self._h)))
 
@@ -270,6 +285,12 @@ class Stepper:
 
     def F4(self, y1, y2, y3, y4):
         return self.rhs(y1, y2, y3, y4)[3]
+
+    @property
+    def device_bytes(self):
+        b = C.c_int64(0)
+        check(lib().hqt_ctx_device_bytes(self._h, C.byref(b)))
+        return int(b.value)
 
     def order_parameter(self):
         """O = (func_Lz(Z1))(:,:,end) of the current state, complex ``[nx, ny]``."""
@@ -361,6 +382,94 @@ class Ensemble:
             self._h = C.c_void_p()
             for m in self.members:
                 m._ensemble = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Server:
+    """The serving loop (``hqt_server_*``): evolution segments and stationary Newton solves
+    submitted at any time, admitted by HBM budget and batched as they arrive.
+
+    ``submit_evolution`` takes the :class:`Stepper` arguments plus an initial state and a step
+    count; ``submit_newton`` takes ``NewtonRaphson_vy``'s arguments.  ``result(id)`` waits and
+    returns ``(status, latency_s, payload)``: the state tuple (evolution) or
+    ``(z1, z2, VY1, it, err)`` (Newton)."""
+
+    def __init__(self, hbm_budget, quantum=25):
+        self._h = C.c_void_p()
+        self._kind = {}
+        self._shape = {}
+        check(lib().hqt_server_create(int(hbm_budget), int(quantum), C.byref(self._h)))
+
+    def submit_evolution(self, state, nsteps, nx, ny, nz, Lx, Ly, **kw):
+        p, keep = make_params(nx, ny, nz, Lx, Ly, **kw)
+        s = (int(nx), int(ny), int(nz))
+        Z1, Z2, Z3, Z4, rh = state
+        z1 = _c128F(Z1, s)
+        args = [dptr(z1)] + [dptr(_f64F(a, s)) for a in (Z2, Z3, Z4, rh)]
+        rid = C.c_int64(0)
+        check(lib().hqt_server_submit_evolution(self._h, C.byref(p), *args, int(nsteps), C.byref(rid)))
+        self._kind[rid.value] = "evo"
+        self._shape[rid.value] = s
+        return int(rid.value)
+
+    def submit_newton(self, nz, zh, small, D, Z1, Z2, vy1, vy):
+        a, b, v = (np.ascontiguousarray(np.ravel(x), dtype=np.float64) for x in (Z1, Z2, vy1))
+        rid = C.c_int64(0)
+        check(lib().hqt_server_submit_newton(self._h, int(nz), float(zh), float(small), float(D), dptr(a), dptr(b),
+                                             dptr(v), float(vy), C.byref(rid)))
+        self._kind[rid.value] = "newton"
+        self._shape[rid.value] = int(nz)
+        return int(rid.value)
+
+    def result(self, rid, timeout=-1.0):
+        lat = C.c_double(0.0)
+        st = lib().hqt_server_wait(self._h, int(rid), float(timeout), C.byref(lat))
+        if st == HQT_ERR_TIMEOUT or st == HQT_ERR_ARG:
+            check(st)
+        kind = self._kind.pop(rid)
+        shp = self._shape.pop(rid)
+        if kind == "evo":
+            Z1 = np.empty(shp, dtype=np.complex128, order="F")
+            Z2, Z3, Z4, rh = (np.empty(shp, order="F") for _ in range(4))
+            ns = C.c_int32(0)
+            lib().hqt_server_evolution_result(self._h, int(rid), dptr(Z1), dptr(Z2), dptr(Z3), dptr(Z4), dptr(rh),
+                                              C.byref(ns))
+            return st, lat.value, (Z1, Z2, Z3, Z4, rh)
+        z1, z2, v = np.empty(shp), np.empty(shp), np.empty(shp)
+        it = C.c_int32(0)
+        err = C.c_double(0.0)
+        lib().hqt_server_newton_result(self._h, int(rid), dptr(z1), dptr(z2), dptr(v), C.byref(it), C.byref(err))
+        return st, lat.value, (z1, z2, v, int(it.value), float(err.value))
+
+    def stats(self, kind=2):
+        """``(p50, p99, completed)`` latency over completed requests: kind 0 evolutions, 1
+        Newton solves, 2 both."""
+        p50, p99, n = C.c_double(0), C.c_double(0), C.c_int64(0)
+        check(lib().hqt_server_stats(self._h, int(kind), C.byref(p50), C.byref(p99), C.byref(n)))
+        return p50.value, p99.value, int(n.value)
+
+    def counters(self):
+        out = (C.c_int64 * 8)()
+        check(lib().hqt_server_counters(self._h, out))
+        keys = ("admitted", "evolutions_done", "newton_done", "ensemble_builds", "newton_batches", "peak_bytes",
+                "peak_resident", "steps")
+        return dict(zip(keys, [int(x) for x in out]))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().hqt_server_destroy(self._h)
+            self._h = C.c_void_p()
 
     def __del__(self):
         try:

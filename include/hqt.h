@@ -34,6 +34,7 @@ extern "C" {
 #define HQT_ERR_OOM     4   /* device allocation failed */
 #define HQT_ERR_STATE   5   /* call out of order (e.g. step before set_state) */
 #define HQT_ERR_NOCONV  6   /* Newton did not converge within it_max */
+#define HQT_ERR_TIMEOUT 7   /* hqt_server_wait: the request is not done yet */
 
 #define HQT_NO_GRAPH 1      /* hqt_params.flags */
 
@@ -122,6 +123,47 @@ int hqt_ensemble_sync(hqt_ensemble* ens, int32_t* nan_steps);
 int hqt_ensemble_launches(hqt_ensemble* ens, int32_t* n);
 /* Release the ensemble (members stay valid and are released from it). */
 int hqt_ensemble_destroy(hqt_ensemble* ens);
+
+/* ---- serving loop ------------------------------------------------------------------------ */
+/* A host scheduler thread serving two request types with continuous batching:
+ *   - evolution segments: a context's parameters, initial state and step count
+ *     (tevol_driven_cluster.m:117-187 for nsteps steps, then the checkpoint download :178-181);
+ *   - stationary Newton solves (src/solvers/NewtonRaphson_vy.m:1-105, it_max 1000, tol 1e-6).
+ * Evolutions are admitted in arrival order while the sum of their device footprints
+ * (hqt_ctx_footprint) stays within hbm_budget bytes; admitted ones stay device-resident and
+ * advance together as one ensemble, `quantum` steps per scheduling round, and leave (state
+ * downloaded, memory freed) when done or when their NaN guard trips.  Newton requests pending
+ * at a round are solved as batches on a second stream while the round's steps run.  Every
+ * result equals what the request gives alone (hqt_step_n / hqt_newton_vy_batch).  Requests
+ * copy their inputs at submission.  No reference counterpart: the reference runs one script
+ * per parameter point. */
+typedef struct hqt_server hqt_server;
+int hqt_server_create(int64_t hbm_budget, int32_t quantum, hqt_server** out);
+/* Stops the scheduler; requests not finished end with HQT_ERR_STATE. */
+int hqt_server_destroy(hqt_server* srv);
+int hqt_server_submit_evolution(hqt_server* srv, const hqt_params* p, const double* Z1, const double* Z2,
+                                const double* Z3, const double* Z4, const double* rhoi, int32_t nsteps,
+                                int64_t* id);
+int hqt_server_submit_newton(hqt_server* srv, int32_t nz, double zh, double small, double D, const double* Z1,
+                             const double* Z2, const double* vy1, double vy, int64_t* id);
+/* Wait until request id is done (timeout_s < 0: no limit; HQT_ERR_TIMEOUT otherwise) and
+ * return its status; *latency_s = completion - submission on the host clock. */
+int hqt_server_wait(hqt_server* srv, int64_t id, double timeout_s, double* latency_s);
+/* Results of a done request (then released).  Evolution: the state after its steps (or at
+ * the NaN stop; *nan_step as hqt_sync).  Newton: z1, z2, VY1 ([nz]), it, err. */
+int hqt_server_evolution_result(hqt_server* srv, int64_t id, double* Z1, double* Z2, double* Z3, double* Z4,
+                                double* rhoi, int32_t* nan_step);
+int hqt_server_newton_result(hqt_server* srv, int64_t id, double* z1, double* z2, double* VY1, int32_t* it,
+                             double* err);
+/* Latency percentiles (nearest rank, seconds) over completed requests of kind 0 (evolution),
+ * 1 (Newton) or 2 (both). */
+int hqt_server_stats(hqt_server* srv, int32_t kind, double* p50, double* p99, int64_t* completed);
+/* out[8]: evolutions admitted, evolutions done, Newton done, ensemble (re)builds, Newton
+ * batches, peak resident device bytes, peak resident evolutions, ensemble steps run. */
+int hqt_server_counters(hqt_server* srv, int64_t* out);
+/* Device bytes a context with parameters p allocates, and those a context holds. */
+int hqt_ctx_footprint(const hqt_params* p, int64_t* bytes);
+int hqt_ctx_device_bytes(hqt_ctx* ctx, int64_t* bytes);
 
 /* ---- stand-alone operators ---------------------------------------------------------- */
 /* w = ff(x, fx|fy): src/numerics/ff.m:7-11 with the mask of tevol_driven_cluster.m:27-31.

@@ -238,3 +238,23 @@ def test_continue_mu_stays_on_superfluid_branch():
         z1, z2, v, it, err = hqt.NewtonRaphson_vy(30, 1, 0, 0, p["Z1"], p["Z2"], p["Vy1"], 0.0)
         assert it == 0
 
+
+def test_footprint_formula_scales_with_points():
+    """hqt_ctx_footprint (host only): 42 state-precision arrays plus 4 FP64 arrays per grid
+    point, plus per-plane, per-line and constant terms; mixed precision halves the first part."""
+    import hqt
+    a = hqt.footprint(64, 64, 12, "fp64")
+    b = hqt.footprint(64, 64, 24, "fp64")
+    per_z = b - a
+    assert per_z == 12 * 64 * 64 * (42 * 8 + 32) + ((24 * 24 + 23 * 23 + 120) - (12 * 12 + 11 * 11 + 60)) * 8
+    m = hqt.footprint(64, 64, 12, "mixed")
+    assert a - m > 64 * 64 * 12 * 42 * 4
+
+
+def test_server_rejects_bad_arguments():
+    import ctypes as C
+    import hqt
+    h = C.c_void_p()
+    assert hqt.lib().hqt_server_create(0, 10, C.byref(h)) == hqt.HQT_ERR_ARG
+    assert hqt.lib().hqt_server_create(1 << 30, 0, C.byref(h)) == hqt.HQT_ERR_ARG
+

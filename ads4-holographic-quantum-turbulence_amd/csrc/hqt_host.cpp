@@ -13,7 +13,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "hqt_internal.h"
@@ -92,6 +95,14 @@ int download_as(double* dst, const void* src, size_t n, int dt) {
 }
 
 // ---- line-pass descriptor helpers --------------------------------------------------------
+// Lines per work-group (16 or 32) and the grid that follows from it.
+void set_lb(LPass& P, int lb, int dir) {
+    P.lb = lb;
+    P.nxc = (P.nx + lb - 1) / lb;
+    if (dir == 0) P.ntiles = (int)(((int64_t)P.ny * P.nz + lb - 1) / lb);
+    else P.ntiles = P.nxc * P.nz;
+}
+
 LPass base_pass(int N, int nx, int ny, int nz, int dir, void* const* ops, int32_t* flags) {
     LPass P;
     std::memset(&P, 0, sizeof(P));
@@ -102,9 +113,7 @@ LPass base_pass(int N, int nx, int ny, int nz, int dir, void* const* ops, int32_
     P.TL = lg.TL;
     P.S = P.Np + 1;
     P.nx = nx; P.ny = ny; P.nz = nz;
-    P.nxc = (nx + 15) / 16;
-    if (dir == 0) P.ntiles = (int)(((int64_t)ny * nz + 15) / 16);
-    else P.ntiles = P.nxc * nz;
+    set_lb(P, 16, dir);
     for (int i = 0; i < OP_COUNT; ++i) P.ops[i] = ops[i];
     P.flags = flags;
     return P;
@@ -191,8 +200,8 @@ int line_kind(int name) {
 int line(hqt_ctx* c, int pass, int dir, int name) {
     const LPass& H = c->hp[pass];
     return run(c, name, [&] {
-        return launch_line_pass(c->dp + pass, kOne, H.ntiles, H.ngroups, line_pass_lds_bytes(H, c->fp32), H.Mp / 64, dir,
-                                c->fp32, line_kind(name), c->stream);
+        return launch_line_pass(c->dp + pass, kOne, H.ntiles, H.ngroups, line_pass_lds_bytes(H, c->fp32), H.Mp / 64,
+                                H.lb / 16, dir, c->fp32, line_kind(name), c->stream);
     });
 }
 
@@ -207,6 +216,68 @@ int upload_desc(hqt_ctx* c) {
     h.wind = c->wind;
     h.nan = c->nan;
     HIPC(hipMemcpy(c->dd, &h, sizeof(h), hipMemcpyHostToDevice));
+    return HQT_OK;
+}
+
+int pass_dir(int p) { return ((p >= PY1 && p <= PY4) || p == PYC || p == PYF || p == PYR) ? 1 : 0; }
+
+constexpr size_t kLdsCap = 160 * 1024;
+
+// Lines per work-group for every line pass: forced by the params flags, else measured once per
+// (pass, precision, grid) and kept for the process.  Both choices give bitwise identical
+// results (each output's MFMA accumulation order is the same), so tuning never changes numbers.
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int, int>, int> g_tune;
+
+int tune_lines(hqt_ctx* c) {
+    const int forced = (c->p.flags & HQT_LINES32) ? 32 : (c->p.flags & HQT_LINES16) ? 16 : 0;
+    hipEvent_t ea = nullptr, eb = nullptr;
+    int rc = HQT_OK;
+    for (int p = 0; p < NPASS && !rc; ++p) {
+        LPass& H = c->hp[p];
+        const int dir = pass_dir(p);
+        int lb = forced;
+        const auto key = std::make_tuple(p, c->fp32, c->nx, c->ny, c->nz);
+        if (!lb) {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tune.find(key);
+            if (it != g_tune.end()) lb = it->second;
+        }
+        if (!lb) {
+            if (!ea && (hipEventCreate(&ea) != hipSuccess || hipEventCreate(&eb) != hipSuccess)) {
+                rc = fail(HQT_ERR_HIP, "hipEventCreate");
+                break;
+            }
+            float best = 0;
+            for (int cand : {16, 32}) {
+                set_lb(H, cand, dir);
+                const size_t lds = line_pass_lds_bytes(H, c->fp32);
+                if (lds > kLdsCap) continue;
+                float ms = 0;
+                hipError_t e = hipMemcpy(c->dp + p, &H, sizeof(H), hipMemcpyHostToDevice);
+                for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {
+                    if (rep == 1) e = hipEventRecord(ea, c->stream);
+                    if (e == hipSuccess)
+                        e = launch_line_pass(c->dp + p, kOne, H.ntiles, H.ngroups, lds, H.Mp / 64, cand / 16, dir, c->fp32,
+                                             0, c->stream);
+                }
+                if (e == hipSuccess) e = hipEventRecord(eb, c->stream);
+                if (e == hipSuccess) e = hipEventSynchronize(eb);
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, ea, eb);
+                if (e != hipSuccess) { rc = fail(HQT_ERR_HIP, std::string("line-pass tuning: ") + hipGetErrorString(e)); break; }
+                if (!lb || ms < best) { best = ms; lb = cand; }
+            }
+            if (rc) break;
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tune[key] = lb;
+        }
+        set_lb(H, lb, dir);
+        if (line_pass_lds_bytes(H, c->fp32) > kLdsCap) set_lb(H, 16, dir);
+    }
+    if (ea) (void)hipEventDestroy(ea);
+    if (eb) (void)hipEventDestroy(eb);
+    if (rc) return rc;
+    HIPC(hipMemcpy(c->dp, c->hp, sizeof(c->hp), hipMemcpyHostToDevice));
     return HQT_OK;
 }
 
@@ -489,6 +560,7 @@ int hqt_ctx_create(const hqt_params* p, hqt_ctx** out) {
     hqt_params tmp = *p;
     c->p = tmp;     // keep the caller's boundary pointers only during build_context
     rc = build_context(c);
+    if (!rc) rc = tune_lines(c);
     c->p.ax_b = nullptr;
     c->p.ay_b = nullptr;
     if (rc) { hqt_ctx_destroy(c); return rc; }
@@ -761,10 +833,10 @@ int hqt_ff(int32_t nx, int32_t ny, int32_t nslices, const double* x_re, const do
         if (hipMemcpy(dp, hp, sizeof(hp), hipMemcpyHostToDevice) != hipSuccess) { rc = fail(HQT_ERR_HIP, "hipMemcpy pass"); break; }
         if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) { rc = fail(HQT_ERR_HIP, "stream"); break; }
         hipError_t e = launch_line_pass(dp, kOne, hp[0].ntiles, hp[0].ngroups, line_pass_lds_bytes(hp[0], 0), hp[0].Mp / 64,
-                                        0, 0, 2, s);
+                                        1, 0, 0, 2, s);
         if (e == hipSuccess)
             e = launch_line_pass(dp + 1, kOne, hp[1].ntiles, hp[1].ngroups, line_pass_lds_bytes(hp[1], 0), hp[1].Mp / 64, 1,
-                                 0, 2, s);
+                                 1, 0, 2, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) { rc = fail(HQT_ERR_HIP, std::string("hqt_ff: ") + hipGetErrorString(e)); break; }
         if (hipMemcpy(w, out, n * 8, hipMemcpyDeviceToHost) != hipSuccess) { rc = fail(HQT_ERR_HIP, "hipMemcpy w"); break; }

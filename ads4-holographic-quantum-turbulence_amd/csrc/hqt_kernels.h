@@ -62,8 +62,9 @@ struct LPass {
     int32_t N, Np, S;          // line length, padded length (multiple of 16), LDS row stride
     int32_t Mp, TL;            // output positions padded to 64 (4 waves x Q tiles), Toeplitz table length
     int32_t nx, ny, nz;
-    int32_t ntiles;            // line tiles (grid.x)
-    int32_t nxc;               // dir 1: x chunks of 16
+    int32_t lb;                // lines per work-group: 16 or 32
+    int32_t ntiles;            // line tiles of lb lines (grid.x)
+    int32_t nxc;               // dir 1: x chunks of lb
     int32_t ngroups;           // grid.y
     LGroup g[LP_MAXG];
     const void* ops[LP_MAXOPS]; // Toeplitz tables: tab[i - j + Np + 4] = T[i][j], TL entries
@@ -163,8 +164,8 @@ struct Census {
 // Line passes: grid.y = ngroups (members with fewer groups skip the extra rows); lds bytes
 // must cover every member; Q = Mp/64 is shared by the members of one launch.
 // kind: 0 RHS stage, 1 constraint, 2 filter (separate instantiations for profiling).
-hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, size_t lds, int Q, int dir, int fp32,
-                            int kind, hipStream_t s);
+hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, size_t lds, int Q, int nt, int dir,
+                            int fp32, int kind, hipStream_t s);
 size_t line_pass_lds_bytes(const LPass& P, int fp32);
 // Radial kernels: grid.x per member = zblocks(nxy); nz is shared by the members of one launch.
 inline int zblocks(int64_t nxy) { return (int)((nxy + 63) / 64); }

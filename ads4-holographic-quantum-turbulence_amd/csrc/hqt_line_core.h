@@ -1,0 +1,255 @@
+// Line-operator passes for gfx950 (MI355X): the x/y spectral derivatives and the Fourier
+// filter of the AdS4 stepper, applied as Toeplitz matrix products on MFMA.
+//
+// Reference: tevol_driven_cluster.m:60-61 (func_Lx / func_Ly: dense fourdif matrix times
+// every line), src/numerics/ff.m:7-11 (filter).  A work-group owns 16 lines (the MFMA
+// N/M dimension) of one output group and stages those lines in LDS once; each wave then
+// sweeps its output position tiles over the full line with 16x16x4 MFMAs.  The operator
+// matrix is never stored: its 16x4 fragments are read from a 2*Np-entry Toeplitz table
+// (T[i][j] = tab[i - j + Np]) kept in LDS.
+//
+// Orientation: x-lines are contiguous in memory, so the output tile is (line x position)
+// and every 16 lanes write 16 consecutive x values; y-lines are strided by nx, so the tile
+// is (position x line) with lines = 16 consecutive x values, which again makes every 16
+// lanes write 128 contiguous bytes.  The MFMA operand order is the only difference.
+#pragma once
+#include "hqt_kernels.h"
+
+namespace hqt {
+
+// 16x16x4 MFMA operand layouts (A[m][k] in lane m + 16k, B[k][n] in lane n + 16k).  The
+// accumulator row held by lane group g = lane/16 in register r differs between the two
+// instructions on gfx950 (measured): f64 rows are 4r + g, f32 rows are 4g + r.
+template <typename T> struct MF;
+template <> struct MF<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int g, int r) { return 4 * r + g; }
+};
+template <> struct MF<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
+};
+
+template <typename T>
+__device__ __forceinline__ T ldv(const void* p, int dt, int64_t i) {
+    return dt == DT_F64 ? (T)(static_cast<const double*>(p)[i]) : (T)(static_cast<const float*>(p)[i]);
+}
+template <typename T>
+__device__ __forceinline__ void stv(void* p, int dt, int64_t i, T v) {
+    if (dt == DT_F64) static_cast<double*>(p)[i] = (double)v;
+    else static_cast<float*>(p)[i] = (float)v;
+}
+
+// KIND only tags the instantiation (0: RHS stage, 1: constraint, 2: filter) so that
+// profilers report the three uses separately; the code is identical.
+// NT: 16-line tiles per work-group (LB = 16*NT lines).  With NT = 2 every operator fragment
+// read from LDS feeds two MFMAs (one per line tile), halving the LDS operand traffic per MFMA
+// at twice the accumulator registers and LDS line storage; which one is faster depends on the
+// line length and precision, and the autotuner picks per pass.
+template <typename T, int DIR, int Q, int KIND, int NT>
+__global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restrict__ Pp, const Members M) {
+    constexpr int LB = 16 * NT;
+    const MemberBlock mb = member_block(M);
+    const LPass& P = Pp[mb.m];
+    if ((int)blockIdx.y >= P.ngroups) return;
+    if (__builtin_nontemporal_load(&P.flags[0]) != 0) return;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    const int S = P.S, Np = P.Np, N = P.N, TL = P.TL;
+    T* V = reinterpret_cast<T*>(smem_raw);
+    T* C = V + LB * S;
+    const LGroup& G = P.g[blockIdx.y];
+    const int tid = threadIdx.x;
+
+    for (int s = 0; s < G.nslots; ++s) {
+        const T* src = static_cast<const T*>(P.ops[G.slot_op[s]]);
+        for (int t = tid; t < TL; t += LP_THREADS) C[s * TL + t] = src[t];
+    }
+
+    int64_t line0 = 0, nl = 0;
+    int z = 0, x0 = 0;
+    if (DIR == 0) {
+        line0 = (int64_t)mb.bx * LB;
+        nl = (int64_t)P.ny * P.nz;
+    } else {
+        const int xc = mb.bx % P.nxc;
+        z = mb.bx / P.nxc;
+        x0 = xc * LB;
+    }
+
+    typedef typename MF<T>::v4 v4;
+    v4 acc[2][NT][Q];
+#pragma unroll
+    for (int o = 0; o < 2; ++o)
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+#pragma unroll
+            for (int q = 0; q < Q; ++q) acc[o][u][q] = v4{0, 0, 0, 0};
+
+    const int lane = tid & 63, ln = lane & 15, lk = lane >> 4;
+    // wave index and its tile count are wave-uniform: keep them in scalar registers so the
+    // per-tile guards below are scalar branches, not exec-masked ones
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+
+    for (int t = 0; t < G.ntiles; ++t) {
+        __syncthreads();
+        const void* src = G.src[t];
+        const int dt = G.src_dt[t];
+        if (DIR == 0) {
+            for (int idx = tid; idx < LB * Np; idx += LP_THREADS) {
+                const int m = idx / Np, j = idx - m * Np;
+                const int64_t line = line0 + m;
+                T v = 0;
+                if (line < nl && j < N) v = ldv<T>(src, dt, line * P.nx + j);
+                V[m * S + j] = v;
+            }
+        } else {
+            for (int idx = tid; idx < LB * Np; idx += LP_THREADS) {
+                const int j = idx / LB, m = idx % LB;
+                const int x = x0 + m;
+                T v = 0;
+                if (x < P.nx && j < N) v = ldv<T>(src, dt, (int64_t)x + (int64_t)P.nx * (j + (int64_t)P.ny * z));
+                V[m * S + j] = v;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int o = 0; o < 2; ++o) {
+            if (o >= G.nout) break;
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                if (e >= G.out[o].nterms || G.out[o].term[e].tile != t) continue;
+                // table entry of T[i][j] is tab[i - j + Np + 4]; lane (ln, lk) reads i = i0 + ln, j = j0 + lk
+                const T* Cs = C + G.out[o].term[e].slot * TL + Np + 4 + ln - lk + wave * 16;
+                const T* Vr = V + ln * S + lk;
+                // two k-steps per iteration with ping-pong operand registers: the loads of
+                // one k-step are in flight while the other's MFMAs issue.  The last loads run
+                // one k-step past the tile (values unused; the table is padded for them and
+                // the line rows are followed by the table in LDS).
+                T va0[NT], va1[NT];
+                T cb0[Q], cb1[Q];
+#pragma unroll
+                for (int u = 0; u < NT; ++u) va0[u] = Vr[16 * u * S];
+#pragma unroll
+                for (int q = 0; q < Q; ++q) cb0[q] = Cs[64 * q];
+                for (int j0 = 0; j0 < Np; j0 += 8) {
+#pragma unroll
+                    for (int u = 0; u < NT; ++u) va1[u] = Vr[16 * u * S + j0 + 4];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) cb1[q] = Cs[64 * q - j0 - 4];
+#pragma unroll
+                    for (int u = 0; u < NT; ++u)
+#pragma unroll
+                        for (int q = 0; q < Q; ++q)
+                            acc[o][u][q] = DIR == 0 ? MF<T>::mma(va0[u], cb0[q], acc[o][u][q])
+                                                    : MF<T>::mma(cb0[q], va0[u], acc[o][u][q]);
+#pragma unroll
+                    for (int u = 0; u < NT; ++u) va0[u] = Vr[16 * u * S + j0 + 8];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) cb0[q] = Cs[64 * q - j0 - 8];
+#pragma unroll
+                    for (int u = 0; u < NT; ++u)
+#pragma unroll
+                        for (int q = 0; q < Q; ++q)
+                            acc[o][u][q] = DIR == 0 ? MF<T>::mma(va1[u], cb1[q], acc[o][u][q])
+                                                    : MF<T>::mma(cb1[q], va1[u], acc[o][u][q]);
+                }
+            }
+        }
+    }
+
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+        if (o >= G.nout) break;
+        const LOut& O = G.out[o];
+        bool bad = false;
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int i0 = (wave + 4 * q) * 16;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int64_t idx;
+                bool ok;
+                if (DIR == 0) {
+                    const int64_t line = line0 + 16 * u + MF<T>::row(lk, r);
+                    const int i = i0 + ln;
+                    ok = line < nl && i < N;
+                    idx = line * P.nx + i;
+                } else {
+                    const int x = x0 + 16 * u + ln;
+                    const int i = i0 + MF<T>::row(lk, r);
+                    ok = x < P.nx && i < N;
+                    idx = (int64_t)x + (int64_t)P.nx * (i + (int64_t)P.ny * z);
+                }
+                if (ok) {
+                    T v = acc[o][u][q][r];
+                    if (O.addend) v += ldv<T>(O.addend, O.add_dt, idx);
+                    stv<T>(O.out, O.out_dt, idx, v);
+                    if (v != v) bad = true;
+                }
+            }
+        }
+        if (O.nan_check) {
+            if (__any(bad) && lane == 0) {
+                const int step = __atomic_load_n(&P.flags[1], __ATOMIC_RELAXED);
+                atomicCAS(&P.flags[0], 0, step > 0 ? step : 1);
+            }
+        }
+    }
+}
+
+// Launch of one (precision, direction) family: Q = Mp/64 output tiles per wave, NT line tiles
+// per work-group, KIND the profiling tag.  Dynamic LDS above 64 KB is enabled per instance on
+// first use.
+template <typename T, int DIR, int KIND, int NT, int Q>
+hipError_t line_launch_one(const LPass* d, Members M, dim3 grid, size_t lds, hipStream_t s) {
+    const void* fn = reinterpret_cast<const void*>(&k_line_pass<T, DIR, Q, KIND, NT>);
+    if (lds > 64 * 1024) {
+        static bool done = false;
+        if (!done) {
+            const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            if (e != hipSuccess) return e;
+            done = true;
+        }
+    }
+    hipLaunchKernelGGL((k_line_pass<T, DIR, Q, KIND, NT>), grid, dim3(LP_THREADS), lds, s, d, M);
+    return hipGetLastError();
+}
+
+template <typename T, int DIR, int KIND, int NT>
+hipError_t line_launch_q(const LPass* d, Members M, dim3 grid, int Q, size_t lds, hipStream_t s) {
+    switch (Q) {
+        case 1: return line_launch_one<T, DIR, KIND, NT, 1>(d, M, grid, lds, s);
+        case 2: return line_launch_one<T, DIR, KIND, NT, 2>(d, M, grid, lds, s);
+        case 3: return line_launch_one<T, DIR, KIND, NT, 3>(d, M, grid, lds, s);
+        case 4: return line_launch_one<T, DIR, KIND, NT, 4>(d, M, grid, lds, s);
+        case 5: return line_launch_one<T, DIR, KIND, NT, 5>(d, M, grid, lds, s);
+        case 6: return line_launch_one<T, DIR, KIND, NT, 6>(d, M, grid, lds, s);
+        case 7: return line_launch_one<T, DIR, KIND, NT, 7>(d, M, grid, lds, s);
+        case 8: return line_launch_one<T, DIR, KIND, NT, 8>(d, M, grid, lds, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// Every instance of one (precision, direction) family; explicitly instantiated by the
+// hqt_line_*.hip translation units so that the four families compile in parallel.
+template <typename T, int DIR>
+hipError_t line_launch_td(const LPass* d, Members M, dim3 grid, int Q, int nt, int kind, size_t lds, hipStream_t s) {
+    if (nt == 2) {
+        if (kind == 0) return line_launch_q<T, DIR, 0, 2>(d, M, grid, Q, lds, s);
+        if (kind == 1) return line_launch_q<T, DIR, 1, 2>(d, M, grid, Q, lds, s);
+        return line_launch_q<T, DIR, 2, 2>(d, M, grid, Q, lds, s);
+    }
+    if (kind == 0) return line_launch_q<T, DIR, 0, 1>(d, M, grid, Q, lds, s);
+    if (kind == 1) return line_launch_q<T, DIR, 1, 1>(d, M, grid, Q, lds, s);
+    return line_launch_q<T, DIR, 2, 1>(d, M, grid, Q, lds, s);
+}
+
+}  // namespace hqt

@@ -14,6 +14,8 @@ import numpy as np
 HQT_OK, HQT_ERR_ARG, HQT_ERR_NAN, HQT_ERR_HIP, HQT_ERR_OOM, HQT_ERR_STATE, HQT_ERR_NOCONV, HQT_ERR_TIMEOUT = range(8)
 HQT_FP64, HQT_MIXED = 0, 1
 HQT_NO_GRAPH = 1
+HQT_LINES16 = 2
+HQT_LINES32 = 4
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "csrc", "libhqt.so")

@@ -571,6 +571,7 @@ int hqt_ctx_create(const hqt_params* p, hqt_ctx** out) {
 int hqt_ctx_destroy(hqt_ctx* c) {
     if (!c) return HQT_OK;
     if (c->owner) return fail(HQT_ERR_STATE, "context is an ensemble member; destroy the ensemble first");
+    checkpoint_writer_release(c);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     for (auto& r : c->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }

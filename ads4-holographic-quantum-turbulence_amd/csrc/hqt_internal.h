@@ -9,6 +9,8 @@
 
 using namespace hqt;
 
+struct CheckpointWriter;
+
 // Sets the thread-local message returned by hqt_last_error() and returns code.
 int hqt_fail(int code, const std::string& msg);
 
@@ -64,6 +66,7 @@ struct hqt_ctx {
     struct DevDesc { ZStage zs[5]; ZPost zp; ZPrep zpre[2]; Wind wind; NanCheck nan; };
     DevDesc* dd = nullptr;
     hipStream_t owner = nullptr;     // stream of an ensemble that steps this context, if any
+    CheckpointWriter* ckpt = nullptr;  // asynchronous checkpoint writer (created on first use)
     double* Od = nullptr;            // [2*nxy] condensate of the current state (diagnostics)
     int8_t* cmap = nullptr;          // [nxy] plaquette charges
     int32_t* ccount = nullptr;       // [2] +1 / -1 plaquette counts
@@ -99,3 +102,5 @@ int newton_batch_run(hipStream_t s, NewtonWork& w, int nz, double zh, double sma
 // tables, descriptors), as hqt_ctx_create would; used for admission by HBM budget.
 int64_t ctx_footprint(const hqt_params& p);
 
+// Joins a pending asynchronous checkpoint write and frees the writer's buffers.
+void checkpoint_writer_release(hqt_ctx* c);

@@ -26,7 +26,9 @@ EXPORTS = (
     "hqt_newton_vy", "hqt_newton_vy_batch", "hqt_vortex_census", "hqt_ensemble_create", "hqt_ensemble_step_n", "hqt_ensemble_sync",
     "hqt_ensemble_launches", "hqt_ensemble_destroy", "hqt_server_create", "hqt_server_destroy",
     "hqt_server_submit_evolution", "hqt_server_submit_newton", "hqt_server_wait", "hqt_server_evolution_result",
-    "hqt_server_newton_result", "hqt_server_stats", "hqt_server_counters", "hqt_ctx_footprint", "hqt_ctx_device_bytes", "hqt_debug_field", "hqt_profile", "hqt_kernel_stats", "hqt_calibration_copy",
+    "hqt_server_newton_result", "hqt_server_stats", "hqt_server_counters", "hqt_ctx_footprint", "hqt_ctx_device_bytes",
+    "hqt_checkpoint_name", "hqt_checkpoint_write_host", "hqt_checkpoint_save", "hqt_checkpoint_save_async",
+    "hqt_checkpoint_wait", "hqt_debug_field", "hqt_profile", "hqt_kernel_stats", "hqt_calibration_copy",
     "hqt_last_error", "hqt_version",
 )
 
@@ -92,6 +94,11 @@ def lib():
     L.hqt_server_counters.argtypes = [vp, C.POINTER(C.c_int64)]
     L.hqt_ctx_footprint.argtypes = [C.POINTER(Params), C.POINTER(C.c_int64)]
     L.hqt_ctx_device_bytes.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.hqt_checkpoint_name.argtypes = [C.c_double, C.c_char_p, i32]
+    L.hqt_checkpoint_write_host.argtypes = [C.c_char_p, i32, i32, i32, dp, dp, dp, dp, dp]
+    L.hqt_checkpoint_save.argtypes = [vp, C.c_char_p]
+    L.hqt_checkpoint_save_async.argtypes = [vp, C.c_char_p]
+    L.hqt_checkpoint_wait.argtypes = [vp]
     L.hqt_vortex_census.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_int8), dp]
     L.hqt_ff.argtypes = [i32, i32, i32, dp, dp, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), dp]
     L.hqt_fourdif.argtypes = [i32, i32, dp, dp]

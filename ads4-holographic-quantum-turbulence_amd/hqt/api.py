@@ -20,7 +20,7 @@ import numpy as np
 
 from ._lib import (HQT_ERR_ARG, HQT_ERR_NAN, HQT_ERR_NOCONV, HQT_ERR_TIMEOUT, HQT_FP64, HQT_LINES16, HQT_LINES32, HQT_MIXED, HQT_NO_GRAPH, HQTError, Params, check, dptr, lib)
 
-__all__ = ["fourdif", "chebdif", "ff", "NewtonRaphson_vy", "NewtonRaphson_vy_batch", "Stepper", "Ensemble", "Server", "footprint", "make_params", "HQTError", "mask_factors"]
+__all__ = ["fourdif", "chebdif", "ff", "NewtonRaphson_vy", "NewtonRaphson_vy_batch", "Stepper", "Ensemble", "Server", "footprint", "make_params", "checkpoint_name", "write_checkpoint", "HQTError", "mask_factors"]
 
 
 def fourdif(N: int, m: int):
@@ -174,6 +174,21 @@ def make_params(nx, ny, nz, Lx, Ly, mu=6.0, dt=0.02, winding=True, filter=True, 
     return p, keep
 
 
+def checkpoint_name(t):
+    """``['check_point_t=', num2str(t), '.mat']`` (``tevol_driven_cluster.m:179``)."""
+    buf = C.create_string_buffer(128)
+    check(lib().hqt_checkpoint_name(float(t), buf, 128))
+    return buf.value.decode()
+
+
+def write_checkpoint(path, Z1, Z2, Z3, Z4, rhoi):
+    """``save(path, 'Z1','Z2','Z3','Z4','rhoi')`` of host arrays [nx, ny, nz] (MAT level 5)."""
+    s = np.shape(Z2)
+    z1 = _c128F(Z1, s)
+    arrs = [_f64F(a, s) for a in (Z2, Z3, Z4, rhoi)]
+    check(lib().hqt_checkpoint_write_host(str(path).encode(), s[0], s[1], s[2], dptr(z1), *[dptr(a) for a in arrs]))
+
+
 def footprint(nx, ny, nz, precision="fp64"):
     """Device bytes a context of this size allocates (``hqt_ctx_footprint``)."""
     p, _ = make_params(nx, ny, nz, 50.0, 50.0, precision=precision)
@@ -286,6 +301,18 @@ class Stepper:
 
     def F4(self, y1, y2, y3, y4):
         return self.rhs(y1, y2, y3, y4)[3]
+
+    def save_checkpoint(self, path, wait=True):
+        """Write the current state as a reference checkpoint file.  ``wait=False`` snapshots the
+        state on the device and writes the file from a host thread while stepping goes on;
+        :meth:`checkpoint_wait` joins it."""
+        if wait:
+            check(lib().hqt_checkpoint_save(self._h, str(path).encode()))
+        else:
+            check(lib().hqt_checkpoint_save_async(self._h, str(path).encode()))
+
+    def checkpoint_wait(self):
+        check(lib().hqt_checkpoint_wait(self._h))
 
     @property
     def device_bytes(self):

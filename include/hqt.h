@@ -105,6 +105,21 @@ int hqt_order_parameter(hqt_ctx* ctx, double* O);
  * are downloaded. */
 int hqt_vortex_census(hqt_ctx* ctx, int32_t* counts, int8_t* charge, double* Oabs);
 
+/* ---- checkpoints: tevol_driven_cluster.m:178-181, docs/DATA_DICTIONARY.md "Checkpoint files" -- */
+/* File name of the checkpoint at time t: "check_point_t=" num2str(t) ".mat" (num2str of a
+ * non-integer scalar prints %.<d>g with d = max(floor(log10|t|), 0) + 5). */
+int hqt_checkpoint_name(double t, char* buf, int32_t len);
+/* save(path, 'Z1','Z2','Z3','Z4','rhoi') of host arrays: a MAT level 5 file (uncompressed,
+ * as save -v6) with Z1 complex and all five [nx,ny,nz] double. */
+int hqt_checkpoint_write_host(const char* path, int32_t nx, int32_t ny, int32_t nz, const double* Z1,
+                              const double* Z2, const double* Z3, const double* Z4, const double* rhoi);
+/* The same for the context's current state: synchronous, or asynchronous (the state is
+ * snapshotted on the device and written by a host thread while stepping goes on; one write in
+ * flight per context, hqt_checkpoint_wait joins it and returns its status). */
+int hqt_checkpoint_save(hqt_ctx* ctx, const char* path);
+int hqt_checkpoint_save_async(hqt_ctx* ctx, const char* path);
+int hqt_checkpoint_wait(hqt_ctx* ctx);
+
 /* ---- batched ensemble ------------------------------------------------------------------ */
 /* Many independent evolutions stepped together: the contexts may differ in nx, ny, nz, mu,
  * boundary drive, winding/filter switches and precision.  Every kernel of the step runs once

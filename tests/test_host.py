@@ -258,3 +258,39 @@ def test_server_rejects_bad_arguments():
     assert hqt.lib().hqt_server_create(0, 10, C.byref(h)) == hqt.HQT_ERR_ARG
     assert hqt.lib().hqt_server_create(1 << 30, 0, C.byref(h)) == hqt.HQT_ERR_ARG
 
+
+
+@pytest.mark.parametrize("t,name", [(0.0, "0"), (1.0, "1"), (0.02, "0.02"), (1.5, "1.5"), (math.pi, "3.1416"),
+                                    (123.456, "123.456"), (12345.678, "12345.678"), (2000.0, "2000")])
+def test_checkpoint_name_is_num2str(t, name):
+    """num2str of a scalar double (MATLAB's documented %.<d>g rule) as used in the
+    checkpoint file name (tevol_driven_cluster.m:179)."""
+    import hqt
+    assert hqt.checkpoint_name(t) == f"check_point_t={name}.mat"
+
+
+def test_checkpoint_name_of_accumulated_time():
+    """t = t + h with h = 0.02, 50 steps, as the reference's loop accumulates it (:110,126)."""
+    import hqt
+    t = 0.0
+    for _ in range(50):
+        t = t + 0.02
+    assert t != 1.0 and hqt.checkpoint_name(t) == "check_point_t=1.mat"
+
+
+def test_checkpoint_file_reads_back_with_scipy(tmp_path):
+    """The MAT level 5 writer: scipy.io.loadmat returns the five variables with the reference's
+    names, shapes, complex Z1, and the exact values."""
+    import hqt
+    import scipy.io
+    rng = np.random.default_rng(3)
+    nx, ny, nz = 5, 7, 3
+    Z1 = rng.standard_normal((nx, ny, nz)) + 1j * rng.standard_normal((nx, ny, nz))
+    Z2, Z3, Z4, rh = (rng.standard_normal((nx, ny, nz)) for _ in range(4))
+    path = tmp_path / hqt.checkpoint_name(1.0)
+    hqt.write_checkpoint(path, Z1, Z2, Z3, Z4, rh)
+    d = scipy.io.loadmat(str(path))
+    assert {"Z1", "Z2", "Z3", "Z4", "rhoi"} <= set(d)
+    assert d["Z1"].dtype == np.complex128 and d["Z1"].shape == (nx, ny, nz)
+    for k, v in (("Z1", Z1), ("Z2", Z2), ("Z3", Z3), ("Z4", Z4), ("rhoi", rh)):
+        assert np.array_equal(d[k], v), k

@@ -6,9 +6,9 @@ Python mirror of the reference's MATLAB interface (``fourdif``, ``chebdif``, ``f
 """
 
 from ._lib import HQT_ERR_ARG, HQT_ERR_HIP, HQT_ERR_NAN, HQT_ERR_NOCONV, HQT_ERR_OOM, HQT_ERR_TIMEOUT, HQT_ERR_STATE, HQTError, LIB_PATH, lib
-from .api import Ensemble, NewtonRaphson_vy, NewtonRaphson_vy_batch, Server, Stepper, checkpoint_name, footprint, write_checkpoint, chebdif, ff, fourdif, mask_factors
+from .api import Ensemble, NewtonRaphson_vy, NewtonRaphson_vy_batch, Server, Stepper, checkpoint_name, footprint, spline, write_checkpoint, chebdif, ff, fourdif, mask_factors
 
 __all__ = [
-    "Stepper", "Ensemble", "Server", "footprint", "checkpoint_name", "write_checkpoint", "fourdif", "chebdif", "ff", "NewtonRaphson_vy", "NewtonRaphson_vy_batch", "mask_factors", "HQTError", "lib", "LIB_PATH",
+    "Stepper", "Ensemble", "Server", "footprint", "checkpoint_name", "write_checkpoint", "spline", "fourdif", "chebdif", "ff", "NewtonRaphson_vy", "NewtonRaphson_vy_batch", "mask_factors", "HQTError", "lib", "LIB_PATH",
     "HQT_ERR_ARG", "HQT_ERR_NAN", "HQT_ERR_HIP", "HQT_ERR_OOM", "HQT_ERR_STATE", "HQT_ERR_NOCONV",
 ]

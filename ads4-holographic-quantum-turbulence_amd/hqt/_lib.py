@@ -28,7 +28,7 @@ EXPORTS = (
     "hqt_server_submit_evolution", "hqt_server_submit_newton", "hqt_server_wait", "hqt_server_evolution_result",
     "hqt_server_newton_result", "hqt_server_stats", "hqt_server_counters", "hqt_ctx_footprint", "hqt_ctx_device_bytes",
     "hqt_checkpoint_name", "hqt_checkpoint_write_host", "hqt_checkpoint_save", "hqt_checkpoint_save_async",
-    "hqt_checkpoint_wait", "hqt_debug_field", "hqt_profile", "hqt_kernel_stats", "hqt_calibration_copy",
+    "hqt_checkpoint_wait", "hqt_init_state", "hqt_spline", "hqt_debug_field", "hqt_profile", "hqt_kernel_stats", "hqt_calibration_copy",
     "hqt_last_error", "hqt_version",
 )
 
@@ -49,6 +49,19 @@ class Params(C.Structure):
         ("winding", C.c_int32), ("filter", C.c_int32), ("precision", C.c_int32), ("flags", C.c_int32),
         ("ax_b", C.POINTER(C.c_double)), ("ay_b", C.POINTER(C.c_double)),
         ("ax_b_const", C.c_double), ("ay_b_const", C.c_double),
+    ]
+
+
+class Init(C.Structure):
+    """Mirror of ``hqt_init`` (include/hqt.h)."""
+    _fields_ = [
+        ("nz0", C.c_int32),
+        ("Z1", C.POINTER(C.c_double)), ("Z2", C.POINTER(C.c_double)), ("Vy1", C.POINTER(C.c_double)),
+        ("rhoi", C.POINTER(C.c_double)),
+        ("amp_noise", C.c_double), ("phase_noise", C.c_double), ("seed", C.c_uint64),
+        ("nv", C.c_int32),
+        ("vx", C.POINTER(C.c_double)), ("vy", C.POINTER(C.c_double)), ("q", C.POINTER(C.c_double)),
+        ("sigma", C.c_double),
     ]
 
 
@@ -99,6 +112,8 @@ def lib():
     L.hqt_checkpoint_save.argtypes = [vp, C.c_char_p]
     L.hqt_checkpoint_save_async.argtypes = [vp, C.c_char_p]
     L.hqt_checkpoint_wait.argtypes = [vp]
+    L.hqt_init_state.argtypes = [vp, C.POINTER(Init)]
+    L.hqt_spline.argtypes = [i32, dp, dp, i32, dp, dp]
     L.hqt_vortex_census.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_int8), dp]
     L.hqt_ff.argtypes = [i32, i32, i32, dp, dp, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), dp]
     L.hqt_fourdif.argtypes = [i32, i32, dp, dp]

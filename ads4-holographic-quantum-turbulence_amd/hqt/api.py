@@ -18,9 +18,9 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (HQT_ERR_ARG, HQT_ERR_NAN, HQT_ERR_NOCONV, HQT_ERR_TIMEOUT, HQT_FP64, HQT_LINES16, HQT_LINES32, HQT_MIXED, HQT_NO_GRAPH, HQTError, Params, check, dptr, lib)
+from ._lib import (HQT_ERR_ARG, HQT_ERR_NAN, HQT_ERR_NOCONV, HQT_ERR_TIMEOUT, HQT_FP64, HQT_LINES16, HQT_LINES32, HQT_MIXED, HQT_NO_GRAPH, HQTError, Init, Params, check, dptr, lib)
 
-__all__ = ["fourdif", "chebdif", "ff", "NewtonRaphson_vy", "NewtonRaphson_vy_batch", "Stepper", "Ensemble", "Server", "footprint", "make_params", "checkpoint_name", "write_checkpoint", "HQTError", "mask_factors"]
+__all__ = ["fourdif", "chebdif", "ff", "NewtonRaphson_vy", "NewtonRaphson_vy_batch", "Stepper", "Ensemble", "Server", "footprint", "make_params", "checkpoint_name", "write_checkpoint", "spline", "HQTError", "mask_factors"]
 
 
 def fourdif(N: int, m: int):
@@ -174,6 +174,16 @@ def make_params(nx, ny, nz, Lx, Ly, mu=6.0, dt=0.02, winding=True, filter=True, 
     return p, keep
 
 
+def spline(X, Y, Xi):
+    """``interp1(X, Y, Xi, 'spline')``: not-a-knot cubic spline (``hqt_spline``)."""
+    X = np.ascontiguousarray(X, dtype=np.float64).ravel()
+    Y = np.ascontiguousarray(Y, dtype=np.float64).ravel()
+    Xi = np.ascontiguousarray(Xi, dtype=np.float64).ravel()
+    out = np.empty_like(Xi)
+    check(lib().hqt_spline(X.size, dptr(X), dptr(Y), Xi.size, dptr(Xi), dptr(out)))
+    return out
+
+
 def checkpoint_name(t):
     """``['check_point_t=', num2str(t), '.mat']`` (``tevol_driven_cluster.m:179``)."""
     buf = C.create_string_buffer(128)
@@ -301,6 +311,27 @@ class Stepper:
 
     def F4(self, y1, y2, y3, y4):
         return self.rhs(y1, y2, y3, y4)[3]
+
+    def init_state(self, prof, amp_noise=0.0, phase_noise=0.0, seed=0, vortices=None, sigma=0.0):
+        """Build the initial fields on the device (``hqt_init_state``): the stationary profile
+        ``prof`` (dict of nz0-vectors ``Z1, Z2, Vy1, rhoi`` on chebdif(nz0) nodes) splined to
+        this grid and gauge-rotated, times amp = (1 + amp_noise g1)(1 - sum of Gaussian dips of
+        width sigma) and exp(i phase), phase = phase_noise g2 + sum q atan2(y - Y, x - X).
+        ``vortices`` = (X, Y, q) arrays."""
+        arr = [np.ascontiguousarray(prof[k], dtype=np.float64).ravel() for k in ("Z1", "Z2", "Vy1", "rhoi")]
+        d = Init()
+        d.nz0 = arr[0].size
+        d.Z1, d.Z2, d.Vy1, d.rhoi = (dptr(a) for a in arr)
+        d.amp_noise, d.phase_noise, d.seed = float(amp_noise), float(phase_noise), int(seed) & ((1 << 64) - 1)
+        keep = []
+        if vortices is not None and len(vortices[0]):
+            keep = [np.ascontiguousarray(v, dtype=np.float64).ravel() for v in vortices]
+            d.nv = keep[0].size
+            d.vx, d.vy, d.q = (dptr(a) for a in keep)
+        else:
+            d.nv = 0
+        d.sigma = float(sigma)
+        check(lib().hqt_init_state(self._h, C.byref(d)))
 
     def save_checkpoint(self, path, wait=True):
         """Write the current state as a reference checkpoint file.  ``wait=False`` snapshots the

@@ -105,6 +105,30 @@ int hqt_order_parameter(hqt_ctx* ctx, double* O);
  * are downloaded. */
 int hqt_vortex_census(hqt_ctx* ctx, int32_t* counts, int8_t* charge, double* Oabs);
 
+/* ---- initial state on the device: tevol_driven_cluster.m:81-102, tevol_lattice.m:101-135 -- */
+/* The stationary profile (Z1, Z2, Vy1, rhoi on the chebdif(nz0) nodes, as the reference's
+ * data/reference/*.mat hold it) is splined to the run's nz nodes (interp1 'spline', :84),
+ * gauge-rotated by theta = -Lzr(A_t/f) (:90-93) and spread over the plane with the pattern
+ *   amp(x,y)   = (1 + amp_noise g1) (1 - sum_v exp(-|r - r_v|^2/sigma^2))    (sigma <= 0: no dips)
+ *   phase(x,y) = phase_noise g2 + sum_v q_v atan2(y - vy_v, x - vx_v)
+ * (:101 is amp_noise = 1e-2; the lattice of tevol_lattice.m:104-135 is sigma = 3 with q = +-6).
+ * g1, g2: standard normals from Philox4x32-10 keyed by seed and counted by the column index
+ * i + nx*j (Box-Muller), independent of the device and launch shape.  A_t = Z2, A_x = 0,
+ * A_y = -Vy1, rhoi = rhoi(z=0) replicated (:94-98).  Replaces hqt_set_state; the fields are
+ * written in place on the device. */
+typedef struct hqt_init {
+    int32_t nz0;
+    const double *Z1, *Z2, *Vy1, *rhoi;  /* [nz0] stationary profile */
+    double amp_noise, phase_noise;
+    uint64_t seed;
+    int32_t nv;                          /* windings / dips */
+    const double *vx, *vy, *q;           /* [nv] positions (box units) and charges */
+    double sigma;
+} hqt_init;
+int hqt_init_state(hqt_ctx* ctx, const hqt_init* in);
+/* yi = interp1(X, Y, Xi, 'spline'): not-a-knot cubic spline, X strictly monotone, n >= 4. */
+int hqt_spline(int32_t n, const double* X, const double* Y, int32_t m, const double* Xi, double* out);
+
 /* ---- checkpoints: tevol_driven_cluster.m:178-181, docs/DATA_DICTIONARY.md "Checkpoint files" -- */
 /* File name of the checkpoint at time t: "check_point_t=" num2str(t) ".mat" (num2str of a
  * non-integer scalar prints %.<d>g with d = max(floor(log10|t|), 0) + 5). */

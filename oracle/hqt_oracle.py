@@ -537,3 +537,72 @@ def imprint(ops: Operators, z1, phase_xy=None, amp_xy=None):
     if phase_xy is not None:
         out = out * np.exp(1j * phase_xy)[:, :, None]
     return out
+
+
+# --------------------------------------------------------------------------------------
+# Device initial state (tevol_driven_cluster.m:81-102 with the plane pattern of :101 and
+# tevol_lattice.m:113-135): restatement of the product's documented construction
+# --------------------------------------------------------------------------------------
+
+_M0, _M1 = np.uint64(0xD2511F53), np.uint64(0xCD9E8D57)
+_W0, _W1 = np.uint64(0x9E3779B9), np.uint64(0xBB67AE85)
+_MASK32 = np.uint64(0xFFFFFFFF)
+
+
+def philox4x32_10(c0, c1, c2, c3, k0, k1):
+    """Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) on uint32 counter arrays."""
+    c = [np.asarray(x, dtype=np.uint64) & _MASK32 for x in (c0, c1, c2, c3)]
+    k0 = np.uint64(k0) & _MASK32
+    k1 = np.uint64(k1) & _MASK32
+    for _ in range(10):
+        p0 = _M0 * c[0]
+        p1 = _M1 * c[2]
+        hi0, lo0 = p0 >> np.uint64(32), p0 & _MASK32
+        hi1, lo1 = p1 >> np.uint64(32), p1 & _MASK32
+        c = [(hi1 ^ c[1] ^ k0) & _MASK32, lo1, (hi0 ^ c[3] ^ k1) & _MASK32, lo0]
+        k0 = (k0 + _W0) & _MASK32
+        k1 = (k1 + _W1) & _MASK32
+    return c
+
+
+def column_normals(nx, ny, seed):
+    """(g1, g2) per column i + nx*j: Philox keyed by the seed, two 53-bit uniforms, Box-Muller."""
+    col = np.arange(nx * ny, dtype=np.uint64)
+    seed = int(seed)
+    u = philox4x32_10(col & _MASK32, col >> np.uint64(32), 0, 0, seed & 0xFFFFFFFF, seed >> 32)
+    r1 = (((u[0] >> np.uint64(6)) << np.uint64(27)) | (u[1] >> np.uint64(5))).astype(np.float64)
+    r2 = (((u[2] >> np.uint64(6)) << np.uint64(27)) | (u[3] >> np.uint64(5))).astype(np.float64)
+    r1 = (r1 + 0.5) * 2.0 ** -53
+    r2 = (r2 + 0.5) * 2.0 ** -53
+    rad = np.sqrt(-2.0 * np.log(r1))
+    g1 = (rad * np.cos(2 * math.pi * r2)).reshape((ny, nx)).T
+    g2 = (rad * np.sin(2 * math.pi * r2)).reshape((ny, nx)).T
+    return g1, g2
+
+
+def device_initial_state(ops: Operators, prof: dict, amp_noise=0.0, phase_noise=0.0, seed=0, vortices=None,
+                         sigma=0.0):
+    """Fields of the device initial-state builder: stationary profile (:82-98) times
+    amp = (1 + amp_noise g1)(1 - sum exp(-r^2/sigma^2)) and exp(i phase),
+    phase = phase_noise g2 + sum q atan2(y - Y, x - X); A_x set to 0 (func_Lx of an
+    x-independent theta)."""
+    z1, z2, z3, z4, rhoi = stationary_fields(ops, prof)
+    nx, ny = ops.nx, ops.ny
+    X = ops.xx1[:, None]                                  # xx, yy (:42-43)
+    Y = ops.yy1[None, :]
+    amp = np.ones((nx, ny))
+    phase = np.zeros((nx, ny))
+    if amp_noise or phase_noise:
+        g1, g2 = column_normals(nx, ny, seed)
+        amp = amp + amp_noise * g1
+        phase = phase + phase_noise * g2
+    dip = np.zeros((nx, ny))
+    if vortices is not None:
+        for x0, y0, q in zip(*vortices):
+            phase = phase + q * np.arctan2(Y - y0, X - x0)
+            if sigma > 0:
+                dip = dip + np.exp(-((X - x0) ** 2 + (Y - y0) ** 2) / sigma ** 2)
+    amp = amp * (1 - dip)
+    z1 = imprint(ops, z1, phase, amp)
+    return z1, z2, np.zeros_like(z3), z4, rhoi
+

@@ -294,3 +294,19 @@ def test_checkpoint_file_reads_back_with_scipy(tmp_path):
     assert d["Z1"].dtype == np.complex128 and d["Z1"].shape == (nx, ny, nz)
     for k, v in (("Z1", Z1), ("Z2", Z2), ("Z3", Z3), ("Z4", Z4), ("rhoi", rh)):
         assert np.array_equal(d[k], v), k
+
+
+def test_spline_matches_scipy_not_a_knot():
+    """hqt_spline (interp1 'spline') against scipy's not-a-knot CubicSpline on the reference's
+    radial use (chebdif(30) nodes to chebdif(20) nodes), and exactness on a cubic."""
+    import hqt
+    from scipy.interpolate import CubicSpline
+    from oracle import hqt_oracle as O
+    z0 = (O.chebdif(30, 1)[0] - 1) / 2 + 1
+    z = (O.chebdif(20, 1)[0] - 1) / 2 + 1
+    y = np.sin(3 * z0) + z0 ** 5
+    o = np.argsort(z0)
+    ref = CubicSpline(z0[o], y[o], bc_type="not-a-knot")(z)
+    assert np.max(np.abs(hqt.spline(z0, y, z) - ref)) < 1e-13
+    c = 1 - 2 * z0 + 0.5 * z0 ** 2 - 3 * z0 ** 3
+    assert np.max(np.abs(hqt.spline(z0, c, z) - (1 - 2 * z + 0.5 * z ** 2 - 3 * z ** 3))) < 1e-13

@@ -12,6 +12,11 @@
  *   [Z1,Z2,Z3,Z4,rhoi] = hqt_mex('step', h, nsteps)      % :120-142 iterated nsteps times
  *   [F1,F2,F3,F4] = hqt_mex('rhs', h, Z1, Z2, Z3, Z4)     % the closures of :72-76
  *   w = hqt_mex('ff', x, fx, fy)                          % src/numerics/ff.m with mask fx|fy
+ *   hqt_mex('init', h, Z1, Z2, Vy1, rhoi, amp_noise, phase_noise, seed)
+ *         % :81-102 on the device from the nz0-point profile (1e-2 amplitude noise as :101)
+ *   hqt_mex('checkpoint', h, fileName)                    % save(fileName,'Z1',...,'rhoi') of :178-181,
+ *                                                         % written by a host thread while stepping goes on
+ *   [n, np, nm] = hqt_mex('countN', h)                    % src/diagnostics/countN.m on the current state
  *   hqt_mex('destroy', h)
  *
  * Errors (bad arguments, the NaN guard of :182-185, HIP failures) are raised with
@@ -32,8 +37,9 @@ static void cleanup(void) {
 }
 
 static void fail(int rc) {
-    static const char* ids[] = {"hqt:ok", "hqt:arg", "hqt:nan", "hqt:hip", "hqt:oom", "hqt:state", "hqt:noconv"};
-    mexErrMsgIdAndTxt(rc >= 0 && rc < 7 ? ids[rc] : "hqt:error", "%s", hqt_last_error());
+    static const char* ids[] = {"hqt:ok", "hqt:arg", "hqt:nan", "hqt:hip", "hqt:oom", "hqt:state", "hqt:noconv",
+                                "hqt:timeout"};
+    mexErrMsgIdAndTxt(rc >= 0 && rc < 8 ? ids[rc] : "hqt:error", "%s", hqt_last_error());
 }
 
 static hqt_ctx* handle(const mxArray* a) {
@@ -138,7 +144,39 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     uint64_t k = *(const uint64_t*)mxGetData(prhs[1]);
     dims_t dm = g_dims[k - 1];
     size_t n = (size_t)dm.nx * dm.ny * dm.nz;
+    if (!strcmp(cmd, "init")) {
+        if (nrhs != 9) mexErrMsgIdAndTxt("hqt:arg", "init needs Z1, Z2, Vy1, rhoi, amp_noise, phase_noise, seed");
+        const size_t nz0 = mxGetNumberOfElements(prhs[2]);
+        hqt_init in;
+        memset(&in, 0, sizeof(in));
+        in.nz0 = (int32_t)nz0;
+        in.Z1 = real_field(prhs[2], nz0, "Z1");
+        in.Z2 = real_field(prhs[3], nz0, "Z2");
+        in.Vy1 = real_field(prhs[4], nz0, "Vy1");
+        in.rhoi = real_field(prhs[5], nz0, "rhoi");
+        in.amp_noise = mxGetScalar(prhs[6]);
+        in.phase_noise = mxGetScalar(prhs[7]);
+        in.seed = (uint64_t)mxGetScalar(prhs[8]);
+        int rc = hqt_init_state(c, &in);
+        if (rc) fail(rc);
+        return;
+    }
+    if (!strcmp(cmd, "checkpoint")) {
+        char path[4096];
+        if (nrhs != 3 || mxGetString(prhs[2], path, sizeof(path))) mexErrMsgIdAndTxt("hqt:arg", "checkpoint needs a file name");
+        int rc = hqt_checkpoint_save_async(c, path);
+        if (rc) fail(rc);
+        return;
+    }
+    if (!strcmp(cmd, "countN")) {
+        int32_t counts[3];
+        int rc = hqt_vortex_census(c, counts, NULL, NULL);
+        if (rc) fail(rc);
+        for (int i = 0; i < 3 && i < (nlhs > 0 ? nlhs : 1); ++i) plhs[i] = mxCreateDoubleScalar((double)counts[i]);
+        return;
+    }
     if (!strcmp(cmd, "destroy")) {
+        hqt_checkpoint_wait(c);
         hqt_ctx_destroy(c);
         g_ctx[k - 1] = NULL;
         return;

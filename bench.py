@@ -7,7 +7,10 @@ stationary profile with 1e-3 phase noise and 4 windings.  One "step" is one full
 removal, filter, NaN guard) with the state resident in HBM.
 
 Prints ONE JSON line.  ``value`` is FP64 steps/s; the mixed-precision form is reported beside
-it with its per-field error against the FP64 run after the same steps.  ``roofline`` is the
+it with its per-field error against the FP64 run after the same steps.  Beside them:
+``ensemble`` (many independent evolutions of uneven size stepped as one batch) and
+``serving`` (a Poisson stream of Newton solves and evolution segments through the serving
+loop, with p50/p99 latency).  ``roofline`` is the
 dominant kernel's algorithmic FLOP rate from HIP events recorded around its launches on the
 library's stream (a profiled repeat of the timed steps); ``cpu_baseline`` is the NumPy oracle
 (the reference's dense-matrix form) timed on a bounded sample on this host.
@@ -116,6 +119,83 @@ def cpu_baseline(nx, ny, nz, L, state, axb, budget_s=20.0):
     return n / dt, n
 
 
+def ensemble_leg(members, steps, seed=4):
+    """Batched ensemble of independent evolutions of uneven size: nx = ny in {64, 96, 128, 192,
+    256}, nz in {12, 16, 20, 24}, mu ~ U[4.1, 8] (below mu ~ 4.07 the zero-flow state is the
+    normal phase), kk in 3..9 drive, 8-64 windings, each built on the device.  Timed: `steps`
+    steps of the whole ensemble (one graph per step) vs the same members stepped one by one
+    (each its own graph and stream)."""
+    import hqt
+    from hqt import ic
+    rng = np.random.default_rng(seed)
+    L = 50.0
+    specs = []
+    for i in range(members):
+        n = int(rng.choice([64, 96, 128, 192, 256]))
+        nz = int(rng.choice([12, 16, 20, 24]))
+        specs.append((n, nz, float(rng.uniform(4.1, 8.0)), int(rng.integers(3, 10)), int(rng.integers(4, 33)) * 2,
+                      int(rng.integers(1 << 30))))
+    pts = sum(n * n * nz for n, nz, *_ in specs)
+    out = {"members": members, "steps": steps, "point_count": pts,
+           "sizes": "nx=ny in {64,96,128,192,256}, nz in {12,16,20,24}, mu~U[4.1,8], kk 3..9, 8-64 windings"}
+    for prec in ("fp64", "mixed"):
+        sts = []
+        for n, nz, mu, kk, nv, s in specs:
+            st = hqt.Stepper(n, n, nz, L, L, mu=mu, dt=0.02, precision=prec, ax_b=ic.sinusoidal_drive(n, n, L, kk, 0.1))
+            r = np.random.default_rng(s)
+            vort = (r.uniform(0, L, nv), r.uniform(0, L, nv), np.where(np.arange(nv) % 2 == 0, 1.0, -1.0))
+            st.init_state(ic.zero_flow_profile(mu), phase_noise=1e-3, seed=s, vortices=vort)
+            sts.append(st)
+        for st in sts:
+            st.step(2)
+        t0 = time.perf_counter()
+        for st in sts:
+            st.step_async(steps)
+        for st in sts:
+            st.sync()
+        tseq = (time.perf_counter() - t0) / steps
+        ens = hqt.Ensemble(sts)
+        ens.step(2)
+        t0 = time.perf_counter()
+        ens.step(steps)
+        tens = (time.perf_counter() - t0) / steps
+        out[prec] = {"ms_per_step": round(1e3 * tens, 3), "member_steps_per_s": round(members / tens, 1),
+                     "point_updates_per_s": round(pts / tens, 1), "launches_per_step": ens.launches_per_step,
+                     "members_one_by_one_ms_per_step": round(1e3 * tseq, 3), "speedup": round(tseq / tens, 3)}
+        ens.close()
+        for st in sts:
+            st.close()
+    # the batching case proper: many small evolutions (the 64x64x12 FP64 workload, 48 copies with
+    # their own seeds), where a single evolution cannot fill the GPU
+    sts = []
+    for i in range(members):
+        st = hqt.Stepper(64, 64, 12, L, L, mu=6.0, dt=0.02)
+        r = np.random.default_rng(1000 + i)
+        st.init_state(ic.zero_flow_profile(6.0), phase_noise=1e-3, seed=1000 + i,
+                      vortices=(r.uniform(0, L, 4), r.uniform(0, L, 4), np.array([1.0, -1.0, 1.0, -1.0])))
+        sts.append(st)
+    for st in sts:
+        st.step(2)
+    t0 = time.perf_counter()
+    for st in sts:
+        st.step_async(steps)
+    for st in sts:
+        st.sync()
+    tseq = (time.perf_counter() - t0) / steps
+    with hqt.Ensemble(sts) as ens:
+        ens.step(2)
+        t0 = time.perf_counter()
+        ens.step(steps)
+        tens = (time.perf_counter() - t0) / steps
+    out["small_64x64x12_fp64"] = {"members": members, "ms_per_step": round(1e3 * tens, 3),
+                                  "member_steps_per_s": round(members / tens, 1),
+                                  "members_one_by_one_ms_per_step": round(1e3 * tseq, 3),
+                                  "speedup": round(tseq / tens, 3)}
+    for st in sts:
+        st.close()
+    return out
+
+
 def field_errors(a, b):
     names = ("Psi", "A_t", "A_x", "A_y", "rho")
     out = {}
@@ -133,6 +213,12 @@ def main():
     ap.add_argument("--nz", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving (Poisson arrivals) leg")
+    ap.add_argument("--ensemble-members", type=int, default=48)
+    ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--serving-rate", type=float, default=2.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serving-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="also run the plain copy kernel on MB megabytes (traffic-counter calibration)")
     args = ap.parse_args()
@@ -222,6 +308,13 @@ def main():
             mixed["O_abs_rel_err"] = float(np.max(np.abs(np.abs(Om) - np.abs(O64))) / np.max(np.abs(O64)))
             mixed["vortices"] = diagnostics.countN(Om)
         out["mixed"] = mixed
+
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_leg(args.ensemble_members, args.ensemble_steps)
+
+    if not args.no_serving:
+        from hqt import loadgen
+        out["serving"] = loadgen.run(rate=args.serving_rate, duration=args.serving_seconds, seed=0)
 
     if not args.no_cpu:
         cps, ncs = cpu_baseline(nx, ny, nz, L, state, axb)

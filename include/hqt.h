@@ -107,7 +107,7 @@ int hqt_vortex_census(hqt_ctx* ctx, int32_t* counts, int8_t* charge, double* Oab
 
 /* ---- initial state on the device: tevol_driven_cluster.m:81-102, tevol_lattice.m:101-135 -- */
 /* The stationary profile (Z1, Z2, Vy1, rhoi on the chebdif(nz0) nodes, as the reference's
- * data/reference/*.mat hold it) is splined to the run's nz nodes (interp1 'spline', :84),
+ * data/reference .mat files hold it) is splined to the run's nz nodes (interp1 'spline', :84),
  * gauge-rotated by theta = -Lzr(A_t/f) (:90-93) and spread over the plane with the pattern
  *   amp(x,y)   = (1 + amp_noise g1) (1 - sum_v exp(-|r - r_v|^2/sigma^2))    (sigma <= 0: no dips)
  *   phase(x,y) = phase_noise g2 + sum_v q_v atan2(y - vy_v, x - vx_v)

@@ -21,7 +21,7 @@ enum LKind { K_LINE, K_ZPREP, K_ZSTAGE, K_ZPOST, K_WPART, K_WAPPLY, K_NAN };
 
 struct Launch {
     int kind = 0;
-    int dir = 0, lkind = 0, Q = 1, nt = 1, ngroups = 1, nz = 0, fp32 = 0, blocks = 0, nmem = 0;
+    int dir = 0, lkind = 0, Q = 1, nt = 1, ngroups = 1, nz = 0, fp32 = 0, wpe = 2, blocks = 0, nmem = 0;
     size_t lds = 0;
     size_t desc_off = 0, boff_off = 0;   // offsets into the device plan buffer
 };
@@ -110,12 +110,14 @@ bool without_filter(const hqt_ctx* c) { return c->p.filter == 0; }
 template <class D, class DescFn, class BlocksFn>
 void add_z(std::vector<Launch>& plan, PlanImage& img, const std::vector<hqt_ctx*>& mem, int kind, bool by_nz,
            bool (*use)(const hqt_ctx*), DescFn desc, BlocksFn blocks) {
-    auto cls = classes<std::pair<int, int>>(
-        mem, [&](hqt_ctx* c) { return std::make_pair(c->fp32, by_nz ? c->nz : 0); }, [&](hqt_ctx* c) { return use(c); });
+    auto cls = classes<std::tuple<int, int, int>>(
+        mem, [&](hqt_ctx* c) { return std::make_tuple(c->fp32, by_nz ? c->nz : 0, kind == K_ZSTAGE ? c->zwpe : 2); },
+        [&](hqt_ctx* c) { return use(c); });
     for (auto& kv : cls) {
         Launch L = make_launch<D>(img, kind, kv.second, desc, blocks);
-        L.fp32 = kv.first.first;
-        L.nz = kv.first.second;
+        L.fp32 = std::get<0>(kv.first);
+        L.nz = std::get<1>(kv.first);
+        L.wpe = std::get<2>(kv.first);
         plan.push_back(L);
     }
 }
@@ -153,7 +155,7 @@ hipError_t issue(const hqt_ensemble* e, const Launch& L) {
             return launch_line_pass(static_cast<const LPass*>(d), M, L.blocks, L.ngroups, L.lds, L.Q, L.nt, L.dir, L.fp32,
                                     L.lkind, s);
         case K_ZPREP: return launch_zprep(static_cast<const ZPrep*>(d), M, L.blocks, L.nz, s);
-        case K_ZSTAGE: return launch_zstage(static_cast<const ZStage*>(d), M, L.blocks, L.nz, L.fp32, s);
+        case K_ZSTAGE: return launch_zstage(static_cast<const ZStage*>(d), M, L.blocks, L.nz, L.fp32, L.wpe, s);
         case K_ZPOST: return launch_zpost(static_cast<const ZPost*>(d), M, L.blocks, L.nz, L.fp32, s);
         case K_WPART: return launch_wind_partial(static_cast<const Wind*>(d), M, L.blocks, s);
         case K_WAPPLY: return launch_wind_apply(static_cast<const Wind*>(d), M, L.blocks, L.fp32, s);

@@ -274,6 +274,11 @@ int tune_lines(hqt_ctx* c) {
         set_lb(H, lb, dir);
         if (line_pass_lds_bytes(H, c->fp32) > kLdsCap) set_lb(H, 16, dir);
     }
+    // occupancy target of the radial stage kernel: a fixed rule, so that results never depend on a
+    // timing.  Measured on the MI355X (301^2 and 256^2, both precisions): 3 waves per SIMD is 8-10%
+    // faster for nz = 17..20 (two 16-row tiles, few spills), equal below (the kernel fits 3 waves
+    // anyway) and 25-50% slower above (heavy spills).  The params flags can force either.
+    c->zwpe = (c->p.flags & HQT_RADIAL_W3) ? 3 : (c->p.flags & HQT_RADIAL_W2) ? 2 : ((c->nz >= 17 && c->nz <= 20) ? 3 : 2);
     if (ea) (void)hipEventDestroy(ea);
     if (eb) (void)hipEventDestroy(eb);
     if (rc) return rc;
@@ -289,7 +294,7 @@ int enqueue_step(hqt_ctx* c) {
     for (int s = 0; s < 4; ++s) {
         if ((rc = line(c, PX1 + s, 0, KN_LX_STAGE))) return rc;
         if ((rc = line(c, PY1 + s, 1, KN_LY_STAGE))) return rc;
-        if ((rc = run(c, KN_ZSTAGE, [&] { return launch_zstage(&c->dd->zs[s], kOne, zb, nz, f32, st); }))) return rc;
+        if ((rc = run(c, KN_ZSTAGE, [&] { return launch_zstage(&c->dd->zs[s], kOne, zb, nz, f32, c->zwpe, st); }))) return rc;
     }
     if ((rc = line(c, PXC, 0, KN_LX_CONS))) return rc;
     if ((rc = line(c, PYC, 1, KN_LY_CONS))) return rc;
@@ -697,7 +702,7 @@ int hqt_rhs(hqt_ctx* c, const double* Z1, const double* Z2, const double* Z3, co
     if ((rc = run(c, KN_ZPREP, [&] { return launch_zprep(&c->dd->zpre[1], kOne, zb, c->nz, c->stream); }))) return rc;
     if ((rc = line(c, PXR, 0, KN_LX_RHS))) return rc;
     if ((rc = line(c, PYR, 1, KN_LY_RHS))) return rc;
-    if ((rc = run(c, KN_ZRHS, [&] { return launch_zstage(&c->dd->zs[4], kOne, zb, c->nz, c->fp32, c->stream); }))) return rc;
+    if ((rc = run(c, KN_ZRHS, [&] { return launch_zstage(&c->dd->zs[4], kOne, zb, c->nz, c->fp32, c->zwpe, c->stream); }))) return rc;
     if (int rs_ = hqt_sync_streams(c)) return rs_;
     HIPC(hipMemcpy(c->flags, saved, sizeof(saved), hipMemcpyHostToDevice));
     if (F1) {

@@ -42,6 +42,7 @@ struct hqt_ctx {
     hipStream_t stream = nullptr;
     std::vector<void*> allocs;
     int64_t bytes = 0;               // device bytes allocated by this context
+    int zwpe = 2;                    // occupancy target of the radial stage kernel (tuned)
 
     void *ZR, *ZI, *ZAx, *ZAy;
     void *SA[4], *SB[4], *AC[4];

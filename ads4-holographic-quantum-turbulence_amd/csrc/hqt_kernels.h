@@ -169,7 +169,8 @@ hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, 
 size_t line_pass_lds_bytes(const LPass& P, int fp32);
 // Radial kernels: grid.x per member = zblocks(nxy); nz is shared by the members of one launch.
 inline int zblocks(int64_t nxy) { return (int)((nxy + 63) / 64); }
-hipError_t launch_zstage(const ZStage* d, Members M, int blocks, int nz, int fp32, hipStream_t s);
+// wpe: 2 or 3 waves per SIMD targeted by the register allocation (see k_zstage).
+hipError_t launch_zstage(const ZStage* d, Members M, int blocks, int nz, int fp32, int wpe, hipStream_t s);
 hipError_t launch_zpost(const ZPost* d, Members M, int blocks, int nz, int fp32, hipStream_t s);
 hipError_t launch_zprep(const ZPrep* d, Members M, int blocks, int nz, hipStream_t s);
 // Winding: partial sums (grid.x per member = ny), then the removal (wind_apply_blocks per member).

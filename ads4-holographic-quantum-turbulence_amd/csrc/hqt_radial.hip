@@ -64,15 +64,15 @@ __global__ __launch_bounds__(256) void k_wind_apply(const Wind* __restrict__ Ap,
 
 bool zkernels_support_nz(int nz) { return nz >= 4 && nz <= 32; }
 
-#define HQT_DECL(n) extern template hipError_t zstage_launch_nz<n>(const ZStage*, Members, int, int, hipStream_t); \
+#define HQT_DECL(n) extern template hipError_t zstage_launch_nz<n>(const ZStage*, Members, int, int, int, hipStream_t); \
                     extern template hipError_t zpost_launch_nz<n>(const ZPost*, Members, int, int, hipStream_t); \
                     extern template hipError_t zprep_launch_nz<n>(const ZPrep*, Members, int, hipStream_t);
 HQT_NZ_LIST(HQT_DECL)
 #undef HQT_DECL
 
-hipError_t launch_zstage(const ZStage* d, Members M, int blocks, int nz, int fp32, hipStream_t s) {
+hipError_t launch_zstage(const ZStage* d, Members M, int blocks, int nz, int fp32, int wpe, hipStream_t s) {
     switch (nz) {
-#define HQT_ZS(n) case n: return zstage_launch_nz<n>(d, M, blocks, fp32, s);
+#define HQT_ZS(n) case n: return zstage_launch_nz<n>(d, M, blocks, fp32, wpe, s);
         HQT_NZ_LIST(HQT_ZS)
 #undef HQT_ZS
         default: return hipErrorInvalidValue;

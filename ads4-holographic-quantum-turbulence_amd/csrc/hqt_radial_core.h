@@ -158,10 +158,12 @@ __device__ __forceinline__ void zstage_a(const ZStage& A, int64_t c, bool cok, i
     }
 }
 
-// grid: x = ceil(nxy/64) blocks of 4 waves, 16 columns per wave.
-template <int NZ, typename TS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_zstage(const ZStage* __restrict__ Ap,
-                                                                                          const Members M) {
+// grid: x = ceil(nxy/64) blocks of 4 waves, 16 columns per wave.  WPE: the occupancy the
+// register allocation targets (2 waves per SIMD fit every nz without spills; 3 hide more memory
+// latency at the cost of some spills from nz = 18 up); the host picks per (nz, precision) by timing.
+template <int NZ, typename TS, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_zstage(const ZStage* __restrict__ Ap,
+                                                                                            const Members M) {
     const MemberBlock mb = member_block(M);
     const ZStage& A = Ap[mb.m];
     if (stopped(A.flags)) return;
@@ -339,9 +341,14 @@ __global__ __launch_bounds__(256) void k_zprep(const ZPrep* __restrict__ Ap, con
 
 // Per-nz launchers; each hqt_radial_nz_*.hip translation unit instantiates a range of nz.
 template <int NZ>
-hipError_t zstage_launch_nz(const ZStage* d, Members M, int blocks, int fp32, hipStream_t s) {
-    if (fp32) hipLaunchKernelGGL((k_zstage<NZ, float>), dim3(blocks), dim3(256), 0, s, d, M);
-    else hipLaunchKernelGGL((k_zstage<NZ, double>), dim3(blocks), dim3(256), 0, s, d, M);
+hipError_t zstage_launch_nz(const ZStage* d, Members M, int blocks, int fp32, int wpe, hipStream_t s) {
+    if (wpe == 3) {
+        if (fp32) hipLaunchKernelGGL((k_zstage<NZ, float, 3>), dim3(blocks), dim3(256), 0, s, d, M);
+        else hipLaunchKernelGGL((k_zstage<NZ, double, 3>), dim3(blocks), dim3(256), 0, s, d, M);
+    } else {
+        if (fp32) hipLaunchKernelGGL((k_zstage<NZ, float, 2>), dim3(blocks), dim3(256), 0, s, d, M);
+        else hipLaunchKernelGGL((k_zstage<NZ, double, 2>), dim3(blocks), dim3(256), 0, s, d, M);
+    }
     return hipGetLastError();
 }
 template <int NZ>

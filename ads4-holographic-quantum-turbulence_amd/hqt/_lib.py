@@ -16,6 +16,8 @@ HQT_FP64, HQT_MIXED = 0, 1
 HQT_NO_GRAPH = 1
 HQT_LINES16 = 2
 HQT_LINES32 = 4
+HQT_RADIAL_W2 = 8
+HQT_RADIAL_W3 = 16
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "csrc", "libhqt.so")

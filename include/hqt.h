@@ -40,6 +40,8 @@ extern "C" {
 #define HQT_NO_GRAPH 1      /* launch every kernel of a step directly instead of replaying the step's graph */
 #define HQT_LINES16  2      /* x/y line passes with 16 lines per work-group (no tuning) */
 #define HQT_LINES32  4      /* x/y line passes with 32 lines per work-group where LDS allows (no tuning) */
+#define HQT_RADIAL_W2 8     /* radial stage kernel compiled for 2 waves per SIMD (default except nz = 17..20) */
+#define HQT_RADIAL_W3 16    /* radial stage kernel compiled for 3 waves per SIMD (default for nz = 17..20) */
 
 #define HQT_FP64   0        /* plain form: FP64 end to end */
 #define HQT_MIXED  1        /* x/y operators, filter, stage arithmetic FP32; z operators FP64 */
@@ -55,8 +57,10 @@ typedef struct hqt_params {
     int32_t winding;        /* 1: winding removal after each step (:136-139); 0: off (tevol_lattice.m:168-170) */
     int32_t filter;         /* 1: Fourier filter after each step (:142); 0: off */
     int32_t precision;      /* HQT_FP64 or HQT_MIXED */
-    int32_t flags;          /* HQT_NO_GRAPH, HQT_LINES16, HQT_LINES32 (default: lines per work-group
-                               measured per pass at hqt_ctx_create and kept for the process) */
+    int32_t flags;          /* HQT_NO_GRAPH, HQT_LINES16/32 (default: lines per work-group measured per
+                               pass at hqt_ctx_create and kept for the process; results are the same
+                               bitwise), HQT_RADIAL_W2/W3 (radial kernel occupancy; results may differ
+                               in the last bits between the two) */
     const double* ax_b;     /* [nx*ny] A_x(z=0) restored each step, or NULL for ax_b_const (:133: 0) */
     const double* ay_b;     /* [nx*ny] A_y(z=0) restored each step, or NULL for ay_b_const (:133: -vy) */
     double  ax_b_const, ay_b_const;

@@ -88,7 +88,7 @@ void add_line(std::vector<Launch>& plan, PlanImage& img, const std::vector<hqt_c
         [&](hqt_ctx* c) { return use(c); });
     for (auto& kv : cls) {
         Launch L = make_launch<LPass>(img, K_LINE, kv.second, [&](hqt_ctx* c) { return c->hp[pass]; },
-                                      [&](hqt_ctx* c) { return c->hp[pass].ntiles; });
+                                      [&](hqt_ctx* c) { return line_blocks(c->hp[pass]); });
         L.dir = dir;
         L.fp32 = std::get<0>(kv.first);
         L.Q = std::get<1>(kv.first);

@@ -96,8 +96,9 @@ int download_as(double* dst, const void* src, size_t n, int dt) {
 
 // ---- line-pass descriptor helpers --------------------------------------------------------
 // Lines per work-group (16 or 32) and the grid that follows from it.
-void set_lb(LPass& P, int lb, int dir) {
+void set_lb(LPass& P, int lb, int dir, int tpb = 1) {
     P.lb = lb;
+    P.tpb = tpb;
     P.nxc = (P.nx + lb - 1) / lb;
     if (dir == 0) P.ntiles = (int)(((int64_t)P.ny * P.nz + lb - 1) / lb);
     else P.ntiles = P.nxc * P.nz;
@@ -200,7 +201,7 @@ int line_kind(int name) {
 int line(hqt_ctx* c, int pass, int dir, int name) {
     const LPass& H = c->hp[pass];
     return run(c, name, [&] {
-        return launch_line_pass(c->dp + pass, kOne, H.ntiles, H.ngroups, line_pass_lds_bytes(H, c->fp32), H.Mp / 64,
+        return launch_line_pass(c->dp + pass, kOne, line_blocks(H), H.ngroups, line_pass_lds_bytes(H, c->fp32), H.Mp / 64,
                                 H.lb / 16, dir, c->fp32, line_kind(name), c->stream);
     });
 }
@@ -223,11 +224,12 @@ int pass_dir(int p) { return ((p >= PY1 && p <= PY4) || p == PYC || p == PYF || 
 
 constexpr size_t kLdsCap = 160 * 1024;
 
-// Lines per work-group for every line pass: forced by the params flags, else measured once per
-// (pass, precision, grid) and kept for the process.  Both choices give bitwise identical
-// results (each output's MFMA accumulation order is the same), so tuning never changes numbers.
+// Line tiles of every line pass: lines per tile (16 or 32) and tiles per work-group (1 or 4),
+// forced by the params flags (then 1 tile per work-group), else measured once per (pass,
+// precision, grid) and kept for the process.  Every choice gives bitwise identical results (each
+// output's MFMA accumulation order is the same), so tuning never changes numbers.
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int, int, int>, int> g_tune;
+std::map<std::tuple<int, int, int, int, int>, int> g_tune;   // -> 100 * lb + tpb
 
 int tune_lines(hqt_ctx* c) {
     const int forced = (c->p.flags & HQT_LINES32) ? 32 : (c->p.flags & HQT_LINES16) ? 16 : 0;
@@ -236,43 +238,47 @@ int tune_lines(hqt_ctx* c) {
     for (int p = 0; p < NPASS && !rc; ++p) {
         LPass& H = c->hp[p];
         const int dir = pass_dir(p);
-        int lb = forced;
+        int choice = forced ? 100 * forced + 1 : 0;
         const auto key = std::make_tuple(p, c->fp32, c->nx, c->ny, c->nz);
-        if (!lb) {
+        if (!choice) {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             auto it = g_tune.find(key);
-            if (it != g_tune.end()) lb = it->second;
+            if (it != g_tune.end()) choice = it->second;
         }
-        if (!lb) {
+        if (!choice) {
             if (!ea && (hipEventCreate(&ea) != hipSuccess || hipEventCreate(&eb) != hipSuccess)) {
                 rc = fail(HQT_ERR_HIP, "hipEventCreate");
                 break;
             }
             float best = 0;
-            for (int cand : {16, 32}) {
-                set_lb(H, cand, dir);
-                const size_t lds = line_pass_lds_bytes(H, c->fp32);
-                if (lds > kLdsCap) continue;
-                float ms = 0;
-                hipError_t e = hipMemcpy(c->dp + p, &H, sizeof(H), hipMemcpyHostToDevice);
-                for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {
-                    if (rep == 1) e = hipEventRecord(ea, c->stream);
-                    if (e == hipSuccess)
-                        e = launch_line_pass(c->dp + p, kOne, H.ntiles, H.ngroups, lds, H.Mp / 64, cand / 16, dir, c->fp32,
-                                             0, c->stream);
+            for (int lb : {16, 32}) {
+                for (int tpb : {1, 4}) {
+                    if (dir == 1 && tpb != 1) continue;      // y-lines: one tile per work-group
+                    set_lb(H, lb, dir, tpb);
+                    const size_t lds = line_pass_lds_bytes(H, c->fp32);
+                    if (lds > kLdsCap) continue;
+                    float ms = 0;
+                    hipError_t e = hipMemcpy(c->dp + p, &H, sizeof(H), hipMemcpyHostToDevice);
+                    for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {
+                        if (rep == 1) e = hipEventRecord(ea, c->stream);
+                        if (e == hipSuccess)
+                            e = launch_line_pass(c->dp + p, kOne, line_blocks(H), H.ngroups, lds, H.Mp / 64, lb / 16, dir,
+                                                 c->fp32, 0, c->stream);
+                    }
+                    if (e == hipSuccess) e = hipEventRecord(eb, c->stream);
+                    if (e == hipSuccess) e = hipEventSynchronize(eb);
+                    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ea, eb);
+                    if (e != hipSuccess) { rc = fail(HQT_ERR_HIP, std::string("line-pass tuning: ") + hipGetErrorString(e)); break; }
+                    if (!choice || ms < best) { best = ms; choice = 100 * lb + tpb; }
                 }
-                if (e == hipSuccess) e = hipEventRecord(eb, c->stream);
-                if (e == hipSuccess) e = hipEventSynchronize(eb);
-                if (e == hipSuccess) e = hipEventElapsedTime(&ms, ea, eb);
-                if (e != hipSuccess) { rc = fail(HQT_ERR_HIP, std::string("line-pass tuning: ") + hipGetErrorString(e)); break; }
-                if (!lb || ms < best) { best = ms; lb = cand; }
+                if (rc) break;
             }
             if (rc) break;
             std::lock_guard<std::mutex> lk(g_tune_mu);
-            g_tune[key] = lb;
+            g_tune[key] = choice;
         }
-        set_lb(H, lb, dir);
-        if (line_pass_lds_bytes(H, c->fp32) > kLdsCap) set_lb(H, 16, dir);
+        set_lb(H, choice / 100, dir, choice % 100);
+        if (line_pass_lds_bytes(H, c->fp32) > kLdsCap) set_lb(H, 16, dir, 1);
     }
     // occupancy target of the radial stage kernel: a fixed rule, so that results never depend on a
     // timing.  Measured on the MI355X (301^2 and 256^2, both precisions): 3 waves per SIMD is 8-10%
@@ -838,10 +844,10 @@ int hqt_ff(int32_t nx, int32_t ny, int32_t nslices, const double* x_re, const do
         add_out(h, out, DT_F64, 0, OP_QRY, 1, OP_NQIY);     // Re(Py (a + i b)) = Qr a - Qi b
         if (hipMemcpy(dp, hp, sizeof(hp), hipMemcpyHostToDevice) != hipSuccess) { rc = fail(HQT_ERR_HIP, "hipMemcpy pass"); break; }
         if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) { rc = fail(HQT_ERR_HIP, "stream"); break; }
-        hipError_t e = launch_line_pass(dp, kOne, hp[0].ntiles, hp[0].ngroups, line_pass_lds_bytes(hp[0], 0), hp[0].Mp / 64,
+        hipError_t e = launch_line_pass(dp, kOne, line_blocks(hp[0]), hp[0].ngroups, line_pass_lds_bytes(hp[0], 0), hp[0].Mp / 64,
                                         1, 0, 0, 2, s);
         if (e == hipSuccess)
-            e = launch_line_pass(dp + 1, kOne, hp[1].ntiles, hp[1].ngroups, line_pass_lds_bytes(hp[1], 0), hp[1].Mp / 64, 1,
+            e = launch_line_pass(dp + 1, kOne, line_blocks(hp[1]), hp[1].ngroups, line_pass_lds_bytes(hp[1], 0), hp[1].Mp / 64, 1,
                                  1, 0, 2, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) { rc = fail(HQT_ERR_HIP, std::string("hqt_ff: ") + hipGetErrorString(e)); break; }

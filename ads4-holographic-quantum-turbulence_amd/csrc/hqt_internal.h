@@ -105,3 +105,6 @@ int64_t ctx_footprint(const hqt_params& p);
 
 // Joins a pending asynchronous checkpoint write and frees the writer's buffers.
 void checkpoint_writer_release(hqt_ctx* c);
+
+// Work-groups of a line pass along grid.x (line tiles / tiles per work-group, rounded up).
+inline int line_blocks(const LPass& P) { return (P.ntiles + P.tpb - 1) / P.tpb; }

@@ -62,8 +62,9 @@ struct LPass {
     int32_t N, Np, S;          // line length, padded length (multiple of 16), LDS row stride
     int32_t Mp, TL;            // output positions padded to 64 (4 waves x Q tiles), Toeplitz table length
     int32_t nx, ny, nz;
-    int32_t lb;                // lines per work-group: 16 or 32
-    int32_t ntiles;            // line tiles of lb lines (grid.x)
+    int32_t lb;                // lines per line tile: 16 or 32
+    int32_t tpb;               // line tiles per work-group (grid.x = ceil(ntiles / tpb))
+    int32_t ntiles;            // line tiles of lb lines
     int32_t nxc;               // dir 1: x chunks of lb
     int32_t ngroups;           // grid.y
     LGroup g[LP_MAXG];

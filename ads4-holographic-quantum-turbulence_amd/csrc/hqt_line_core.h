@@ -50,10 +50,219 @@ __device__ __forceinline__ void stv(void* p, int dt, int64_t i, T v) {
 // profilers report the three uses separately; the code is identical.
 // NT: 16-line tiles per work-group (LB = 16*NT lines).  With NT = 2 every operator fragment
 // read from LDS feeds two MFMAs (one per line tile), halving the LDS operand traffic per MFMA
-// at twice the accumulator registers and LDS line storage; which one is faster depends on the
-// line length and precision, and the autotuner picks per pass.
+// at twice the accumulator registers and LDS line storage.
+// This form serves the x-lines: a work-group processes P.tpb consecutive line tiles, the
+// operator tables are staged once, and the lines of the next (tile, source) pair are loaded into
+// registers while the MFMAs of the current pair run, so global-load latency overlaps the matrix
+// work (k_line_pass_one below serves the y-lines).  Which NT and tpb are faster depends on the
+// line length and precision; the host measures and picks per pass.  Every output is accumulated
+// by the same MFMA sequence in all variants.
 template <typename T, int DIR, int Q, int KIND, int NT>
 __global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restrict__ Pp, const Members M) {
+    constexpr int LB = 16 * NT;
+    constexpr int PF = 4 * Q * NT;            // staged elements per thread: LB*Np/256 <= LB*Mp/256 = 4*Q*NT
+    const MemberBlock mb = member_block(M);
+    const LPass& P = Pp[mb.m];
+    if ((int)blockIdx.y >= P.ngroups) return;
+    if (__builtin_nontemporal_load(&P.flags[0]) != 0) return;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    const int S = P.S, Np = P.Np, N = P.N, TL = P.TL;
+    T* V = reinterpret_cast<T*>(smem_raw);
+    T* C = V + LB * S;
+    const LGroup& G = P.g[blockIdx.y];
+    const int tid = threadIdx.x;
+
+    for (int s = 0; s < G.nslots; ++s) {
+        const T* src = static_cast<const T*>(P.ops[G.slot_op[s]]);
+        for (int t = tid; t < TL; t += LP_THREADS) C[s * TL + t] = src[t];
+    }
+
+    const int64_t nl = (int64_t)P.ny * P.nz;
+    const int tile0 = mb.bx * P.tpb;
+    const int ntile = min(P.tpb, P.ntiles - tile0);
+    const int npair = ntile * G.ntiles;
+
+    typedef typename MF<T>::v4 v4;
+    v4 acc[2][NT][Q];
+#pragma unroll
+    for (int o = 0; o < 2; ++o)
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+#pragma unroll
+            for (int q = 0; q < Q; ++q) acc[o][u][q] = v4{0, 0, 0, 0};
+
+    const int lane = tid & 63, ln = lane & 15, lk = lane >> 4;
+    // wave index and its tile count are wave-uniform: keep them in scalar registers so the
+    // per-tile guards below are scalar branches, not exec-masked ones
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+
+    // Lines of (tile, source) pair `pr` into registers.  x-lines: thread -> line m = tid/16, line
+    // points j = tid%16 + 16 r (16 lanes read 128 contiguous bytes of one line); y-lines: thread
+    // -> x offset m = tid%LB, line points j = tid/LB + (256/LB) r (LB lanes read LB contiguous
+    // x values).  Either way a thread's elements are a fixed stride apart: one base offset, no
+    // divisions, PF registers for the data.
+    constexpr int JS = DIR == 0 ? 16 : LP_THREADS / LB;     // j step between a thread's elements
+    const int mrow = DIR == 0 ? (tid >> 4) : (tid % LB);
+    const int jt = DIR == 0 ? (tid & 15) : (tid / LB);
+    T pf[PF];
+    auto fetch = [&](int pr) {
+        const int lt = tile0 + pr / G.ntiles, t = pr % G.ntiles;
+        const void* src = G.src[t];
+        const int dt = G.src_dt[t];
+        bool okm;
+        int64_t base, step;
+        if (DIR == 0) {
+            // LB lines per tile; with LB = 32 a thread covers lines mrow and mrow + 16
+            const int64_t line = (int64_t)lt * LB + mrow;
+            okm = line < nl;
+            base = line * P.nx;
+            step = 0;
+        } else {
+            const int x = (lt % P.nxc) * LB + mrow, zz = lt / P.nxc;
+            okm = x < P.nx;
+            base = (int64_t)x + (int64_t)P.nx * P.ny * zz;
+            step = (int64_t)P.nx;
+        }
+#pragma unroll
+        for (int r = 0; r < PF; ++r) {
+            T v = 0;
+            if (DIR == 0) {
+                const int half = r / (PF / NT), rr = r % (PF / NT);     // NT = 2: second line at +16
+                const int j = jt + JS * rr;
+                const int64_t line = (int64_t)lt * LB + mrow + 16 * half;
+                if (line < nl && j < N && j < Np) v = ldv<T>(src, dt, line * P.nx + j);
+            } else {
+                const int j = jt + JS * r;
+                if (okm && j < N) v = ldv<T>(src, dt, base + step * j);
+            }
+            pf[r] = v;
+        }
+        (void)base;
+    };
+    auto deposit = [&]() {
+#pragma unroll
+        for (int r = 0; r < PF; ++r) {
+            if (DIR == 0) {
+                const int half = r / (PF / NT), rr = r % (PF / NT);
+                const int j = jt + JS * rr;
+                if (j < Np) V[(mrow + 16 * half) * S + j] = pf[r];
+            } else {
+                const int j = jt + JS * r;
+                if (j < Np) V[mrow * S + j] = pf[r];
+            }
+        }
+    };
+
+    // x-lines prefetch the next pair into registers; y-lines use k_line_pass_one (launchers)
+    if (npair > 0) fetch(0);
+    for (int pr = 0; pr < npair; ++pr) {
+        const int t = pr % G.ntiles;
+        __syncthreads();                 // the previous pair's MFMAs are done with V
+        deposit();
+        __syncthreads();
+        if (pr + 1 < npair) fetch(pr + 1);
+#pragma unroll
+        for (int o = 0; o < 2; ++o) {
+            if (o >= G.nout) break;
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                if (e >= G.out[o].nterms || G.out[o].term[e].tile != t) continue;
+                // table entry of T[i][j] is tab[i - j + Np + 4]; lane (ln, lk) reads i = i0 + ln, j = j0 + lk
+                const T* Cs = C + G.out[o].term[e].slot * TL + Np + 4 + ln - lk + wave * 16;
+                const T* Vr = V + ln * S + lk;
+                // two k-steps per iteration with ping-pong operand registers: the loads of
+                // one k-step are in flight while the other's MFMAs issue.  The last loads run
+                // one k-step past the tile (values unused; the table is padded for them and
+                // the line rows are followed by the table in LDS).
+                T va0[NT], va1[NT];
+                T cb0[Q], cb1[Q];
+#pragma unroll
+                for (int u = 0; u < NT; ++u) va0[u] = Vr[16 * u * S];
+#pragma unroll
+                for (int q = 0; q < Q; ++q) cb0[q] = Cs[64 * q];
+                for (int j0 = 0; j0 < Np; j0 += 8) {
+#pragma unroll
+                    for (int u = 0; u < NT; ++u) va1[u] = Vr[16 * u * S + j0 + 4];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) cb1[q] = Cs[64 * q - j0 - 4];
+#pragma unroll
+                    for (int u = 0; u < NT; ++u)
+#pragma unroll
+                        for (int q = 0; q < Q; ++q)
+                            acc[o][u][q] = DIR == 0 ? MF<T>::mma(va0[u], cb0[q], acc[o][u][q])
+                                                    : MF<T>::mma(cb0[q], va0[u], acc[o][u][q]);
+#pragma unroll
+                    for (int u = 0; u < NT; ++u) va0[u] = Vr[16 * u * S + j0 + 8];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) cb0[q] = Cs[64 * q - j0 - 8];
+#pragma unroll
+                    for (int u = 0; u < NT; ++u)
+#pragma unroll
+                        for (int q = 0; q < Q; ++q)
+                            acc[o][u][q] = DIR == 0 ? MF<T>::mma(va1[u], cb1[q], acc[o][u][q])
+                                                    : MF<T>::mma(cb1[q], va1[u], acc[o][u][q]);
+                }
+            }
+        }
+        if (t != G.ntiles - 1) continue;
+        // all sources of this line tile are in: write its outputs and restart the accumulators
+        const int lt = tile0 + pr / G.ntiles;
+        int64_t line0 = 0;
+        int z = 0, x0 = 0;
+        if (DIR == 0) {
+            line0 = (int64_t)lt * LB;
+        } else {
+            x0 = (lt % P.nxc) * LB;
+            z = lt / P.nxc;
+        }
+#pragma unroll
+        for (int o = 0; o < 2; ++o) {
+            if (o >= G.nout) break;
+            const LOut& O = G.out[o];
+            bool bad = false;
+#pragma unroll
+            for (int u = 0; u < NT; ++u)
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int i0 = (wave + 4 * q) * 16;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    int64_t idx;
+                    bool ok;
+                    if (DIR == 0) {
+                        const int64_t line = line0 + 16 * u + MF<T>::row(lk, r);
+                        const int i = i0 + ln;
+                        ok = line < nl && i < N;
+                        idx = line * P.nx + i;
+                    } else {
+                        const int x = x0 + 16 * u + ln;
+                        const int i = i0 + MF<T>::row(lk, r);
+                        ok = x < P.nx && i < N;
+                        idx = (int64_t)x + (int64_t)P.nx * (i + (int64_t)P.ny * z);
+                    }
+                    if (ok) {
+                        T v = acc[o][u][q][r];
+                        if (O.addend) v += ldv<T>(O.addend, O.add_dt, idx);
+                        stv<T>(O.out, O.out_dt, idx, v);
+                        if (v != v) bad = true;
+                    }
+                    acc[o][u][q][r] = 0;
+                }
+            }
+            if (O.nan_check) {
+                if (__any(bad) && lane == 0) {
+                    const int step = __atomic_load_n(&P.flags[1], __ATOMIC_RELAXED);
+                    atomicCAS(&P.flags[0], 0, step > 0 ? step : 1);
+                }
+            }
+        }
+    }
+}
+
+// One line tile per work-group, staged global to LDS inside the tile loop: the y-line form
+// (lower register use keeps more waves resident, which the strided y accesses need).
+template <typename T, int DIR, int Q, int KIND, int NT>
+__global__ __launch_bounds__(LP_THREADS) void k_line_pass_one(const LPass* __restrict__ Pp, const Members M) {
     constexpr int LB = 16 * NT;
     const MemberBlock mb = member_block(M);
     const LPass& P = Pp[mb.m];
@@ -210,7 +419,9 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restric
 // first use.
 template <typename T, int DIR, int KIND, int NT, int Q>
 hipError_t line_launch_one(const LPass* d, Members M, dim3 grid, size_t lds, hipStream_t s) {
-    const void* fn = reinterpret_cast<const void*>(&k_line_pass<T, DIR, Q, KIND, NT>);
+    const void* fn;
+    if constexpr (DIR == 0) fn = reinterpret_cast<const void*>(&k_line_pass<T, DIR, Q, KIND, NT>);
+    else fn = reinterpret_cast<const void*>(&k_line_pass_one<T, DIR, Q, KIND, NT>);
     if (lds > 64 * 1024) {
         static bool done = false;
         if (!done) {
@@ -219,7 +430,8 @@ hipError_t line_launch_one(const LPass* d, Members M, dim3 grid, size_t lds, hip
             done = true;
         }
     }
-    hipLaunchKernelGGL((k_line_pass<T, DIR, Q, KIND, NT>), grid, dim3(LP_THREADS), lds, s, d, M);
+    if constexpr (DIR == 0) hipLaunchKernelGGL((k_line_pass<T, DIR, Q, KIND, NT>), grid, dim3(LP_THREADS), lds, s, d, M);
+    else hipLaunchKernelGGL((k_line_pass_one<T, DIR, Q, KIND, NT>), grid, dim3(LP_THREADS), lds, s, d, M);
     return hipGetLastError();
 }
 

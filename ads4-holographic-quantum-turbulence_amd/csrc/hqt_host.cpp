@@ -47,6 +47,13 @@ int64_t ctx_footprint(const hqt_params& p) {
     b += 2 * nxy * 8 + nxy + 8 + 2 * (int64_t)census_blocks(nxy) * 8;   // diagnostics
     b += (nz * nz + (nz - 1) * (nz - 1) + 5 * nz) * 8;   // radial operators
     b += 6 * ((int64_t)line_geom(p.nx).TL + (int64_t)line_geom(p.ny).TL) * es;   // x/y operator tables
+    for (int n : {p.nx, p.ny}) {                          // rank-4 filter tables, when used
+        if (p.flags & HQT_DENSE_FILTER) break;
+        std::vector<uint8_t> mk;
+        matlab_masks(n, mk);
+        std::vector<double> t;
+        if (filter_lowrank(n, mk.data(), line_geom(n), 1.0, t)) b += 2 * (int64_t)t.size() * es;
+    }
     return b;
 }
 
@@ -136,13 +143,16 @@ int slot_for(LGroup& g, int op) {
     return g.nslots++;
 }
 
+// A term with lr = 1 is a rank-4 operator: its `slot` is the ops index of the [V | U] table
+// (no LDS slot), applied by the kernel as U (V^T line).
 void add_out(LGroup& g, void* out, int odt, int tile0, int op0, int tile1 = -1, int op1 = -1,
-             const void* addend = nullptr, int adt = DT_F64, int nan_check = 0) {
+             const void* addend = nullptr, int adt = DT_F64, int nan_check = 0, int lr0 = 0, int lr1 = 0) {
     LOut& o = g.out[g.nout++];
     std::memset(&o, 0, sizeof(o));
     o.out = out; o.out_dt = odt; o.addend = addend; o.add_dt = adt; o.nan_check = nan_check;
-    o.term[o.nterms++] = LTerm{tile0, slot_for(g, op0)};
-    if (tile1 >= 0) o.term[o.nterms++] = LTerm{tile1, slot_for(g, op1)};
+    o.term[o.nterms++] = LTerm{tile0, lr0 ? op0 : slot_for(g, op0), lr0, 0};
+    if (tile1 >= 0) o.term[o.nterms++] = LTerm{tile1, lr1 ? op1 : slot_for(g, op1), lr1, 0};
+    if (lr0 || lr1) g.has_lr = 1;
 }
 
 // Toeplitz tables of the x/y operators in the context precision.
@@ -171,6 +181,19 @@ int build_tables(hqt_ctx* c, int N, double L, const uint8_t* mask, int base) {
         if (rc) return rc;
         rc = upload_as(c, c->ops[base + t], tabs[t].data(), tabs[t].size(), c->tdt);
         if (rc) return rc;
+    }
+    const int dir = base == OP_D1X ? 0 : 1, lrbase = dir ? OP_UVY : OP_UVX;
+    std::vector<double> uv, nuv;
+    c->lr[dir] = !(c->p.flags & HQT_DENSE_FILTER) && filter_lowrank(N, mask, Np, 1.0, uv) &&
+                 filter_lowrank(N, mask, Np, -1.0, nuv);
+    if (c->lr[dir]) {
+        const std::vector<double>* tt[2] = {&uv, &nuv};
+        for (int t = 0; t < 2; ++t) {
+            int rc = dalloc(c, tt[t]->size() * c->es, &c->ops[lrbase + t]);
+            if (rc) return rc;
+            rc = upload_as(c, c->ops[lrbase + t], tt[t]->data(), tt[t]->size(), c->tdt);
+            if (rc) return rc;
+        }
     }
     return HQT_OK;
 }
@@ -472,12 +495,15 @@ int build_context(hqt_ctx* c) {
         P = base_pass(nx, nx, ny, nz, 0, ops, c->flags);
         LPass& Q = c->hp[PYF];
         Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags);
+        // Im(P) has rank 4 for the reference's masks: applied as U (V^T x) instead of a dense product
         for (int i = 0; i < 5; ++i) {
             LGroup& g = add_group(P, f[i], fdt[i]);
             add_out(g, c->fa[i], T, 0, OP_PRX);
-            add_out(g, c->fb[i], T, 0, OP_PIX);
+            if (c->lr[0]) add_out(g, c->fb[i], T, 0, OP_UVX, -1, -1, nullptr, DT_F64, 0, 1);
+            else add_out(g, c->fb[i], T, 0, OP_PIX);
             LGroup& h = add_group(Q, c->fa[i], T, c->fb[i], T);
-            add_out(h, f[i], fdt[i], 0, OP_QRY, 1, OP_NQIY, nullptr, DT_F64, 1);
+            if (c->lr[1]) add_out(h, f[i], fdt[i], 0, OP_QRY, 1, OP_NUVY, nullptr, DT_F64, 1, 0, 1);
+            else add_out(h, f[i], fdt[i], 0, OP_QRY, 1, OP_NQIY, nullptr, DT_F64, 1);
         }
     }
     HIPC(hipMemcpy(c->dp, c->hp, sizeof(c->hp), hipMemcpyHostToDevice));

@@ -28,7 +28,11 @@ inline constexpr const char* kNames[KN_COUNT] = {
     "zprep", "line_x_stage", "line_y_stage", "zstage", "line_x_cons", "line_y_cons", "zpost", "winding",
     "line_x_filter", "line_y_filter", "nancheck", "line_x_rhs", "line_y_rhs", "zrhs"};
 
-enum Op { OP_D1X, OP_D2X, OP_ND1X, OP_PRX, OP_PIX, OP_NPIX, OP_D1Y, OP_D2Y, OP_ND1Y, OP_QRY, OP_QIY, OP_NQIY, OP_COUNT };
+enum Op {
+    OP_D1X, OP_D2X, OP_ND1X, OP_PRX, OP_PIX, OP_NPIX, OP_D1Y, OP_D2Y, OP_ND1Y, OP_QRY, OP_QIY, OP_NQIY,
+    OP_UVX, OP_NUVX, OP_UVY, OP_NUVY,   // rank-4 forms of +-Im(P_x), +-Im(P_y) ([V | U] tables)
+    OP_COUNT
+};
 
 enum Pass { PX1, PX2, PX3, PX4, PY1, PY2, PY3, PY4, PXC, PYC, PXF, PYF, PXR, PYR, NPASS };
 
@@ -43,6 +47,7 @@ struct hqt_ctx {
     std::vector<void*> allocs;
     int64_t bytes = 0;               // device bytes allocated by this context
     int zwpe = 2;                    // occupancy target of the radial stage kernel (tuned)
+    bool lr[2] = {false, false};     // rank-4 tables of the filter's imaginary part exist (x, y)
 
     void *ZR, *ZI, *ZAx, *ZAy;
     void *SA[4], *SB[4], *AC[4];

@@ -43,10 +43,12 @@ __device__ __forceinline__ MemberBlock member_block(const Members M) {
 // of up to two terms op(tile) plus an optional point-wise addend field.
 // ---------------------------------------------------------------------------------------
 constexpr int LP_MAXG = 8;      // groups per pass
-constexpr int LP_MAXOPS = 12;   // operator table entries per pass
+constexpr int LP_MAXOPS = 16;   // operator table entries per pass
 constexpr int LP_THREADS = 256;
 
-struct LTerm { int32_t tile; int32_t slot; };
+// A term of an output: op(tile).  lr = 0: dense Toeplitz operator in LDS slot `slot`;
+// lr = 1: rank-4 operator U V^T with the [V (Np x 4) | U (Mp x 4)] row-major table at P.ops[slot].
+struct LTerm { int32_t tile; int32_t slot; int32_t lr; int32_t pad; };
 struct LOut {
     void* out; const void* addend;
     int32_t out_dt, add_dt;
@@ -54,7 +56,7 @@ struct LOut {
     int32_t nan_check;
 };
 struct LGroup {
-    int32_t ntiles; int32_t src_dt[2]; const void* src[2];
+    int32_t ntiles; int32_t has_lr; int32_t src_dt[2]; const void* src[2];
     int32_t nslots; int32_t slot_op[4];
     int32_t nout; LOut out[2];
 };

@@ -13,7 +13,7 @@ size_t line_pass_lds_bytes(const LPass& P, int fp32) {
     int maxslots = 0;
     for (int g = 0; g < P.ngroups; ++g) maxslots = P.g[g].nslots > maxslots ? P.g[g].nslots : maxslots;
     const size_t es = fp32 ? sizeof(float) : sizeof(double);
-    return es * ((size_t)P.lb * P.S + (size_t)maxslots * P.TL);
+    return es * ((size_t)P.lb * P.S + (size_t)maxslots * P.TL + 6 * 4 * (size_t)P.lb);   // + rank-4 dots
 }
 
 hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, size_t lds, int Q, int nt, int dir,

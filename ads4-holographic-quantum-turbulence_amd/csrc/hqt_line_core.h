@@ -46,6 +46,71 @@ __device__ __forceinline__ void stv(void* p, int dt, int64_t i, T v) {
     else static_cast<float*>(p)[i] = (float)v;
 }
 
+// Rank-4 terms (the filter's imaginary part, U V^T with U, V of 4 columns; filter kernels only).
+// Dots D^T (lines x 4) = X V on MFMA: the 4 waves split the k range (ranks padded to 16 MFMA
+// columns), their partial tiles go to LDS and are summed in wave order, so the result does not
+// depend on the lines per tile.  Table layout: V (Np x 4) then U (Mp x 4), row-major.
+template <typename T, int NT>
+__device__ __forceinline__ void lr_dots(const T* __restrict__ X, int S, int Np, const T* __restrict__ vt, T* DLp,
+                                        T* dl, int wave, int lane) {
+    typedef typename MF<T>::v4 v4;
+    constexpr int LB = 16 * NT;
+    const int ln = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+        v4 acc = v4{0, 0, 0, 0};
+        for (int j0 = 4 * wave; j0 < Np; j0 += 16) {
+            const T a = X[(16 * u + ln) * S + j0 + lk];
+            const T b = ln < 4 ? vt[4 * (j0 + lk) + ln] : (T)0;
+            acc = MF<T>::mma(a, b, acc);
+        }
+        if (ln < 4) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) DLp[(wave * LB + 16 * u + MF<T>::row(lk, r)) * 4 + ln] = acc[r];
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < LB * 4; idx += LP_THREADS)
+        dl[idx] = ((DLp[idx] + DLp[LB * 4 + idx]) + DLp[2 * LB * 4 + idx]) + DLp[3 * LB * 4 + idx];
+}
+
+// Dots of every rank-4 term whose source is tile t (one per output at most); ends with a barrier.
+template <typename T, int NT>
+__device__ __forceinline__ void lr_terms(const LPass& P, const LGroup& G, int t, const T* X, int S, int Np, T* DLp,
+                                         T* DL, int wave, int lane) {
+    constexpr int LB = 16 * NT;
+    for (int o = 0; o < G.nout; ++o)
+        for (int e = 0; e < G.out[o].nterms; ++e) {
+            const LTerm& tm = G.out[o].term[e];
+            if (tm.lr && tm.tile == t) {
+                lr_dots<T, NT>(X, S, Np, static_cast<const T*>(P.ops[tm.slot]), DLp, DL + o * LB * 4, wave, lane);
+                __syncthreads();
+            }
+        }
+}
+
+// acc += U D for output o's rank-4 term: one MFMA k-step per output tile (the 4 ranks are K).
+template <typename T, int DIR, int Q, int NT>
+__device__ __forceinline__ void lr_expand(const LPass& P, const LOut& O, int o, int Np, const T* DL, int wave, int lane,
+                                          typename MF<T>::v4 (&acc)[NT][Q]) {
+    constexpr int LB = 16 * NT;
+    const int ln = lane & 15, lk = lane >> 4;
+    for (int e = 0; e < O.nterms; ++e) {
+        if (!O.term[e].lr) continue;
+        const T* ut = static_cast<const T*>(P.ops[O.term[e].slot]) + 4 * Np;
+#pragma unroll
+        for (int u = 0; u < NT; ++u) {
+            const T d = DL[o * LB * 4 + (16 * u + ln) * 4 + lk];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int i0 = (wave + 4 * q) * 16;
+                const T uu = ut[4 * (i0 + ln) + lk];
+                acc[u][q] = DIR == 0 ? MF<T>::mma(d, uu, acc[u][q]) : MF<T>::mma(uu, d, acc[u][q]);
+            }
+        }
+    }
+}
+
 // KIND only tags the instantiation (0: RHS stage, 1: constraint, 2: filter) so that
 // profilers report the three uses separately; the code is identical.
 // NT: 16-line tiles per work-group (LB = 16*NT lines).  With NT = 2 every operator fragment
@@ -70,6 +135,8 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restric
     T* V = reinterpret_cast<T*>(smem_raw);
     T* C = V + LB * S;
     const LGroup& G = P.g[blockIdx.y];
+    T* DL = C + G.nslots * TL;               // rank-4 dots: 2 outputs x LB lines x 4
+    T* DLp = DL + 2 * LB * 4;                // rank-4 partial dots: 4 waves x LB lines x 4
     const int tid = threadIdx.x;
 
     for (int s = 0; s < G.nslots; ++s) {
@@ -161,12 +228,15 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restric
         deposit();
         __syncthreads();
         if (pr + 1 < npair) fetch(pr + 1);
+        if constexpr (KIND == 2) {
+            if (G.has_lr) lr_terms<T, NT>(P, G, t, V, S, Np, DLp, DL, wave, lane);
+        }
 #pragma unroll
         for (int o = 0; o < 2; ++o) {
             if (o >= G.nout) break;
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
-                if (e >= G.out[o].nterms || G.out[o].term[e].tile != t) continue;
+                if (e >= G.out[o].nterms || G.out[o].term[e].tile != t || G.out[o].term[e].lr) continue;
                 // table entry of T[i][j] is tab[i - j + Np + 4]; lane (ln, lk) reads i = i0 + ln, j = j0 + lk
                 const T* Cs = C + G.out[o].term[e].slot * TL + Np + 4 + ln - lk + wave * 16;
                 const T* Vr = V + ln * S + lk;
@@ -220,6 +290,9 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restric
             if (o >= G.nout) break;
             const LOut& O = G.out[o];
             bool bad = false;
+            if constexpr (KIND == 2) {
+                if (G.has_lr) lr_expand<T, DIR, Q, NT>(P, O, o, Np, DL, wave, lane, acc[o]);
+            }
 #pragma unroll
             for (int u = 0; u < NT; ++u)
 #pragma unroll
@@ -273,6 +346,8 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass_one(const LPass* __res
     T* V = reinterpret_cast<T*>(smem_raw);
     T* C = V + LB * S;
     const LGroup& G = P.g[blockIdx.y];
+    T* DL = C + G.nslots * TL;               // rank-4 dots: 2 outputs x LB lines x 4
+    T* DLp = DL + 2 * LB * 4;                // rank-4 partial dots: 4 waves x LB lines x 4
     const int tid = threadIdx.x;
 
     for (int s = 0; s < G.nslots; ++s) {
@@ -327,12 +402,15 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass_one(const LPass* __res
             }
         }
         __syncthreads();
+        if constexpr (KIND == 2) {
+            if (G.has_lr) lr_terms<T, NT>(P, G, t, V, S, Np, DLp, DL, wave, lane);
+        }
 #pragma unroll
         for (int o = 0; o < 2; ++o) {
             if (o >= G.nout) break;
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
-                if (e >= G.out[o].nterms || G.out[o].term[e].tile != t) continue;
+                if (e >= G.out[o].nterms || G.out[o].term[e].tile != t || G.out[o].term[e].lr) continue;
                 // table entry of T[i][j] is tab[i - j + Np + 4]; lane (ln, lk) reads i = i0 + ln, j = j0 + lk
                 const T* Cs = C + G.out[o].term[e].slot * TL + Np + 4 + ln - lk + wave * 16;
                 const T* Vr = V + ln * S + lk;
@@ -377,6 +455,9 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass_one(const LPass* __res
         if (o >= G.nout) break;
         const LOut& O = G.out[o];
         bool bad = false;
+        if constexpr (KIND == 2) {
+            if (G.has_lr) lr_expand<T, DIR, Q, NT>(P, O, o, Np, DL, wave, lane, acc[o]);
+        }
 #pragma unroll
         for (int u = 0; u < NT; ++u)
 #pragma unroll

@@ -193,6 +193,30 @@ void filter_tables(int N, const uint8_t* mask, std::vector<double>& pr, std::vec
     }
 }
 
+bool filter_lowrank(int N, const uint8_t* mask, const LineGeom& g, double scale, std::vector<double>& tab) {
+    // Im P[d] = -(1/N) sum_{k masked} sin(2 pi k d / N): a masked k whose conjugate N-k is also
+    // masked cancels; an asymmetric pair leaves c sin(theta d), c = -(mask[k] - mask[N-k]) / N,
+    // and sin(theta (i - j)) = sin(theta i) cos(theta j) - cos(theta i) sin(theta j).
+    std::vector<int> ks;
+    std::vector<long double> cs;
+    for (int k = 1; 2 * k < N; ++k) {
+        const int a = mask[k] ? 1 : 0, b = mask[N - k] ? 1 : 0;
+        if (a != b) { ks.push_back(k); cs.push_back(-(long double)(a - b) / N); }
+    }
+    if (ks.size() > 2) return false;
+    tab.assign(4 * (size_t)g.Np + 4 * (size_t)g.Mp, 0.0);
+    for (size_t p = 0; p < ks.size(); ++p) {
+        for (int j = 0; j < N; ++j) {
+            const long double th = 2 * (long double)PI * (long double)(((long long)ks[p] * j) % N) / N;
+            tab[4 * (size_t)j + 2 * p] = (double)std::cos(th);                                        // V
+            tab[4 * (size_t)j + 2 * p + 1] = (double)std::sin(th);
+            tab[4 * (size_t)g.Np + 4 * (size_t)j + 2 * p] = (double)(scale * cs[p] * std::sin(th));      // U
+            tab[4 * (size_t)g.Np + 4 * (size_t)j + 2 * p + 1] = (double)(-scale * cs[p] * std::cos(th));
+        }
+    }
+    return true;
+}
+
 void circulant_square(const std::vector<double>& c, std::vector<double>& c2) {
     const int N = (int)c.size();
     c2.assign(N, 0.0);

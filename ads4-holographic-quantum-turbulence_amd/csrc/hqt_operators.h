@@ -30,4 +30,10 @@ int newton_vy(int nz, double zh, double small, double D, const double* Z1, const
               double vy, double* z1, double* z2, double* VY1, int* it, double* err, int it_max = 1000,
               double tol = 1e-6);
 
+// Rank-4 factorisation of the imaginary part of the filter circulant: Im P[i][j] = sum_k
+// U[i][k] V[j][k] when at most two masked modes have an unmasked conjugate (every other masked
+// mode pairs with a masked conjugate and cancels).  tab gets Np x 4 V then Mp x 4 U (row-major,
+// zero past N), U scaled by `scale`.  Returns false (tab untouched) for more asymmetric pairs.
+bool filter_lowrank(int N, const uint8_t* mask, const LineGeom& g, double scale, std::vector<double>& tab);
+
 }  // namespace hqt

@@ -18,6 +18,7 @@ HQT_LINES16 = 2
 HQT_LINES32 = 4
 HQT_RADIAL_W2 = 8
 HQT_RADIAL_W3 = 16
+HQT_DENSE_FILTER = 32
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "csrc", "libhqt.so")

@@ -303,11 +303,11 @@ int tune_lines(hqt_ctx* c) {
         set_lb(H, choice / 100, dir, choice % 100);
         if (line_pass_lds_bytes(H, c->fp32) > kLdsCap) set_lb(H, 16, dir, 1);
     }
-    // occupancy target of the radial stage kernel: a fixed rule, so that results never depend on a
-    // timing.  Measured on the MI355X (301^2 and 256^2, both precisions): 3 waves per SIMD is 8-10%
-    // faster for nz = 17..20 (two 16-row tiles, few spills), equal below (the kernel fits 3 waves
-    // anyway) and 25-50% slower above (heavy spills).  The params flags can force either.
-    c->zwpe = (c->p.flags & HQT_RADIAL_W3) ? 3 : (c->p.flags & HQT_RADIAL_W2) ? 2 : ((c->nz >= 17 && c->nz <= 20) ? 3 : 2);
+    // occupancy target of the radial stage kernel: fixed, so that results never depend on a timing.
+    // Measured on the MI355X (301^2 for nz = 12..20, 256^2 for nz = 21..32, both precisions): 2 waves
+    // per SIMD is as fast as 3 up to nz = 16 and 4-55% faster above (3 waves spill).  The params
+    // flags can force 3.
+    c->zwpe = (c->p.flags & HQT_RADIAL_W3) ? 3 : 2;
     if (ea) (void)hipEventDestroy(ea);
     if (eb) (void)hipEventDestroy(eb);
     if (rc) return rc;

@@ -6,8 +6,9 @@
 // registers as "slots" (slot s, lane group g = lane/16 holds row 4s + g of column lane%16),
 // which is both the B-operand layout of v_mfma_f64_16x16x4 for k-block s and the layout of
 // its accumulator (gfx950: accumulator register r of row tile t holds row 16t + 4r + g).
-// A matvec is therefore RT x KS MFMAs with the operator tiles kept in registers, and
-// matvecs chain (Lz(Lz Psi), Lzr(E)) without leaving registers.  The point-wise parts of
+// A matvec is therefore RT x KS MFMAs, and matvecs chain (Lz(Lz Psi), Lzr(E)) without
+// leaving registers.  The stage kernel reads its operator tiles from LDS; the small post-step
+// kernels keep them in registers.  The point-wise parts of
 // the right-hand sides run on the same slots, with 16 consecutive columns per lane group,
 // so every global access is a 128-byte coalesced segment.
 //
@@ -71,6 +72,37 @@ __device__ __forceinline__ void mv(const double (&A)[ZG<NZ>::RT][ZG<NZ>::KS], co
     }
 }
 
+// Operator tiles in LDS, one entry per lane: T[(t*KS + s)*64 + lane] = A[t][s] of that lane.
+template <int NZ>
+__device__ __forceinline__ void stage_op(const double* __restrict__ M, int ldm, int n, double* T) {
+    constexpr int KS = ZG<NZ>::KS, RT = ZG<NZ>::RT;
+    for (int idx = threadIdx.x; idx < RT * KS * 64; idx += blockDim.x) {
+        const int lane = idx & 63, ts = idx >> 6, t = ts / KS, s = ts % KS;
+        const int i = 16 * t + (lane & 15), j = 4 * s + (lane >> 4);
+        T[idx] = (i < n && j < n) ? M[i * ldm + j] : 0.0;
+    }
+}
+// out = A * in with the operator tiles read from LDS at every use (volatile: no reuse across
+// calls, so the tiles occupy no registers between matvecs; the explicit LDS address space keeps
+// them ds_reads, since address-space inference leaves volatile accesses generic).
+template <int NZ>
+__device__ __forceinline__ void mvl(const double* T, int lane, const double (&in)[ZG<NZ>::KS], double (&out)[ZG<NZ>::KS]) {
+    constexpr int KS = ZG<NZ>::KS;
+#pragma unroll
+    for (int t = 0; t < ZG<NZ>::RT; ++t) {
+        v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const double a = *(const volatile __attribute__((address_space(3))) double*)(
+                (const __attribute__((address_space(3))) double*)(T) + (t * KS + s) * 64 + lane);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, in[s], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            if (4 * t + r < KS) out[4 * t + r] = acc[r];
+    }
+}
+
 // Load a [nxy, nz] field into slots (0 outside the grid).
 template <int NZ, typename TS>
 __device__ __forceinline__ void load_slots(const void* f, int64_t c, bool cok, int g, int64_t n, double (&v)[ZG<NZ>::KS]) {
@@ -110,152 +142,293 @@ __device__ __forceinline__ void rk_update(int stage, double kv, void* acc, void*
 //   E4 = 0.5*Lz(dyAt + f*Lz Ay) + 0.5*curl4 + Im(conj(Psi)*dyPsi) - Ay*|Psi|^2
 //   F1 = Lzr(E1), F3 = Lzr(E3), F4 = Lzr(E4), F2 = (1-z^3)*Lz(dxAx + dyAy)
 // (the reference's current term -0.5i*(conj(Psi)*dPsi - Psi*d conj(Psi)) equals Im(conj(Psi)*dPsi)).
-// Every input field is read once; the stage state Psi and the x/y derivatives of Psi stay in
-// registers across the three equations.
-template <int NZ, typename TS, int DIR>
-__device__ __forceinline__ void zstage_a(const ZStage& A, int64_t c, bool cok, int g, const double (&Lz)[ZG<NZ>::RT][ZG<NZ>::KS],
-                                         const double (&L1)[ZG<NZ>::RT][ZG<NZ>::KS], const double (&sR)[ZG<NZ>::KS],
-                                         const double (&sI)[ZG<NZ>::KS], const double (&dR)[ZG<NZ>::KS],
-                                         const double (&dI)[ZG<NZ>::KS]) {
+//
+// Each wave's run time is a chain of memory round trips (a 301 x 301 grid is two rounds of
+// resident blocks, so that chain, not bandwidth or occupancy, sets the kernel time).  The
+// kernel therefore has few serial memory phases, and each phase's loads are issued before the
+// previous phase's arithmetic: every field access is a buffer load/store whose out-of-range
+// offset reads 0 or drops the write (columns past nxy, rows past nz: no branches), the RK
+// bookkeeping selects buffer resources and values by stage instead of branching, the operator
+// tiles and row coefficients sit in LDS, and scheduling barriers pin each prefetch to its phase.
+// Order: F2 (and the charge update); one point-wise pass over every term of E1, E3, E4 without
+// a z derivative; F3; F4; F1.
+typedef __amdgpu_buffer_rsrc_t Rs;
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+// Element index past every field: nx, ny <= 512 and nz <= 32 (checked at context creation) keep
+// every real index below 2^23, and 8 * Z_OOB still fits the 31-bit buffer offset.
+constexpr uint32_t Z_OOB = 0x0FFFFFFFu;
+
+__device__ __forceinline__ Rs rs_make(const void* p, int64_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T> __device__ __forceinline__ double bld(Rs r, uint32_t e) {
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(e * 8u), 0, 0));
+    else
+        return (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(e * 4u), 0, 0));
+}
+template <typename T> __device__ __forceinline__ void bst(Rs r, uint32_t e, double v) {
+    if constexpr (sizeof(T) == 8)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)(e * 8u), 0, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, (float)v), r, (int)(e * 4u), 0, 0);
+}
+
+// Resources of one evolved field's RK bookkeeping at this stage (:120-128):
+//   stage 1: acc = k;        next = z + k/2
+//   stage 2: acc = acc + 2k; next = z + k/2
+//   stage 3: acc = acc + 2k; next = z + k
+//   stage 4: z = z + (1/6)*(acc + k)
+struct RkRs { Rs acc_ld, z_ld, st1, st2; };
+template <typename TS>
+__device__ __forceinline__ RkRs rk_rs(int stage, void* acc, void* z, void* nxt, int64_t bytes) {
+    const Rs ra = rs_make(acc, bytes), rz = rs_make(z, bytes), rn = rs_make(nxt, bytes), r0 = rs_make(acc, 0);
+    return RkRs{stage == 1 ? r0 : ra, rz, stage == 4 ? rz : ra, stage == 4 ? r0 : rn};
+}
+template <typename TS>
+__device__ __forceinline__ void rk_apply(int stage, const RkRs& R, uint32_t e, double kv) {
+    const double a0 = bld<TS>(R.acc_ld, e), z0 = bld<TS>(R.z_ld, e);
+    const double v1 = stage == 4 ? z0 + (1.0 / 6.0) * (a0 + kv) : (stage == 1 ? kv : a0 + 2.0 * kv);
+    const double v2 = z0 + (stage == 3 ? kv : kv * 0.5);
+    bst<TS>(R.st1, e, v1);
+    bst<TS>(R.st2, e, v2);
+}
+
+template <int NZ> struct ZLds {
+    static constexpr int KS = ZG<NZ>::KS, RT = ZG<NZ>::RT, OPN = RT * KS * 64, NZP = 4 * KS;
+    double lz[OPN], l1[OPN];
+    double hf[NZP], cd[NZP], cp[NZP], f1[NZP], f2[NZP];
+};
+
+template <typename T> __device__ __forceinline__ T bldr(Rs r, uint32_t e) {
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(e * 8u), 0, 0));
+    else
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(e * 4u), 0, 0));
+}
+// no instruction crosses this point: keeps each prefetch within its phase
+#define ZSB() __builtin_amdgcn_sched_barrier(0)
+
+template <int NZ, typename TS, int MODE>
+__device__ __forceinline__ void zstage_body(const ZStage& A, const ZLds<NZ>& Z, int lane, int g, int64_t c, bool cok,
+                                            const uint32_t (&e)[ZG<NZ>::KS], double (&sR)[ZG<NZ>::KS],
+                                            double (&sI)[ZG<NZ>::KS], double (&dv)[ZG<NZ>::KS]) {
     constexpr int KS = ZG<NZ>::KS;
-    const int64_t n = A.nxy;
-    const void* sA = DIR == 0 ? A.sAx : A.sAy;
-    const void* dAt = DIR == 0 ? A.dxAt : A.dyAt;
-    const void* curl = DIR == 0 ? A.curl3 : A.curl4;
-    double a[KS], u[KS], e[KS];
-    load_slots<NZ, TS>(sA, c, cok, g, n, a);
-    mv<NZ>(Lz, a, u);
+    const int64_t bt = A.nxy * NZ * (int64_t)sizeof(TS), b8 = A.nxy * NZ * 8;
+    const int stage = A.stage;
+    const double h = A.h;
+    double w[KS], u[KS], p1r[KS], p1i[KS], p3[KS], p4[KS];
+    // point-wise operands in flight during F2
+    TS ax[KS], ay[KS], lr[KS], li[KS], xr[KS], xi[KS], yr[KS], yi[KS], c3[KS], c4[KS];
+    double dat[KS];
+    {
+        const Rs rdat = rs_make(A.dzAt, b8), rax = rs_make(A.sAx, bt), ray = rs_make(A.sAy, bt);
+        const Rs rlr = rs_make(A.lapR, bt), rli = rs_make(A.lapI, bt);
+        const Rs rxr = rs_make(A.dxR, bt), rxi = rs_make(A.dxI, bt), ryr = rs_make(A.dyR, bt), ryi = rs_make(A.dyI, bt);
+        const Rs rc3 = rs_make(A.curl3, bt), rc4 = rs_make(A.curl4, bt);
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
-        const int k = 4 * s + g;
-        u[s] = (cok && k < NZ) ? ld<TS>(dAt, c + n * k) + A.zo.f1[k] * u[s] : 0.0;
-    }
-    mv<NZ>(Lz, u, e);
-#pragma unroll
-    for (int s = 0; s < KS; ++s) {
-        const int k = 4 * s + g;
-        double ev = 0.0;
-        if (cok && k < NZ) {
-            const int64_t p = c + n * k;
-            const double sr = sR[s], si = sI[s], av = a[s];
-            const double cur = sr * dI[s] - si * dR[s];
-            ev = 0.5 * e[s] + 0.5 * ld<TS>(curl, p) + cur - ((av * sr) * sr + (av * si) * si);
+        for (int s = 0; s < KS; ++s) {
+            dat[s] = bldr<double>(rdat, e[s]);
+            ax[s] = bldr<TS>(rax, e[s]); ay[s] = bldr<TS>(ray, e[s]);
+            lr[s] = bldr<TS>(rlr, e[s]); li[s] = bldr<TS>(rli, e[s]);
+            xr[s] = bldr<TS>(rxr, e[s]); xi[s] = bldr<TS>(rxi, e[s]);
+            yr[s] = bldr<TS>(ryr, e[s]); yi[s] = bldr<TS>(ryi, e[s]);
+            c3[s] = bldr<TS>(rc3, e[s]); c4[s] = bldr<TS>(rc4, e[s]);
         }
-        u[s] = ev;
     }
-    mv<NZ>(L1, u, e);
-    void* acc = DIR == 0 ? A.aAx : A.aAy;
-    void* zf = DIR == 0 ? A.zAx : A.zAy;
-    void* nx = DIR == 0 ? A.nAx : A.nAy;
-    double* Fo = DIR == 0 ? A.F3 : A.F4;
+    ZSB();
+    // ---- F2 (z=0 row feeds the charge update at :128) ----
+    mvl<NZ>(Z.lz, lane, dv, w);
+    if (MODE == 1) {
+        const Rs r = rs_make(A.F2, b8);
+#pragma unroll
+        for (int s = 0; s < KS; ++s) bst<double>(r, e[s], Z.f2[4 * s + g] * w[s]);
+    } else {
+        constexpr int sl = (NZ - 1) / 4, gl = (NZ - 1) % 4;
+        const double kvl = h * (Z.f2[NZ - 1] * w[sl]);
+        const double kv = __shfl(kvl, (lane & 15) + 16 * gl);
+        const Rs arl = rs_make(A.arho, stage == 1 ? 0 : A.nxy * 8);
+        const Rs ars = rs_make(A.arho, stage == 4 ? 0 : A.nxy * 8);
+        const uint32_t ec = cok ? (uint32_t)c : Z_OOB;
+        const double ar0 = bld<double>(arl, ec);
+        bst<double>(ars, g == 0 ? ec : Z_OOB, stage == 1 ? kv : ar0 + 2.0 * kv);
+        const double fin = (1.0 / 6.0) * (ar0 + kv);
+        const Rs rr = rs_make(A.rho, stage == 4 ? b8 : 0);
+#pragma unroll
+        for (int s = 0; s < KS; ++s) bst<double>(rr, e[s], bld<double>(rr, e[s]) - fin);
+    }
+    ZSB();
+    // F3 operands in flight during the point-wise pass
+    TS fa[KS], fd[KS];
+    {
+        const Rs ra = rs_make(A.sAx, bt), rd = rs_make(A.dxAt, bt);
+#pragma unroll
+        for (int s = 0; s < KS; ++s) { fa[s] = bldr<TS>(ra, e[s]); fd[s] = bldr<TS>(rd, e[s]); }
+    }
+    ZSB();
+    // ---- point-wise parts of E1, E3, E4 ----
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
-        const int k = 4 * s + g;
-        if (!cok || k >= NZ) continue;
-        const int64_t p = c + n * k;
-        if (A.mode == 1) { Fo[p] = e[s]; continue; }
-        rk_update<TS>(A.stage, A.h * e[s], acc, zf, nx, p);
+        const double sr = sR[s], si = sI[s];
+        const double axv = ax[s], ayv = ay[s], dvs = dv[s];
+        const double xrv = xr[s], xiv = xi[s], yrv = yr[s], yiv = yi[s];
+        const double cp = Z.cp[4 * s + g];
+        double er = (-0.5 * si) * dat[s], ei = (0.5 * sr) * dat[s];
+        er += cp * sr;               ei += cp * si;
+        const double a2 = (-0.5 * axv) * axv, b2 = (-0.5 * ayv) * ayv;
+        er += a2 * sr;               ei += a2 * si;
+        er += b2 * sr;               ei += b2 * si;
+        er += 0.5 * (double)lr[s];   ei += 0.5 * (double)li[s];
+        er += (0.5 * si) * dvs;      ei += (-0.5 * sr) * dvs;
+        const double qr = axv * xrv + ayv * yrv, qi = axv * xiv + ayv * yiv;
+        er += qi;                    ei -= qr;
+        p1r[s] = er;
+        p1i[s] = ei;
+        p3[s] = 0.5 * (double)c3[s] + (sr * xiv - si * xrv) - ((axv * sr) * sr + (axv * si) * si);
+        p4[s] = 0.5 * (double)c4[s] + (sr * yiv - si * yrv) - ((ayv * sr) * sr + (ayv * si) * si);
+    }
+    ZSB();
+    // ---- F3, F4 ----
+#pragma unroll
+    for (int dir = 0; dir < 2; ++dir) {
+        const double (&pa)[KS] = dir == 0 ? p3 : p4;
+        const RkRs R = dir == 0 ? rk_rs<TS>(stage, A.aAx, A.zAx, A.nAx, bt) : rk_rs<TS>(stage, A.aAy, A.zAy, A.nAy, bt);
+        TS ra0[KS], rz0[KS];
+        if (MODE == 0) {
+#pragma unroll
+            for (int s = 0; s < KS; ++s) { ra0[s] = bldr<TS>(R.acc_ld, e[s]); rz0[s] = bldr<TS>(R.z_ld, e[s]); }
+        }
+#pragma unroll
+        for (int s = 0; s < KS; ++s) w[s] = fa[s];
+        mvl<NZ>(Z.lz, lane, w, u);
+#pragma unroll
+        for (int s = 0; s < KS; ++s) w[s] = (double)fd[s] + Z.f1[4 * s + g] * u[s];
+        if (dir == 0) {
+            // F4 operands in flight during the F3 matvecs
+            const Rs ra = rs_make(A.sAy, bt), rd = rs_make(A.dyAt, bt);
+#pragma unroll
+            for (int s = 0; s < KS; ++s) { fa[s] = bldr<TS>(ra, e[s]); fd[s] = bldr<TS>(rd, e[s]); }
+        }
+        mvl<NZ>(Z.lz, lane, w, u);
+#pragma unroll
+        for (int s = 0; s < KS; ++s) w[s] = 0.5 * u[s] + pa[s];
+        mvl<NZ>(Z.l1, lane, w, u);
+        if (MODE == 1) {
+            const Rs r = rs_make(dir == 0 ? A.F3 : A.F4, b8);
+#pragma unroll
+            for (int s = 0; s < KS; ++s) bst<double>(r, e[s], u[s]);
+        } else {
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const double kv = h * u[s], a0 = ra0[s], z0 = rz0[s];
+                const double v1 = stage == 4 ? z0 + (1.0 / 6.0) * (a0 + kv) : (stage == 1 ? kv : a0 + 2.0 * kv);
+                const double v2 = z0 + (stage == 3 ? kv : kv * 0.5);
+                bst<TS>(R.st1, e[s], v1);
+                bst<TS>(R.st2, e[s], v2);
+            }
+        }
+        ZSB();
+    }
+    // ---- F1 ----
+    {
+        const Rs rat = rs_make(A.At, b8);
+        const RkRs RR = rk_rs<TS>(stage, A.aR, A.zR, A.nR, bt), RI = rk_rs<TS>(stage, A.aI, A.zI, A.nI, bt);
+        double atv[KS];
+        TS aR0[KS], zR0[KS], aI0[KS], zI0[KS];
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            atv[s] = bldr<double>(rat, e[s]);
+            if (MODE == 0) {
+                aR0[s] = bldr<TS>(RR.acc_ld, e[s]); zR0[s] = bldr<TS>(RR.z_ld, e[s]);
+                aI0[s] = bldr<TS>(RI.acc_ld, e[s]); zI0[s] = bldr<TS>(RI.z_ld, e[s]);
+            }
+        }
+        double dR[KS], dI[KS], d2R[KS], d2I[KS];
+        mvl<NZ>(Z.lz, lane, sR, dR);
+        mvl<NZ>(Z.lz, lane, sI, dI);
+        mvl<NZ>(Z.lz, lane, dR, d2R);
+        mvl<NZ>(Z.lz, lane, dI, d2I);
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int k = 4 * s + g;
+            const double at = atv[s];
+            const double hf = Z.hf[k], cd = Z.cd[k];
+            double er = p1r[s] + hf * d2R[s], ei = p1i[s] + hf * d2I[s];
+            er += cd * dR[s];            ei += cd * dI[s];
+            er -= at * dI[s];            ei += at * dR[s];
+            // rows past nz carry no data: keep them exactly zero for the Lzr matvec
+            p1r[s] = k < NZ ? er : 0.0;
+            p1i[s] = k < NZ ? ei : 0.0;
+        }
+        mvl<NZ>(Z.l1, lane, p1r, dR);     // F1 = Lzr(E1); the last row of Lzr is zero
+        mvl<NZ>(Z.l1, lane, p1i, dI);
+        if (MODE == 1) {
+            const Rs rr = rs_make(A.F1R, b8), ri = rs_make(A.F1I, b8);
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                bst<double>(rr, e[s], dR[s]);
+                bst<double>(ri, e[s], dI[s]);
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const RkRs& R = q == 0 ? RR : RI;
+                    const double kv = h * (q == 0 ? dR[s] : dI[s]);
+                    const double a0 = q == 0 ? aR0[s] : aI0[s], z0 = q == 0 ? zR0[s] : zI0[s];
+                    const double v1 = stage == 4 ? z0 + (1.0 / 6.0) * (a0 + kv) : (stage == 1 ? kv : a0 + 2.0 * kv);
+                    const double v2 = z0 + (stage == 3 ? kv : kv * 0.5);
+                    bst<TS>(R.st1, e[s], v1);
+                    bst<TS>(R.st2, e[s], v2);
+                }
+            }
+        }
     }
 }
 
 // grid: x = ceil(nxy/64) blocks of 4 waves, 16 columns per wave.  WPE: the occupancy the
-// register allocation targets (2 waves per SIMD fit every nz without spills; 3 hide more memory
-// latency at the cost of some spills from nz = 18 up); the host picks per (nz, precision) by timing.
+// register allocation targets (2: no spills up to nz = 20; 3 trades spills for occupancy).
 template <int NZ, typename TS, int WPE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_zstage(const ZStage* __restrict__ Ap,
                                                                                             const Members M) {
+    constexpr int KS = ZG<NZ>::KS;
     const MemberBlock mb = member_block(M);
     const ZStage& A = Ap[mb.m];
-    if (stopped(A.flags)) return;
-    constexpr int KS = ZG<NZ>::KS;
+    const int stop = __builtin_amdgcn_readfirstlane(__builtin_nontemporal_load(&A.flags[0]));
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int m = lane & 15, g = lane >> 4;
     const int64_t c = (int64_t)mb.bx * 64 + wave * 16 + m;
     const bool cok = c < A.nxy;
-    const int64_t n = A.nxy;
-    double Lz[ZG<NZ>::RT][KS], L1[ZG<NZ>::RT][KS];
-    load_op<NZ>(A.zo.Lz, NZ, NZ, Lz, m, g);
-    load_op<NZ>(A.zo.Lz1, NZ - 1, NZ - 1, L1, m, g);
-    double sR[KS], sI[KS], xR[KS], xI[KS], yR[KS], yI[KS], dv[KS];
-    load_slots<NZ, TS>(A.sR, c, cok, g, n, sR);
-    load_slots<NZ, TS>(A.sI, c, cok, g, n, sI);
-    load_slots<NZ, TS>(A.dxR, c, cok, g, n, xR);
-    load_slots<NZ, TS>(A.dxI, c, cok, g, n, xI);
-    load_slots<NZ, TS>(A.dyR, c, cok, g, n, yR);
-    load_slots<NZ, TS>(A.dyI, c, cok, g, n, yI);
-    load_slots<NZ, TS>(A.divA, c, cok, g, n, dv);
-    // ---- F1 ----
-    {
-        double dR[KS], dI[KS], eR[KS], eI[KS];
-        mv<NZ>(Lz, sR, dR);
-        mv<NZ>(Lz, sI, dI);
-        {
-            double d2R[KS], d2I[KS];
-            mv<NZ>(Lz, dR, d2R);
-            mv<NZ>(Lz, dI, d2I);
+    uint32_t e[KS];
 #pragma unroll
-            for (int s = 0; s < KS; ++s) {
-                const int k = 4 * s + g;
-                double er = 0.0, ei = 0.0;
-                if (cok && k < NZ) {
-                    const int64_t p = c + n * k;
-                    const double sr = sR[s], si = sI[s];
-                    const double at = A.At[p], dat = A.dzAt[p];
-                    const double ax = ld<TS>(A.sAx, p), ay = ld<TS>(A.sAy, p);
-                    const double lpr = ld<TS>(A.lapR, p), lpi = ld<TS>(A.lapI, p);
-                    const double hf = A.zo.hf[k], cd = A.zo.cd[k], cp = A.zo.cp[k];
-                    er = -at * dI[s];            ei = at * dR[s];
-                    er += (-0.5 * si) * dat;     ei += (0.5 * sr) * dat;
-                    er += hf * d2R[s];           ei += hf * d2I[s];
-                    er += cd * dR[s];            ei += cd * dI[s];
-                    er += cp * sr;               ei += cp * si;
-                    const double a2 = (-0.5 * ax) * ax, b2 = (-0.5 * ay) * ay;
-                    er += a2 * sr;               ei += a2 * si;
-                    er += b2 * sr;               ei += b2 * si;
-                    er += 0.5 * lpr;             ei += 0.5 * lpi;
-                    er += (0.5 * si) * dv[s];    ei += (-0.5 * sr) * dv[s];
-                    const double qr = ax * xR[s] + ay * yR[s], qi = ax * xI[s] + ay * yI[s];
-                    er += qi;                    ei -= qr;
-                }
-                eR[s] = er;
-                eI[s] = ei;
-            }
-        }
-        mv<NZ>(L1, eR, dR);     // F1 = Lzr(E1); the last row of Lzr is zero
-        mv<NZ>(L1, eI, dI);
-#pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            const int k = 4 * s + g;
-            if (!cok || k >= NZ) continue;
-            const int64_t p = c + n * k;
-            if (A.mode == 1) { A.F1R[p] = dR[s]; A.F1I[p] = dI[s]; continue; }
-            rk_update<TS>(A.stage, A.h * dR[s], A.aR, A.zR, A.nR, p);
-            rk_update<TS>(A.stage, A.h * dI[s], A.aI, A.zI, A.nI, p);
-        }
+    for (int s = 0; s < KS; ++s) {
+        const int k = 4 * s + g;
+        e[s] = (cok && k < NZ) ? (uint32_t)(c + A.nxy * k) : Z_OOB;
     }
-    // ---- F3, F4 ----
-    zstage_a<NZ, TS, 0>(A, c, cok, g, Lz, L1, sR, sI, xR, xI);
-    zstage_a<NZ, TS, 1>(A, c, cok, g, Lz, L1, sR, sI, yR, yI);
-    // ---- F2 (z=0 row feeds the charge update at :128) ----
-    {
-        double w[KS];
-        mv<NZ>(Lz, dv, w);
+    // first operands in flight before the operator staging
+    const int64_t bt = A.nxy * NZ * (int64_t)sizeof(TS);
+    const Rs rsr = rs_make(A.sR, bt), rsi = rs_make(A.sI, bt), rdv = rs_make(A.divA, bt);
+    double sR[KS], sI[KS], dv[KS];
 #pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            const int k = 4 * s + g;
-            if (!cok || k >= NZ) continue;
-            if (A.mode == 1) { A.F2[c + n * k] = A.zo.f2[k] * w[s]; continue; }
-            if (k != NZ - 1) continue;
-            const double kv = A.h * (A.zo.f2[k] * w[s]);
-            double* ar = A.arho;
-            if (A.stage == 1) ar[c] = kv;
-            else if (A.stage < 4) ar[c] = ar[c] + 2.0 * kv;
-            else {
-                const double fin = (1.0 / 6.0) * (ar[c] + kv);
-                for (int q = 0; q < NZ; ++q) A.rho[c + n * q] = A.rho[c + n * q] - fin;
-            }
-        }
+    for (int s = 0; s < KS; ++s) {
+        dv[s] = bld<TS>(rdv, e[s]);
+        sR[s] = bld<TS>(rsr, e[s]);
+        sI[s] = bld<TS>(rsi, e[s]);
     }
+    __shared__ ZLds<NZ> Z;
+    stage_op<NZ>(A.zo.Lz, NZ, NZ, Z.lz);
+    stage_op<NZ>(A.zo.Lz1, NZ - 1, NZ - 1, Z.l1);
+    for (int i = threadIdx.x; i < 5 * ZLds<NZ>::NZP; i += 256) {
+        const int q = i / ZLds<NZ>::NZP, k = i % ZLds<NZ>::NZP;
+        const double* src = q == 0 ? A.zo.hf : q == 1 ? A.zo.cd : q == 2 ? A.zo.cp : q == 3 ? A.zo.f1 : A.zo.f2;
+        double* dst = q == 0 ? Z.hf : q == 1 ? Z.cd : q == 2 ? Z.cp : q == 3 ? Z.f1 : Z.f2;
+        dst[k] = k < NZ ? src[k] : 0.0;
+    }
+    __syncthreads();
+    if (stop != 0) return;           // same value for the whole grid: the block returns together
+    if (A.mode == 1) zstage_body<NZ, TS, 1>(A, Z, lane, g, c, cok, e, sR, sI, dv);
+    else zstage_body<NZ, TS, 0>(A, Z, lane, g, c, cok, e, sR, sI, dv);
 }
 
 // ---- post-step: A_t constraint solve and boundary reset (:129-133), O for the winding (:136)

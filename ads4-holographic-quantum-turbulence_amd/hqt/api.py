@@ -153,7 +153,7 @@ def make_params(nx, ny, nz, Lx, Ly, mu=6.0, dt=0.02, winding=True, filter=True, 
     """``hqt_params`` for a run, plus the arrays its boundary pointers borrow (keep them alive
     while the params are in use).  ``lines`` = 16 or 32 fixes the lines per work-group of the
     x/y passes; None lets the library measure and pick per pass.  ``radial_waves`` = 2 or 3 fixes the
-    occupancy the radial stage kernel is compiled for (None: the library's rule by nz).
+    occupancy the radial stage kernel is compiled for (None: 2).
     ``dense_filter`` applies the filter's imaginary part as a dense Toeplitz product instead of
     its rank-4 form."""
     p = Params()

@@ -40,8 +40,8 @@ extern "C" {
 #define HQT_NO_GRAPH 1      /* launch every kernel of a step directly instead of replaying the step's graph */
 #define HQT_LINES16  2      /* x/y line passes with 16 lines per work-group (no tuning) */
 #define HQT_LINES32  4      /* x/y line passes with 32 lines per work-group where LDS allows (no tuning) */
-#define HQT_RADIAL_W2 8     /* radial stage kernel compiled for 2 waves per SIMD (default except nz = 17..20) */
-#define HQT_RADIAL_W3 16    /* radial stage kernel compiled for 3 waves per SIMD (default for nz = 17..20) */
+#define HQT_RADIAL_W2 8     /* radial stage kernel compiled for 2 waves per SIMD (the default) */
+#define HQT_RADIAL_W3 16    /* radial stage kernel compiled for 3 waves per SIMD (spills from nz = 13) */
 #define HQT_DENSE_FILTER 32 /* filter's imaginary part as a dense Toeplitz product instead of its rank-4 form
                                (the default; ff.m's 2/3 mask leaves at most two unpaired modes) */
 

@@ -111,6 +111,35 @@ __device__ __forceinline__ void lr_expand(const LPass& P, const LOut& O, int o, 
     }
 }
 
+// Operator tables of a group into LDS: every thread issues all of its table loads (up to 4 slots
+// of TL <= Np + Mp + 8 <= 128 Q + 8 entries) before the first LDS write.
+template <typename T, int Q>
+__device__ __forceinline__ void stage_tables(const LPass& P, const LGroup& G, T* C, int TL, int tid) {
+    constexpr int TPT = (128 * Q + 8 + LP_THREADS - 1) / LP_THREADS;
+    T tv[4][TPT];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        if (s < G.nslots) {
+            const T* src = static_cast<const T*>(P.ops[G.slot_op[s]]);
+#pragma unroll
+            for (int r = 0; r < TPT; ++r) {
+                const int t = tid + r * LP_THREADS;
+                tv[s][r] = t < TL ? src[t] : (T)0;
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        if (s < G.nslots) {
+#pragma unroll
+            for (int r = 0; r < TPT; ++r) {
+                const int t = tid + r * LP_THREADS;
+                if (t < TL) C[s * TL + t] = tv[s][r];
+            }
+        }
+    }
+}
+
 // KIND only tags the instantiation (0: RHS stage, 1: constraint, 2: filter) so that
 // profilers report the three uses separately; the code is identical.
 // NT: 16-line tiles per work-group (LB = 16*NT lines).  With NT = 2 every operator fragment
@@ -139,10 +168,7 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restric
     T* DLp = DL + 2 * LB * 4;                // rank-4 partial dots: 4 waves x LB lines x 4
     const int tid = threadIdx.x;
 
-    for (int s = 0; s < G.nslots; ++s) {
-        const T* src = static_cast<const T*>(P.ops[G.slot_op[s]]);
-        for (int t = tid; t < TL; t += LP_THREADS) C[s * TL + t] = src[t];
-    }
+    stage_tables<T, Q>(P, G, C, TL, tid);
 
     const int64_t nl = (int64_t)P.ny * P.nz;
     const int tile0 = mb.bx * P.tpb;
@@ -350,10 +376,7 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass_one(const LPass* __res
     T* DLp = DL + 2 * LB * 4;                // rank-4 partial dots: 4 waves x LB lines x 4
     const int tid = threadIdx.x;
 
-    for (int s = 0; s < G.nslots; ++s) {
-        const T* src = static_cast<const T*>(P.ops[G.slot_op[s]]);
-        for (int t = tid; t < TL; t += LP_THREADS) C[s * TL + t] = src[t];
-    }
+    stage_tables<T, Q>(P, G, C, TL, tid);
 
     int64_t line0 = 0, nl = 0;
     int z = 0, x0 = 0;
@@ -380,25 +403,59 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass_one(const LPass* __res
     // per-tile guards below are scalar branches, not exec-masked ones
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
+    // Staging of one tile: a thread's PF elements are a fixed stride apart (x-lines: line tid/16,
+    // points tid%16 + 16 r; y-lines: x offset tid%LB, points tid/LB + (256/LB) r), and all PF loads
+    // are issued before the first LDS write, so a tile costs one memory round trip per thread.
+    constexpr int PF = 4 * Q * NT;            // LB*Mp/256 >= LB*Np/256 elements per thread
+    constexpr int JS = DIR == 0 ? 16 : LP_THREADS / LB;
+    const int mrow = DIR == 0 ? (tid >> 4) : (tid % LB);
+    const int jt = DIR == 0 ? (tid & 15) : (tid / LB);
     for (int t = 0; t < G.ntiles; ++t) {
         __syncthreads();
         const void* src = G.src[t];
         const int dt = G.src_dt[t];
+        T pf[PF];
+        int64_t base;
+        int64_t step;
+        bool okm;
         if (DIR == 0) {
-            for (int idx = tid; idx < LB * Np; idx += LP_THREADS) {
-                const int m = idx / Np, j = idx - m * Np;
-                const int64_t line = line0 + m;
-                T v = 0;
-                if (line < nl && j < N) v = ldv<T>(src, dt, line * P.nx + j);
-                V[m * S + j] = v;
-            }
+            okm = true;
+            base = 0;
+            step = 1;
         } else {
-            for (int idx = tid; idx < LB * Np; idx += LP_THREADS) {
-                const int j = idx / LB, m = idx % LB;
-                const int x = x0 + m;
+            okm = x0 + mrow < P.nx;
+            base = (int64_t)(x0 + mrow) + (int64_t)P.nx * P.ny * z;
+            step = (int64_t)P.nx;
+        }
+        auto load = [&](auto tag) {
+            using S_ = decltype(tag);
+            const S_* sp = static_cast<const S_*>(src);
+#pragma unroll
+            for (int r = 0; r < PF; ++r) {
                 T v = 0;
-                if (x < P.nx && j < N) v = ldv<T>(src, dt, (int64_t)x + (int64_t)P.nx * (j + (int64_t)P.ny * z));
-                V[m * S + j] = v;
+                if (DIR == 0) {
+                    const int half = r / (PF / NT), rr = r % (PF / NT);
+                    const int j = jt + JS * rr;
+                    const int64_t line = line0 + mrow + 16 * half;
+                    if (line < nl && j < N) v = (T)sp[line * P.nx + j];
+                } else {
+                    const int j = jt + JS * r;
+                    if (okm && j < N) v = (T)sp[base + step * j];
+                }
+                pf[r] = v;
+            }
+        };
+        if (dt == DT_F64) load(double{});
+        else load(float{});
+#pragma unroll
+        for (int r = 0; r < PF; ++r) {
+            if (DIR == 0) {
+                const int half = r / (PF / NT), rr = r % (PF / NT);
+                const int j = jt + JS * rr;
+                if (j < Np) V[(mrow + 16 * half) * S + j] = pf[r];
+            } else {
+                const int j = jt + JS * r;
+                if (j < Np) V[mrow * S + j] = pf[r];
             }
         }
         __syncthreads();

@@ -13,6 +13,7 @@
 // is (position x line) with lines = 16 consecutive x values, which again makes every 16
 // lanes write 128 contiguous bytes.  The MFMA operand order is the only difference.
 #pragma once
+#include "hqt_buffer.h"
 #include "hqt_kernels.h"
 
 namespace hqt {
@@ -140,6 +141,85 @@ __device__ __forceinline__ void stage_tables(const LPass& P, const LGroup& G, T*
     }
 }
 
+// Occupancy the line kernels' register allocation targets (the branch-free loads would otherwise
+// be hoisted until one wave per SIMD fits).
+template <typename T> constexpr int LP_WPE = sizeof(T) == 8 ? 2 : 3;          // k_line_pass
+template <typename T> constexpr int LP1_WPE = sizeof(T) == 8 ? 2 : 4;         // k_line_pass_one
+
+// Bytes of one [nx, ny, nz] field of element type S (the buffer bound of every line-pass access).
+template <typename S>
+__device__ __forceinline__ int64_t fbytes(const LPass& P) {
+    return (int64_t)P.nx * P.ny * P.nz * (int64_t)sizeof(S);
+}
+
+// Outputs of a finished line tile: out = acc (+ addend).  The addends are all loaded before the
+// first store and the data-type dispatch sits outside the unrolled loops, so a tile's output
+// costs one memory round trip.  Returns whether any stored value is NaN; clears acc.
+template <typename T, int DIR, int Q, int NT>
+__device__ __forceinline__ bool tile_out(const LPass& P, const LOut& O, typename MF<T>::v4 (&acc)[NT][Q], int64_t line0,
+                                         int x0, int z, int64_t nl, int N, int wave, int lane) {
+    const int ln = lane & 15, lk = lane >> 4;
+    // element index of accumulator (u, q, r) or Z_OOB outside the grid (32-bit: fields hold
+    // fewer than 2^23 elements)
+    const int l0 = (int)line0, nlv = (int)nl, nx = P.nx, pl = P.nx * P.ny * z;
+    auto at = [&](int u, int q, int r) -> uint32_t {
+        const int i0 = (wave + 4 * q) * 16;
+        if (DIR == 0) {
+            const int line = l0 + 16 * u + MF<T>::row(lk, r), i = i0 + ln;
+            return (line < nlv && i < N) ? (uint32_t)(line * nx + i) : Z_OOB;
+        } else {
+            const int x = x0 + 16 * u + ln, i = i0 + MF<T>::row(lk, r);
+            return (x < nx && i < N) ? (uint32_t)(x + nx * i + pl) : Z_OOB;
+        }
+    };
+    T add[NT][Q][4];
+    auto load_add = [&](auto tag) {
+        using S_ = decltype(tag);
+        const Rs rs = rs_make(O.addend, fbytes<S_>(P));
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    add[u][q][r] = (T)bldr<S_>(rs, at(u, q, r));
+                }
+    };
+    if (!O.addend) {
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) add[u][q][r] = 0;
+    } else if (O.add_dt == DT_F64) {
+        load_add(double{});
+    } else {
+        load_add(float{});
+    }
+    bool bad = false;
+    // out-of-grid elements are dropped by the buffer bounds check
+    auto store = [&](auto tag) {
+        using S_ = decltype(tag);
+        const Rs rs = rs_make(O.out, fbytes<S_>(P));
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const uint32_t e = at(u, q, r);
+                    const T v = O.addend ? acc[u][q][r] + add[u][q][r] : acc[u][q][r];
+                    bstr<S_>(rs, e, (S_)v);
+                    bad |= e != Z_OOB && v != v;
+                    acc[u][q][r] = 0;
+                }
+    };
+    if (O.out_dt == DT_F64) store(double{});
+    else store(float{});
+    return bad;
+}
+
 // KIND only tags the instantiation (0: RHS stage, 1: constraint, 2: filter) so that
 // profilers report the three uses separately; the code is identical.
 // NT: 16-line tiles per work-group (LB = 16*NT lines).  With NT = 2 every operator fragment
@@ -152,7 +232,7 @@ __device__ __forceinline__ void stage_tables(const LPass& P, const LGroup& G, T*
 // line length and precision; the host measures and picks per pass.  Every output is accumulated
 // by the same MFMA sequence in all variants.
 template <typename T, int DIR, int Q, int KIND, int NT>
-__global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restrict__ Pp, const Members M) {
+__global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(LP_WPE<T>))) void k_line_pass(const LPass* __restrict__ Pp, const Members M) {
     constexpr int LB = 16 * NT;
     constexpr int PF = 4 * Q * NT;            // staged elements per thread: LB*Np/256 <= LB*Mp/256 = 4*Q*NT
     const MemberBlock mb = member_block(M);
@@ -216,21 +296,27 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restric
             base = (int64_t)x + (int64_t)P.nx * P.ny * zz;
             step = (int64_t)P.nx;
         }
+        // out-of-grid elements read 0 through the buffer bounds check (no branches around the loads)
+        auto load = [&](auto tag) {
+            using S_ = decltype(tag);
+            const Rs rs = rs_make(src, fbytes<S_>(P));
 #pragma unroll
-        for (int r = 0; r < PF; ++r) {
-            T v = 0;
-            if (DIR == 0) {
-                const int half = r / (PF / NT), rr = r % (PF / NT);     // NT = 2: second line at +16
-                const int j = jt + JS * rr;
-                const int64_t line = (int64_t)lt * LB + mrow + 16 * half;
-                if (line < nl && j < N && j < Np) v = ldv<T>(src, dt, line * P.nx + j);
-            } else {
-                const int j = jt + JS * r;
-                if (okm && j < N) v = ldv<T>(src, dt, base + step * j);
+            for (int r = 0; r < PF; ++r) {
+                uint32_t e;
+                if (DIR == 0) {
+                    const int half = r / (PF / NT), rr = r % (PF / NT);     // NT = 2: second line at +16
+                    const int j = jt + JS * rr;
+                    const int line = lt * LB + mrow + 16 * half;
+                    e = (line < (int)nl && j < N) ? (uint32_t)(line * P.nx + j) : Z_OOB;
+                } else {
+                    const int j = jt + JS * r;
+                    e = (okm && j < N) ? (uint32_t)((int)base + (int)step * j) : Z_OOB;
+                }
+                pf[r] = (T)bldr<S_>(rs, e);
             }
-            pf[r] = v;
-        }
-        (void)base;
+        };
+        if (dt == DT_F64) load(double{});
+        else load(float{});
     };
     auto deposit = [&]() {
 #pragma unroll
@@ -315,39 +401,10 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restric
         for (int o = 0; o < 2; ++o) {
             if (o >= G.nout) break;
             const LOut& O = G.out[o];
-            bool bad = false;
             if constexpr (KIND == 2) {
                 if (G.has_lr) lr_expand<T, DIR, Q, NT>(P, O, o, Np, DL, wave, lane, acc[o]);
             }
-#pragma unroll
-            for (int u = 0; u < NT; ++u)
-#pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                const int i0 = (wave + 4 * q) * 16;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    int64_t idx;
-                    bool ok;
-                    if (DIR == 0) {
-                        const int64_t line = line0 + 16 * u + MF<T>::row(lk, r);
-                        const int i = i0 + ln;
-                        ok = line < nl && i < N;
-                        idx = line * P.nx + i;
-                    } else {
-                        const int x = x0 + 16 * u + ln;
-                        const int i = i0 + MF<T>::row(lk, r);
-                        ok = x < P.nx && i < N;
-                        idx = (int64_t)x + (int64_t)P.nx * (i + (int64_t)P.ny * z);
-                    }
-                    if (ok) {
-                        T v = acc[o][u][q][r];
-                        if (O.addend) v += ldv<T>(O.addend, O.add_dt, idx);
-                        stv<T>(O.out, O.out_dt, idx, v);
-                        if (v != v) bad = true;
-                    }
-                    acc[o][u][q][r] = 0;
-                }
-            }
+            const bool bad = tile_out<T, DIR, Q, NT>(P, O, acc[o], line0, x0, z, nl, N, wave, lane);
             if (O.nan_check) {
                 if (__any(bad) && lane == 0) {
                     const int step = __atomic_load_n(&P.flags[1], __ATOMIC_RELAXED);
@@ -361,7 +418,7 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass(const LPass* __restric
 // One line tile per work-group, staged global to LDS inside the tile loop: the y-line form
 // (lower register use keeps more waves resident, which the strided y accesses need).
 template <typename T, int DIR, int Q, int KIND, int NT>
-__global__ __launch_bounds__(LP_THREADS) void k_line_pass_one(const LPass* __restrict__ Pp, const Members M) {
+__global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(LP1_WPE<T>))) void k_line_pass_one(const LPass* __restrict__ Pp, const Members M) {
     constexpr int LB = 16 * NT;
     const MemberBlock mb = member_block(M);
     const LPass& P = Pp[mb.m];
@@ -429,20 +486,20 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass_one(const LPass* __res
         }
         auto load = [&](auto tag) {
             using S_ = decltype(tag);
-            const S_* sp = static_cast<const S_*>(src);
+            const Rs rs = rs_make(src, fbytes<S_>(P));
 #pragma unroll
             for (int r = 0; r < PF; ++r) {
-                T v = 0;
+                uint32_t e;
                 if (DIR == 0) {
                     const int half = r / (PF / NT), rr = r % (PF / NT);
                     const int j = jt + JS * rr;
-                    const int64_t line = line0 + mrow + 16 * half;
-                    if (line < nl && j < N) v = (T)sp[line * P.nx + j];
+                    const int line = (int)line0 + mrow + 16 * half;
+                    e = (line < (int)nl && j < N) ? (uint32_t)(line * P.nx + j) : Z_OOB;
                 } else {
                     const int j = jt + JS * r;
-                    if (okm && j < N) v = (T)sp[base + step * j];
+                    e = (okm && j < N) ? (uint32_t)((int)base + (int)step * j) : Z_OOB;
                 }
-                pf[r] = v;
+                pf[r] = (T)bldr<S_>(rs, e);
             }
         };
         if (dt == DT_F64) load(double{});
@@ -511,38 +568,10 @@ __global__ __launch_bounds__(LP_THREADS) void k_line_pass_one(const LPass* __res
     for (int o = 0; o < 2; ++o) {
         if (o >= G.nout) break;
         const LOut& O = G.out[o];
-        bool bad = false;
         if constexpr (KIND == 2) {
             if (G.has_lr) lr_expand<T, DIR, Q, NT>(P, O, o, Np, DL, wave, lane, acc[o]);
         }
-#pragma unroll
-        for (int u = 0; u < NT; ++u)
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const int i0 = (wave + 4 * q) * 16;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                int64_t idx;
-                bool ok;
-                if (DIR == 0) {
-                    const int64_t line = line0 + 16 * u + MF<T>::row(lk, r);
-                    const int i = i0 + ln;
-                    ok = line < nl && i < N;
-                    idx = line * P.nx + i;
-                } else {
-                    const int x = x0 + 16 * u + ln;
-                    const int i = i0 + MF<T>::row(lk, r);
-                    ok = x < P.nx && i < N;
-                    idx = (int64_t)x + (int64_t)P.nx * (i + (int64_t)P.ny * z);
-                }
-                if (ok) {
-                    T v = acc[o][u][q][r];
-                    if (O.addend) v += ldv<T>(O.addend, O.add_dt, idx);
-                    stv<T>(O.out, O.out_dt, idx, v);
-                    if (v != v) bad = true;
-                }
-            }
-        }
+        const bool bad = tile_out<T, DIR, Q, NT>(P, O, acc[o], line0, x0, z, nl, N, wave, lane);
         if (O.nan_check) {
             if (__any(bad) && lane == 0) {
                 const int step = __atomic_load_n(&P.flags[1], __ATOMIC_RELAXED);

@@ -21,6 +21,7 @@
 //   RK4 stage states / combine / update                  :120-128
 //   A_t constraint solve, boundary reset, O              :129-136
 #pragma once
+#include "hqt_buffer.h"
 #include "hqt_kernels.h"
 #include <math.h>
 
@@ -152,28 +153,6 @@ __device__ __forceinline__ void rk_update(int stage, double kv, void* acc, void*
 // tiles and row coefficients sit in LDS, and scheduling barriers pin each prefetch to its phase.
 // Order: F2 (and the charge update); one point-wise pass over every term of E1, E3, E4 without
 // a z derivative; F3; F4; F1.
-typedef __amdgpu_buffer_rsrc_t Rs;
-typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-// Element index past every field: nx, ny <= 512 and nz <= 32 (checked at context creation) keep
-// every real index below 2^23, and 8 * Z_OOB still fits the 31-bit buffer offset.
-constexpr uint32_t Z_OOB = 0x0FFFFFFFu;
-
-__device__ __forceinline__ Rs rs_make(const void* p, int64_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
-}
-template <typename T> __device__ __forceinline__ double bld(Rs r, uint32_t e) {
-    if constexpr (sizeof(T) == 8)
-        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(e * 8u), 0, 0));
-    else
-        return (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(e * 4u), 0, 0));
-}
-template <typename T> __device__ __forceinline__ void bst(Rs r, uint32_t e, double v) {
-    if constexpr (sizeof(T) == 8)
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)(e * 8u), 0, 0);
-    else
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, (float)v), r, (int)(e * 4u), 0, 0);
-}
-
 // Resources of one evolved field's RK bookkeeping at this stage (:120-128):
 //   stage 1: acc = k;        next = z + k/2
 //   stage 2: acc = acc + 2k; next = z + k/2
@@ -200,12 +179,6 @@ template <int NZ> struct ZLds {
     double hf[NZP], cd[NZP], cp[NZP], f1[NZP], f2[NZP];
 };
 
-template <typename T> __device__ __forceinline__ T bldr(Rs r, uint32_t e) {
-    if constexpr (sizeof(T) == 8)
-        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(e * 8u), 0, 0));
-    else
-        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(e * 4u), 0, 0));
-}
 // no instruction crosses this point: keeps each prefetch within its phase
 #define ZSB() __builtin_amdgcn_sched_barrier(0)
 

@@ -141,6 +141,9 @@ __device__ __forceinline__ void stage_tables(const LPass& P, const LGroup& G, T*
     }
 }
 
+// Keeps the operand loads of one k-step ahead of the MFMAs of the other (no instruction crosses it).
+#define LSB() __builtin_amdgcn_sched_barrier(0)
+
 // Occupancy the line kernels' register allocation targets (the branch-free loads would otherwise
 // be hoisted until one wave per SIMD fits).
 template <typename T> constexpr int LP_WPE = sizeof(T) == 8 ? 2 : 3;          // k_line_pass
@@ -367,22 +370,26 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(LP_W
                     for (int u = 0; u < NT; ++u) va1[u] = Vr[16 * u * S + j0 + 4];
 #pragma unroll
                     for (int q = 0; q < Q; ++q) cb1[q] = Cs[64 * q - j0 - 4];
+                    LSB();
 #pragma unroll
                     for (int u = 0; u < NT; ++u)
 #pragma unroll
                         for (int q = 0; q < Q; ++q)
                             acc[o][u][q] = DIR == 0 ? MF<T>::mma(va0[u], cb0[q], acc[o][u][q])
                                                     : MF<T>::mma(cb0[q], va0[u], acc[o][u][q]);
+                    LSB();
 #pragma unroll
                     for (int u = 0; u < NT; ++u) va0[u] = Vr[16 * u * S + j0 + 8];
 #pragma unroll
                     for (int q = 0; q < Q; ++q) cb0[q] = Cs[64 * q - j0 - 8];
+                    LSB();
 #pragma unroll
                     for (int u = 0; u < NT; ++u)
 #pragma unroll
                         for (int q = 0; q < Q; ++q)
                             acc[o][u][q] = DIR == 0 ? MF<T>::mma(va1[u], cb1[q], acc[o][u][q])
                                                     : MF<T>::mma(cb1[q], va1[u], acc[o][u][q]);
+                    LSB();
                 }
             }
         }
@@ -543,22 +550,26 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(LP1_
                     for (int u = 0; u < NT; ++u) va1[u] = Vr[16 * u * S + j0 + 4];
 #pragma unroll
                     for (int q = 0; q < Q; ++q) cb1[q] = Cs[64 * q - j0 - 4];
+                    LSB();
 #pragma unroll
                     for (int u = 0; u < NT; ++u)
 #pragma unroll
                         for (int q = 0; q < Q; ++q)
                             acc[o][u][q] = DIR == 0 ? MF<T>::mma(va0[u], cb0[q], acc[o][u][q])
                                                     : MF<T>::mma(cb0[q], va0[u], acc[o][u][q]);
+                    LSB();
 #pragma unroll
                     for (int u = 0; u < NT; ++u) va0[u] = Vr[16 * u * S + j0 + 8];
 #pragma unroll
                     for (int q = 0; q < Q; ++q) cb0[q] = Cs[64 * q - j0 - 8];
+                    LSB();
 #pragma unroll
                     for (int u = 0; u < NT; ++u)
 #pragma unroll
                         for (int q = 0; q < Q; ++q)
                             acc[o][u][q] = DIR == 0 ? MF<T>::mma(va1[u], cb1[q], acc[o][u][q])
                                                     : MF<T>::mma(cb1[q], va1[u], acc[o][u][q]);
+                    LSB();
                 }
             }
         }

@@ -1,0 +1,482 @@
+// Folded line-operator passes for gfx950 (MI355X): the x/y spectral derivatives and the
+// Fourier filter applied through the even/odd split of their circulant matrices.
+//
+// Reference: tevol_driven_cluster.m:60-61 (func_Lx / func_Ly: dense fourdif matrix times every
+// line), src/numerics/ff.m:7-11 (filter).  Every line operator of the stepper is a circulant
+// C[i][j] = c[(i - j) mod N] whose column is even (d_xx, the filter's real part: c[-k] = c[k])
+// or odd (d_x, the filter's imaginary part: c[-k] = -c[k]).  With
+//     f_j = x_j + x_{N-j},  g_j = x_j - x_{N-j}      (0 < j < N/2;  f_0 = x_0, f_{N/2} = x_{N/2}, g = 0 there)
+// the line y = C x splits, for i = 0 .. N/2, into
+//     A_i = sum_j (c[i-j] + c[i+j])/2 f_j,   B_i = sum_j (c[i-j] - c[i+j])/2 g_j      (j = 0 .. N/2)
+//     even c:  y_i = A_i + B_i,  y_{N-i} = A_i - B_i;     odd c:  y_i = B_i + A_i,  y_{N-i} = B_i - A_i
+// i.e. E (the even part of y) and O (its odd part) are each one product of a half-size matrix
+// with half of the folded line: half the multiply-adds of the dense product.  The half-size
+// matrices are Toeplitz + Hankel; their 16x4 fragments are formed in registers from two reads of
+// one table tab[d + Z] = c[d mod N] / 2 (T part at d = i - j, H part at d = i + j).
+//
+// Each line enters the fold relative to its first value x_0 (f is formed from x - x_0; g does not
+// change): every operator here annihilates constants except the filter's real part, whose row sum
+// (1) times x_0 is added back at the output.  In FP32 this keeps the rounding of the summed
+// Toeplitz + Hankel coefficients from acting on a line's mean (A_t is about mu along every line).
+//
+// Work decomposition (as the dense form): a work-group owns LB = 16 NT lines and one output
+// group; each wave owns the position tiles wave + 4 q (q < QF) of the half line and keeps the E
+// and O accumulators of its tiles, so y_i and y_{N-i} are finished in registers.
+#pragma once
+#include "hqt_line_core.h"
+
+namespace hqt {
+
+// t + s h with s = +-1 (exact), in the element type.
+__device__ __forceinline__ double fadd_s(double s, double h, double t) { return __builtin_fma(s, h, t); }
+__device__ __forceinline__ float fadd_s(float s, float h, float t) { return __builtin_fmaf(s, h, t); }
+
+// Occupancy the folded kernels' register allocation targets.
+template <typename T> constexpr int FP_WPE = sizeof(T) == 8 ? 2 : 3;
+
+// Largest folded table (entries) for QF position tiles per wave: Z + 64 QF + Kp + 4 with
+// Kp <= 64 QF + 8, Z = Kp + 4.
+template <int QF> constexpr int fold_tlmax() { return 192 * QF + 32; }
+
+template <typename T, int QF>
+__device__ __forceinline__ void stage_ftables(const LPass& P, const LGroup& G, T* C, int TL, int tid) {
+    constexpr int TPT = (fold_tlmax<QF>() + LP_THREADS - 1) / LP_THREADS;
+    T tv[4][TPT];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        if (s < G.nslots) {
+            const T* src = static_cast<const T*>(P.ops[G.slot_op[s]]);
+#pragma unroll
+            for (int r = 0; r < TPT; ++r) {
+                const int t = tid + r * LP_THREADS;
+                tv[s][r] = t < TL ? src[t] : (T)0;
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        if (s < G.nslots) {
+#pragma unroll
+            for (int r = 0; r < TPT; ++r) {
+                const int t = tid + r * LP_THREADS;
+                if (t < TL) C[s * TL + t] = tv[s][r];
+            }
+        }
+    }
+}
+
+// Rank-4 dots of the folded lines: D (lines x 4) = F Vf + G Vg on MFMA, the 4 waves splitting the
+// k range; partial tiles are summed in wave order (independent of the lines per tile).
+// Folded rank-4 table: Vf (Kp x 4) | Vg (Kp x 4) | UE (Mp x 4) | UO (Mp x 4), row-major.
+template <typename T, int NT>
+__device__ __forceinline__ void fold_lr_dots(const T* __restrict__ F, const T* __restrict__ G, int S, int Kp,
+                                             const T* __restrict__ tab, T* DLp, T* dl, int wave, int lane) {
+    typedef typename MF<T>::v4 v4;
+    constexpr int LB = 16 * NT;
+    const int ln = lane & 15, lk = lane >> 4;
+    const T* vf = tab;
+    const T* vg = tab + 4 * Kp;
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+        v4 acc = v4{0, 0, 0, 0};
+        for (int j0 = 4 * wave; j0 < Kp; j0 += 16) {
+            const T a = F[(16 * u + ln) * S + j0 + lk];
+            const T b = ln < 4 ? vf[4 * (j0 + lk) + ln] : (T)0;
+            acc = MF<T>::mma(a, b, acc);
+        }
+        for (int j0 = 4 * wave; j0 < Kp; j0 += 16) {
+            const T a = G[(16 * u + ln) * S + j0 + lk];
+            const T b = ln < 4 ? vg[4 * (j0 + lk) + ln] : (T)0;
+            acc = MF<T>::mma(a, b, acc);
+        }
+        if (ln < 4) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) DLp[(wave * LB + 16 * u + MF<T>::row(lk, r)) * 4 + ln] = acc[r];
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < LB * 4; idx += LP_THREADS)
+        dl[idx] = ((DLp[idx] + DLp[LB * 4 + idx]) + DLp[2 * LB * 4 + idx]) + DLp[3 * LB * 4 + idx];
+}
+
+// Output of a finished line tile: y_i = E + O (+ addend) at i, then y_{N-i} = E - O (+ addend) at
+// N - i, where E includes the line's constant term cst (row sums times reference values).  Each
+// half loads all of its addends before its first store.  Returns whether a stored value is NaN;
+// clears the accumulators.
+template <typename T, int DIR, int QF, int NT>
+__device__ __forceinline__ bool fold_out(const LPass& P, const LOut& O, typename MF<T>::v4 (&aE)[NT][QF],
+                                         typename MF<T>::v4 (&aO)[NT][QF], const T (&cst)[NT][4], int line0, int x0,
+                                         int z, int nl, int N, int wave, int lane) {
+    const int ln = lane & 15, lk = lane >> 4;
+    const int nx = P.nx, pl = P.nx * P.ny * z, Mh = N >> 1;
+    // element index of position i (direct) or N - i (mirror) of accumulator (u, q, r)
+    auto idx = [&](int u, int q, int r, bool mirror) -> uint32_t {
+        const int i0 = (wave + 4 * q) * 16;
+        int line, i;
+        if (DIR == 0) {
+            line = line0 + 16 * u + MF<T>::row(lk, r);
+            i = i0 + ln;
+        } else {
+            line = x0 + 16 * u + ln;
+            i = i0 + MF<T>::row(lk, r);
+        }
+        const bool okl = DIR == 0 ? line < nl : line < nx;
+        const bool ok = mirror ? (okl && i >= 1 && 2 * i < N) : (okl && i <= Mh);
+        const int pos = mirror ? N - i : i;
+        if (!ok) return Z_OOB;
+        return DIR == 0 ? (uint32_t)(line * nx + pos) : (uint32_t)(line + nx * pos + pl);
+    };
+    bool bad = false;
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        const bool mirror = half == 1;
+        const T sO = mirror ? (T)-1 : (T)1;
+        T add[NT][QF][4];
+        auto load_add = [&](auto tag) {
+            using S_ = decltype(tag);
+            const Rs rs = rs_make(O.addend, fbytes<S_>(P));
+#pragma unroll
+            for (int u = 0; u < NT; ++u)
+#pragma unroll
+                for (int q = 0; q < QF; ++q)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) add[u][q][r] = (T)bldr<S_>(rs, idx(u, q, r, mirror));
+        };
+        if (!O.addend) {
+#pragma unroll
+            for (int u = 0; u < NT; ++u)
+#pragma unroll
+                for (int q = 0; q < QF; ++q)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) add[u][q][r] = 0;
+        } else if (O.add_dt == DT_F64) {
+            load_add(double{});
+        } else {
+            load_add(float{});
+        }
+        auto store = [&](auto tag) {
+            using S_ = decltype(tag);
+            const Rs rs = rs_make(O.out, fbytes<S_>(P));
+#pragma unroll
+            for (int u = 0; u < NT; ++u)
+#pragma unroll
+                for (int q = 0; q < QF; ++q)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const uint32_t e = idx(u, q, r, mirror);
+                        const T v = ((aE[u][q][r] + cst[u][r]) + sO * aO[u][q][r]) + add[u][q][r];
+                        bstr<S_>(rs, e, (S_)v);
+                        bad |= e != Z_OOB && v != v;
+                    }
+        };
+        if (O.out_dt == DT_F64) store(double{});
+        else store(float{});
+    }
+#pragma unroll
+    for (int u = 0; u < NT; ++u)
+#pragma unroll
+        for (int q = 0; q < QF; ++q) aE[u][q] = aO[u][q] = typename MF<T>::v4{0, 0, 0, 0};
+    return bad;
+}
+
+// KIND tags the instantiation only (0: RHS stage, 1: constraint, 2: filter), as in k_line_pass.
+// A work-group processes P.tpb consecutive line tiles.  Its work is a sequence of units, one per
+// (line tile, source): a unit's lines are staged folded into LDS, then every output term on that
+// source runs its k-loop; an output is written as soon as its last source is in.  The host only
+// folds groups with one source, or two sources and one output, so one output's accumulators are
+// live at a time.  The global loads of the next unit are issued right after a unit is staged and
+// stay in flight in registers during the unit's MFMAs.
+template <typename T, int DIR, int QF, int KIND, int NT>
+__global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_WPE<T>))) void k_fold_pass(const LPass* __restrict__ Pp, const Members M) {
+    constexpr int LB = 16 * NT;
+    constexpr int JS = DIR == 0 ? 16 : LP_THREADS / LB;            // j step between a thread's elements
+    constexpr int PFJ = ((64 * QF + 8) + JS - 1) / JS;             // folded positions per thread and line half
+    constexpr int NH = DIR == 0 ? NT : 1;                          // line halves per thread (x-lines, LB = 32)
+    const MemberBlock mb = member_block(M);
+    const LPass& P = Pp[mb.m];
+    if ((int)blockIdx.y >= P.ngroups) return;
+    if (__builtin_nontemporal_load(&P.flags[0]) != 0) return;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    const int S = P.S, Kp = P.Kp, N = P.N, TL = P.TL, Z = P.Z, Mh = N >> 1;
+    T* Vf = reinterpret_cast<T*>(smem_raw);
+    T* Vg = Vf + LB * S;
+    T* C = Vg + LB * S;
+    const LGroup& G = P.g[blockIdx.y];
+    T* DL = C + G.nslots * TL;               // rank-4 dots: 2 outputs x LB lines x 4
+    T* DLp = DL + 2 * LB * 4;                // rank-4 partial dots: 4 waves x LB lines x 4
+    T* X0 = DLp + 4 * LB * 4;                // reference value x_0 of every line of each source: 2 x LB
+    const int tid = threadIdx.x;
+
+    stage_ftables<T, QF>(P, G, C, TL, tid);
+
+    const int nl = P.ny * P.nz;
+    const int tile0 = mb.bx * P.tpb;
+    const int ntile = min(P.tpb, P.ntiles - tile0);
+    const int nunit = ntile * G.ntiles;
+
+    typedef typename MF<T>::v4 v4;
+    v4 aE[NT][QF], aO[NT][QF];
+#pragma unroll
+    for (int u = 0; u < NT; ++u)
+#pragma unroll
+        for (int q = 0; q < QF; ++q) aE[u][q] = aO[u][q] = v4{0, 0, 0, 0};
+
+    const int lane = tid & 63, ln = lane & 15, lk = lane >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    // staging map: x-lines: line tid/16 (+16 for the second half), positions tid%16 + 16 r;
+    // y-lines: x offset tid%LB, positions tid/LB + (256/LB) r
+    const int mrow = DIR == 0 ? (tid >> 4) : (tid % LB);
+    const int jt = DIR == 0 ? (tid & 15) : (tid / LB);
+
+    // Registers of the unit in flight: x_j, x_{N-j} and x_0 of the thread's lines, widened to
+    // double (every source is double or float; the fold is formed in the source's precision).
+    double pa[NH][PFJ], pb[NH][PFJ], pref[NH];
+    auto fetch = [&](int unit) {
+        const int lt = tile0 + unit / G.ntiles, t = unit % G.ntiles;
+        const void* src = G.src[t];
+        const int dt = G.src_dt[t];
+        int line0 = 0, z = 0, x0 = 0;
+        if (DIR == 0) {
+            line0 = lt * LB;
+        } else {
+            x0 = (lt % P.nxc) * LB;
+            z = lt / P.nxc;
+        }
+        auto load = [&](auto tag) {
+            using S_ = decltype(tag);
+            const Rs rs = rs_make(src, fbytes<S_>(P));
+#pragma unroll
+            for (int h = 0; h < NH; ++h) {
+                uint32_t e;
+                if (DIR == 0) {
+                    const int line = line0 + mrow + 16 * h;
+                    e = line < nl ? (uint32_t)(line * P.nx) : Z_OOB;
+                } else {
+                    const int x = x0 + mrow;
+                    e = x < P.nx ? (uint32_t)(x + P.nx * P.ny * z) : Z_OOB;
+                }
+                pref[h] = (double)bldr<S_>(rs, e);
+            }
+#pragma unroll
+            for (int h = 0; h < NH; ++h)
+#pragma unroll
+                for (int r = 0; r < PFJ; ++r) {
+                    const int j = jt + JS * r;
+                    uint32_t e0, e1;
+                    if (DIR == 0) {
+                        const int line = line0 + mrow + 16 * h;
+                        const bool okl = line < nl;
+                        e0 = (okl && j <= Mh) ? (uint32_t)(line * P.nx + j) : Z_OOB;
+                        e1 = (okl && j >= 1 && 2 * j < N) ? (uint32_t)(line * P.nx + N - j) : Z_OOB;
+                    } else {
+                        const int x = x0 + mrow;
+                        const bool okl = x < P.nx;
+                        const int base = x + P.nx * P.ny * z;
+                        e0 = (okl && j <= Mh) ? (uint32_t)(base + P.nx * j) : Z_OOB;
+                        e1 = (okl && j >= 1 && 2 * j < N) ? (uint32_t)(base + P.nx * (N - j)) : Z_OOB;
+                    }
+                    pa[h][r] = (double)bldr<S_>(rs, e0);
+                    pb[h][r] = (double)bldr<S_>(rs, e1);
+                }
+        };
+        if (dt == DT_F64) load(double{});
+        else load(float{});
+    };
+    // f = (x_j - x_0) + (x_{N-j} - x_0), g = x_j - x_{N-j} into LDS (zero past N/2 and where
+    // x_{N-j} is x_j itself); float sources are exact in double, so this is the source precision.
+    auto deposit = [&](int t) {
+#pragma unroll
+        for (int h = 0; h < NH; ++h)
+#pragma unroll
+            for (int r = 0; r < PFJ; ++r) {
+                const int j = jt + JS * r;
+                const bool pair = j >= 1 && 2 * j < N;
+                const double da = j <= Mh ? pa[h][r] - pref[h] : 0.0;
+                const double db = pair ? pb[h][r] - pref[h] : 0.0;
+                if (j < Kp) {
+                    Vf[(mrow + 16 * h) * S + j] = (T)(da + db);
+                    Vg[(mrow + 16 * h) * S + j] = pair ? (T)(pa[h][r] - pb[h][r]) : (T)0;
+                }
+            }
+        if (jt == 0) {
+#pragma unroll
+            for (int h = 0; h < NH; ++h) X0[t * LB + mrow + 16 * h] = (T)pref[h];
+        }
+    };
+
+    if (nunit > 0) fetch(0);
+    for (int unit = 0; unit < nunit; ++unit) {
+        const int lt = tile0 + unit / G.ntiles, t = unit % G.ntiles;
+        int line0 = 0, z = 0, x0 = 0;
+        if (DIR == 0) {
+            line0 = lt * LB;
+        } else {
+            x0 = (lt % P.nxc) * LB;
+            z = lt / P.nxc;
+        }
+        __syncthreads();                 // the previous unit's MFMAs and outputs are done with LDS
+        deposit(t);
+        __syncthreads();
+        if (unit + 1 < nunit) fetch(unit + 1);
+        if constexpr (KIND == 2) {
+            if (G.has_lr) {
+                for (int o = 0; o < G.nout; ++o)
+                    for (int e = 0; e < G.out[o].nterms; ++e) {
+                        const LTerm& tm = G.out[o].term[e];
+                        if (tm.lr && tm.tile == t) {
+                            fold_lr_dots<T, NT>(Vf, Vg, S, Kp, static_cast<const T*>(P.ops[tm.slot]), DLp, DL + o * LB * 4,
+                                                wave, lane);
+                            __syncthreads();
+                        }
+                    }
+            }
+        }
+        for (int o = 0; o < G.nout; ++o) {
+            const LOut& O = G.out[o];
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                if (e >= O.nterms || O.term[e].tile != t || O.term[e].lr) continue;
+                const int sym = O.term[e].pad;              // +1: even column, -1: odd column
+                // T part tab[i - j + Z], H part tab[i + j + Z]; lane (ln, lk) reads i = i0 + ln, j = j0 + lk
+                const T* Ct = C + O.term[e].slot * TL + Z + ln - lk + wave * 16;
+                const T* Ch = C + O.term[e].slot * TL + Z + ln + lk + wave * 16;
+                // E gets (T + sg H) times f (even column) or g (odd column), O the other pair
+                const T* Be = (sym > 0 ? Vf : Vg) + ln * S + lk;
+                const T* Bo = (sym > 0 ? Vg : Vf) + ln * S + lk;
+                const T sg = sym > 0 ? (T)1 : (T)-1;
+                T ae0[NT], ao0[NT], ae1[NT], ao1[NT];
+                T t0[QF], h0[QF], t1[QF], h1[QF];
+#pragma unroll
+                for (int u = 0; u < NT; ++u) { ae0[u] = Be[16 * u * S]; ao0[u] = Bo[16 * u * S]; }
+#pragma unroll
+                for (int q = 0; q < QF; ++q) { t0[q] = Ct[64 * q]; h0[q] = Ch[64 * q]; }
+                // two k-steps per iteration with ping-pong operand registers; the last loads run
+                // one k-step past Kp (values unused: rows are padded, the table is padded)
+                for (int j0 = 0; j0 < Kp; j0 += 8) {
+#pragma unroll
+                    for (int u = 0; u < NT; ++u) { ae1[u] = Be[16 * u * S + j0 + 4]; ao1[u] = Bo[16 * u * S + j0 + 4]; }
+#pragma unroll
+                    for (int q = 0; q < QF; ++q) { t1[q] = Ct[64 * q - j0 - 4]; h1[q] = Ch[64 * q + j0 + 4]; }
+                    LSB();
+#pragma unroll
+                    for (int q = 0; q < QF; ++q) {
+                        const T ke = fadd_s(sg, h0[q], t0[q]);
+                        const T ko = fadd_s(-sg, h0[q], t0[q]);
+#pragma unroll
+                        for (int u = 0; u < NT; ++u) {
+                            aE[u][q] = DIR == 0 ? MF<T>::mma(ae0[u], ke, aE[u][q]) : MF<T>::mma(ke, ae0[u], aE[u][q]);
+                            aO[u][q] = DIR == 0 ? MF<T>::mma(ao0[u], ko, aO[u][q]) : MF<T>::mma(ko, ao0[u], aO[u][q]);
+                        }
+                    }
+                    LSB();
+#pragma unroll
+                    for (int u = 0; u < NT; ++u) { ae0[u] = Be[16 * u * S + j0 + 8]; ao0[u] = Bo[16 * u * S + j0 + 8]; }
+#pragma unroll
+                    for (int q = 0; q < QF; ++q) { t0[q] = Ct[64 * q - j0 - 8]; h0[q] = Ch[64 * q + j0 + 8]; }
+                    LSB();
+#pragma unroll
+                    for (int q = 0; q < QF; ++q) {
+                        const T ke = fadd_s(sg, h1[q], t1[q]);
+                        const T ko = fadd_s(-sg, h1[q], t1[q]);
+#pragma unroll
+                        for (int u = 0; u < NT; ++u) {
+                            aE[u][q] = DIR == 0 ? MF<T>::mma(ae1[u], ke, aE[u][q]) : MF<T>::mma(ke, ae1[u], aE[u][q]);
+                            aO[u][q] = DIR == 0 ? MF<T>::mma(ao1[u], ko, aO[u][q]) : MF<T>::mma(ko, ao1[u], aO[u][q]);
+                        }
+                    }
+                    LSB();
+                }
+            }
+            if (t != G.ntiles - 1) continue;
+            // all sources of output o are in: rank-4 expansion, constant term, store
+            if constexpr (KIND == 2) {
+                if (G.has_lr) {
+                    // E += UE D, O += UO D for the rank-4 term (one MFMA k-step per tile: the ranks are K)
+                    for (int e = 0; e < O.nterms; ++e) {
+                        if (!O.term[e].lr) continue;
+                        const T* ue = static_cast<const T*>(P.ops[O.term[e].slot]) + 8 * Kp;
+                        const T* uo = ue + 4 * P.Mp;
+#pragma unroll
+                        for (int u = 0; u < NT; ++u) {
+                            const T d = DL[o * LB * 4 + (16 * u + ln) * 4 + lk];
+#pragma unroll
+                            for (int q = 0; q < QF; ++q) {
+                                const int i0 = (wave + 4 * q) * 16;
+                                const T we = ue[4 * (i0 + ln) + lk], wo = uo[4 * (i0 + ln) + lk];
+                                aE[u][q] = DIR == 0 ? MF<T>::mma(d, we, aE[u][q]) : MF<T>::mma(we, d, aE[u][q]);
+                                aO[u][q] = DIR == 0 ? MF<T>::mma(d, wo, aO[u][q]) : MF<T>::mma(wo, d, aO[u][q]);
+                            }
+                        }
+                    }
+                }
+            }
+            // constant term of every line: sum over the terms of rsum times the source's x_0
+            T cst[NT][4];
+#pragma unroll
+            for (int u = 0; u < NT; ++u)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) cst[u][r] = 0;
+            for (int e = 0; e < O.nterms; ++e) {
+                if (O.term[e].lr || O.term[e].rsum == 0) continue;
+                const T rsv = (T)O.term[e].rsum;
+#pragma unroll
+                for (int u = 0; u < NT; ++u)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int l = DIR == 0 ? 16 * u + MF<T>::row(lk, r) : 16 * u + ln;
+                        cst[u][r] += rsv * X0[O.term[e].tile * LB + l];
+                    }
+            }
+            const bool bad = fold_out<T, DIR, QF, NT>(P, O, aE, aO, cst, line0, x0, z, nl, N, wave, lane);
+            if (O.nan_check) {
+                if (__any(bad) && lane == 0) {
+                    const int step = __atomic_load_n(&P.flags[1], __ATOMIC_RELAXED);
+                    atomicCAS(&P.flags[0], 0, step > 0 ? step : 1);
+                }
+            }
+        }
+    }
+}
+
+template <typename T, int DIR, int KIND, int NT, int QF>
+hipError_t fold_launch_one(const LPass* d, Members M, dim3 grid, size_t lds, hipStream_t s) {
+    const void* fn = reinterpret_cast<const void*>(&k_fold_pass<T, DIR, QF, KIND, NT>);
+    if (lds > 64 * 1024) {
+        static bool done = false;
+        if (!done) {
+            const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            if (e != hipSuccess) return e;
+            done = true;
+        }
+    }
+    hipLaunchKernelGGL((k_fold_pass<T, DIR, QF, KIND, NT>), grid, dim3(LP_THREADS), lds, s, d, M);
+    return hipGetLastError();
+}
+
+template <typename T, int DIR, int KIND, int NT>
+hipError_t fold_launch_q(const LPass* d, Members M, dim3 grid, int QF, size_t lds, hipStream_t s) {
+    switch (QF) {
+        case 1: return fold_launch_one<T, DIR, KIND, NT, 1>(d, M, grid, lds, s);
+        case 2: return fold_launch_one<T, DIR, KIND, NT, 2>(d, M, grid, lds, s);
+        case 3: return fold_launch_one<T, DIR, KIND, NT, 3>(d, M, grid, lds, s);
+        case 4: return fold_launch_one<T, DIR, KIND, NT, 4>(d, M, grid, lds, s);
+        case 5: return fold_launch_one<T, DIR, KIND, NT, 5>(d, M, grid, lds, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// Every folded instance of one (precision, direction) family; explicitly instantiated by the
+// hqt_fold_*.hip translation units.
+template <typename T, int DIR>
+hipError_t fold_launch_td(const LPass* d, Members M, dim3 grid, int QF, int nt, int kind, size_t lds, hipStream_t s) {
+    if (nt == 2) {
+        if (kind == 0) return fold_launch_q<T, DIR, 0, 2>(d, M, grid, QF, lds, s);
+        if (kind == 1) return fold_launch_q<T, DIR, 1, 2>(d, M, grid, QF, lds, s);
+        return fold_launch_q<T, DIR, 2, 2>(d, M, grid, QF, lds, s);
+    }
+    if (kind == 0) return fold_launch_q<T, DIR, 0, 1>(d, M, grid, QF, lds, s);
+    if (kind == 1) return fold_launch_q<T, DIR, 1, 1>(d, M, grid, QF, lds, s);
+    return fold_launch_q<T, DIR, 2, 1>(d, M, grid, QF, lds, s);
+}
+
+}  // namespace hqt

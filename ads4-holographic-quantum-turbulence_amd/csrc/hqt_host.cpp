@@ -47,12 +47,17 @@ int64_t ctx_footprint(const hqt_params& p) {
     b += 2 * nxy * 8 + nxy + 8 + 2 * (int64_t)census_blocks(nxy) * 8;   // diagnostics
     b += (nz * nz + (nz - 1) * (nz - 1) + 5 * nz) * 8;   // radial operators
     b += 6 * ((int64_t)line_geom(p.nx).TL + (int64_t)line_geom(p.ny).TL) * es;   // x/y operator tables
+    const bool fold = !(p.flags & HQT_DENSE_LINES);
+    if (fold) b += 6 * ((int64_t)fold_geom(p.nx).TL + (int64_t)fold_geom(p.ny).TL) * es;   // folded tables
     for (int n : {p.nx, p.ny}) {                          // rank-4 filter tables, when used
         if (p.flags & HQT_DENSE_FILTER) break;
         std::vector<uint8_t> mk;
         matlab_masks(n, mk);
         std::vector<double> t;
-        if (filter_lowrank(n, mk.data(), line_geom(n), 1.0, t)) b += 2 * (int64_t)t.size() * es;
+        if (filter_lowrank(n, mk.data(), line_geom(n), 1.0, t)) {
+            b += 2 * (int64_t)t.size() * es;
+            if (fold) b += 2 * (8 * (int64_t)fold_geom(n).Kp + 8 * (int64_t)fold_geom(n).Mp) * es;
+        }
     }
     return b;
 }
@@ -111,7 +116,7 @@ void set_lb(LPass& P, int lb, int dir, int tpb = 1) {
     else P.ntiles = P.nxc * P.nz;
 }
 
-LPass base_pass(int N, int nx, int ny, int nz, int dir, void* const* ops, int32_t* flags) {
+LPass base_pass(int N, int nx, int ny, int nz, int dir, void* const* ops, int32_t* flags, int fold = 0) {
     LPass P;
     std::memset(&P, 0, sizeof(P));
     const LineGeom lg = line_geom(N);
@@ -120,6 +125,15 @@ LPass base_pass(int N, int nx, int ny, int nz, int dir, void* const* ops, int32_
     P.Mp = lg.Mp;
     P.TL = lg.TL;
     P.S = P.Np + 1;
+    if (fold) {
+        const FoldGeom fg = fold_geom(N);
+        P.fold = 1;
+        P.Mp = fg.Mp;
+        P.TL = fg.TL;
+        P.S = fg.S;
+        P.Kp = fg.Kp;
+        P.Z = fg.Z;
+    }
     P.nx = nx; P.ny = ny; P.nz = nz;
     set_lb(P, 16, dir);
     for (int i = 0; i < OP_COUNT; ++i) P.ops[i] = ops[i];
@@ -143,6 +157,19 @@ int slot_for(LGroup& g, int op) {
     return g.nslots++;
 }
 
+// Parity of an operator's circulant column: d_x and the filter's imaginary part are odd, d_xx and
+// the filter's real part even.
+int op_parity(int op) {
+    switch (op) {
+        case OP_D2X: case OP_PRX: case OP_D2Y: case OP_QRY: return 1;
+        default: return -1;
+    }
+}
+
+// Row sum of an operator (its value on a constant line): 1 for the filter's real part (mode 0 is
+// kept), 0 for the derivatives and the filter's imaginary part.
+int op_rowsum(int op) { return (op == OP_PRX || op == OP_QRY) ? 1 : 0; }
+
 // A term with lr = 1 is a rank-4 operator: its `slot` is the ops index of the [V | U] table
 // (no LDS slot), applied by the kernel as U (V^T line).
 void add_out(LGroup& g, void* out, int odt, int tile0, int op0, int tile1 = -1, int op1 = -1,
@@ -150,8 +177,9 @@ void add_out(LGroup& g, void* out, int odt, int tile0, int op0, int tile1 = -1, 
     LOut& o = g.out[g.nout++];
     std::memset(&o, 0, sizeof(o));
     o.out = out; o.out_dt = odt; o.addend = addend; o.add_dt = adt; o.nan_check = nan_check;
-    o.term[o.nterms++] = LTerm{tile0, lr0 ? op0 : slot_for(g, op0), lr0, 0};
-    if (tile1 >= 0) o.term[o.nterms++] = LTerm{tile1, lr1 ? op1 : slot_for(g, op1), lr1, 0};
+    o.term[o.nterms++] = LTerm{tile0, lr0 ? op0 : slot_for(g, op0), lr0, lr0 ? 0 : op_parity(op0), lr0 ? 0 : op_rowsum(op0)};
+    if (tile1 >= 0)
+        o.term[o.nterms++] = LTerm{tile1, lr1 ? op1 : slot_for(g, op1), lr1, lr1 ? 0 : op_parity(op1), lr1 ? 0 : op_rowsum(op1)};
     if (lr0 || lr1) g.has_lr = 1;
 }
 
@@ -172,6 +200,21 @@ int build_tables(hqt_ctx* c, int N, double L, const uint8_t* mask, int base) {
     toeplitz_table(col, row, Np, -px, tabs[2]);
     std::vector<double> pr, pi, prr(N), pir(N);
     filter_tables(N, mask, pr, pi);
+    if (c->fold) {
+        // folded tables of the same six operators (columns and parities as above)
+        const FoldGeom fg = fold_geom(N);
+        std::vector<double> ft;
+        const std::vector<double>* fc[6] = {&col, &c2, &col, &pr, &pi, &pi};
+        const int fsym[6] = {-1, 1, -1, 1, -1, -1};
+        const double fsc[6] = {px, 1.0, -px, 1.0, 1.0, -1.0};
+        for (int t = 0; t < 6; ++t) {
+            fold_table(*fc[t], fsym[t], fg, fsc[t], ft);
+            int rc = dalloc(c, ft.size() * c->es, &c->fops[base + t]);
+            if (rc) return rc;
+            rc = upload_as(c, c->fops[base + t], ft.data(), ft.size(), c->tdt);
+            if (rc) return rc;
+        }
+    }
     for (int m = 0; m < N; ++m) { prr[m] = pr[(N - m) % N]; pir[m] = pi[(N - m) % N]; }
     toeplitz_table(pr, prr, Np, 1.0, tabs[3]);
     toeplitz_table(pi, pir, Np, 1.0, tabs[4]);
@@ -193,6 +236,14 @@ int build_tables(hqt_ctx* c, int N, double L, const uint8_t* mask, int base) {
             if (rc) return rc;
             rc = upload_as(c, c->ops[lrbase + t], tt[t]->data(), tt[t]->size(), c->tdt);
             if (rc) return rc;
+            if (c->fold) {
+                std::vector<double> ft;
+                fold_lowrank(*tt[t], Np, fold_geom(N), ft);
+                rc = dalloc(c, ft.size() * c->es, &c->fops[lrbase + t]);
+                if (rc) return rc;
+                rc = upload_as(c, c->fops[lrbase + t], ft.data(), ft.size(), c->tdt);
+                if (rc) return rc;
+            }
         }
     }
     return HQT_OK;
@@ -225,7 +276,7 @@ int line(hqt_ctx* c, int pass, int dir, int name) {
     const LPass& H = c->hp[pass];
     return run(c, name, [&] {
         return launch_line_pass(c->dp + pass, kOne, line_blocks(H), H.ngroups, line_pass_lds_bytes(H, c->fp32), H.Mp / 64,
-                                H.lb / 16, dir, c->fp32, line_kind(name), c->stream);
+                                H.lb / 16, dir, c->fp32, line_kind(name), c->stream, H.fold);
     });
 }
 
@@ -252,7 +303,7 @@ constexpr size_t kLdsCap = 160 * 1024;
 // precision, grid) and kept for the process.  Every choice gives bitwise identical results (each
 // output's MFMA accumulation order is the same), so tuning never changes numbers.
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int, int, int>, int> g_tune;   // -> 100 * lb + tpb
+std::map<std::tuple<int, int, int, int, int, int>, int> g_tune;   // -> 100 * lb + tpb
 
 int tune_lines(hqt_ctx* c) {
     const int forced = (c->p.flags & HQT_LINES32) ? 32 : (c->p.flags & HQT_LINES16) ? 16 : 0;
@@ -262,7 +313,7 @@ int tune_lines(hqt_ctx* c) {
         LPass& H = c->hp[p];
         const int dir = pass_dir(p);
         int choice = forced ? 100 * forced + 1 : 0;
-        const auto key = std::make_tuple(p, c->fp32, c->nx, c->ny, c->nz);
+        const auto key = std::make_tuple(p, c->fp32, c->nx, c->ny, c->nz, (int)H.fold);
         if (!choice) {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             auto it = g_tune.find(key);
@@ -276,7 +327,7 @@ int tune_lines(hqt_ctx* c) {
             float best = 0;
             for (int lb : {16, 32}) {
                 for (int tpb : {1, 4}) {
-                    if (dir == 1 && tpb != 1) continue;      // y-lines: one tile per work-group
+                    if (dir == 1 && tpb != 1 && !H.fold) continue;   // dense y-lines: one tile per work-group
                     set_lb(H, lb, dir, tpb);
                     const size_t lds = line_pass_lds_bytes(H, c->fp32);
                     if (lds > kLdsCap) continue;
@@ -286,7 +337,7 @@ int tune_lines(hqt_ctx* c) {
                         if (rep == 1) e = hipEventRecord(ea, c->stream);
                         if (e == hipSuccess)
                             e = launch_line_pass(c->dp + p, kOne, line_blocks(H), H.ngroups, lds, H.Mp / 64, lb / 16, dir,
-                                                 c->fp32, 0, c->stream);
+                                                 c->fp32, 0, c->stream, H.fold);
                     }
                     if (e == hipSuccess) e = hipEventRecord(eb, c->stream);
                     if (e == hipSuccess) e = hipEventSynchronize(eb);
@@ -300,7 +351,8 @@ int tune_lines(hqt_ctx* c) {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             g_tune[key] = choice;
         }
-        set_lb(H, choice / 100, dir, choice % 100);
+        // the dense y-line kernel processes one tile per work-group
+        set_lb(H, choice / 100, dir, (dir == 1 && !H.fold) ? 1 : choice % 100);
         if (line_pass_lds_bytes(H, c->fp32) > kLdsCap) set_lb(H, 16, dir, 1);
     }
     // occupancy target of the radial stage kernel: fixed, so that results never depend on a timing.
@@ -440,13 +492,14 @@ int build_context(hqt_ctx* c) {
 
     // ---- line-pass descriptors ----
     const int T = c->tdt;
-    void* const* ops = c->ops;
+    void* const* ops = c->fold ? c->fops : c->ops;
+    const int fo = c->fold;
     void* Sin[4][4] = {{c->ZR, c->ZI, c->ZAx, c->ZAy},
                        {c->SA[0], c->SA[1], c->SA[2], c->SA[3]},
                        {c->SB[0], c->SB[1], c->SB[2], c->SB[3]},
                        {c->SA[0], c->SA[1], c->SA[2], c->SA[3]}};
     auto stage_x = [&](LPass& P, void* const* s, const double* at) {
-        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags);
+        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo);
         LGroup* g = &add_group(P, s[0], T);
         add_out(*g, c->dxR, T, 0, OP_D1X); add_out(*g, c->dxxR, T, 0, OP_D2X);
         g = &add_group(P, s[1], T);
@@ -458,7 +511,7 @@ int build_context(hqt_ctx* c) {
         if (at) { g = &add_group(P, at, DT_F64); add_out(*g, c->dxAt, T, 0, OP_D1X); }
     };
     auto stage_y = [&](LPass& P, void* const* s, const double* at) {
-        P = base_pass(ny, nx, ny, nz, 1, ops, c->flags);
+        P = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo);
         LGroup* g = &add_group(P, s[0], T);
         add_out(*g, c->dyR, T, 0, OP_D1Y); add_out(*g, c->lapR, T, 0, OP_D2Y, -1, -1, c->dxxR, T);
         g = &add_group(P, s[1], T);
@@ -480,11 +533,11 @@ int build_context(hqt_ctx* c) {
     stage_y(c->hp[PYR], SR_, c->AtR);
     {
         LPass& P = c->hp[PXC];
-        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags);
+        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo);
         LGroup& g = add_group(P, c->ZAx, T);
         add_out(g, c->dxAx, T, 0, OP_D1X);
         LPass& Q = c->hp[PYC];
-        Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags);
+        Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo);
         LGroup& h = add_group(Q, c->ZAy, T);
         add_out(h, c->divA, T, 0, OP_D1Y, -1, -1, c->dxAx, T);
     }
@@ -492,9 +545,9 @@ int build_context(hqt_ctx* c) {
         void* f[5] = {c->ZR, c->ZI, c->At, c->ZAx, c->ZAy};
         int fdt[5] = {T, T, DT_F64, T, T};
         LPass& P = c->hp[PXF];
-        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags);
+        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo);
         LPass& Q = c->hp[PYF];
-        Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags);
+        Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo);
         // Im(P) has rank 4 for the reference's masks: applied as U (V^T x) instead of a dense product
         for (int i = 0; i < 5; ++i) {
             LGroup& g = add_group(P, f[i], fdt[i]);
@@ -506,6 +559,11 @@ int build_context(hqt_ctx* c) {
             else add_out(h, f[i], fdt[i], 0, OP_QRY, 1, OP_NQIY, nullptr, DT_F64, 1);
         }
     }
+    // the folded kernel keeps one output's accumulators: a group has one source or one output
+    for (int p = 0; p < NPASS; ++p)
+        for (int g = 0; g < c->hp[p].ngroups; ++g)
+            if (c->hp[p].fold && c->hp[p].g[g].ntiles > 1 && c->hp[p].g[g].nout > 1)
+                return fail(HQT_ERR_ARG, "internal: folded line pass group with two sources and two outputs");
     HIPC(hipMemcpy(c->dp, c->hp, sizeof(c->hp), hipMemcpyHostToDevice));
 
     // ---- radial kernel descriptors ----
@@ -591,6 +649,7 @@ int hqt_ctx_create(const hqt_params* p, hqt_ctx** out) {
     c->npts = c->nxy * p->nz;
     c->fp32 = p->precision == HQT_MIXED;
     c->tdt = c->fp32 ? DT_F32 : DT_F64;
+    c->fold = (p->flags & HQT_DENSE_LINES) ? 0 : 1;
     c->es = c->fp32 ? 4 : 8;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return fail(HQT_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }

@@ -59,6 +59,8 @@ struct hqt_ctx {
     double *F1R = nullptr, *F1I = nullptr, *F2 = nullptr, *F3 = nullptr, *F4 = nullptr;
     int32_t* flags = nullptr;
     void* ops[OP_COUNT];
+    void* fops[OP_COUNT];            // folded tables of the same operators (NULL where unused)
+    int fold = 1;                    // line passes in the even/odd folded form
     LPass hp[NPASS];
     LPass* dp = nullptr;
     ZOps zo{};

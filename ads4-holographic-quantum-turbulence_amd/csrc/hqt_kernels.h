@@ -48,7 +48,9 @@ constexpr int LP_THREADS = 256;
 
 // A term of an output: op(tile).  lr = 0: dense Toeplitz operator in LDS slot `slot`;
 // lr = 1: rank-4 operator U V^T with the [V (Np x 4) | U (Mp x 4)] row-major table at P.ops[slot].
-struct LTerm { int32_t tile; int32_t slot; int32_t lr; int32_t pad; };
+// pad: parity of a dense operator's column (+1 even, -1 odd); rsum: the operator's row sum
+// (C 1 = rsum 1: 1 for the filter's real part, 0 for the derivatives).  Both used by the folded form.
+struct LTerm { int32_t tile; int32_t slot; int32_t lr; int32_t pad; int32_t rsum; };
 struct LOut {
     void* out; const void* addend;
     int32_t out_dt, add_dt;
@@ -71,6 +73,9 @@ struct LPass {
     int32_t ngroups;           // grid.y
     LGroup g[LP_MAXG];
     const void* ops[LP_MAXOPS]; // Toeplitz tables: tab[i - j + Np + 4] = T[i][j], TL entries
+                                // (folded form: tab[d + Z] = c[d mod N] / 2, TL entries)
+    int32_t fold;              // 1: even/odd folded form (hqt_fold_core.h); Mp = 64 QF, S its LDS row stride
+    int32_t Kp, Z;             // folded form: padded folded positions, table offset
     int32_t* flags;            // [0] = NaN step (0: none), [1] = step counter
 };
 
@@ -167,8 +172,9 @@ struct Census {
 // Line passes: grid.y = ngroups (members with fewer groups skip the extra rows); lds bytes
 // must cover every member; Q = Mp/64 is shared by the members of one launch.
 // kind: 0 RHS stage, 1 constraint, 2 filter (separate instantiations for profiling).
+// fold: the even/odd folded kernels (descriptors built with fold = 1; Q is then QF).
 hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, size_t lds, int Q, int nt, int dir,
-                            int fp32, int kind, hipStream_t s);
+                            int fp32, int kind, hipStream_t s, int fold = 0);
 size_t line_pass_lds_bytes(const LPass& P, int fp32);
 // Radial kernels: grid.x per member = zblocks(nxy); nz is shared by the members of one launch.
 inline int zblocks(int64_t nxy) { return (int)((nxy + 63) / 64); }

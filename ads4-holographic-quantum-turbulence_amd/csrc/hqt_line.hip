@@ -1,6 +1,6 @@
 // Line-operator passes: dispatch over the (precision, direction) families whose kernels are
 // compiled in hqt_line_*.hip (the kernel itself is in hqt_line_core.h).
-#include "hqt_line_core.h"
+#include "hqt_fold_core.h"
 
 namespace hqt {
 
@@ -8,18 +8,31 @@ extern template hipError_t line_launch_td<double, 0>(const LPass*, Members, dim3
 extern template hipError_t line_launch_td<double, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t);
 extern template hipError_t line_launch_td<float, 0>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t);
 extern template hipError_t line_launch_td<float, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t);
+extern template hipError_t fold_launch_td<double, 0>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t);
+extern template hipError_t fold_launch_td<double, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t);
+extern template hipError_t fold_launch_td<float, 0>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t);
+extern template hipError_t fold_launch_td<float, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t);
 
 size_t line_pass_lds_bytes(const LPass& P, int fp32) {
     int maxslots = 0;
     for (int g = 0; g < P.ngroups; ++g) maxslots = P.g[g].nslots > maxslots ? P.g[g].nslots : maxslots;
     const size_t es = fp32 ? sizeof(float) : sizeof(double);
-    return es * ((size_t)P.lb * P.S + (size_t)maxslots * P.TL + 6 * 4 * (size_t)P.lb);   // + rank-4 dots
+    // folded: f and g rows, and each source's per-line reference values
+    const size_t lines = P.fold ? 2 * (size_t)P.lb * P.S + 2 * (size_t)P.lb : (size_t)P.lb * P.S;
+    return es * (lines + (size_t)maxslots * P.TL + 6 * 4 * (size_t)P.lb);        // + rank-4 dots
 }
 
 hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, size_t lds, int Q, int nt, int dir,
-                            int fp32, int kind, hipStream_t s) {
+                            int fp32, int kind, hipStream_t s, int fold) {
     if (Q < 1 || Q > 8 || blocks <= 0 || (nt != 1 && nt != 2) || lds > 160 * 1024) return hipErrorInvalidValue;
     const dim3 grid((unsigned)blocks, (unsigned)ngroups);
+    if (fold) {
+        if (Q > 5) return hipErrorInvalidValue;
+        if (fp32) return dir ? fold_launch_td<float, 1>(d, M, grid, Q, nt, kind, lds, s)
+                             : fold_launch_td<float, 0>(d, M, grid, Q, nt, kind, lds, s);
+        return dir ? fold_launch_td<double, 1>(d, M, grid, Q, nt, kind, lds, s)
+                   : fold_launch_td<double, 0>(d, M, grid, Q, nt, kind, lds, s);
+    }
     if (fp32) return dir ? line_launch_td<float, 1>(d, M, grid, Q, nt, kind, lds, s)
                          : line_launch_td<float, 0>(d, M, grid, Q, nt, kind, lds, s);
     return dir ? line_launch_td<double, 1>(d, M, grid, Q, nt, kind, lds, s)

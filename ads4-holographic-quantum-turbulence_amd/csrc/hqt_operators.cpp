@@ -236,6 +236,53 @@ LineGeom line_geom(int N) {
     return g;
 }
 
+FoldGeom fold_geom(int N) {
+    FoldGeom g;
+    g.N = N;
+    g.R = N / 2 + 1;
+    g.Kp = 8 * ((g.R + 7) / 8);
+    g.QF = ((g.R + 15) / 16 + 3) / 4;
+    g.Mp = 64 * g.QF;
+    g.Z = g.Kp + 4;
+    g.TL = 4 * ((g.Z + g.Mp + g.Kp + 4 + 3) / 4);
+    g.S = 16 * ((g.Kp + 4 + 15) / 16) + 1;
+    return g;
+}
+
+void fold_table(const std::vector<double>& c, int sym, const FoldGeom& g, double scale, std::vector<double>& tab) {
+    const int N = g.N;
+    tab.assign((size_t)g.TL, 0.0);
+    for (int t = 0; t < g.TL; ++t) {
+        const int d = ((t - g.Z) % N + N) % N;
+        const double v = 2 * d <= N ? c[d] : sym * c[N - d];
+        tab[t] = 0.5 * scale * v;
+    }
+}
+
+void fold_lowrank(const std::vector<double>& dense, const LineGeom& lg, const FoldGeom& g, std::vector<double>& tab) {
+    const int N = g.N;
+    const double* V = dense.data();
+    const double* U = dense.data() + 4 * (size_t)lg.Np;
+    tab.assign(8 * (size_t)g.Kp + 8 * (size_t)g.Mp, 0.0);
+    double* vf = tab.data();
+    double* vg = vf + 4 * (size_t)g.Kp;
+    double* ue = vg + 4 * (size_t)g.Kp;
+    double* uo = ue + 4 * (size_t)g.Mp;
+    for (int j = 0; 2 * j <= N; ++j)
+        for (int r = 0; r < 4; ++r) {
+            const size_t a = 4 * (size_t)j + r, b = 4 * (size_t)((N - j) % N) + r;
+            if (j == 0 || 2 * j == N) {
+                vf[a] = V[a];
+                ue[a] = U[a];
+            } else {
+                vf[a] = 0.5 * (V[a] + V[b]);
+                vg[a] = 0.5 * (V[a] - V[b]);
+                ue[a] = 0.5 * (U[a] + U[b]);
+                uo[a] = 0.5 * (U[a] - U[b]);
+            }
+        }
+}
+
 void toeplitz_table(const std::vector<double>& col, const std::vector<double>& row, const LineGeom& g, double scale,
                     std::vector<double>& tab) {
     const int N = (int)col.size();

@@ -25,6 +25,18 @@ LineGeom line_geom(int N);
 // Toeplitz lookup table tab[u] = scale * T[i][j] with u = i - j + Np + 4 (zero outside |i-j| < N).
 void toeplitz_table(const std::vector<double>& col, const std::vector<double>& row, const LineGeom& g, double scale,
                     std::vector<double>& tab);
+// Folded (even/odd) form of a circulant line operator: rows i = 0 .. N/2 (R of them, in QF
+// position tiles of 16 per wave, Mp = 64 QF), folded positions j = 0 .. N/2 padded to Kp (a
+// multiple of 8), LDS row stride S, table tab[d + Z] = c[d mod N] / 2 of TL entries.
+struct FoldGeom { int N, R, Kp, QF, Mp, Z, TL, S; };
+FoldGeom fold_geom(int N);
+// Folded table of the circulant with first column c (C[i][j] = c[(i - j) mod N]) and parity
+// sym (+1: c[-k] = c[k], -1: c[-k] = -c[k]): tab[t] = scale * c[(t - Z) mod N] / 2, where c[d]
+// for d > N/2 is taken as sym * c[N - d] (the table is exactly even or odd).
+void fold_table(const std::vector<double>& c, int sym, const FoldGeom& g, double scale, std::vector<double>& tab);
+// Folded form of a rank-4 [V (Np x 4) | U (Mp x 4)] table (filter_lowrank): Vf | Vg (Kp x 4 each),
+// UE | UO (Mp x 4 each) with x.V = f.Vf + g.Vg and y_i = E_i + O_i, y_{N-i} = E_i - O_i.
+void fold_lowrank(const std::vector<double>& dense, const LineGeom& lg, const FoldGeom& g, std::vector<double>& tab);
 // NewtonRaphson_vy; returns 0 when the residual test passed.
 int newton_vy(int nz, double zh, double small, double D, const double* Z1, const double* Z2, const double* vy1,
               double vy, double* z1, double* z2, double* VY1, int* it, double* err, int it_max = 1000,

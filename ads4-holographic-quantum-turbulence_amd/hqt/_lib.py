@@ -19,6 +19,7 @@ HQT_LINES32 = 4
 HQT_RADIAL_W2 = 8
 HQT_RADIAL_W3 = 16
 HQT_DENSE_FILTER = 32
+HQT_DENSE_LINES = 64
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "csrc", "libhqt.so")

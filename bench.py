@@ -299,10 +299,12 @@ def main():
         out["fp64_final"] = {"error": repr(e)}
 
     if not args.no_mixed:
-        tm, finm, _ = run_gpu(nx, ny, nz, L, "mixed", args.steps, args.warmup, state, axb, 0)
+        tm, finm, mstats = run_gpu(nx, ny, nz, L, "mixed", args.steps, args.warmup, state, axb, prof_steps)
         err = field_errors(finm, fin64)
         mixed = {"value": round(args.steps / tm, 3), "unit": "steps/s", "speedup_vs_fp64": round(t64 / tm, 3),
-                 "err_vs_fp64_after_steps": err}
+                 "err_vs_fp64_after_steps": err,
+                 "kernels": {k: {"launches": c, "avg_us": 1e3 * ms / c, "ms_per_step": ms / prof_steps}
+                             for k, (c, ms) in mstats.items()}}
         if O64 is not None:
             Om = diagnostics.order_parameter(g, finm[0])
             mixed["O_abs_rel_err"] = float(np.max(np.abs(np.abs(Om) - np.abs(O64))) / np.max(np.abs(O64)))

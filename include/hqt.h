@@ -44,6 +44,9 @@ extern "C" {
 #define HQT_RADIAL_W3 16    /* radial stage kernel compiled for 3 waves per SIMD (spills from nz = 13) */
 #define HQT_DENSE_FILTER 32 /* filter's imaginary part as a dense Toeplitz product instead of its rank-4 form
                                (the default; ff.m's 2/3 mask leaves at most two unpaired modes) */
+#define HQT_DENSE_LINES 64  /* x/y operators and filter as dense Toeplitz products (func_Lx's own form)
+                               instead of their even/odd folded form (the default: half the
+                               multiply-adds; results agree to rounding) */
 
 #define HQT_FP64   0        /* plain form: FP64 end to end */
 #define HQT_MIXED  1        /* x/y operators, filter, stage arithmetic FP32; z operators FP64 */

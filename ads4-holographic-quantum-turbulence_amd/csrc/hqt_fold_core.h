@@ -20,9 +20,11 @@
 // Toeplitz + Hankel coefficients from acting on a line's mean (A_t is about mu along every line).
 //
 // Work decomposition (as the dense form): a work-group owns LB = 16 NT lines and one output
-// group; each wave owns the position tiles wave + 4 q (q < QF) of the half line and keeps the E
+// group; each wave owns the position tiles role + 4 q (q < QF) of the half line and keeps the E
 // and O accumulators of its tiles, so y_i and y_{N-i} are finished in registers.
 #pragma once
+#include <type_traits>
+
 #include "hqt_line_core.h"
 
 namespace hqt {
@@ -106,12 +108,12 @@ __device__ __forceinline__ void fold_lr_dots(const T* __restrict__ F, const T* _
 template <typename T, int DIR, int QF, int NT>
 __device__ __forceinline__ bool fold_out(const LPass& P, const LOut& O, typename MF<T>::v4 (&aE)[NT][QF],
                                          typename MF<T>::v4 (&aO)[NT][QF], const T (&cst)[NT][4], int line0, int x0,
-                                         int z, int nl, int N, int wave, int lane) {
+                                         int z, int nl, int N, int role, int lane) {
     const int ln = lane & 15, lk = lane >> 4;
     const int nx = P.nx, pl = P.nx * P.ny * z, Mh = N >> 1;
     // element index of position i (direct) or N - i (mirror) of accumulator (u, q, r)
     auto idx = [&](int u, int q, int r, bool mirror) -> uint32_t {
-        const int i0 = (wave + 4 * q) * 16;
+        const int i0 = (role + 4 * q) * 16;
         int line, i;
         if (DIR == 0) {
             line = line0 + 16 * u + MF<T>::row(lk, r);
@@ -223,6 +225,12 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
 
     const int lane = tid & 63, ln = lane & 15, lk = lane >> 4;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    // Position tiles of this wave: role + 4 q, with the role rotated by the work-group index so that
+    // the waves with one tile fewer (their last tile past the half line) fall on different SIMDs
+    // from one work-group to the next.  nq: tiles of the wave that hold output rows (QF or QF - 1).
+    const int role = (wave + mb.bx) & 3;
+    const int rt = (Mh + 1 + 15) >> 4;
+    const int nq = role + 4 * (QF - 1) < rt ? QF : QF - 1;
     // staging map: x-lines: line tid/16 (+16 for the second half), positions tid%16 + 16 r;
     // y-lines: x offset tid%LB, positions tid/LB + (256/LB) r
     const int mrow = DIR == 0 ? (tid >> 4) : (tid % LB);
@@ -338,54 +346,63 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
                 if (e >= O.nterms || O.term[e].tile != t || O.term[e].lr) continue;
                 const int sym = O.term[e].pad;              // +1: even column, -1: odd column
                 // T part tab[i - j + Z], H part tab[i + j + Z]; lane (ln, lk) reads i = i0 + ln, j = j0 + lk
-                const T* Ct = C + O.term[e].slot * TL + Z + ln - lk + wave * 16;
-                const T* Ch = C + O.term[e].slot * TL + Z + ln + lk + wave * 16;
+                const T* Ct = C + O.term[e].slot * TL + Z + ln - lk + role * 16;
+                const T* Ch = C + O.term[e].slot * TL + Z + ln + lk + role * 16;
                 // E gets (T + sg H) times f (even column) or g (odd column), O the other pair
                 const T* Be = (sym > 0 ? Vf : Vg) + ln * S + lk;
                 const T* Bo = (sym > 0 ? Vg : Vf) + ln * S + lk;
                 const T sg = sym > 0 ? (T)1 : (T)-1;
-                T ae0[NT], ao0[NT], ae1[NT], ao1[NT];
-                T t0[QF], h0[QF], t1[QF], h1[QF];
+                // the k-loop over the wave's first NQ position tiles
+                auto kloop = [&](auto nq_c) {
+                    constexpr int NQ = decltype(nq_c)::value;
+                    if constexpr (NQ > 0) {
+                        T ae0[NT], ao0[NT], ae1[NT], ao1[NT];
+                        T t0[NQ], h0[NQ], t1[NQ], h1[NQ];
 #pragma unroll
-                for (int u = 0; u < NT; ++u) { ae0[u] = Be[16 * u * S]; ao0[u] = Bo[16 * u * S]; }
+                        for (int u = 0; u < NT; ++u) { ae0[u] = Be[16 * u * S]; ao0[u] = Bo[16 * u * S]; }
 #pragma unroll
-                for (int q = 0; q < QF; ++q) { t0[q] = Ct[64 * q]; h0[q] = Ch[64 * q]; }
-                // two k-steps per iteration with ping-pong operand registers; the last loads run
-                // one k-step past Kp (values unused: rows are padded, the table is padded)
-                for (int j0 = 0; j0 < Kp; j0 += 8) {
+                        for (int q = 0; q < NQ; ++q) { t0[q] = Ct[64 * q]; h0[q] = Ch[64 * q]; }
+                        // two k-steps per iteration with ping-pong operand registers; the last loads
+                        // run one k-step past Kp (values unused: rows are padded, the table is padded)
+                        for (int j0 = 0; j0 < Kp; j0 += 8) {
 #pragma unroll
-                    for (int u = 0; u < NT; ++u) { ae1[u] = Be[16 * u * S + j0 + 4]; ao1[u] = Bo[16 * u * S + j0 + 4]; }
+                            for (int u = 0; u < NT; ++u) { ae1[u] = Be[16 * u * S + j0 + 4]; ao1[u] = Bo[16 * u * S + j0 + 4]; }
 #pragma unroll
-                    for (int q = 0; q < QF; ++q) { t1[q] = Ct[64 * q - j0 - 4]; h1[q] = Ch[64 * q + j0 + 4]; }
-                    LSB();
+                            for (int q = 0; q < NQ; ++q) { t1[q] = Ct[64 * q - j0 - 4]; h1[q] = Ch[64 * q + j0 + 4]; }
+                            LSB();
 #pragma unroll
-                    for (int q = 0; q < QF; ++q) {
-                        const T ke = fadd_s(sg, h0[q], t0[q]);
-                        const T ko = fadd_s(-sg, h0[q], t0[q]);
+                            for (int q = 0; q < NQ; ++q) {
+                                const T ke = fadd_s(sg, h0[q], t0[q]);
+                                const T ko = fadd_s(-sg, h0[q], t0[q]);
 #pragma unroll
-                        for (int u = 0; u < NT; ++u) {
-                            aE[u][q] = DIR == 0 ? MF<T>::mma(ae0[u], ke, aE[u][q]) : MF<T>::mma(ke, ae0[u], aE[u][q]);
-                            aO[u][q] = DIR == 0 ? MF<T>::mma(ao0[u], ko, aO[u][q]) : MF<T>::mma(ko, ao0[u], aO[u][q]);
+                                for (int u = 0; u < NT; ++u) {
+                                    aE[u][q] = DIR == 0 ? MF<T>::mma(ae0[u], ke, aE[u][q]) : MF<T>::mma(ke, ae0[u], aE[u][q]);
+                                    aO[u][q] = DIR == 0 ? MF<T>::mma(ao0[u], ko, aO[u][q]) : MF<T>::mma(ko, ao0[u], aO[u][q]);
+                                }
+                            }
+                            LSB();
+#pragma unroll
+                            for (int u = 0; u < NT; ++u) { ae0[u] = Be[16 * u * S + j0 + 8]; ao0[u] = Bo[16 * u * S + j0 + 8]; }
+#pragma unroll
+                            for (int q = 0; q < NQ; ++q) { t0[q] = Ct[64 * q - j0 - 8]; h0[q] = Ch[64 * q + j0 + 8]; }
+                            LSB();
+#pragma unroll
+                            for (int q = 0; q < NQ; ++q) {
+                                const T ke = fadd_s(sg, h1[q], t1[q]);
+                                const T ko = fadd_s(-sg, h1[q], t1[q]);
+#pragma unroll
+                                for (int u = 0; u < NT; ++u) {
+                                    aE[u][q] = DIR == 0 ? MF<T>::mma(ae1[u], ke, aE[u][q]) : MF<T>::mma(ke, ae1[u], aE[u][q]);
+                                    aO[u][q] = DIR == 0 ? MF<T>::mma(ao1[u], ko, aO[u][q]) : MF<T>::mma(ko, ao1[u], aO[u][q]);
+                                }
+                            }
+                            LSB();
                         }
                     }
-                    LSB();
-#pragma unroll
-                    for (int u = 0; u < NT; ++u) { ae0[u] = Be[16 * u * S + j0 + 8]; ao0[u] = Bo[16 * u * S + j0 + 8]; }
-#pragma unroll
-                    for (int q = 0; q < QF; ++q) { t0[q] = Ct[64 * q - j0 - 8]; h0[q] = Ch[64 * q + j0 + 8]; }
-                    LSB();
-#pragma unroll
-                    for (int q = 0; q < QF; ++q) {
-                        const T ke = fadd_s(sg, h1[q], t1[q]);
-                        const T ko = fadd_s(-sg, h1[q], t1[q]);
-#pragma unroll
-                        for (int u = 0; u < NT; ++u) {
-                            aE[u][q] = DIR == 0 ? MF<T>::mma(ae1[u], ke, aE[u][q]) : MF<T>::mma(ke, ae1[u], aE[u][q]);
-                            aO[u][q] = DIR == 0 ? MF<T>::mma(ao1[u], ko, aO[u][q]) : MF<T>::mma(ko, ao1[u], aO[u][q]);
-                        }
-                    }
-                    LSB();
-                }
+                };
+                // tiles past the half line hold no output rows: a wave whose last tile lies there skips it
+                if (nq == QF) kloop(std::integral_constant<int, QF>{});
+                else kloop(std::integral_constant<int, (QF > 1 ? QF - 1 : 0)>{});
             }
             if (t != G.ntiles - 1) continue;
             // all sources of output o are in: rank-4 expansion, constant term, store
@@ -401,7 +418,7 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
                             const T d = DL[o * LB * 4 + (16 * u + ln) * 4 + lk];
 #pragma unroll
                             for (int q = 0; q < QF; ++q) {
-                                const int i0 = (wave + 4 * q) * 16;
+                                const int i0 = (role + 4 * q) * 16;
                                 const T we = ue[4 * (i0 + ln) + lk], wo = uo[4 * (i0 + ln) + lk];
                                 aE[u][q] = DIR == 0 ? MF<T>::mma(d, we, aE[u][q]) : MF<T>::mma(we, d, aE[u][q]);
                                 aO[u][q] = DIR == 0 ? MF<T>::mma(d, wo, aO[u][q]) : MF<T>::mma(wo, d, aO[u][q]);
@@ -427,7 +444,7 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
                         cst[u][r] += rsv * X0[O.term[e].tile * LB + l];
                     }
             }
-            const bool bad = fold_out<T, DIR, QF, NT>(P, O, aE, aO, cst, line0, x0, z, nl, N, wave, lane);
+            const bool bad = fold_out<T, DIR, QF, NT>(P, O, aE, aO, cst, line0, x0, z, nl, N, role, lane);
             if (O.nan_check) {
                 if (__any(bad) && lane == 0) {
                     const int step = __atomic_load_n(&P.flags[1], __ATOMIC_RELAXED);

@@ -33,8 +33,9 @@ namespace hqt {
 __device__ __forceinline__ double fadd_s(double s, double h, double t) { return __builtin_fma(s, h, t); }
 __device__ __forceinline__ float fadd_s(float s, float h, float t) { return __builtin_fmaf(s, h, t); }
 
-// Occupancy the folded kernels' register allocation targets.
-template <typename T> constexpr int FP_WPE = sizeof(T) == 8 ? 2 : 3;
+// Occupancy the folded kernels' register allocation targets (T: element type, R: type of the
+// prefetch registers, double when a source of the pass is double).
+template <typename T, typename R> constexpr int FP_WPE = sizeof(T) == 8 ? 2 : 3;
 
 // Largest folded table (entries) for QF position tiles per wave: Z + 64 QF + Kp + 4 with
 // Kp <= 64 QF + 8, Z = Kp + 4.
@@ -188,8 +189,8 @@ __device__ __forceinline__ bool fold_out(const LPass& P, const LOut& O, typename
 // folds groups with one source, or two sources and one output, so one output's accumulators are
 // live at a time.  The global loads of the next unit are issued right after a unit is staged and
 // stay in flight in registers during the unit's MFMAs.
-template <typename T, int DIR, int QF, int KIND, int NT>
-__global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_WPE<T>))) void k_fold_pass(const LPass* __restrict__ Pp, const Members M) {
+template <typename T, int DIR, int QF, int KIND, int NT, typename R>
+__global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_WPE<T, R>))) void k_fold_pass(const LPass* __restrict__ Pp, const Members M) {
     constexpr int LB = 16 * NT;
     constexpr int JS = DIR == 0 ? 16 : LP_THREADS / LB;            // j step between a thread's elements
     constexpr int PFJ = ((64 * QF + 8) + JS - 1) / JS;             // folded positions per thread and line half
@@ -208,9 +209,6 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
     T* DLp = DL + 2 * LB * 4;                // rank-4 partial dots: 4 waves x LB lines x 4
     T* X0 = DLp + 4 * LB * 4;                // reference value x_0 of every line of each source: 2 x LB
     const int tid = threadIdx.x;
-
-    stage_ftables<T, QF>(P, G, C, TL, tid);
-
     const int nl = P.ny * P.nz;
     const int tile0 = mb.bx * P.tpb;
     const int ntile = min(P.tpb, P.ntiles - tile0);
@@ -236,9 +234,9 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
     const int mrow = DIR == 0 ? (tid >> 4) : (tid % LB);
     const int jt = DIR == 0 ? (tid & 15) : (tid / LB);
 
-    // Registers of the unit in flight: x_j, x_{N-j} and x_0 of the thread's lines, widened to
-    // double (every source is double or float; the fold is formed in the source's precision).
-    double pa[NH][PFJ], pb[NH][PFJ], pref[NH];
+    // Registers of the unit in flight: x_j, x_{N-j} and x_0 of the thread's lines in R (double when
+    // any source of the pass is double, so the fold is formed in each source's own precision).
+    R pa[NH][PFJ], pb[NH][PFJ], pref[NH];
     auto fetch = [&](int unit) {
         const int lt = tile0 + unit / G.ntiles, t = unit % G.ntiles;
         const void* src = G.src[t];
@@ -263,7 +261,7 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
                     const int x = x0 + mrow;
                     e = x < P.nx ? (uint32_t)(x + P.nx * P.ny * z) : Z_OOB;
                 }
-                pref[h] = (double)bldr<S_>(rs, e);
+                pref[h] = (R)bldr<S_>(rs, e);
             }
 #pragma unroll
             for (int h = 0; h < NH; ++h)
@@ -283,15 +281,15 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
                         e0 = (okl && j <= Mh) ? (uint32_t)(base + P.nx * j) : Z_OOB;
                         e1 = (okl && j >= 1 && 2 * j < N) ? (uint32_t)(base + P.nx * (N - j)) : Z_OOB;
                     }
-                    pa[h][r] = (double)bldr<S_>(rs, e0);
-                    pb[h][r] = (double)bldr<S_>(rs, e1);
+                    pa[h][r] = (R)bldr<S_>(rs, e0);
+                    pb[h][r] = (R)bldr<S_>(rs, e1);
                 }
         };
         if (dt == DT_F64) load(double{});
         else load(float{});
     };
     // f = (x_j - x_0) + (x_{N-j} - x_0), g = x_j - x_{N-j} into LDS (zero past N/2 and where
-    // x_{N-j} is x_j itself); float sources are exact in double, so this is the source precision.
+    // x_{N-j} is x_j itself), formed in R.
     auto deposit = [&](int t) {
 #pragma unroll
         for (int h = 0; h < NH; ++h)
@@ -299,8 +297,8 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
             for (int r = 0; r < PFJ; ++r) {
                 const int j = jt + JS * r;
                 const bool pair = j >= 1 && 2 * j < N;
-                const double da = j <= Mh ? pa[h][r] - pref[h] : 0.0;
-                const double db = pair ? pb[h][r] - pref[h] : 0.0;
+                const R da = j <= Mh ? pa[h][r] - pref[h] : (R)0;
+                const R db = pair ? pb[h][r] - pref[h] : (R)0;
                 if (j < Kp) {
                     Vf[(mrow + 16 * h) * S + j] = (T)(da + db);
                     Vg[(mrow + 16 * h) * S + j] = pair ? (T)(pa[h][r] - pb[h][r]) : (T)0;
@@ -312,7 +310,9 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
         }
     };
 
+    // the first unit's line loads and the operator tables are in flight together
     if (nunit > 0) fetch(0);
+    stage_ftables<T, QF>(P, G, C, TL, tid);
     for (int unit = 0; unit < nunit; ++unit) {
         const int lt = tile0 + unit / G.ntiles, t = unit % G.ntiles;
         int line0 = 0, z = 0, x0 = 0;
@@ -455,9 +455,9 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
     }
 }
 
-template <typename T, int DIR, int KIND, int NT, int QF>
+template <typename T, int DIR, int KIND, int NT, int QF, typename R>
 hipError_t fold_launch_one(const LPass* d, Members M, dim3 grid, size_t lds, hipStream_t s) {
-    const void* fn = reinterpret_cast<const void*>(&k_fold_pass<T, DIR, QF, KIND, NT>);
+    const void* fn = reinterpret_cast<const void*>(&k_fold_pass<T, DIR, QF, KIND, NT, R>);
     if (lds > 64 * 1024) {
         static bool done = false;
         if (!done) {
@@ -466,34 +466,47 @@ hipError_t fold_launch_one(const LPass* d, Members M, dim3 grid, size_t lds, hip
             done = true;
         }
     }
-    hipLaunchKernelGGL((k_fold_pass<T, DIR, QF, KIND, NT>), grid, dim3(LP_THREADS), lds, s, d, M);
+    hipLaunchKernelGGL((k_fold_pass<T, DIR, QF, KIND, NT, R>), grid, dim3(LP_THREADS), lds, s, d, M);
     return hipGetLastError();
 }
 
-template <typename T, int DIR, int KIND, int NT>
+template <typename T, int DIR, int KIND, int NT, typename R>
 hipError_t fold_launch_q(const LPass* d, Members M, dim3 grid, int QF, size_t lds, hipStream_t s) {
     switch (QF) {
-        case 1: return fold_launch_one<T, DIR, KIND, NT, 1>(d, M, grid, lds, s);
-        case 2: return fold_launch_one<T, DIR, KIND, NT, 2>(d, M, grid, lds, s);
-        case 3: return fold_launch_one<T, DIR, KIND, NT, 3>(d, M, grid, lds, s);
-        case 4: return fold_launch_one<T, DIR, KIND, NT, 4>(d, M, grid, lds, s);
-        case 5: return fold_launch_one<T, DIR, KIND, NT, 5>(d, M, grid, lds, s);
+        case 1: return fold_launch_one<T, DIR, KIND, NT, 1, R>(d, M, grid, lds, s);
+        case 2: return fold_launch_one<T, DIR, KIND, NT, 2, R>(d, M, grid, lds, s);
+        case 3: return fold_launch_one<T, DIR, KIND, NT, 3, R>(d, M, grid, lds, s);
+        case 4: return fold_launch_one<T, DIR, KIND, NT, 4, R>(d, M, grid, lds, s);
+        case 5: return fold_launch_one<T, DIR, KIND, NT, 5, R>(d, M, grid, lds, s);
         default: return hipErrorInvalidValue;
     }
 }
 
-// Every folded instance of one (precision, direction) family; explicitly instantiated by the
-// hqt_fold_*.hip translation units.
-template <typename T, int DIR>
-hipError_t fold_launch_td(const LPass* d, Members M, dim3 grid, int QF, int nt, int kind, size_t lds, hipStream_t s) {
+template <typename T, int DIR, typename R>
+hipError_t fold_launch_tr(const LPass* d, Members M, dim3 grid, int QF, int nt, int kind, size_t lds, hipStream_t s) {
     if (nt == 2) {
-        if (kind == 0) return fold_launch_q<T, DIR, 0, 2>(d, M, grid, QF, lds, s);
-        if (kind == 1) return fold_launch_q<T, DIR, 1, 2>(d, M, grid, QF, lds, s);
-        return fold_launch_q<T, DIR, 2, 2>(d, M, grid, QF, lds, s);
+        if (kind == 0) return fold_launch_q<T, DIR, 0, 2, R>(d, M, grid, QF, lds, s);
+        if (kind == 1) return fold_launch_q<T, DIR, 1, 2, R>(d, M, grid, QF, lds, s);
+        return fold_launch_q<T, DIR, 2, 2, R>(d, M, grid, QF, lds, s);
     }
-    if (kind == 0) return fold_launch_q<T, DIR, 0, 1>(d, M, grid, QF, lds, s);
-    if (kind == 1) return fold_launch_q<T, DIR, 1, 1>(d, M, grid, QF, lds, s);
-    return fold_launch_q<T, DIR, 2, 1>(d, M, grid, QF, lds, s);
+    if (kind == 0) return fold_launch_q<T, DIR, 0, 1, R>(d, M, grid, QF, lds, s);
+    if (kind == 1) return fold_launch_q<T, DIR, 1, 1, R>(d, M, grid, QF, lds, s);
+    return fold_launch_q<T, DIR, 2, 1, R>(d, M, grid, QF, lds, s);
+}
+
+// Every folded instance of one (precision, direction) family; explicitly instantiated by the
+// hqt_fold_*.hip translation units.  wide: prefetch registers in double (a double source in an
+// FP32 pass); FP64 passes always use double.
+template <typename T, int DIR>
+hipError_t fold_launch_td(const LPass* d, Members M, dim3 grid, int QF, int nt, int kind, size_t lds, hipStream_t s,
+                          int wide) {
+    if constexpr (sizeof(T) == 8) {
+        (void)wide;
+        return fold_launch_tr<T, DIR, double>(d, M, grid, QF, nt, kind, lds, s);
+    } else {
+        if (wide) return fold_launch_tr<T, DIR, double>(d, M, grid, QF, nt, kind, lds, s);
+        return fold_launch_tr<T, DIR, float>(d, M, grid, QF, nt, kind, lds, s);
+    }
 }
 
 }  // namespace hqt

@@ -116,7 +116,7 @@ void set_lb(LPass& P, int lb, int dir, int tpb = 1) {
     else P.ntiles = P.nxc * P.nz;
 }
 
-LPass base_pass(int N, int nx, int ny, int nz, int dir, void* const* ops, int32_t* flags, int fold = 0) {
+LPass base_pass(int N, int nx, int ny, int nz, int dir, void* const* ops, int32_t* flags, int fold = 0, int es = 8) {
     LPass P;
     std::memset(&P, 0, sizeof(P));
     const LineGeom lg = line_geom(N);
@@ -126,7 +126,7 @@ LPass base_pass(int N, int nx, int ny, int nz, int dir, void* const* ops, int32_
     P.TL = lg.TL;
     P.S = P.Np + 1;
     if (fold) {
-        const FoldGeom fg = fold_geom(N);
+        const FoldGeom fg = fold_geom(N, es);
         P.fold = 1;
         P.Mp = fg.Mp;
         P.TL = fg.TL;
@@ -499,7 +499,7 @@ int build_context(hqt_ctx* c) {
                        {c->SB[0], c->SB[1], c->SB[2], c->SB[3]},
                        {c->SA[0], c->SA[1], c->SA[2], c->SA[3]}};
     auto stage_x = [&](LPass& P, void* const* s, const double* at) {
-        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo);
+        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo, (int)c->es);
         LGroup* g = &add_group(P, s[0], T);
         add_out(*g, c->dxR, T, 0, OP_D1X); add_out(*g, c->dxxR, T, 0, OP_D2X);
         g = &add_group(P, s[1], T);
@@ -511,7 +511,7 @@ int build_context(hqt_ctx* c) {
         if (at) { g = &add_group(P, at, DT_F64); add_out(*g, c->dxAt, T, 0, OP_D1X); }
     };
     auto stage_y = [&](LPass& P, void* const* s, const double* at) {
-        P = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo);
+        P = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo, (int)c->es);
         LGroup* g = &add_group(P, s[0], T);
         add_out(*g, c->dyR, T, 0, OP_D1Y); add_out(*g, c->lapR, T, 0, OP_D2Y, -1, -1, c->dxxR, T);
         g = &add_group(P, s[1], T);
@@ -533,11 +533,11 @@ int build_context(hqt_ctx* c) {
     stage_y(c->hp[PYR], SR_, c->AtR);
     {
         LPass& P = c->hp[PXC];
-        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo);
+        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo, (int)c->es);
         LGroup& g = add_group(P, c->ZAx, T);
         add_out(g, c->dxAx, T, 0, OP_D1X);
         LPass& Q = c->hp[PYC];
-        Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo);
+        Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo, (int)c->es);
         LGroup& h = add_group(Q, c->ZAy, T);
         add_out(h, c->divA, T, 0, OP_D1Y, -1, -1, c->dxAx, T);
     }
@@ -545,9 +545,9 @@ int build_context(hqt_ctx* c) {
         void* f[5] = {c->ZR, c->ZI, c->At, c->ZAx, c->ZAy};
         int fdt[5] = {T, T, DT_F64, T, T};
         LPass& P = c->hp[PXF];
-        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo);
+        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo, (int)c->es);
         LPass& Q = c->hp[PYF];
-        Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo);
+        Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo, (int)c->es);
         // Im(P) has rank 4 for the reference's masks: applied as U (V^T x) instead of a dense product
         for (int i = 0; i < 5; ++i) {
             LGroup& g = add_group(P, f[i], fdt[i]);
@@ -559,11 +559,16 @@ int build_context(hqt_ctx* c) {
             else add_out(h, f[i], fdt[i], 0, OP_QRY, 1, OP_NQIY, nullptr, DT_F64, 1);
         }
     }
-    // the folded kernel keeps one output's accumulators: a group has one source or one output
+    // the folded kernel keeps one output's accumulators: a group has one source or one output;
+    // an FP32 pass with a double source (A_t) stages through double registers
     for (int p = 0; p < NPASS; ++p)
-        for (int g = 0; g < c->hp[p].ngroups; ++g)
-            if (c->hp[p].fold && c->hp[p].g[g].ntiles > 1 && c->hp[p].g[g].nout > 1)
+        for (int g = 0; g < c->hp[p].ngroups; ++g) {
+            LPass& H = c->hp[p];
+            if (H.fold && H.g[g].ntiles > 1 && H.g[g].nout > 1)
                 return fail(HQT_ERR_ARG, "internal: folded line pass group with two sources and two outputs");
+            for (int t = 0; t < H.g[g].ntiles; ++t)
+                if (H.fold && c->fp32 && H.g[g].src_dt[t] == DT_F64) H.fold = 2;
+        }
     HIPC(hipMemcpy(c->dp, c->hp, sizeof(c->hp), hipMemcpyHostToDevice));
 
     // ---- radial kernel descriptors ----

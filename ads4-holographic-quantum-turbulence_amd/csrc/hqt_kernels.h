@@ -74,7 +74,8 @@ struct LPass {
     LGroup g[LP_MAXG];
     const void* ops[LP_MAXOPS]; // Toeplitz tables: tab[i - j + Np + 4] = T[i][j], TL entries
                                 // (folded form: tab[d + Z] = c[d mod N] / 2, TL entries)
-    int32_t fold;              // 1: even/odd folded form (hqt_fold_core.h); Mp = 64 QF, S its LDS row stride
+    int32_t fold;              // even/odd folded form (hqt_fold_core.h; Mp = 64 QF, S its LDS row stride):
+                               // 1, or 2 when a source is double in an FP32 pass (wide prefetch registers)
     int32_t Kp, Z;             // folded form: padded folded positions, table offset
     int32_t* flags;            // [0] = NaN step (0: none), [1] = step counter
 };
@@ -172,7 +173,7 @@ struct Census {
 // Line passes: grid.y = ngroups (members with fewer groups skip the extra rows); lds bytes
 // must cover every member; Q = Mp/64 is shared by the members of one launch.
 // kind: 0 RHS stage, 1 constraint, 2 filter (separate instantiations for profiling).
-// fold: the even/odd folded kernels (descriptors built with fold = 1; Q is then QF).
+// fold: the even/odd folded kernels (descriptors built with fold = 1 or 2; Q is then QF).
 hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, size_t lds, int Q, int nt, int dir,
                             int fp32, int kind, hipStream_t s, int fold = 0);
 size_t line_pass_lds_bytes(const LPass& P, int fp32);

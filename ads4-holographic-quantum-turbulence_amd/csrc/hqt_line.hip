@@ -8,10 +8,10 @@ extern template hipError_t line_launch_td<double, 0>(const LPass*, Members, dim3
 extern template hipError_t line_launch_td<double, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t);
 extern template hipError_t line_launch_td<float, 0>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t);
 extern template hipError_t line_launch_td<float, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t);
-extern template hipError_t fold_launch_td<double, 0>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t);
-extern template hipError_t fold_launch_td<double, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t);
-extern template hipError_t fold_launch_td<float, 0>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t);
-extern template hipError_t fold_launch_td<float, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t);
+extern template hipError_t fold_launch_td<double, 0>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int);
+extern template hipError_t fold_launch_td<double, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int);
+extern template hipError_t fold_launch_td<float, 0>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int);
+extern template hipError_t fold_launch_td<float, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int);
 
 size_t line_pass_lds_bytes(const LPass& P, int fp32) {
     int maxslots = 0;
@@ -28,10 +28,11 @@ hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, 
     const dim3 grid((unsigned)blocks, (unsigned)ngroups);
     if (fold) {
         if (Q > 5) return hipErrorInvalidValue;
-        if (fp32) return dir ? fold_launch_td<float, 1>(d, M, grid, Q, nt, kind, lds, s)
-                             : fold_launch_td<float, 0>(d, M, grid, Q, nt, kind, lds, s);
-        return dir ? fold_launch_td<double, 1>(d, M, grid, Q, nt, kind, lds, s)
-                   : fold_launch_td<double, 0>(d, M, grid, Q, nt, kind, lds, s);
+        const int wide = fold == 2;
+        if (fp32) return dir ? fold_launch_td<float, 1>(d, M, grid, Q, nt, kind, lds, s, wide)
+                             : fold_launch_td<float, 0>(d, M, grid, Q, nt, kind, lds, s, wide);
+        return dir ? fold_launch_td<double, 1>(d, M, grid, Q, nt, kind, lds, s, wide)
+                   : fold_launch_td<double, 0>(d, M, grid, Q, nt, kind, lds, s, wide);
     }
     if (fp32) return dir ? line_launch_td<float, 1>(d, M, grid, Q, nt, kind, lds, s)
                          : line_launch_td<float, 0>(d, M, grid, Q, nt, kind, lds, s);

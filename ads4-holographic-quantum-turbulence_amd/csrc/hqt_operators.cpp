@@ -236,7 +236,7 @@ LineGeom line_geom(int N) {
     return g;
 }
 
-FoldGeom fold_geom(int N) {
+FoldGeom fold_geom(int N, int es) {
     FoldGeom g;
     g.N = N;
     g.R = N / 2 + 1;
@@ -245,7 +245,12 @@ FoldGeom fold_geom(int N) {
     g.Mp = 64 * g.QF;
     g.Z = g.Kp + 4;
     g.TL = 4 * ((g.Z + g.Mp + g.Kp + 4 + 3) / 4);
-    g.S = 16 * ((g.Kp + 4 + 15) / 16) + 1;
+    // Row stride of the folded lines in LDS, at least Kp + 4.  A k-step
+    // reads element (line ln, k lk) in lane ln + 16 lk: with S = 2 (mod 32) doubles or 4 (mod 64)
+    // floats, each group of lanes served together (32 of 8 bytes or 64 of 4 bytes over 64 banks of
+    // 4 bytes) hits distinct banks.
+    const int m = es == 8 ? 32 : 64, r = es == 8 ? 2 : 4;
+    g.S = r + m * ((g.Kp + 4 - r + m - 1) / m);
     return g;
 }
 

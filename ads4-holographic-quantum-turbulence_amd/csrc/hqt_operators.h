@@ -29,7 +29,8 @@ void toeplitz_table(const std::vector<double>& col, const std::vector<double>& r
 // position tiles of 16 per wave, Mp = 64 QF), folded positions j = 0 .. N/2 padded to Kp (a
 // multiple of 8), LDS row stride S, table tab[d + Z] = c[d mod N] / 2 of TL entries.
 struct FoldGeom { int N, R, Kp, QF, Mp, Z, TL, S; };
-FoldGeom fold_geom(int N);
+// es: element bytes (8 or 4), which sets the LDS row stride S (see fold_geom).
+FoldGeom fold_geom(int N, int es = 8);
 // Folded table of the circulant with first column c (C[i][j] = c[(i - j) mod N]) and parity
 // sym (+1: c[-k] = c[k], -1: c[-k] = -c[k]): tab[t] = scale * c[(t - Z) mod N] / 2, where c[d]
 // for d > N/2 is taken as sym * c[N - d] (the table is exactly even or odd).

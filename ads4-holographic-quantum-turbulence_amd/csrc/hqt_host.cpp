@@ -294,6 +294,9 @@ int upload_desc(hqt_ctx* c) {
     return HQT_OK;
 }
 
+// Kernel kind of a pass (the instantiation its launches use): 2 filter, 1 constraint, 0 stage/RHS.
+int pass_kind(int p) { return (p == PXF || p == PYF) ? 2 : (p == PXC || p == PYC) ? 1 : 0; }
+
 int pass_dir(int p) { return ((p >= PY1 && p <= PY4) || p == PYC || p == PYF || p == PYR) ? 1 : 0; }
 
 constexpr size_t kLdsCap = 160 * 1024;
@@ -337,7 +340,7 @@ int tune_lines(hqt_ctx* c) {
                         if (rep == 1) e = hipEventRecord(ea, c->stream);
                         if (e == hipSuccess)
                             e = launch_line_pass(c->dp + p, kOne, line_blocks(H), H.ngroups, lds, H.Mp / 64, lb / 16, dir,
-                                                 c->fp32, 0, c->stream, H.fold);
+                                                 c->fp32, pass_kind(p), c->stream, H.fold);
                     }
                     if (e == hipSuccess) e = hipEventRecord(eb, c->stream);
                     if (e == hipSuccess) e = hipEventSynchronize(eb);

@@ -36,25 +36,34 @@ PEAK_FP32_TFLOPS = 157.3     # MI355X dense FP32 matrix
 PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
 
 
-def line_pass_flops(name, nx, ny, nz, stage_mix=(1, 3)):
-    """Algorithmic FLOPs of one launch of a line pass in the reference's dense form
-    (each operator application = one dense N x N matrix times every line: 2*N flops/point)."""
-    S = nx * ny * nz
+def line_pass_apps(name, stage_mix=(1, 3)):
+    """Operator applications of one launch of a line pass (dense and rank-4 terms), averaged over
+    the RK stages for the stage passes (stage 1 adds the derivative of A_t)."""
     if name == "line_x_stage":      # dx, dxx of Re/Im Psi; dx Ax; dx, dxx Ay; (+ dx At in stage 1)
-        apps = (8 * stage_mix[0] + 7 * stage_mix[1]) / sum(stage_mix)
-        return apps * 2 * nx * S
+        return (8 * stage_mix[0] + 7 * stage_mix[1]) / sum(stage_mix), 0
     if name == "line_y_stage":      # dy, dyy of Re/Im Psi; dy Ay; dyy Ax + dy(dxAy); dy(dxAx) (+ dy At)
-        apps = (9 * stage_mix[0] + 8 * stage_mix[1]) / sum(stage_mix)
-        return apps * 2 * ny * S
-    if name == "line_x_filter":     # Pr, Pi on 5 fields
-        return 10 * 2 * nx * S
-    if name == "line_y_filter":     # Qr, -Qi on 5 fields
-        return 10 * 2 * ny * S
-    if name == "line_x_cons":
-        return 1 * 2 * nx * S
-    if name == "line_y_cons":
-        return 1 * 2 * ny * S
-    return None
+        return (9 * stage_mix[0] + 8 * stage_mix[1]) / sum(stage_mix), 0
+    if name in ("line_x_filter", "line_y_filter"):   # real part dense, imaginary part rank 4, on 5 fields
+        return 5, 5
+    if name in ("line_x_cons", "line_y_cons"):
+        return 1, 0
+    return None, None
+
+
+def line_pass_flops(name, nx, ny, nz, folded=True):
+    """Algorithmic FLOPs of one launch of a line pass.  Dense form (the reference's func_Lx: a
+    dense N x N matrix times every line): 2 N flops per point per application.  Folded form (the
+    kernels' even/odd split): each output pair (i, N-i) takes the even and the odd half-size dot
+    products of N/2 + 1 terms, i.e. 2 (N/2 + 1) flops per point per application.  A rank-4 term
+    (the filter's imaginary part) costs 2 x 2 x 4 flops per point (dots and expansion)."""
+    S = nx * ny * nz
+    N = nx if "_x_" in name else ny
+    dense, lr = line_pass_apps(name)
+    if dense is None:
+        return None
+    per = 2 * (N // 2 + 1) if folded else 2 * N
+    lr_flops = 16 if folded else 2 * N          # the dense form applies the imaginary part densely
+    return dense * per * S + lr * lr_flops * S
 
 
 def profiled_traffic(kernel):
@@ -70,18 +79,18 @@ def profiled_traffic(kernel):
     return None
 
 
-def step_flops_dense(nx, ny, nz):
+def step_flops(nx, ny, nz, folded):
     tot = 0.0
     for k, n in (("line_x_stage", 4), ("line_y_stage", 4), ("line_x_filter", 1), ("line_y_filter", 1),
                  ("line_x_cons", 1), ("line_y_cons", 1)):
-        tot += n * line_pass_flops(k, nx, ny, nz)
+        tot += n * line_pass_flops(k, nx, ny, nz, folded)
     return tot
 
 
-def run_gpu(nx, ny, nz, L, precision, steps, warmup, state, axb, profile_steps):
+def run_gpu(nx, ny, nz, L, precision, steps, warmup, state, axb, profile_steps, dense_lines=False):
     import hqt
     st = hqt.Stepper(nx, ny, nz, L, L, mu=6.0, dt=0.02, winding=True, filter=True, precision=precision,
-                     ax_b=axb, ay_b=0.0)
+                     ax_b=axb, ay_b=0.0, dense_lines=dense_lines)
     st.set_state(*state)
     st.step(warmup)
     st.sync()
@@ -219,6 +228,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--serving-rate", type=float, default=2.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serving-seconds", type=float, default=15.0)
+    ap.add_argument("--dense-lines", action="store_true",
+                    help="x/y operators as dense Toeplitz products (the reference's form) instead of the folded form")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="also run the plain copy kernel on MB megabytes (traffic-counter calibration)")
     args = ap.parse_args()
@@ -244,7 +255,7 @@ def main():
     S = nx * ny * nz
 
     prof_steps = max(5, min(args.steps, 50))
-    t64, fin64, stats = run_gpu(nx, ny, nz, L, "fp64", args.steps, args.warmup, state, axb, prof_steps)
+    t64, fin64, stats = run_gpu(nx, ny, nz, L, "fp64", args.steps, args.warmup, state, axb, prof_steps, args.dense_lines)
     sps = args.steps / t64
 
     # dominant kernel roofline (HIP events on the library's stream)
@@ -255,15 +266,18 @@ def main():
     roof = None
     if dom is not None:
         cnt, ms = stats[dom]
-        fl = line_pass_flops(dom, nx, ny, nz)
+        fl = line_pass_flops(dom, nx, ny, nz, folded=not args.dense_lines)
         avg_s = ms / cnt / 1e3
         if fl is not None:
             ach = fl / avg_s / 1e12
-            tr = profiled_traffic(dom) if (nx, ny, nz) == (301, 301, 20) else None
+            tr = profiled_traffic(dom + ("_dense" if args.dense_lines else "")) if (nx, ny, nz) == (301, 301, 20) else None
             roof = {"kernel": dom, "bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": tr,
                     "avg_launch_us": round(avg_s * 1e6, 2),
-                    "flops_per_launch": fl}
+                    "flops_per_launch": fl,
+                    "flops_form": ("dense operator, 2 N flops per point per application" if args.dense_lines else
+                                   "folded (even/odd) operator, 2 (N/2+1) flops per point per application"),
+                    "dense_form_equivalent_TFLOPs": round(line_pass_flops(dom, nx, ny, nz, folded=False) / avg_s / 1e12, 3)}
 
     step_bytes = 368.0 * S          # compulsory FP64 bytes per step (4 stages + post-step)
     out = {
@@ -285,7 +299,7 @@ def main():
         "step_hbm": {"algorithmic_bytes_per_step": step_bytes,
                      "achieved_GBps": round(step_bytes * sps / 1e9, 2),
                      "frac_of_peak": round(step_bytes * sps / 1e9 / PEAK_HBM_GBS, 4)},
-        "step_flops_dense_form": step_flops_dense(nx, ny, nz),
+        "step_flops": {"folded_form": step_flops(nx, ny, nz, True), "dense_form": step_flops(nx, ny, nz, False)},
         "roofline": roof,
         "kernels": kernels,
     }
@@ -299,7 +313,8 @@ def main():
         out["fp64_final"] = {"error": repr(e)}
 
     if not args.no_mixed:
-        tm, finm, mstats = run_gpu(nx, ny, nz, L, "mixed", args.steps, args.warmup, state, axb, prof_steps)
+        tm, finm, mstats = run_gpu(nx, ny, nz, L, "mixed", args.steps, args.warmup, state, axb, prof_steps,
+                                   args.dense_lines)
         err = field_errors(finm, fin64)
         mixed = {"value": round(args.steps / tm, 3), "unit": "steps/s", "speedup_vs_fp64": round(t64 / tm, 3),
                  "err_vs_fp64_after_steps": err,

@@ -21,7 +21,7 @@ enum LKind { K_LINE, K_ZPREP, K_ZSTAGE, K_ZPOST, K_WPART, K_WAPPLY, K_NAN };
 
 struct Launch {
     int kind = 0;
-    int dir = 0, lkind = 0, Q = 1, nt = 1, ngroups = 1, nz = 0, fp32 = 0, wpe = 2, blocks = 0, nmem = 0, fold = 0;
+    int dir = 0, lkind = 0, Q = 1, nt = 1, ngroups = 1, nz = 0, fp32 = 0, wpe = 2, blocks = 0, nmem = 0, fold = 0, order = 0;
     size_t lds = 0;
     size_t desc_off = 0, boff_off = 0;   // offsets into the device plan buffer
 };
@@ -83,8 +83,10 @@ Launch make_launch(PlanImage& img, int kind, const std::vector<hqt_ctx*>& ms, De
 
 void add_line(std::vector<Launch>& plan, PlanImage& img, const std::vector<hqt_ctx*>& mem, int pass, int dir, int name,
               bool (*use)(const hqt_ctx*)) {
-    auto cls = classes<std::tuple<int, int, int, int>>(
-        mem, [&](hqt_ctx* c) { return std::make_tuple(c->fp32, c->hp[pass].Mp / 64, c->hp[pass].lb / 16, c->hp[pass].fold); },
+    auto cls = classes<std::tuple<int, int, int, int, int>>(
+        mem, [&](hqt_ctx* c) {
+            return std::make_tuple(c->fp32, c->hp[pass].Mp / 64, c->hp[pass].lb / 16, c->hp[pass].fold, c->hp[pass].xcd);
+        },
         [&](hqt_ctx* c) { return use(c); });
     for (auto& kv : cls) {
         Launch L = make_launch<LPass>(img, K_LINE, kv.second, [&](hqt_ctx* c) { return c->hp[pass]; },
@@ -94,6 +96,7 @@ void add_line(std::vector<Launch>& plan, PlanImage& img, const std::vector<hqt_c
         L.Q = std::get<1>(kv.first);
         L.nt = std::get<2>(kv.first);
         L.fold = std::get<3>(kv.first);
+        L.order = std::get<4>(kv.first);
         L.lkind = (name == KN_LX_FILT || name == KN_LY_FILT) ? 2 : (name == KN_LX_CONS || name == KN_LY_CONS) ? 1 : 0;
         for (hqt_ctx* c : kv.second) {
             L.ngroups = std::max(L.ngroups, c->hp[pass].ngroups);
@@ -154,7 +157,7 @@ hipError_t issue(const hqt_ensemble* e, const Launch& L) {
     switch (L.kind) {
         case K_LINE:
             return launch_line_pass(static_cast<const LPass*>(d), M, L.blocks, L.ngroups, L.lds, L.Q, L.nt, L.dir, L.fp32,
-                                    L.lkind, s, L.fold);
+                                    L.lkind, s, L.fold, L.order);
         case K_ZPREP: return launch_zprep(static_cast<const ZPrep*>(d), M, L.blocks, L.nz, s);
         case K_ZSTAGE: return launch_zstage(static_cast<const ZStage*>(d), M, L.blocks, L.nz, L.fp32, L.wpe, s);
         case K_ZPOST: return launch_zpost(static_cast<const ZPost*>(d), M, L.blocks, L.nz, L.fp32, s);

@@ -190,21 +190,36 @@ __device__ __forceinline__ bool fold_out(const LPass& P, const LOut& O, typename
 // live at a time.  The global loads of the next unit are issued right after a unit is staged and
 // stay in flight in registers during the unit's MFMAs.
 template <typename T, int DIR, int QF, int KIND, int NT, typename R>
-__global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_WPE<T, R>))) void k_fold_pass(const LPass* __restrict__ Pp, const Members M) {
+__global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_WPE<T, R>))) void k_fold_pass(const LPass* __restrict__ Pp, const Members M, const int xcd_order) {
     constexpr int LB = 16 * NT;
     constexpr int JS = DIR == 0 ? 16 : LP_THREADS / LB;            // j step between a thread's elements
     constexpr int PFJ = ((64 * QF + 8) + JS - 1) / JS;             // folded positions per thread and line half
     constexpr int NH = DIR == 0 ? NT : 1;                          // line halves per thread (x-lines, LB = 32)
-    const MemberBlock mb = member_block(M);
+    // Block order (chosen per pass by the autotuner; it never changes a result).  xcd_order = 0:
+    // the launch order, group-major.  xcd_order = 1: XCD-aware over the whole (tile, group) grid,
+    // tile-major, so that one XCD's L2 serves a run of consecutive tiles (their lines share cache
+    // lines at the edges, which matters for the strided y-lines) with all of their groups.
+    const int nbx = (int)gridDim.x, nby = (int)gridDim.y, lin0 = (int)blockIdx.x + nbx * (int)blockIdx.y;
+    int lin, bxl, gy;
+    if (xcd_order) {
+        lin = xcd_block(lin0, nbx * nby);
+        bxl = lin / nby;
+        gy = lin - bxl * nby;
+    } else {
+        lin = lin0;
+        bxl = (int)blockIdx.x;
+        gy = (int)blockIdx.y;
+    }
+    const MemberBlock mb = member_block_at(M, bxl, nbx);
     const LPass& P = Pp[mb.m];
-    if ((int)blockIdx.y >= P.ngroups) return;
+    if (gy >= P.ngroups) return;
     if (__builtin_nontemporal_load(&P.flags[0]) != 0) return;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     const int S = P.S, Kp = P.Kp, N = P.N, TL = P.TL, Z = P.Z, Mh = N >> 1;
     T* Vf = reinterpret_cast<T*>(smem_raw);
     T* Vg = Vf + LB * S;
     T* C = Vg + LB * S;
-    const LGroup& G = P.g[blockIdx.y];
+    const LGroup& G = P.g[gy];
     T* DL = C + G.nslots * TL;               // rank-4 dots: 2 outputs x LB lines x 4
     T* DLp = DL + 2 * LB * 4;                // rank-4 partial dots: 4 waves x LB lines x 4
     T* X0 = DLp + 4 * LB * 4;                // reference value x_0 of every line of each source: 2 x LB
@@ -223,10 +238,11 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
 
     const int lane = tid & 63, ln = lane & 15, lk = lane >> 4;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    // Position tiles of this wave: role + 4 q, with the role rotated by the work-group index so that
-    // the waves with one tile fewer (their last tile past the half line) fall on different SIMDs
-    // from one work-group to the next.  nq: tiles of the wave that hold output rows (QF or QF - 1).
-    const int role = (wave + mb.bx) & 3;
+    // Position tiles of this wave: role + 4 q, with the role rotated by the logical block index so
+    // that the waves with one tile fewer (their last tile past the half line) fall on different
+    // SIMDs in work-groups that share a CU (an XCD's 32 CUs take its blocks in turn, so blocks 1 and
+    // 32 apart differ in rotation).  nq: tiles of the wave that hold output rows (QF or QF - 1).
+    const int role = (wave + lin + (lin >> 5)) & 3;
     const int rt = (Mh + 1 + 15) >> 4;
     const int nq = role + 4 * (QF - 1) < rt ? QF : QF - 1;
     // staging map: x-lines: line tid/16 (+16 for the second half), positions tid%16 + 16 r;
@@ -456,7 +472,7 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
 }
 
 template <typename T, int DIR, int KIND, int NT, int QF, typename R>
-hipError_t fold_launch_one(const LPass* d, Members M, dim3 grid, size_t lds, hipStream_t s) {
+hipError_t fold_launch_one(const LPass* d, Members M, dim3 grid, size_t lds, hipStream_t s, int order) {
     const void* fn = reinterpret_cast<const void*>(&k_fold_pass<T, DIR, QF, KIND, NT, R>);
     if (lds > 64 * 1024) {
         static bool done = false;
@@ -466,46 +482,47 @@ hipError_t fold_launch_one(const LPass* d, Members M, dim3 grid, size_t lds, hip
             done = true;
         }
     }
-    hipLaunchKernelGGL((k_fold_pass<T, DIR, QF, KIND, NT, R>), grid, dim3(LP_THREADS), lds, s, d, M);
+    hipLaunchKernelGGL((k_fold_pass<T, DIR, QF, KIND, NT, R>), grid, dim3(LP_THREADS), lds, s, d, M, order);
     return hipGetLastError();
 }
 
 template <typename T, int DIR, int KIND, int NT, typename R>
-hipError_t fold_launch_q(const LPass* d, Members M, dim3 grid, int QF, size_t lds, hipStream_t s) {
+hipError_t fold_launch_q(const LPass* d, Members M, dim3 grid, int QF, size_t lds, hipStream_t s, int order) {
     switch (QF) {
-        case 1: return fold_launch_one<T, DIR, KIND, NT, 1, R>(d, M, grid, lds, s);
-        case 2: return fold_launch_one<T, DIR, KIND, NT, 2, R>(d, M, grid, lds, s);
-        case 3: return fold_launch_one<T, DIR, KIND, NT, 3, R>(d, M, grid, lds, s);
-        case 4: return fold_launch_one<T, DIR, KIND, NT, 4, R>(d, M, grid, lds, s);
-        case 5: return fold_launch_one<T, DIR, KIND, NT, 5, R>(d, M, grid, lds, s);
+        case 1: return fold_launch_one<T, DIR, KIND, NT, 1, R>(d, M, grid, lds, s, order);
+        case 2: return fold_launch_one<T, DIR, KIND, NT, 2, R>(d, M, grid, lds, s, order);
+        case 3: return fold_launch_one<T, DIR, KIND, NT, 3, R>(d, M, grid, lds, s, order);
+        case 4: return fold_launch_one<T, DIR, KIND, NT, 4, R>(d, M, grid, lds, s, order);
+        case 5: return fold_launch_one<T, DIR, KIND, NT, 5, R>(d, M, grid, lds, s, order);
         default: return hipErrorInvalidValue;
     }
 }
 
 template <typename T, int DIR, typename R>
-hipError_t fold_launch_tr(const LPass* d, Members M, dim3 grid, int QF, int nt, int kind, size_t lds, hipStream_t s) {
+hipError_t fold_launch_tr(const LPass* d, Members M, dim3 grid, int QF, int nt, int kind, size_t lds, hipStream_t s,
+                          int order) {
     if (nt == 2) {
-        if (kind == 0) return fold_launch_q<T, DIR, 0, 2, R>(d, M, grid, QF, lds, s);
-        if (kind == 1) return fold_launch_q<T, DIR, 1, 2, R>(d, M, grid, QF, lds, s);
-        return fold_launch_q<T, DIR, 2, 2, R>(d, M, grid, QF, lds, s);
+        if (kind == 0) return fold_launch_q<T, DIR, 0, 2, R>(d, M, grid, QF, lds, s, order);
+        if (kind == 1) return fold_launch_q<T, DIR, 1, 2, R>(d, M, grid, QF, lds, s, order);
+        return fold_launch_q<T, DIR, 2, 2, R>(d, M, grid, QF, lds, s, order);
     }
-    if (kind == 0) return fold_launch_q<T, DIR, 0, 1, R>(d, M, grid, QF, lds, s);
-    if (kind == 1) return fold_launch_q<T, DIR, 1, 1, R>(d, M, grid, QF, lds, s);
-    return fold_launch_q<T, DIR, 2, 1, R>(d, M, grid, QF, lds, s);
+    if (kind == 0) return fold_launch_q<T, DIR, 0, 1, R>(d, M, grid, QF, lds, s, order);
+    if (kind == 1) return fold_launch_q<T, DIR, 1, 1, R>(d, M, grid, QF, lds, s, order);
+    return fold_launch_q<T, DIR, 2, 1, R>(d, M, grid, QF, lds, s, order);
 }
 
 // Every folded instance of one (precision, direction) family; explicitly instantiated by the
 // hqt_fold_*.hip translation units.  wide: prefetch registers in double (a double source in an
-// FP32 pass); FP64 passes always use double.
+// FP32 pass); FP64 passes always use double.  order: the block order (k_fold_pass's xcd_order).
 template <typename T, int DIR>
 hipError_t fold_launch_td(const LPass* d, Members M, dim3 grid, int QF, int nt, int kind, size_t lds, hipStream_t s,
-                          int wide) {
+                          int wide, int order) {
     if constexpr (sizeof(T) == 8) {
         (void)wide;
-        return fold_launch_tr<T, DIR, double>(d, M, grid, QF, nt, kind, lds, s);
+        return fold_launch_tr<T, DIR, double>(d, M, grid, QF, nt, kind, lds, s, order);
     } else {
-        if (wide) return fold_launch_tr<T, DIR, double>(d, M, grid, QF, nt, kind, lds, s);
-        return fold_launch_tr<T, DIR, float>(d, M, grid, QF, nt, kind, lds, s);
+        if (wide) return fold_launch_tr<T, DIR, double>(d, M, grid, QF, nt, kind, lds, s, order);
+        return fold_launch_tr<T, DIR, float>(d, M, grid, QF, nt, kind, lds, s, order);
     }
 }
 

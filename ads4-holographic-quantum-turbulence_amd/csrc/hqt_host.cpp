@@ -276,7 +276,7 @@ int line(hqt_ctx* c, int pass, int dir, int name) {
     const LPass& H = c->hp[pass];
     return run(c, name, [&] {
         return launch_line_pass(c->dp + pass, kOne, line_blocks(H), H.ngroups, line_pass_lds_bytes(H, c->fp32), H.Mp / 64,
-                                H.lb / 16, dir, c->fp32, line_kind(name), c->stream, H.fold);
+                                H.lb / 16, dir, c->fp32, line_kind(name), c->stream, H.fold, H.xcd);
     });
 }
 
@@ -301,12 +301,13 @@ int pass_dir(int p) { return ((p >= PY1 && p <= PY4) || p == PYC || p == PYF || 
 
 constexpr size_t kLdsCap = 160 * 1024;
 
-// Line tiles of every line pass: lines per tile (16 or 32) and tiles per work-group (1 or 4),
-// forced by the params flags (then 1 tile per work-group), else measured once per (pass,
-// precision, grid) and kept for the process.  Every choice gives bitwise identical results (each
-// output's MFMA accumulation order is the same), so tuning never changes numbers.
+// Line tiles of every line pass: lines per tile (16 or 32), tiles per work-group (1 or 4) and, for
+// the folded kernels, the block order (launch order or XCD-aware), forced by the params flags (then
+// 1 tile per work-group, launch order), else measured once per (pass, precision, grid, form) and
+// kept for the process.  Every choice gives bitwise identical results (each output's MFMA
+// accumulation order is the same), so tuning never changes numbers.
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int, int, int, int>, int> g_tune;   // -> 100 * lb + tpb
+std::map<std::tuple<int, int, int, int, int, int>, int> g_tune;   // -> 10000 * order + 100 * lb + tpb
 
 int tune_lines(hqt_ctx* c) {
     const int forced = (c->p.flags & HQT_LINES32) ? 32 : (c->p.flags & HQT_LINES16) ? 16 : 0;
@@ -328,7 +329,9 @@ int tune_lines(hqt_ctx* c) {
                 break;
             }
             float best = 0;
-            for (int lb : {16, 32}) {
+            for (int order : {0, 1}) {
+              if (order && !H.fold) continue;                      // the dense kernels keep the launch order
+              for (int lb : {16, 32}) {
                 for (int tpb : {1, 4}) {
                     if (dir == 1 && tpb != 1 && !H.fold) continue;   // dense y-lines: one tile per work-group
                     set_lb(H, lb, dir, tpb);
@@ -340,22 +343,26 @@ int tune_lines(hqt_ctx* c) {
                         if (rep == 1) e = hipEventRecord(ea, c->stream);
                         if (e == hipSuccess)
                             e = launch_line_pass(c->dp + p, kOne, line_blocks(H), H.ngroups, lds, H.Mp / 64, lb / 16, dir,
-                                                 c->fp32, pass_kind(p), c->stream, H.fold);
+                                                 c->fp32, pass_kind(p), c->stream, H.fold, order);
                     }
                     if (e == hipSuccess) e = hipEventRecord(eb, c->stream);
                     if (e == hipSuccess) e = hipEventSynchronize(eb);
                     if (e == hipSuccess) e = hipEventElapsedTime(&ms, ea, eb);
                     if (e != hipSuccess) { rc = fail(HQT_ERR_HIP, std::string("line-pass tuning: ") + hipGetErrorString(e)); break; }
-                    if (!choice || ms < best) { best = ms; choice = 100 * lb + tpb; }
+                    if (!choice || ms < best) { best = ms; choice = 10000 * order + 100 * lb + tpb; }
                 }
                 if (rc) break;
+              }
+              if (rc) break;
             }
             if (rc) break;
             std::lock_guard<std::mutex> lk(g_tune_mu);
             g_tune[key] = choice;
         }
-        // the dense y-line kernel processes one tile per work-group
-        set_lb(H, choice / 100, dir, (dir == 1 && !H.fold) ? 1 : choice % 100);
+        // choice = 10000 order + 100 lines + tiles per work-group; the dense y-line kernel processes one
+        // tile per work-group and keeps the launch order
+        set_lb(H, (choice % 10000) / 100, dir, (dir == 1 && !H.fold) ? 1 : choice % 100);
+        H.xcd = H.fold ? choice / 10000 : 0;
         if (line_pass_lds_bytes(H, c->fp32) > kLdsCap) set_lb(H, 16, dir, 1);
     }
     // occupancy target of the radial stage kernel: fixed, so that results never depend on a timing.

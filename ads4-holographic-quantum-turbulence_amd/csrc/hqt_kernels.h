@@ -20,6 +20,33 @@ struct Members {
 };
 struct MemberBlock { int m, bx, nb; };   // member, block within the member, member's block count
 
+// XCD-aware block order.  The dispatcher hands linear block i of a launch to XCD i mod 8 (8 XCDs, each
+// with its own L2), so consecutive block indices land on different L2s.  xcd_block(i, n) is a
+// permutation of [0, n) that gives the blocks one XCD receives consecutive logical indices: tiles
+// next to each other in memory (which share cache lines at their edges) then run on one L2.
+__device__ __forceinline__ int xcd_block(int i, int n) {
+    const int q = n >> 3, rem = n & 7, x = i & 7, local = i >> 3;
+    return x * q + (x < rem ? x : rem) + local;
+}
+
+// member_block for an explicit (logical) block index bx of a grid of nb blocks along x.
+__device__ __forceinline__ MemberBlock member_block_at(const Members M, int bx, int nb) {
+    MemberBlock r{0, bx, nb};
+    if (M.nmem > 1) {
+        int lo = 0, hi = M.nmem - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (M.boff[mid] <= r.bx) lo = mid;
+            else hi = mid - 1;
+        }
+        lo = __builtin_amdgcn_readfirstlane(lo);
+        r.m = lo;
+        r.bx -= M.boff[lo];
+        r.nb = M.boff[lo + 1] - M.boff[lo];
+    }
+    return r;
+}
+
 __device__ __forceinline__ MemberBlock member_block(const Members M) {
     MemberBlock r{0, (int)blockIdx.x, (int)gridDim.x};
     if (M.nmem > 1) {
@@ -77,6 +104,7 @@ struct LPass {
     int32_t fold;              // even/odd folded form (hqt_fold_core.h; Mp = 64 QF, S its LDS row stride):
                                // 1, or 2 when a source is double in an FP32 pass (wide prefetch registers)
     int32_t Kp, Z;             // folded form: padded folded positions, table offset
+    int32_t xcd;               // folded form: block order of the launch (1: XCD-aware), set by the autotuner
     int32_t* flags;            // [0] = NaN step (0: none), [1] = step counter
 };
 
@@ -174,8 +202,9 @@ struct Census {
 // must cover every member; Q = Mp/64 is shared by the members of one launch.
 // kind: 0 RHS stage, 1 constraint, 2 filter (separate instantiations for profiling).
 // fold: the even/odd folded kernels (descriptors built with fold = 1 or 2; Q is then QF).
+// order (folded kernels only): 0 launch order, 1 XCD-aware block order; results are the same.
 hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, size_t lds, int Q, int nt, int dir,
-                            int fp32, int kind, hipStream_t s, int fold = 0);
+                            int fp32, int kind, hipStream_t s, int fold = 0, int order = 0);
 size_t line_pass_lds_bytes(const LPass& P, int fp32);
 // Radial kernels: grid.x per member = zblocks(nxy); nz is shared by the members of one launch.
 inline int zblocks(int64_t nxy) { return (int)((nxy + 63) / 64); }

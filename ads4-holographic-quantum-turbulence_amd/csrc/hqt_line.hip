@@ -8,10 +8,10 @@ extern template hipError_t line_launch_td<double, 0>(const LPass*, Members, dim3
 extern template hipError_t line_launch_td<double, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t);
 extern template hipError_t line_launch_td<float, 0>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t);
 extern template hipError_t line_launch_td<float, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t);
-extern template hipError_t fold_launch_td<double, 0>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int);
-extern template hipError_t fold_launch_td<double, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int);
-extern template hipError_t fold_launch_td<float, 0>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int);
-extern template hipError_t fold_launch_td<float, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int);
+extern template hipError_t fold_launch_td<double, 0>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int, int);
+extern template hipError_t fold_launch_td<double, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int, int);
+extern template hipError_t fold_launch_td<float, 0>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int, int);
+extern template hipError_t fold_launch_td<float, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int, int);
 
 size_t line_pass_lds_bytes(const LPass& P, int fp32) {
     int maxslots = 0;
@@ -23,16 +23,16 @@ size_t line_pass_lds_bytes(const LPass& P, int fp32) {
 }
 
 hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, size_t lds, int Q, int nt, int dir,
-                            int fp32, int kind, hipStream_t s, int fold) {
+                            int fp32, int kind, hipStream_t s, int fold, int order) {
     if (Q < 1 || Q > 8 || blocks <= 0 || (nt != 1 && nt != 2) || lds > 160 * 1024) return hipErrorInvalidValue;
     const dim3 grid((unsigned)blocks, (unsigned)ngroups);
     if (fold) {
         if (Q > 5) return hipErrorInvalidValue;
         const int wide = fold == 2;
-        if (fp32) return dir ? fold_launch_td<float, 1>(d, M, grid, Q, nt, kind, lds, s, wide)
-                             : fold_launch_td<float, 0>(d, M, grid, Q, nt, kind, lds, s, wide);
-        return dir ? fold_launch_td<double, 1>(d, M, grid, Q, nt, kind, lds, s, wide)
-                   : fold_launch_td<double, 0>(d, M, grid, Q, nt, kind, lds, s, wide);
+        if (fp32) return dir ? fold_launch_td<float, 1>(d, M, grid, Q, nt, kind, lds, s, wide, order)
+                             : fold_launch_td<float, 0>(d, M, grid, Q, nt, kind, lds, s, wide, order);
+        return dir ? fold_launch_td<double, 1>(d, M, grid, Q, nt, kind, lds, s, wide, order)
+                   : fold_launch_td<double, 0>(d, M, grid, Q, nt, kind, lds, s, wide, order);
     }
     if (fp32) return dir ? line_launch_td<float, 1>(d, M, grid, Q, nt, kind, lds, s)
                          : line_launch_td<float, 0>(d, M, grid, Q, nt, kind, lds, s);

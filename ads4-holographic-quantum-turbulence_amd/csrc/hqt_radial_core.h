@@ -366,7 +366,8 @@ template <int NZ, typename TS, int WPE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_zstage(const ZStage* __restrict__ Ap,
                                                                                             const Members M) {
     constexpr int KS = ZG<NZ>::KS;
-    const MemberBlock mb = member_block(M);
+    // XCD-aware block order: neighbouring column blocks share cache lines at their edges
+    const MemberBlock mb = member_block_at(M, xcd_block((int)blockIdx.x, (int)gridDim.x), (int)gridDim.x);
     const ZStage& A = Ap[mb.m];
     const int stop = __builtin_amdgcn_readfirstlane(__builtin_nontemporal_load(&A.flags[0]));
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -407,7 +408,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
 // ---- post-step: A_t constraint solve and boundary reset (:129-133), O for the winding (:136)
 template <int NZ, typename TS>
 __global__ __launch_bounds__(256) void k_zpost(const ZPost* __restrict__ Ap, const Members M) {
-    const MemberBlock mb = member_block(M);
+    const MemberBlock mb = member_block_at(M, xcd_block((int)blockIdx.x, (int)gridDim.x), (int)gridDim.x);
     const ZPost& A = Ap[mb.m];
     if (stopped(A.flags)) return;
     constexpr int KS = ZG<NZ>::KS;

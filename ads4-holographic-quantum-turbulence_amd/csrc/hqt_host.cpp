@@ -316,7 +316,7 @@ int tune_lines(hqt_ctx* c) {
     for (int p = 0; p < NPASS && !rc; ++p) {
         LPass& H = c->hp[p];
         const int dir = pass_dir(p);
-        int choice = forced ? 100 * forced + 1 : 0;
+        int choice = forced ? 100 * forced + 1 + ((c->p.flags & HQT_ORDER_XCD) ? 10000 : 0) : 0;
         const auto key = std::make_tuple(p, c->fp32, c->nx, c->ny, c->nz, (int)H.fold);
         if (!choice) {
             std::lock_guard<std::mutex> lk(g_tune_mu);

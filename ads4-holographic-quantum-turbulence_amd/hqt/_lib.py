@@ -20,6 +20,7 @@ HQT_RADIAL_W2 = 8
 HQT_RADIAL_W3 = 16
 HQT_DENSE_FILTER = 32
 HQT_DENSE_LINES = 64
+HQT_ORDER_XCD = 128
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "csrc", "libhqt.so")

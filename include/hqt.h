@@ -44,6 +44,8 @@ extern "C" {
 #define HQT_RADIAL_W3 16    /* radial stage kernel compiled for 3 waves per SIMD (spills from nz = 13) */
 #define HQT_DENSE_FILTER 32 /* filter's imaginary part as a dense Toeplitz product instead of its rank-4 form
                                (the default; ff.m's 2/3 mask leaves at most two unpaired modes) */
+#define HQT_ORDER_XCD  128  /* with HQT_LINES16/32: folded line passes in the XCD-aware block order (default with
+                               a forced line count: launch order; results are the same bitwise either way) */
 #define HQT_DENSE_LINES 64  /* x/y operators and filter as dense Toeplitz products (func_Lx's own form)
                                instead of their even/odd folded form (the default: half the
                                multiply-adds; results agree to rounding) */

@@ -236,23 +236,27 @@ def test_graph_replays_match_eager():
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_lines_per_workgroup_do_not_change_results(precision):
-    """16 and 32 lines per work-group (and the tuned choice) accumulate every output in the
-    same MFMA order: states after 5 steps are bitwise equal."""
+    """16 and 32 lines per work-group, the launch and the XCD-aware block order (and the tuned
+    choice) accumulate every output in the same MFMA order: states after 5 steps are bitwise
+    equal, for the folded and for the dense line form."""
     import hqt
     from hqt import ic
-    out = []
     for n, nz in ((301, 20), (77, 9)):
         state = ic.initial_state(n, n, nz, 50.0, 50.0, mu=6.0, seed=4, n_vortices=4)
         axb = ic.sinusoidal_drive(n, n, 50.0, 7, 0.1)
-        res = []
-        for lines in (16, 32, None):
-            with hqt.Stepper(n, n, nz, 50.0, 50.0, mu=6.0, dt=0.02, precision=precision, ax_b=axb, lines=lines) as st:
-                st.set_state(*state)
-                st.step(5)
-                res.append(st.get_state())
-        for other in res[1:]:
-            for a, b in zip(res[0], other):
-                assert np.array_equal(a, b)
+        for dense, variants in ((False, ((16, False), (32, False), (16, True), (32, True), (None, False))),
+                                (True, ((16, False), (32, False), (None, False)))):
+            print("tiling variants", precision, n, nz, "dense" if dense else "folded", flush=True)
+            res = []
+            for lines, xcd in variants:
+                with hqt.Stepper(n, n, nz, 50.0, 50.0, mu=6.0, dt=0.02, precision=precision, ax_b=axb, lines=lines,
+                                 xcd_order=xcd, dense_lines=dense) as st:
+                    st.set_state(*state)
+                    st.step(5)
+                    res.append(st.get_state())
+            for other in res[1:]:
+                for a, b in zip(res[0], other):
+                    assert np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

@@ -301,7 +301,7 @@ int pass_dir(int p) { return ((p >= PY1 && p <= PY4) || p == PYC || p == PYF || 
 
 constexpr size_t kLdsCap = 160 * 1024;
 
-// Line tiles of every line pass: lines per tile (16 or 32), tiles per work-group (1 or 4) and, for
+// Line tiles of every line pass: lines per tile (16 or 32), tiles per work-group (1, 2 or 4) and, for
 // the folded kernels, the block order (launch order or XCD-aware), forced by the params flags (then
 // 1 tile per work-group, launch order), else measured once per (pass, precision, grid, form) and
 // kept for the process.  Every choice gives bitwise identical results (each output's MFMA
@@ -332,7 +332,7 @@ int tune_lines(hqt_ctx* c) {
             for (int order : {0, 1}) {
               if (order && !H.fold) continue;                      // the dense kernels keep the launch order
               for (int lb : {16, 32}) {
-                for (int tpb : {1, 4}) {
+                for (int tpb : {1, 2, 4}) {
                     if (dir == 1 && tpb != 1 && !H.fold) continue;   // dense y-lines: one tile per work-group
                     set_lb(H, lb, dir, tpb);
                     const size_t lds = line_pass_lds_bytes(H, c->fp32);

@@ -205,6 +205,56 @@ def ensemble_leg(members, steps, seed=4):
     return out
 
 
+def mixed_accuracy_leg(checkpoints=(500, 5000), seed=3):
+    """The mixed-precision contract on the vortex-lattice workload: nx = ny = 64, nz = 12, L = 50,
+    mu = 6, dt = 0.02, 64 +-1 windings on an 8 x 8 jittered square lattice (jitter 0.5), winding
+    removal off (tevol_lattice.m:168-170).  The FP64 and mixed forms step side by side; at each
+    checkpoint (t = 10 and t = 100) the per-field relative max-norm error of the mixed state against
+    the FP64 one, the condensate |O| error, and the vortex counts and plaquette positions.  Beside
+    them, ``fp64_perturbed``: the same FP64 evolution from an initial Psi perturbed by 1e-7 relative
+    (seeded, the size of an FP32 rounding), which measures how far the dynamics carry a perturbation
+    of that size by each checkpoint."""
+    import hqt
+    from hqt import diagnostics, ic
+    nx = ny = 64
+    nz, L = 12, 50.0
+    S = ic.initial_state(nx, ny, nz, L, L, mu=6.0, seed=seed, lattice=(8, 0.5))
+    g = ic.Grid(nx, ny, nz, L, L)
+    out = {"workload": "64x64x12 L=50 mu=6 dt=0.02, 8x8 lattice of +-1 windings (jitter 0.5), winding removal off",
+           "checkpoints": {}}
+    sts = {p: hqt.Stepper(nx, ny, nz, L, L, mu=6.0, dt=0.02, precision=p, winding=False) for p in ("fp64", "mixed")}
+    sts["fp64_perturbed"] = hqt.Stepper(nx, ny, nz, L, L, mu=6.0, dt=0.02, precision="fp64", winding=False)
+    Sp = list(S)
+    Sp[0] = S[0] * (1 + 1e-7 * np.random.default_rng(seed + 1).standard_normal(S[0].shape))
+    try:
+        for p, st in sts.items():
+            st.set_state(*(Sp if p == "fp64_perturbed" else S))
+        done = 0
+        for n in checkpoints:
+            for st in sts.values():
+                st.step_async(n - done)
+            states = {}
+            for p, st in sts.items():
+                st.sync()
+                states[p] = st.get_state()
+            done = n
+            O64 = diagnostics.order_parameter(g, states["fp64"][0])
+            Om = diagnostics.order_parameter(g, states["mixed"][0])
+            Op = diagnostics.order_parameter(g, states["fp64_perturbed"][0])
+            c64, cm = diagnostics.countN(O64), diagnostics.countN(Om)
+            out["checkpoints"]["t=%g" % (n * 0.02)] = {
+                "steps": n, "err_vs_fp64": field_errors(states["mixed"], states["fp64"]),
+                "O_abs_rel_err": float(np.max(np.abs(np.abs(Om) - np.abs(O64))) / np.max(np.abs(O64))),
+                "vortices_fp64": c64, "vortices_mixed": cm,
+                "positions_equal": diagnostics.vortex_positions(Om) == diagnostics.vortex_positions(O64),
+                "fp64_perturbed": {"err_vs_fp64": field_errors(states["fp64_perturbed"], states["fp64"]),
+                                   "O_abs_rel_err": float(np.max(np.abs(np.abs(Op) - np.abs(O64))) / np.max(np.abs(O64)))}}
+    finally:
+        for st in sts.values():
+            st.close()
+    return out
+
+
 def field_errors(a, b):
     names = ("Psi", "A_t", "A_x", "A_y", "rho")
     out = {}
@@ -223,6 +273,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the mixed-precision accuracy leg (64^2 vortex lattice to t = 100)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving (Poisson arrivals) leg")
     ap.add_argument("--ensemble-members", type=int, default=48)
     ap.add_argument("--ensemble-steps", type=int, default=20)
@@ -325,6 +377,9 @@ def main():
             mixed["O_abs_rel_err"] = float(np.max(np.abs(np.abs(Om) - np.abs(O64))) / np.max(np.abs(O64)))
             mixed["vortices"] = diagnostics.countN(Om)
         out["mixed"] = mixed
+
+    if not args.no_accuracy and not args.no_mixed:
+        out["mixed_accuracy"] = mixed_accuracy_leg()
 
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(args.ensemble_members, args.ensemble_steps)

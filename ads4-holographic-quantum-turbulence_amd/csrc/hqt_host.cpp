@@ -166,8 +166,9 @@ int op_parity(int op) {
     }
 }
 
-// Row sum of an operator (its value on a constant line): 1 for the filter's real part (mode 0 is
-// kept), 0 for the derivatives and the filter's imaginary part.
+// Row sum of an operator (its value on a constant line) as far as it follows from the operator
+// alone: 0 for the derivatives and the filter's imaginary part.  The filter's real part has row
+// sum 1 - mask[0] (mode 0 is masked for N <= 4); build_context sets it from the masks.
 int op_rowsum(int op) { return (op == OP_PRX || op == OP_QRY) ? 1 : 0; }
 
 // A term with lr = 1 is a rank-4 operator: its `slot` is the ops index of the [V | U] table
@@ -570,10 +571,19 @@ int build_context(hqt_ctx* c) {
         }
     }
     // the folded kernel keeps one output's accumulators: a group has one source or one output;
-    // an FP32 pass with a double source (A_t) stages through double registers
+    // an FP32 pass with a double source (A_t) stages through double registers; the filter's real
+    // part has row sum 1 - mask[0] (ff.m:7-11: the mean is masked when round(round(N/3)) = 1)
     for (int p = 0; p < NPASS; ++p)
         for (int g = 0; g < c->hp[p].ngroups; ++g) {
             LPass& H = c->hp[p];
+            for (int o = 0; o < H.g[g].nout; ++o)
+                for (int e = 0; e < H.g[g].out[o].nterms; ++e) {
+                    LTerm& tm = H.g[g].out[o].term[e];
+                    if (tm.lr) continue;
+                    const int op = H.g[g].slot_op[tm.slot];
+                    if (op == OP_PRX) tm.rsum = fx[0] ? 0 : 1;
+                    if (op == OP_QRY) tm.rsum = fy[0] ? 0 : 1;
+                }
             if (H.fold && H.g[g].ntiles > 1 && H.g[g].nout > 1)
                 return fail(HQT_ERR_ARG, "internal: folded line pass group with two sources and two outputs");
             for (int t = 0; t < H.g[g].ntiles; ++t)

@@ -289,10 +289,12 @@ def test_folded_lines_match_dense_lines():
     """The x/y operators and the filter in their even/odd folded form (the default) and as dense
     Toeplitz products (the reference's func_Lx form) are the same operators: in FP64, F1..F4 on a
     generic state and the state after 5 steps agree to 1e-12 relative, on odd and even line
-    lengths, non-square grids and line lengths below one MFMA tile."""
+    lengths, non-square grids and line lengths below one MFMA tile (down to 2 and 3 points, where
+    the half line is the Nyquist row or a single mirrored row)."""
     import hqt
     from hqt import ic
-    for nx, ny, nz in ((301, 301, 20), (64, 48, 8), (33, 17, 6), (12, 10, 5), (96, 77, 7), (5, 6, 4)):
+    for nx, ny, nz in ((301, 301, 20), (64, 48, 8), (33, 17, 6), (12, 10, 5), (96, 77, 7), (5, 6, 4), (2, 3, 5),
+                       (3, 2, 4)):
         print("folded vs dense lines", nx, ny, nz, flush=True)
         state = ic.initial_state(nx, ny, nz, 50.0, 50.0, mu=6.0, seed=8, n_vortices=2)
         axb = ic.sinusoidal_drive(nx, ny, 50.0, 3, 0.1)
@@ -312,7 +314,13 @@ def test_folded_lines_match_dense_lines():
         for a, b in zip(*rhs):
             assert rel_err(a, b) < 1e-12
         for a, b in zip(*res):
-            assert rel_err(a, b) < 1e-12
+            # on 2- and 3-point lines the reference's mask also removes the mean (round(round(N/3)) = 1),
+            # and the filter leaves Psi at rounding level (fields are O(1)): such a field is compared absolutely
+            # (2- and 3-point grids also leave some fields near 1e-6: 1e-15 absolute on O(1)-scaled fields)
+            if np.max(np.abs(b)) < 1e-12:
+                assert np.max(np.abs(a - b)) < 1e-14
+            else:
+                assert rel_err(a, b) < 1e-12 or np.max(np.abs(a - b)) < 1e-15
 
 
 def test_folded_lines_mixed_accuracy():

@@ -102,6 +102,74 @@ __device__ __forceinline__ void fold_lr_dots(const T* __restrict__ F, const T* _
         dl[idx] = ((DLp[idx] + DLp[LB * 4 + idx]) + DLp[2 * LB * 4 + idx]) + DLp[3 * LB * 4 + idx];
 }
 
+// gmode 1 (x-lines): partials of G[r][s] = sum over the lines of a 16-line block in one z-slice of
+// D[line][r] * V_y[y(line)][s], for the (at most two, ny >= 16) slices the block touches; written as
+// doubles to gp[(block * 2 + slice offset) * 16 + 4 r + s].  Thread (4 r + s, line) forms one product,
+// and each 16-lane group sums its lines in a fixed xor-butterfly order.
+template <typename T, int NT>
+__device__ __forceinline__ void fold_g_partials(const LPass& P, const LOut& O, const T* dl, int line0, int nl) {
+    const T* vy = static_cast<const T*>(O.gtab);
+    double* gp = const_cast<double*>(O.gp);
+    const int tid = threadIdx.x, rs = tid >> 4, l = tid & 15, r = rs >> 2, s = rs & 3;
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+        const int blk = (line0 >> 4) + u;
+        if (blk * 16 >= nl) break;
+        const int z0 = (blk * 16) / P.ny, line = blk * 16 + l;
+        double p0 = 0.0, p1 = 0.0;
+        if (line < nl) {
+            const int z = line / P.ny, y = line - z * P.ny;
+            const double v = (double)dl[(16 * u + l) * 4 + r] * (double)vy[4 * y + s];
+            if (z == z0) p0 = v;
+            else p1 = v;
+        }
+#pragma unroll
+        for (int off = 8; off >= 1; off >>= 1) {
+            p0 += __shfl_xor(p0, off);
+            p1 += __shfl_xor(p1, off);
+        }
+        if (l == 0) {
+            gp[(blk * 2) * 16 + rs] = p0;
+            gp[(blk * 2 + 1) * 16 + rs] = p1;
+        }
+    }
+}
+
+// gmode 2 (y-lines of slice z, x = x0 + l): G of the slice from the partials of the 16-line blocks
+// that touch it, then dl[l][s] = sum_r U_x[x][r] G[r][s].  The partial sum is in a fixed order:
+// thread (block slot b0, 4 r + s) adds blocks b0, b0 + 16, b0 + 32 of the slice, the 4 slots of a
+// wave are combined by an xor butterfly and the 4 waves in wave order.  GL: 80 doubles of LDS.
+// Ends with a barrier.
+template <typename T, int NT>
+__device__ __forceinline__ void fold_lr_from_g(const LPass& P, const LOut& O, T* dl, double* GL, int x0, int z) {
+    constexpr int LB = 16 * NT;
+    const int tid = threadIdx.x, rs = tid & 15, b0 = tid >> 4;
+    const int blo = (z * P.ny) >> 4, bhi = ((z + 1) * P.ny - 1) >> 4;      // at most 33 blocks (ny <= 512)
+    double q = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int blk = blo + b0 + 16 * k;
+        if (blk <= bhi) q += O.gp[(blk * 2 + (z - (blk * 16) / P.ny)) * 16 + rs];
+    }
+    q += __shfl_xor(q, 16);
+    q += __shfl_xor(q, 32);
+    if ((tid & 63) < 16) GL[16 + (tid >> 6) * 16 + rs] = q;
+    __syncthreads();
+    if (tid < 16) GL[tid] = ((GL[16 + tid] + GL[32 + tid]) + GL[48 + tid]) + GL[64 + tid];
+    __syncthreads();
+    const T* ux = static_cast<const T*>(O.gtab);
+    for (int idx = tid; idx < LB * 4; idx += LP_THREADS) {
+        const int l = idx >> 2, s = idx & 3, x = x0 + l;
+        double d = 0.0;
+        if (x < P.nx) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) d += (double)ux[4 * x + r] * GL[4 * r + s];
+        }
+        dl[idx] = (T)d;
+    }
+    __syncthreads();
+}
+
 // Output of a finished line tile: y_i = E + O (+ addend) at i, then y_{N-i} = E - O (+ addend) at
 // N - i, where E includes the line's constant term cst (row sums times reference values).  Each
 // half loads all of its addends before its first store.  Returns whether a stored value is NaN;
@@ -223,6 +291,9 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
     T* DL = C + G.nslots * TL;               // rank-4 dots: 2 outputs x LB lines x 4
     T* DLp = DL + 2 * LB * 4;                // rank-4 partial dots: 4 waves x LB lines x 4
     T* X0 = DLp + 4 * LB * 4;                // reference value x_0 of every line of each source: 2 x LB
+    // the 4 x 4 cross-term matrix of the filter and its per-wave partials (gmode 2), 8-byte aligned after X0
+    double* GL = reinterpret_cast<double*>(
+        smem_raw + ((reinterpret_cast<unsigned char*>(X0 + 2 * LB) - smem_raw + 7) & ~(ptrdiff_t)7));
     const int tid = threadIdx.x;
     const int nl = P.ny * P.nz;
     const int tile0 = mb.bx * P.tpb;
@@ -348,9 +419,13 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
                     for (int e = 0; e < G.out[o].nterms; ++e) {
                         const LTerm& tm = G.out[o].term[e];
                         if (tm.lr && tm.tile == t) {
-                            fold_lr_dots<T, NT>(Vf, Vg, S, Kp, static_cast<const T*>(P.ops[tm.slot]), DLp, DL + o * LB * 4,
-                                                wave, lane);
-                            __syncthreads();
+                            if (DIR == 1 && G.out[o].gmode == 2) {
+                                fold_lr_from_g<T, NT>(P, G.out[o], DL + o * LB * 4, GL, x0, z);
+                            } else {
+                                fold_lr_dots<T, NT>(Vf, Vg, S, Kp, static_cast<const T*>(P.ops[tm.slot]), DLp,
+                                                    DL + o * LB * 4, wave, lane);
+                                __syncthreads();
+                            }
                         }
                     }
             }
@@ -422,6 +497,13 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
             }
             if (t != G.ntiles - 1) continue;
             // all sources of output o are in: rank-4 expansion, constant term, store
+            if constexpr (KIND == 2 && DIR == 0) {
+                if (O.gmode == 1) {
+                    // the x-pass's rank-4 output feeds the cross term: partials of G instead of a field
+                    fold_g_partials<T, NT>(P, O, DL + o * LB * 4, line0, nl);
+                    continue;
+                }
+            }
             if constexpr (KIND == 2) {
                 if (G.has_lr) {
                     // E += UE D, O += UO D for the rank-4 term (one MFMA k-step per tile: the ranks are K)

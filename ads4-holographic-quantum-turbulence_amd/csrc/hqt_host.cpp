@@ -34,6 +34,12 @@ namespace {
 inline int fail(int code, const std::string& msg) { return hqt_fail(code, msg); }
 }  // namespace
 
+// The folded filter takes the cross term through 4 x 4 slice matrices when both directions have the
+// rank-4 imaginary part and a 16-line block of x-lines touches at most two z-slices (ny >= 16).
+bool gfilt_on(const hqt_params& p, bool lrx, bool lry) {
+    return !(p.flags & HQT_DENSE_LINES) && lrx && lry && p.ny >= 16;
+}
+
 int64_t ctx_footprint(const hqt_params& p) {
     const int64_t nxy = (int64_t)p.nx * p.ny, np = nxy * p.nz, nz = p.nz;
     const int64_t es = p.precision == HQT_MIXED ? 4 : 8;
@@ -49,16 +55,20 @@ int64_t ctx_footprint(const hqt_params& p) {
     b += 6 * ((int64_t)line_geom(p.nx).TL + (int64_t)line_geom(p.ny).TL) * es;   // x/y operator tables
     const bool fold = !(p.flags & HQT_DENSE_LINES);
     if (fold) b += 6 * ((int64_t)fold_geom(p.nx).TL + (int64_t)fold_geom(p.ny).TL) * es;   // folded tables
-    for (int n : {p.nx, p.ny}) {                          // rank-4 filter tables, when used
+    bool lr[2] = {false, false};
+    for (int d = 0; d < 2; ++d) {                         // rank-4 filter tables, when used
         if (p.flags & HQT_DENSE_FILTER) break;
+        const int n = d ? p.ny : p.nx;
         std::vector<uint8_t> mk;
         matlab_masks(n, mk);
         std::vector<double> t;
         if (filter_lowrank(n, mk.data(), line_geom(n), 1.0, t)) {
+            lr[d] = true;
             b += 2 * (int64_t)t.size() * es;
             if (fold) b += 2 * (8 * (int64_t)fold_geom(n).Kp + 8 * (int64_t)fold_geom(n).Mp) * es;
         }
     }
+    if (gfilt_on(p, lr[0], lr[1])) b += 5 * (((int64_t)p.ny * p.nz + 15) / 16) * 32 * 8;   // cross-term partials
     return b;
 }
 
@@ -500,6 +510,11 @@ int build_context(hqt_ctx* c) {
     matlab_masks(ny, fy);
     if ((rc = build_tables(c, nx, c->p.Lx, fx.data(), OP_D1X))) return rc;
     if ((rc = build_tables(c, ny, c->p.Ly, fy.data(), OP_D1Y))) return rc;
+    c->gfilt = gfilt_on(c->p, c->lr[0], c->lr[1]);
+    if (c->gfilt) {
+        c->gblk = ((int64_t)ny * nz + 15) / 16;
+        ALLOC(c->Gp, 5 * (size_t)c->gblk * 32 * 8);
+    }
 
     // ---- line-pass descriptors ----
     const int T = c->tdt;
@@ -559,10 +574,26 @@ int build_context(hqt_ctx* c) {
         P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo, (int)c->es);
         LPass& Q = c->hp[PYF];
         Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo, (int)c->es);
-        // Im(P) has rank 4 for the reference's masks: applied as U (V^T x) instead of a dense product
+        // Im(P) has rank 4 for the reference's masks: applied as U (V^T x) instead of a dense product.
+        // Folded with both directions rank 4 (and ny >= 16): the cross term (Im P_x (x) Im P_y) x is
+        // U_x G U_y^T per z-slice, so the x-pass writes partials of G (4 x 4 per slice) instead of the
+        // intermediate field Im(P_x) x, and the y-pass has the one source Re(P_x) x.
         for (int i = 0; i < 5; ++i) {
             LGroup& g = add_group(P, f[i], fdt[i]);
             add_out(g, c->fa[i], T, 0, OP_PRX);
+            if (c->gfilt) {
+                double* gp = c->Gp + (size_t)i * 32 * c->gblk;
+                add_out(g, gp, DT_F64, 0, OP_UVX, -1, -1, nullptr, DT_F64, 0, 1);
+                g.out[1].gmode = 1;
+                g.out[1].gp = gp;
+                g.out[1].gtab = c->ops[OP_UVY];                                        // V_y: ny x 4
+                LGroup& h = add_group(Q, c->fa[i], T);
+                add_out(h, f[i], fdt[i], 0, OP_QRY, 0, OP_NUVY, nullptr, DT_F64, 1, 0, 1);
+                h.out[0].gmode = 2;
+                h.out[0].gp = gp;
+                h.out[0].gtab = static_cast<const char*>(c->ops[OP_UVX]) + 4 * (size_t)line_geom(nx).Np * c->es;  // U_x
+                continue;
+            }
             if (c->lr[0]) add_out(g, c->fb[i], T, 0, OP_UVX, -1, -1, nullptr, DT_F64, 0, 1);
             else add_out(g, c->fb[i], T, 0, OP_PIX);
             LGroup& h = add_group(Q, c->fa[i], T, c->fb[i], T);

@@ -78,11 +78,17 @@ constexpr int LP_THREADS = 256;
 // pad: parity of a dense operator's column (+1 even, -1 odd); rsum: the operator's row sum
 // (C 1 = rsum 1: 1 for the filter's real part, 0 for the derivatives).  Both used by the folded form.
 struct LTerm { int32_t tile; int32_t slot; int32_t lr; int32_t pad; int32_t rsum; };
+// gmode (folded filter, the rank-4 x rank-4 cross term of Re((P_x (x) P_y) x) as U_x G U_y^T per z-slice):
+//   1 (x-pass): instead of storing the output, write per-16-line-block partials of G = D_x V_y, D_x the
+//     output's rank-4 dots, to gp ([block][2 slices][4 x 4] doubles; gtab: V_y, ny x 4);
+//   2 (y-pass): the rank-4 term's dots are U_x G (gtab: U_x, nx x 4), G summed from gp's partials.
 struct LOut {
     void* out; const void* addend;
     int32_t out_dt, add_dt;
     int32_t nterms; LTerm term[2];
     int32_t nan_check;
+    int32_t gmode;
+    const double* gp; const void* gtab;
 };
 struct LGroup {
     int32_t ntiles; int32_t has_lr; int32_t src_dt[2]; const void* src[2];

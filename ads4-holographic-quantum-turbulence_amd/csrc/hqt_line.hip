@@ -19,7 +19,9 @@ size_t line_pass_lds_bytes(const LPass& P, int fp32) {
     const size_t es = fp32 ? sizeof(float) : sizeof(double);
     // folded: f and g rows, and each source's per-line reference values
     const size_t lines = P.fold ? 2 * (size_t)P.lb * P.S + 2 * (size_t)P.lb : (size_t)P.lb * P.S;
-    return es * (lines + (size_t)maxslots * P.TL + 6 * 4 * (size_t)P.lb);        // + rank-4 dots
+    // + rank-4 dots; folded: + the 4 x 4 cross-term matrix and its 4 per-wave partials (80 doubles,
+    // 8-byte aligned)
+    return es * (lines + (size_t)maxslots * P.TL + 6 * 4 * (size_t)P.lb) + (P.fold ? 8 + 80 * sizeof(double) : 0);
 }
 
 hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, size_t lds, int Q, int nt, int dir,

@@ -425,6 +425,9 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
                                 fold_lr_dots<T, NT>(Vf, Vg, S, Kp, static_cast<const T*>(P.ops[tm.slot]), DLp,
                                                     DL + o * LB * 4, wave, lane);
                                 __syncthreads();
+                                // the x-pass's rank-4 output feeds the filter's cross term: partials of G
+                                // (formed here, before any accumulator is live) instead of a field
+                                if (DIR == 0 && G.out[o].gmode == 1) fold_g_partials<T, NT>(P, G.out[o], DL + o * LB * 4, line0, nl);
                             }
                         }
                     }
@@ -498,11 +501,7 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
             if (t != G.ntiles - 1) continue;
             // all sources of output o are in: rank-4 expansion, constant term, store
             if constexpr (KIND == 2 && DIR == 0) {
-                if (O.gmode == 1) {
-                    // the x-pass's rank-4 output feeds the cross term: partials of G instead of a field
-                    fold_g_partials<T, NT>(P, O, DL + o * LB * 4, line0, nl);
-                    continue;
-                }
+                if (O.gmode == 1) continue;     // written as cross-term partials with the dots
             }
             if constexpr (KIND == 2) {
                 if (G.has_lr) {

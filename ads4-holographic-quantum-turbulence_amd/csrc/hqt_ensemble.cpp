@@ -3,7 +3,10 @@
 // class (same template instance: precision and Q = Mp/64 for the line passes, nz for the
 // radial kernels) over all members of that class; a member owns a contiguous block range of
 // the launch and reads its own descriptor (buffers, sizes, flags) from a device array.  The
-// whole step is captured once into a hipGraph and replayed per step.
+// whole step is captured once into a hipGraph and replayed per step.  The launches of one step
+// phase (one kernel of the step sequence, one launch per class) touch disjoint members, so they
+// run on up to kLanes streams at once (fork/join through events around each phase), which
+// overlaps one class's tail with the next class's blocks.
 //
 // The per-member arithmetic is that of hqt_step_n on the same context (same kernels, same
 // operands), so results are identical to stepping each member alone.
@@ -22,6 +25,7 @@ enum LKind { K_LINE, K_ZPREP, K_ZSTAGE, K_ZPOST, K_WPART, K_WAPPLY, K_NAN };
 struct Launch {
     int kind = 0;
     int dir = 0, lkind = 0, Q = 1, nt = 1, ngroups = 1, nz = 0, fp32 = 0, wpe = 2, blocks = 0, nmem = 0, fold = 0, order = 0;
+    int phase = 0;                   // index of the step-sequence kernel this launch belongs to
     size_t lds = 0;
     size_t desc_off = 0, boff_off = 0;   // offsets into the device plan buffer
 };
@@ -37,11 +41,15 @@ struct PlanImage {
     }
 };
 
+constexpr int kLanes = 4;            // streams per phase: the ensemble's own + 3 side streams
+
 }  // namespace
 
 struct hqt_ensemble {
     std::vector<hqt_ctx*> mem;       // in the caller's order
     hipStream_t stream = nullptr;
+    hipStream_t side[kLanes - 1] = {};
+    hipEvent_t fork = nullptr, join[kLanes - 1] = {};
     std::vector<Launch> plan;
     unsigned char* dplan = nullptr;
     hipGraphExec_t graph = nullptr;
@@ -83,20 +91,25 @@ Launch make_launch(PlanImage& img, int kind, const std::vector<hqt_ctx*>& ms, De
 
 void add_line(std::vector<Launch>& plan, PlanImage& img, const std::vector<hqt_ctx*>& mem, int pass, int dir, int name,
               bool (*use)(const hqt_ctx*)) {
-    auto cls = classes<std::tuple<int, int, int, int, int>>(
+    auto cls = classes<std::tuple<int, int, int, int>>(
         mem, [&](hqt_ctx* c) {
-            return std::make_tuple(c->fp32, c->hp[pass].Mp / 64, c->hp[pass].lb / 16, c->hp[pass].fold, c->hp[pass].xcd);
+            return std::make_tuple(c->fp32, c->hp[pass].Mp / 64, c->hp[pass].lb / 16, c->hp[pass].fold);
         },
         [&](hqt_ctx* c) { return use(c); });
+    const int phase = plan.empty() ? 0 : plan.back().phase + 1;
     for (auto& kv : cls) {
         Launch L = make_launch<LPass>(img, K_LINE, kv.second, [&](hqt_ctx* c) { return c->hp[pass]; },
                                       [&](hqt_ctx* c) { return line_blocks(c->hp[pass]); });
+        L.phase = phase;
         L.dir = dir;
         L.fp32 = std::get<0>(kv.first);
         L.Q = std::get<1>(kv.first);
         L.nt = std::get<2>(kv.first);
         L.fold = std::get<3>(kv.first);
-        L.order = std::get<4>(kv.first);
+        // block order: the members' tuned choice by majority (ties: XCD order); it never changes a result
+        int votes = 0;
+        for (hqt_ctx* c : kv.second) votes += c->hp[pass].xcd ? 1 : -1;
+        L.order = L.fold && votes >= 0 ? 1 : 0;
         L.lkind = (name == KN_LX_FILT || name == KN_LY_FILT) ? 2 : (name == KN_LX_CONS || name == KN_LY_CONS) ? 1 : 0;
         for (hqt_ctx* c : kv.second) {
             L.ngroups = std::max(L.ngroups, c->hp[pass].ngroups);
@@ -117,8 +130,10 @@ void add_z(std::vector<Launch>& plan, PlanImage& img, const std::vector<hqt_ctx*
     auto cls = classes<std::tuple<int, int, int>>(
         mem, [&](hqt_ctx* c) { return std::make_tuple(c->fp32, by_nz ? c->nz : 0, kind == K_ZSTAGE ? c->zwpe : 2); },
         [&](hqt_ctx* c) { return use(c); });
+    const int phase = plan.empty() ? 0 : plan.back().phase + 1;
     for (auto& kv : cls) {
         Launch L = make_launch<D>(img, kind, kv.second, desc, blocks);
+        L.phase = phase;
         L.fp32 = std::get<0>(kv.first);
         L.nz = std::get<1>(kv.first);
         L.wpe = std::get<2>(kv.first);
@@ -150,10 +165,9 @@ void build_plan(hqt_ensemble* e, PlanImage& img) {
                     [](hqt_ctx* c) { return nancheck_blocks(c->npts); });
 }
 
-hipError_t issue(const hqt_ensemble* e, const Launch& L) {
+hipError_t issue(const hqt_ensemble* e, const Launch& L, hipStream_t s) {
     const void* d = e->dplan + L.desc_off;
     const Members M{reinterpret_cast<const int32_t*>(e->dplan + L.boff_off), L.nmem};
-    hipStream_t s = e->stream;
     switch (L.kind) {
         case K_LINE:
             return launch_line_pass(static_cast<const LPass*>(d), M, L.blocks, L.ngroups, L.lds, L.Q, L.nt, L.dir, L.fp32,
@@ -169,9 +183,26 @@ hipError_t issue(const hqt_ensemble* e, const Launch& L) {
 }
 
 int enqueue(const hqt_ensemble* e) {
-    for (const Launch& L : e->plan) {
-        const hipError_t r = issue(e, L);
+    const auto& P = e->plan;
+    for (size_t i = 0; i < P.size();) {
+        size_t j = i;
+        while (j < P.size() && P[j].phase == P[i].phase) ++j;
+        const int n = (int)(j - i), lanes = std::min(n, kLanes);
+        hipError_t r = hipSuccess;
+        if (lanes > 1) {
+            r = hipEventRecord(e->fork, e->stream);
+            for (int l = 1; l < lanes && r == hipSuccess; ++l) r = hipStreamWaitEvent(e->side[l - 1], e->fork, 0);
+        }
+        for (int k = 0; k < n && r == hipSuccess; ++k) {
+            const int l = k % lanes;
+            r = issue(e, P[i + k], l == 0 ? e->stream : e->side[l - 1]);
+        }
+        for (int l = 1; l < lanes && r == hipSuccess; ++l) {
+            r = hipEventRecord(e->join[l - 1], e->side[l - 1]);
+            if (r == hipSuccess) r = hipStreamWaitEvent(e->stream, e->join[l - 1], 0);
+        }
         if (r != hipSuccess) return efail(HQT_ERR_HIP, std::string("ensemble launch: ") + hipGetErrorString(r));
+        i = j;
     }
     return HQT_OK;
 }
@@ -193,6 +224,11 @@ void release(hqt_ensemble* e) {
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->graph) (void)hipGraphExecDestroy(e->graph);
     if (e->dplan) (void)hipFree(e->dplan);
+    for (int l = 0; l < kLanes - 1; ++l) {
+        if (e->side[l]) { (void)hipStreamSynchronize(e->side[l]); (void)hipStreamDestroy(e->side[l]); }
+        if (e->join[l]) (void)hipEventDestroy(e->join[l]);
+    }
+    if (e->fork) (void)hipEventDestroy(e->fork);
     for (hqt_ctx* c : e->mem)
         if (c && c->owner == e->stream) c->owner = nullptr;
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -219,6 +255,15 @@ int hqt_ensemble_create(hqt_ctx* const* members, int32_t n, int32_t flags, hqt_e
     e->mem.assign(members, members + n);
     hipError_t r = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
     if (r != hipSuccess) { e->stream = nullptr; release(e); return efail(HQT_ERR_HIP, "hipStreamCreate"); }
+    for (int l = 0; l < kLanes - 1; ++l) {
+        // default priority, like the ensemble's own stream
+        if (hipStreamCreateWithFlags(&e->side[l], hipStreamNonBlocking) != hipSuccess) e->side[l] = nullptr;
+        if (hipEventCreateWithFlags(&e->join[l], hipEventDisableTiming) != hipSuccess) e->join[l] = nullptr;
+        if (!e->side[l] || !e->join[l]) { release(e); return efail(HQT_ERR_HIP, "ensemble side stream/event"); }
+    }
+    if (hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess) {
+        e->fork = nullptr; release(e); return efail(HQT_ERR_HIP, "ensemble fork event");
+    }
     PlanImage img;
     build_plan(e, img);
     r = hipMalloc(&e->dplan, img.bytes.size());

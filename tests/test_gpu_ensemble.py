@@ -56,6 +56,27 @@ def test_ensemble_equals_members_alone(graph):
         st.close()
 
 
+def test_ensemble_many_members_per_class():
+    """Twelve members, two or more per kernel class: the class launches of one kernel run on
+    several streams at once between the phase's fork and join.  Still bitwise equal to each member
+    stepped alone, over repeated graph replays."""
+    import hqt
+    specs = MEMBERS + MEMBERS[::-1]
+    nsteps = 7
+    ref = _alone(specs, nsteps)
+    sts = [_make(spec, 100 + i) for i, spec in enumerate(specs)]
+    with hqt.Ensemble(sts, graph=True) as ens:
+        for _ in range(nsteps):
+            ens.step(1)
+            ens.sync()
+        for i, st in enumerate(sts):
+            got = st.get_state()
+            for a, b in zip(got, ref[i]):
+                assert np.array_equal(a, b), f"member {i} {specs[i]}"
+    for st in sts:
+        st.close()
+
+
 def test_member_calls_are_ordered_after_ensemble_steps():
     """Ensemble steps then the member's own steps: same as stepping the member alone."""
     import hqt

@@ -283,11 +283,11 @@ int line_kind(int name) {
     return (name == KN_LX_FILT || name == KN_LY_FILT) ? 2 : (name == KN_LX_CONS || name == KN_LY_CONS) ? 1 : 0;
 }
 
-int line(hqt_ctx* c, int pass, int dir, int name) {
+int line(hqt_ctx* c, int pass, int dir, int name, hipStream_t s = nullptr) {
     const LPass& H = c->hp[pass];
     return run(c, name, [&] {
         return launch_line_pass(c->dp + pass, kOne, line_blocks(H), H.ngroups, line_pass_lds_bytes(H, c->fp32), H.Mp / 64,
-                                H.lb / 16, dir, c->fp32, line_kind(name), c->stream, H.fold, H.xcd);
+                                H.lb / 16, dir, c->fp32, line_kind(name), s ? s : c->stream, H.fold, H.xcd);
     });
 }
 
@@ -398,8 +398,20 @@ int enqueue_step(hqt_ctx* c) {
         if ((rc = line(c, PY1 + s, 1, KN_LY_STAGE))) return rc;
         if ((rc = run(c, KN_ZSTAGE, [&] { return launch_zstage(&c->dd->zs[s], kOne, zb, nz, f32, c->zwpe, st); }))) return rc;
     }
-    if ((rc = line(c, PXC, 0, KN_LX_CONS))) return rc;
+    // the constraint's x- and y-passes are independent: each alone fills a fraction of the GPU, so
+    // the x-pass runs on the side stream beside the y-pass (not while the per-kernel timers run,
+    // which time one stream)
+    const bool fork = !c->prof;
+    if (fork) {
+        HIPC(hipEventRecord(c->fork_ev, st));
+        HIPC(hipStreamWaitEvent(c->side, c->fork_ev, 0));
+    }
+    if ((rc = line(c, PXC, 0, KN_LX_CONS, fork ? c->side : st))) return rc;
     if ((rc = line(c, PYC, 1, KN_LY_CONS))) return rc;
+    if (fork) {
+        HIPC(hipEventRecord(c->join_ev, c->side));
+        HIPC(hipStreamWaitEvent(st, c->join_ev, 0));
+    }
     if ((rc = run(c, KN_ZPOST, [&] { return launch_zpost(&c->dd->zp, kOne, zb, nz, f32, st); }))) return rc;
     if (c->p.winding)
         if ((rc = run(c, KN_WIND, [&] {
@@ -565,7 +577,9 @@ int build_context(hqt_ctx* c) {
         LPass& Q = c->hp[PYC];
         Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo, (int)c->es);
         LGroup& h = add_group(Q, c->ZAy, T);
-        add_out(h, c->divA, T, 0, OP_D1Y, -1, -1, c->dxAx, T);
+        // d_y A_y into curl3 (a stage temporary, free after the last stage); the post-step kernel
+        // adds d_x A_x, so the two passes are independent
+        add_out(h, c->curl3, T, 0, OP_D1Y);
     }
     {
         void* f[5] = {c->ZR, c->ZI, c->At, c->ZAx, c->ZAy};
@@ -644,7 +658,8 @@ int build_context(hqt_ctx* c) {
     c->zr.At = c->AtR;
     std::memset(&c->zp, 0, sizeof(c->zp));
     c->zp.nxy = c->nxy; c->zp.nz = nz; c->zp.mu = c->p.mu; c->zp.zo = c->zo;
-    c->zp.R = c->ZR; c->zp.I = c->ZI; c->zp.Ax = c->ZAx; c->zp.Ay = c->ZAy; c->zp.divA = c->divA;
+    c->zp.R = c->ZR; c->zp.I = c->ZI; c->zp.Ax = c->ZAx; c->zp.Ay = c->ZAy;
+    c->zp.divA = c->dxAx; c->zp.divB = c->curl3;
     c->zp.rho = c->rho; c->zp.At = c->At; c->zp.axb = c->axb; c->zp.ayb = c->ayb; c->zp.O = c->O;
     c->zp.flags = c->flags;
     c->zpre.nxy = c->nxy; c->zpre.nz = nz; c->zpre.Lz = c->zo.Lz; c->zpre.At = c->At; c->zpre.dzAt = c->dzAt;
@@ -709,6 +724,10 @@ int hqt_ctx_create(const hqt_params* p, hqt_ctx** out) {
     c->es = c->fp32 ? 4 : 8;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return fail(HQT_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+    e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming);
+    if (e != hipSuccess) { hqt_ctx_destroy(c); return fail(HQT_ERR_HIP, std::string("side stream: ") + hipGetErrorString(e)); }
     hqt_params tmp = *p;
     c->p = tmp;     // keep the caller's boundary pointers only during build_context
     rc = build_context(c);
@@ -728,6 +747,9 @@ int hqt_ctx_destroy(hqt_ctx* c) {
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     for (auto& r : c->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (void* q : c->allocs) (void)hipFree(q);
+    if (c->side) { (void)hipStreamSynchronize(c->side); (void)hipStreamDestroy(c->side); }
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+    if (c->join_ev) (void)hipEventDestroy(c->join_ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return HQT_OK;

@@ -147,7 +147,7 @@ struct ZPost {
     int64_t nxy; int32_t nz; double mu;
     ZOps zo;
     void *R, *I, *Ax, *Ay;        // state (TS): read Psi, reset z=0 rows
-    const void* divA;             // d_x A_x + d_y A_y of the updated state (TS)
+    const void *divA, *divB;      // d_x A_x and d_y A_y of the updated state (TS); summed here
     const double* rho;            // [nxy*nz]
     double* At;                   // out
     const double *axb, *ayb;      // [nxy] boundary values

@@ -18,7 +18,6 @@ void fourdif_colrow(int N, int m, std::vector<double>& col, std::vector<double>&
     col.assign(N, 0.0);
     row.assign(N, 0.0);
     const double h = 2 * PI / N;
-    const int n1 = (N - 1) / 2;
     const int n2 = N / 2;  // ceil((N-1)/2)
     auto sgn = [](int k) { return (k % 2) ? -1.0 : 1.0; };
     if (m == 0) {
@@ -92,7 +91,7 @@ void fourdif_dense(int N, int m, double* x, double* DM) {
 }
 
 void chebdif(int N, int M, std::vector<double>& x, std::vector<double>& DM) {
-    const int n1 = N / 2, n2 = (N + 1) / 2;
+    const int n1 = N / 2;
     x.resize(N);
     std::vector<double> th(N), DX((size_t)N * N), C((size_t)N * N), Z((size_t)N * N), D((size_t)N * N, 0.0);
     for (int k = 0; k < N; ++k) th[k] = k * PI / (N - 1);

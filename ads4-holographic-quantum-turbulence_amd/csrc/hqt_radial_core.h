@@ -424,6 +424,9 @@ __global__ __launch_bounds__(256) void k_zpost(const ZPost* __restrict__ Ap, con
     load_slots<NZ, TS>(A.R, c, cok, g, n, R);
     load_slots<NZ, TS>(A.I, c, cok, g, n, I);
     load_slots<NZ, TS>(A.divA, c, cok, g, n, a);
+    load_slots<NZ, TS>(A.divB, c, cok, g, n, b);
+#pragma unroll
+    for (int s = 0; s < KS; ++s) a[s] += b[s];
     mv<NZ>(Lz, a, w);                // Lz(dxAx + dyAy)
     mv<NZ>(Lz, R, a);                // Lz Re Psi
     mv<NZ>(Lz, I, b);                // Lz Im Psi

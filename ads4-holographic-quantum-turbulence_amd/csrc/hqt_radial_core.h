@@ -425,6 +425,10 @@ __global__ __launch_bounds__(256) void k_zpost(const ZPost* __restrict__ Ap, con
     load_slots<NZ, TS>(A.I, c, cok, g, n, I);
     load_slots<NZ, TS>(A.divA, c, cok, g, n, a);
     load_slots<NZ, TS>(A.divB, c, cok, g, n, b);
+    // rho and the boundary values are loaded with the fields: one memory round trip for the kernel
+    double rh[KS];
+    load_slots<NZ, double>(A.rho, c, cok, g, n, rh);
+    const double axb = cok ? A.axb[c] : 0.0, ayb = cok ? A.ayb[c] : 0.0;
 #pragma unroll
     for (int s = 0; s < KS; ++s) a[s] += b[s];
     mv<NZ>(Lz, a, w);                // Lz(dxAx + dyAy)
@@ -440,7 +444,7 @@ __global__ __launch_bounds__(256) void k_zpost(const ZPost* __restrict__ Ap, con
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
         const int k = 4 * s + g;
-        w[s] = (cok && k < NZ) ? a[s] - A.rho[c + n * k] : 0.0;
+        w[s] = (cok && k < NZ) ? a[s] - rh[s] : 0.0;
     }
     mv<NZ>(L1, w, a);                // Lzr(Lzr(cons) - rhoi)
 #pragma unroll
@@ -452,7 +456,7 @@ __global__ __launch_bounds__(256) void k_zpost(const ZPost* __restrict__ Ap, con
             // boundary reset: Z1(:,:,nz)=0; Z3(:,:,nz)=ax_b; Z4(:,:,nz)=ay_b  (Z2 row set above)
             const int64_t pb = c + n * k;
             st<TS>(A.R, pb, 0.0); st<TS>(A.I, pb, 0.0);
-            st<TS>(A.Ax, pb, A.axb[c]); st<TS>(A.Ay, pb, A.ayb[c]);
+            st<TS>(A.Ax, pb, axb); st<TS>(A.Ay, pb, ayb);
             R[s] = 0.0;
             I[s] = 0.0;
         }

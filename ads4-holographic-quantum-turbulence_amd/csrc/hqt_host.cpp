@@ -283,11 +283,11 @@ int line_kind(int name) {
     return (name == KN_LX_FILT || name == KN_LY_FILT) ? 2 : (name == KN_LX_CONS || name == KN_LY_CONS) ? 1 : 0;
 }
 
-int line(hqt_ctx* c, int pass, int dir, int name, hipStream_t s = nullptr) {
+int line(hqt_ctx* c, int pass, int dir, int name) {
     const LPass& H = c->hp[pass];
     return run(c, name, [&] {
         return launch_line_pass(c->dp + pass, kOne, line_blocks(H), H.ngroups, line_pass_lds_bytes(H, c->fp32), H.Mp / 64,
-                                H.lb / 16, dir, c->fp32, line_kind(name), s ? s : c->stream, H.fold, H.xcd);
+                                H.lb / 16, dir, c->fp32, line_kind(name), c->stream, H.fold, H.xcd);
     });
 }
 
@@ -398,20 +398,8 @@ int enqueue_step(hqt_ctx* c) {
         if ((rc = line(c, PY1 + s, 1, KN_LY_STAGE))) return rc;
         if ((rc = run(c, KN_ZSTAGE, [&] { return launch_zstage(&c->dd->zs[s], kOne, zb, nz, f32, c->zwpe, st); }))) return rc;
     }
-    // the constraint's x- and y-passes are independent: each alone fills a fraction of the GPU, so
-    // the x-pass runs on the side stream beside the y-pass (not while the per-kernel timers run,
-    // which time one stream)
-    const bool fork = !c->prof;
-    if (fork) {
-        HIPC(hipEventRecord(c->fork_ev, st));
-        HIPC(hipStreamWaitEvent(c->side, c->fork_ev, 0));
-    }
-    if ((rc = line(c, PXC, 0, KN_LX_CONS, fork ? c->side : st))) return rc;
+    if ((rc = line(c, PXC, 0, KN_LX_CONS))) return rc;
     if ((rc = line(c, PYC, 1, KN_LY_CONS))) return rc;
-    if (fork) {
-        HIPC(hipEventRecord(c->join_ev, c->side));
-        HIPC(hipStreamWaitEvent(st, c->join_ev, 0));
-    }
     if ((rc = run(c, KN_ZPOST, [&] { return launch_zpost(&c->dd->zp, kOne, zb, nz, f32, st); }))) return rc;
     if (c->p.winding)
         if ((rc = run(c, KN_WIND, [&] {
@@ -724,10 +712,6 @@ int hqt_ctx_create(const hqt_params* p, hqt_ctx** out) {
     c->es = c->fp32 ? 4 : 8;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return fail(HQT_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
-    e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming);
-    if (e != hipSuccess) { hqt_ctx_destroy(c); return fail(HQT_ERR_HIP, std::string("side stream: ") + hipGetErrorString(e)); }
     hqt_params tmp = *p;
     c->p = tmp;     // keep the caller's boundary pointers only during build_context
     rc = build_context(c);
@@ -747,9 +731,6 @@ int hqt_ctx_destroy(hqt_ctx* c) {
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     for (auto& r : c->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (void* q : c->allocs) (void)hipFree(q);
-    if (c->side) { (void)hipStreamSynchronize(c->side); (void)hipStreamDestroy(c->side); }
-    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
-    if (c->join_ev) (void)hipEventDestroy(c->join_ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return HQT_OK;

@@ -44,9 +44,6 @@ struct hqt_ctx {
     int tdt = DT_F64;
     size_t es = 8;
     hipStream_t stream = nullptr;
-    // side stream of the step: the constraint's x-pass runs there beside its y-pass (default priority)
-    hipStream_t side = nullptr;
-    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     std::vector<void*> allocs;
     int64_t bytes = 0;               // device bytes allocated by this context
     int zwpe = 2;                    // occupancy target of the radial stage kernel (tuned)

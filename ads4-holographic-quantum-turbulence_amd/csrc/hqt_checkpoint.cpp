@@ -193,12 +193,13 @@ int hqt_checkpoint_save_async(hqt_ctx* c, const char* path) {
     if (!c || !path) return hqt_fail(HQT_ERR_ARG, "NULL argument");
     if (!c->have_state) return hqt_fail(HQT_ERR_STATE, "no state uploaded");
     if (int rc = wait_writer(c)) return rc;
-    const size_t n = (size_t)c->npts, es = c->es;
+    // snapshots keep the device layout (planes ps apart); the copy to the host drops the padding
+    const size_t n = (size_t)c->npts, es = c->es, nv = (size_t)c->nvol, nxy = (size_t)c->nxy, ps = (size_t)c->ps;
     if (!c->ckpt) {
         CheckpointWriter* w = new (std::nothrow) CheckpointWriter();
         if (!w) return hqt_fail(HQT_ERR_OOM, "host allocation failed");
         c->ckpt = w;
-        const size_t sz[6] = {n * es, n * es, n * es, n * es, n * 8, n * 8};
+        const size_t sz[6] = {nv * es, nv * es, nv * es, nv * es, nv * 8, nv * 8};
         for (int k = 0; k < 6; ++k)
             if (hipMalloc(&w->snap[k], sz[k]) != hipSuccess) { w->snap[k] = nullptr; checkpoint_writer_release(c); return hqt_fail(HQT_ERR_OOM, "hipMalloc(checkpoint snapshot)"); }
         if (hipHostMalloc(&w->host, 6 * n * 8, hipHostMallocDefault) != hipSuccess) { w->host = nullptr; checkpoint_writer_release(c); return hqt_fail(HQT_ERR_OOM, "hipHostMalloc(checkpoint staging)"); }
@@ -211,7 +212,7 @@ int hqt_checkpoint_save_async(hqt_ctx* c, const char* path) {
     CheckpointWriter* w = c->ckpt;
     if (int rc = order_with_owner(c, w->done, true)) return rc;
     const void* src[6] = {c->ZR, c->ZI, c->ZAx, c->ZAy, c->At, c->rho};
-    const size_t sz[6] = {n * es, n * es, n * es, n * es, n * 8, n * 8};
+    const size_t sz[6] = {nv * es, nv * es, nv * es, nv * es, nv * 8, nv * 8};
     for (int k = 0; k < 6; ++k) HIPC(hipMemcpyAsync(w->snap[k], src[k], sz[k], hipMemcpyDeviceToDevice, c->stream));
     HIPC(hipEventRecord(w->ready, c->stream));
     if (int rc = order_with_owner(c, w->done, false)) return rc;
@@ -219,14 +220,15 @@ int hqt_checkpoint_save_async(hqt_ctx* c, const char* path) {
     w->msg.clear();
     const int nx = c->nx, ny = c->ny, nz = c->nz, fp32 = c->fp32;
     const std::string file(path);
-    w->th = std::thread([w, n, es, nx, ny, nz, fp32, file] {
+    w->th = std::thread([w, n, es, nxy, ps, nx, ny, nz, fp32, file] {
         double* h = static_cast<double*>(w->host);
         auto fail_w = [w](int code, const std::string& m) { w->status = code; w->msg = m; };
         if (hipStreamWaitEvent(w->copy, w->ready, 0) != hipSuccess) return fail_w(HQT_ERR_HIP, "checkpoint: wait");
         // TS fields land in the first half of each double slot region, FP64 fields directly
         for (int k = 0; k < 6; ++k) {
-            const size_t bytes = k < 4 ? n * es : n * 8;
-            if (hipMemcpyAsync(h + k * n, w->snap[k], bytes, hipMemcpyDeviceToHost, w->copy) != hipSuccess)
+            const size_t e = k < 4 ? es : 8;
+            if (hipMemcpy2DAsync(h + k * n, nxy * e, w->snap[k], ps * e, nxy * e, (size_t)nz, hipMemcpyDeviceToHost,
+                                 w->copy) != hipSuccess)
                 return fail_w(HQT_ERR_HIP, "checkpoint: copy");
         }
         if (hipStreamSynchronize(w->copy) != hipSuccess) return fail_w(HQT_ERR_HIP, "checkpoint: sync");

@@ -19,8 +19,8 @@ __global__ __launch_bounds__(DG_T) void k_order_param(const Census A) {
     double a = 0.0, b = 0.0;
     for (int m = 0; m < A.nz; ++m) {
         const double l = A.Lzb[m];
-        a += l * (double)R[col + A.nxy * m];
-        b += l * (double)I[col + A.nxy * m];
+        a += l * (double)R[col + A.ps * m];
+        b += l * (double)I[col + A.ps * m];
     }
     A.O[2 * col] = a;
     A.O[2 * col + 1] = b;

@@ -162,7 +162,7 @@ void build_plan(hqt_ensemble* e, PlanImage& img) {
     add_line(P, img, M, PXF, 0, KN_LX_FILT, with_filter);
     add_line(P, img, M, PYF, 1, KN_LY_FILT, with_filter);
     add_z<NanCheck>(P, img, M, K_NAN, false, without_filter, [](hqt_ctx* c) { return c->nan; },
-                    [](hqt_ctx* c) { return nancheck_blocks(c->npts); });
+                    [](hqt_ctx* c) { return nancheck_blocks(c->nvol); });
 }
 
 hipError_t issue(const hqt_ensemble* e, const Launch& L, hipStream_t s) {

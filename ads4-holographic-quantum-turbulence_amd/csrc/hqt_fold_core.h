@@ -179,23 +179,29 @@ __device__ __forceinline__ bool fold_out(const LPass& P, const LOut& O, typename
                                          typename MF<T>::v4 (&aO)[NT][QF], const T (&cst)[NT][4], int line0, int x0,
                                          int z, int nl, int N, int role, int lane) {
     const int ln = lane & 15, lk = lane >> 4;
-    const int nx = P.nx, pl = P.nx * P.ny * z, Mh = N >> 1;
+    const int nx = P.nx, pl = P.ps * z, Mh = N >> 1;
+    // x-lines: element offsets of the 4 NT lines this lane stores (z = the z-slice of line0)
+    int xb[NT][4];
+#pragma unroll
+    for (int u = 0; u < NT; ++u)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) xb[u][r] = DIR == 0 ? xline_base_at(P, line0, z, 16 * u + MF<T>::row(lk, r)) : 0;
     // element index of position i (direct) or N - i (mirror) of accumulator (u, q, r)
     auto idx = [&](int u, int q, int r, bool mirror) -> uint32_t {
         const int i0 = (role + 4 * q) * 16;
         int line, i;
         if (DIR == 0) {
-            line = line0 + 16 * u + MF<T>::row(lk, r);
+            line = 16 * u + MF<T>::row(lk, r);      // offset from line0 until the bound check
             i = i0 + ln;
         } else {
             line = x0 + 16 * u + ln;
             i = i0 + MF<T>::row(lk, r);
         }
-        const bool okl = DIR == 0 ? line < nl : line < nx;
+        const bool okl = DIR == 0 ? line0 + line < nl : line < nx;
         const bool ok = mirror ? (okl && i >= 1 && 2 * i < N) : (okl && i <= Mh);
         const int pos = mirror ? N - i : i;
         if (!ok) return Z_OOB;
-        return DIR == 0 ? (uint32_t)(line * nx + pos) : (uint32_t)(line + nx * pos + pl);
+        return DIR == 0 ? (uint32_t)(xb[u][r] + pos) : (uint32_t)(line + nx * pos + pl);
     };
     bool bad = false;
 #pragma unroll
@@ -328,13 +334,18 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
         const int lt = tile0 + unit / G.ntiles, t = unit % G.ntiles;
         const void* src = G.src[t];
         const int dt = G.src_dt[t];
-        int line0 = 0, z = 0, x0 = 0;
+        int line0 = 0, z = 0, x0 = 0;    // x-lines: z = the z-slice of line0
         if (DIR == 0) {
             line0 = lt * LB;
+            z = line0 / P.ny;
         } else {
             x0 = (lt % P.nxc) * LB;
             z = lt / P.nxc;
         }
+        // x-lines: element offsets of this thread's NH lines
+        int xb[NH];
+#pragma unroll
+        for (int h = 0; h < NH; ++h) xb[h] = DIR == 0 ? xline_base_at(P, line0, z, mrow + 16 * h) : 0;
         auto load = [&](auto tag) {
             using S_ = decltype(tag);
             const Rs rs = rs_make(src, fbytes<S_>(P));
@@ -343,10 +354,10 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
                 uint32_t e;
                 if (DIR == 0) {
                     const int line = line0 + mrow + 16 * h;
-                    e = line < nl ? (uint32_t)(line * P.nx) : Z_OOB;
+                    e = line < nl ? (uint32_t)xb[h] : Z_OOB;
                 } else {
                     const int x = x0 + mrow;
-                    e = x < P.nx ? (uint32_t)(x + P.nx * P.ny * z) : Z_OOB;
+                    e = x < P.nx ? (uint32_t)(x + P.ps * z) : Z_OOB;
                 }
                 pref[h] = (R)bldr<S_>(rs, e);
             }
@@ -359,12 +370,13 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
                     if (DIR == 0) {
                         const int line = line0 + mrow + 16 * h;
                         const bool okl = line < nl;
-                        e0 = (okl && j <= Mh) ? (uint32_t)(line * P.nx + j) : Z_OOB;
-                        e1 = (okl && j >= 1 && 2 * j < N) ? (uint32_t)(line * P.nx + N - j) : Z_OOB;
+                        const int lb0 = xb[h];
+                        e0 = (okl && j <= Mh) ? (uint32_t)(lb0 + j) : Z_OOB;
+                        e1 = (okl && j >= 1 && 2 * j < N) ? (uint32_t)(lb0 + N - j) : Z_OOB;
                     } else {
                         const int x = x0 + mrow;
                         const bool okl = x < P.nx;
-                        const int base = x + P.nx * P.ny * z;
+                        const int base = x + P.ps * z;
                         e0 = (okl && j <= Mh) ? (uint32_t)(base + P.nx * j) : Z_OOB;
                         e1 = (okl && j >= 1 && 2 * j < N) ? (uint32_t)(base + P.nx * (N - j)) : Z_OOB;
                     }
@@ -402,9 +414,10 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
     stage_ftables<T, QF>(P, G, C, TL, tid);
     for (int unit = 0; unit < nunit; ++unit) {
         const int lt = tile0 + unit / G.ntiles, t = unit % G.ntiles;
-        int line0 = 0, z = 0, x0 = 0;
+        int line0 = 0, z = 0, x0 = 0;    // x-lines: z = the z-slice of line0
         if (DIR == 0) {
             line0 = lt * LB;
+            z = line0 / P.ny;
         } else {
             x0 = (lt % P.nxc) * LB;
             z = lt / P.nxc;

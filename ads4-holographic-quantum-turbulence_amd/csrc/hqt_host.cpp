@@ -41,7 +41,7 @@ bool gfilt_on(const hqt_params& p, bool lrx, bool lry) {
 }
 
 int64_t ctx_footprint(const hqt_params& p) {
-    const int64_t nxy = (int64_t)p.nx * p.ny, np = nxy * p.nz, nz = p.nz;
+    const int64_t nxy = (int64_t)p.nx * p.ny, np = plane_stride(nxy) * p.nz, nz = p.nz;
     const int64_t es = p.precision == HQT_MIXED ? 4 : 8;
     auto r8 = [](int64_t b) { return b == 0 ? 8 : b; };
     int64_t b = 0;
@@ -93,6 +93,31 @@ int dalloc(hqt_ctx* c, size_t bytes, void** out) {
     return HQT_OK;
 }
 
+// Volume fields: host arrays are dense [nx, ny, nz]; on the device the planes are c->ps elements apart.
+int upload_vol(hqt_ctx* c, void* dst, const double* src, int dt) {
+    const size_t nxy = (size_t)c->nxy, nz = (size_t)c->nz, es = dt == DT_F64 ? 8 : 4, dp = (size_t)c->ps * es;
+    if (dt == DT_F64) {
+        HIPC(hipMemcpy2D(dst, dp, src, nxy * es, nxy * es, nz, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tmp(nxy * nz);
+        for (size_t i = 0; i < tmp.size(); ++i) tmp[i] = (float)src[i];
+        HIPC(hipMemcpy2D(dst, dp, tmp.data(), nxy * es, nxy * es, nz, hipMemcpyHostToDevice));
+    }
+    return HQT_OK;
+}
+
+int download_vol(const hqt_ctx* c, double* dst, const void* src, int dt) {
+    const size_t nxy = (size_t)c->nxy, nz = (size_t)c->nz, es = dt == DT_F64 ? 8 : 4, dp = (size_t)c->ps * es;
+    if (dt == DT_F64) {
+        HIPC(hipMemcpy2D(dst, nxy * es, src, dp, nxy * es, nz, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tmp(nxy * nz);
+        HIPC(hipMemcpy2D(tmp.data(), nxy * es, src, dp, nxy * es, nz, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < tmp.size(); ++i) dst[i] = tmp[i];
+    }
+    return HQT_OK;
+}
+
 int upload_as(hqt_ctx* c, void* dst, const double* src, size_t n, int dt) {
     if (dt == DT_F64) {
         HIPC(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyHostToDevice));
@@ -102,17 +127,6 @@ int upload_as(hqt_ctx* c, void* dst, const double* src, size_t n, int dt) {
         HIPC(hipMemcpy(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
     }
     (void)c;
-    return HQT_OK;
-}
-
-int download_as(double* dst, const void* src, size_t n, int dt) {
-    if (dt == DT_F64) {
-        HIPC(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyDeviceToHost));
-    } else {
-        std::vector<float> tmp(n);
-        HIPC(hipMemcpy(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < n; ++i) dst[i] = tmp[i];
-    }
     return HQT_OK;
 }
 
@@ -126,7 +140,7 @@ void set_lb(LPass& P, int lb, int dir, int tpb = 1) {
     else P.ntiles = P.nxc * P.nz;
 }
 
-LPass base_pass(int N, int nx, int ny, int nz, int dir, void* const* ops, int32_t* flags, int fold = 0, int es = 8) {
+LPass base_pass(int N, int nx, int ny, int nz, int dir, void* const* ops, int32_t* flags, int fold, int es, int64_t ps) {
     LPass P;
     std::memset(&P, 0, sizeof(P));
     const LineGeom lg = line_geom(N);
@@ -145,6 +159,8 @@ LPass base_pass(int N, int nx, int ny, int nz, int dir, void* const* ops, int32_
         P.Z = fg.Z;
     }
     P.nx = nx; P.ny = ny; P.nz = nz;
+    P.ps = (int32_t)ps;
+    P.pad = (int32_t)(ps - (int64_t)nx * ny);
     set_lb(P, 16, dir);
     for (int i = 0; i < OP_COUNT; ++i) P.ops[i] = ops[i];
     P.flags = flags;
@@ -411,7 +427,7 @@ int enqueue_step(hqt_ctx* c) {
         if ((rc = line(c, PXF, 0, KN_LX_FILT))) return rc;
         if ((rc = line(c, PYF, 1, KN_LY_FILT))) return rc;
     } else {
-        if ((rc = run(c, KN_NAN, [&] { return launch_nancheck(&c->dd->nan, kOne, nancheck_blocks(c->npts), f32, st); })))
+        if ((rc = run(c, KN_NAN, [&] { return launch_nancheck(&c->dd->nan, kOne, nancheck_blocks(c->nvol), f32, st); })))
             return rc;
     }
     return HQT_OK;
@@ -432,7 +448,7 @@ int resolve_profile(hqt_ctx* c) {
 
 int build_context(hqt_ctx* c) {
     const int nx = c->nx, ny = c->ny, nz = c->nz;
-    const size_t es = c->es, np = (size_t)c->npts, nxy = (size_t)c->nxy;
+    const size_t es = c->es, np = (size_t)c->nvol, nxy = (size_t)c->nxy;
     int rc;
 #define ALLOC(ptr, bytes) do { void* q_; if ((rc = dalloc(c, (bytes), &q_))) return rc; ptr = (decltype(ptr))q_; } while (0)
     ALLOC(c->ZR, np * es); ALLOC(c->ZI, np * es); ALLOC(c->ZAx, np * es); ALLOC(c->ZAy, np * es);
@@ -525,7 +541,7 @@ int build_context(hqt_ctx* c) {
                        {c->SB[0], c->SB[1], c->SB[2], c->SB[3]},
                        {c->SA[0], c->SA[1], c->SA[2], c->SA[3]}};
     auto stage_x = [&](LPass& P, void* const* s, const double* at) {
-        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo, (int)c->es);
+        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo, (int)c->es, c->ps);
         LGroup* g = &add_group(P, s[0], T);
         add_out(*g, c->dxR, T, 0, OP_D1X); add_out(*g, c->dxxR, T, 0, OP_D2X);
         g = &add_group(P, s[1], T);
@@ -537,7 +553,7 @@ int build_context(hqt_ctx* c) {
         if (at) { g = &add_group(P, at, DT_F64); add_out(*g, c->dxAt, T, 0, OP_D1X); }
     };
     auto stage_y = [&](LPass& P, void* const* s, const double* at) {
-        P = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo, (int)c->es);
+        P = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo, (int)c->es, c->ps);
         LGroup* g = &add_group(P, s[0], T);
         add_out(*g, c->dyR, T, 0, OP_D1Y); add_out(*g, c->lapR, T, 0, OP_D2Y, -1, -1, c->dxxR, T);
         g = &add_group(P, s[1], T);
@@ -559,11 +575,11 @@ int build_context(hqt_ctx* c) {
     stage_y(c->hp[PYR], SR_, c->AtR);
     {
         LPass& P = c->hp[PXC];
-        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo, (int)c->es);
+        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo, (int)c->es, c->ps);
         LGroup& g = add_group(P, c->ZAx, T);
         add_out(g, c->dxAx, T, 0, OP_D1X);
         LPass& Q = c->hp[PYC];
-        Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo, (int)c->es);
+        Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo, (int)c->es, c->ps);
         LGroup& h = add_group(Q, c->ZAy, T);
         // d_y A_y into curl3 (a stage temporary, free after the last stage); the post-step kernel
         // adds d_x A_x, so the two passes are independent
@@ -573,9 +589,9 @@ int build_context(hqt_ctx* c) {
         void* f[5] = {c->ZR, c->ZI, c->At, c->ZAx, c->ZAy};
         int fdt[5] = {T, T, DT_F64, T, T};
         LPass& P = c->hp[PXF];
-        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo, (int)c->es);
+        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo, (int)c->es, c->ps);
         LPass& Q = c->hp[PYF];
-        Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo, (int)c->es);
+        Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo, (int)c->es, c->ps);
         // Im(P) has rank 4 for the reference's masks: applied as U (V^T x) instead of a dense product.
         // Folded with both directions rank 4 (and ny >= 16): the cross term (Im P_x (x) Im P_y) x is
         // U_x G U_y^T per z-slice, so the x-pass writes partials of G (4 x 4 per slice) instead of the
@@ -628,7 +644,7 @@ int build_context(hqt_ctx* c) {
     for (int s = 0; s < 4; ++s) {
         ZStage& Z = c->zs[s];
         std::memset(&Z, 0, sizeof(Z));
-        Z.nxy = c->nxy; Z.nz = nz; Z.stage = s + 1; Z.mode = 0; Z.h = c->p.dt; Z.zo = c->zo;
+        Z.nxy = c->nxy; Z.ps = c->ps; Z.nz = nz; Z.stage = s + 1; Z.mode = 0; Z.h = c->p.dt; Z.zo = c->zo;
         Z.sR = Sin[s][0]; Z.sI = Sin[s][1]; Z.sAx = Sin[s][2]; Z.sAy = Sin[s][3];
         Z.At = c->At; Z.dzAt = c->dzAt; Z.dxAt = c->dxAt; Z.dyAt = c->dyAt;
         Z.dxR = c->dxR; Z.dxI = c->dxI; Z.dyR = c->dyR; Z.dyI = c->dyI; Z.lapR = c->lapR; Z.lapI = c->lapI;
@@ -645,16 +661,16 @@ int build_context(hqt_ctx* c) {
     c->zr.sR = c->SB[0]; c->zr.sI = c->SB[1]; c->zr.sAx = c->SB[2]; c->zr.sAy = c->SB[3];
     c->zr.At = c->AtR;
     std::memset(&c->zp, 0, sizeof(c->zp));
-    c->zp.nxy = c->nxy; c->zp.nz = nz; c->zp.mu = c->p.mu; c->zp.zo = c->zo;
+    c->zp.nxy = c->nxy; c->zp.ps = c->ps; c->zp.nz = nz; c->zp.mu = c->p.mu; c->zp.zo = c->zo;
     c->zp.R = c->ZR; c->zp.I = c->ZI; c->zp.Ax = c->ZAx; c->zp.Ay = c->ZAy;
     c->zp.divA = c->dxAx; c->zp.divB = c->curl3;
     c->zp.rho = c->rho; c->zp.At = c->At; c->zp.axb = c->axb; c->zp.ayb = c->ayb; c->zp.O = c->O;
     c->zp.flags = c->flags;
-    c->zpre.nxy = c->nxy; c->zpre.nz = nz; c->zpre.Lz = c->zo.Lz; c->zpre.At = c->At; c->zpre.dzAt = c->dzAt;
+    c->zpre.nxy = c->nxy; c->zpre.ps = c->ps; c->zpre.nz = nz; c->zpre.Lz = c->zo.Lz; c->zpre.At = c->At; c->zpre.dzAt = c->dzAt;
     c->zpre.flags = c->flags; c->zpre.bump = 1;
     c->zpreR = c->zpre; c->zpreR.At = c->AtR; c->zpreR.bump = 0;
     std::memset(&c->wind, 0, sizeof(c->wind));
-    c->wind.nx = nx; c->wind.ny = ny; c->wind.nz = nz; c->wind.nxy = c->nxy;
+    c->wind.nx = nx; c->wind.ny = ny; c->wind.nz = nz; c->wind.nxy = c->nxy; c->wind.ps = c->ps;
     c->wind.O = c->O; c->wind.part = c->part; c->wind.R = c->ZR; c->wind.I = c->ZI;
     c->wind.yph = c->yph; c->wind.W = c->W; c->wind.flags = c->flags;
     c->nan.R = c->ZR; c->nan.I = c->ZI; c->nan.Ax = c->ZAx; c->nan.Ay = c->ZAy; c->nan.At = c->At;
@@ -706,6 +722,8 @@ int hqt_ctx_create(const hqt_params* p, hqt_ctx** out) {
     c->nx = p->nx; c->ny = p->ny; c->nz = p->nz;
     c->nxy = (int64_t)p->nx * p->ny;
     c->npts = c->nxy * p->nz;
+    c->ps = plane_stride(c->nxy);
+    c->nvol = c->ps * p->nz;
     c->fp32 = p->precision == HQT_MIXED;
     c->tdt = c->fp32 ? DT_F32 : DT_F64;
     c->fold = (p->flags & HQT_DENSE_LINES) ? 0 : 1;
@@ -744,12 +762,12 @@ int hqt_set_state(hqt_ctx* c, const double* Z1, const double* Z2, const double* 
     std::vector<double> re, im;
     split_complex(Z1, n, re, im);
     int rc;
-    if ((rc = upload_as(c, c->ZR, re.data(), n, c->tdt))) return rc;
-    if ((rc = upload_as(c, c->ZI, im.data(), n, c->tdt))) return rc;
-    if ((rc = upload_as(c, c->ZAx, Z3, n, c->tdt))) return rc;
-    if ((rc = upload_as(c, c->ZAy, Z4, n, c->tdt))) return rc;
-    HIPC(hipMemcpy(c->At, Z2, n * 8, hipMemcpyHostToDevice));
-    HIPC(hipMemcpy(c->rho, rhoi, n * 8, hipMemcpyHostToDevice));
+    if ((rc = upload_vol(c, c->ZR, re.data(), c->tdt))) return rc;
+    if ((rc = upload_vol(c, c->ZI, im.data(), c->tdt))) return rc;
+    if ((rc = upload_vol(c, c->ZAx, Z3, c->tdt))) return rc;
+    if ((rc = upload_vol(c, c->ZAy, Z4, c->tdt))) return rc;
+    if ((rc = upload_vol(c, c->At, Z2, DT_F64))) return rc;
+    if ((rc = upload_vol(c, c->rho, rhoi, DT_F64))) return rc;
     HIPC(hipMemset(c->flags, 0, 4 * sizeof(int32_t)));
     c->have_state = true;
     return HQT_OK;
@@ -763,14 +781,14 @@ int hqt_get_state(hqt_ctx* c, double* Z1, double* Z2, double* Z3, double* Z4, do
     int rc;
     if (Z1) {
         std::vector<double> re(n), im(n);
-        if ((rc = download_as(re.data(), c->ZR, n, c->tdt))) return rc;
-        if ((rc = download_as(im.data(), c->ZI, n, c->tdt))) return rc;
+        if ((rc = download_vol(c, re.data(), c->ZR, c->tdt))) return rc;
+        if ((rc = download_vol(c, im.data(), c->ZI, c->tdt))) return rc;
         for (size_t i = 0; i < n; ++i) { Z1[2 * i] = re[i]; Z1[2 * i + 1] = im[i]; }
     }
-    if (Z2) HIPC(hipMemcpy(Z2, c->At, n * 8, hipMemcpyDeviceToHost));
-    if (Z3 && (rc = download_as(Z3, c->ZAx, n, c->tdt))) return rc;
-    if (Z4 && (rc = download_as(Z4, c->ZAy, n, c->tdt))) return rc;
-    if (rhoi) HIPC(hipMemcpy(rhoi, c->rho, n * 8, hipMemcpyDeviceToHost));
+    if (Z2 && (rc = download_vol(c, Z2, c->At, DT_F64))) return rc;
+    if (Z3 && (rc = download_vol(c, Z3, c->ZAx, c->tdt))) return rc;
+    if (Z4 && (rc = download_vol(c, Z4, c->ZAy, c->tdt))) return rc;
+    if (rhoi && (rc = download_vol(c, rhoi, c->rho, DT_F64))) return rc;
     return read_flags(c, nullptr);
 }
 
@@ -832,7 +850,8 @@ int hqt_rhs(hqt_ctx* c, const double* Z1, const double* Z2, const double* Z3, co
     int rc;
     if (!c->F1R) {
 #define ALLOC(ptr, bytes) do { void* q_; if ((rc = dalloc(c, (bytes), &q_))) return rc; ptr = (decltype(ptr))q_; } while (0)
-        ALLOC(c->F1R, n * 8); ALLOC(c->F1I, n * 8); ALLOC(c->F2, n * 8); ALLOC(c->F3, n * 8); ALLOC(c->F4, n * 8);
+        const size_t nv = (size_t)c->nvol;
+        ALLOC(c->F1R, nv * 8); ALLOC(c->F1I, nv * 8); ALLOC(c->F2, nv * 8); ALLOC(c->F3, nv * 8); ALLOC(c->F4, nv * 8);
 #undef ALLOC
         c->zr.F1R = c->F1R; c->zr.F1I = c->F1I; c->zr.F2 = c->F2; c->zr.F3 = c->F3; c->zr.F4 = c->F4;
         if ((rc = upload_desc(c))) return rc;
@@ -840,11 +859,11 @@ int hqt_rhs(hqt_ctx* c, const double* Z1, const double* Z2, const double* Z3, co
     if (int rs_ = hqt_sync_streams(c)) return rs_;
     std::vector<double> re, im;
     split_complex(Z1, n, re, im);
-    if ((rc = upload_as(c, c->SB[0], re.data(), n, c->tdt))) return rc;
-    if ((rc = upload_as(c, c->SB[1], im.data(), n, c->tdt))) return rc;
-    if ((rc = upload_as(c, c->SB[2], Z3, n, c->tdt))) return rc;
-    if ((rc = upload_as(c, c->SB[3], Z4, n, c->tdt))) return rc;
-    HIPC(hipMemcpy(c->AtR, Z2, n * 8, hipMemcpyHostToDevice));
+    if ((rc = upload_vol(c, c->SB[0], re.data(), c->tdt))) return rc;
+    if ((rc = upload_vol(c, c->SB[1], im.data(), c->tdt))) return rc;
+    if ((rc = upload_vol(c, c->SB[2], Z3, c->tdt))) return rc;
+    if ((rc = upload_vol(c, c->SB[3], Z4, c->tdt))) return rc;
+    if ((rc = upload_vol(c, c->AtR, Z2, DT_F64))) return rc;
     int32_t saved[2];
     HIPC(hipMemcpy(saved, c->flags, sizeof(saved), hipMemcpyDeviceToHost));
     HIPC(hipMemset(c->flags, 0, sizeof(saved)));
@@ -857,13 +876,13 @@ int hqt_rhs(hqt_ctx* c, const double* Z1, const double* Z2, const double* Z3, co
     HIPC(hipMemcpy(c->flags, saved, sizeof(saved), hipMemcpyHostToDevice));
     if (F1) {
         std::vector<double> r(n), i(n);
-        HIPC(hipMemcpy(r.data(), c->F1R, n * 8, hipMemcpyDeviceToHost));
-        HIPC(hipMemcpy(i.data(), c->F1I, n * 8, hipMemcpyDeviceToHost));
+        if ((rc = download_vol(c, r.data(), c->F1R, DT_F64))) return rc;
+        if ((rc = download_vol(c, i.data(), c->F1I, DT_F64))) return rc;
         for (size_t k = 0; k < n; ++k) { F1[2 * k] = r[k]; F1[2 * k + 1] = i[k]; }
     }
-    if (F2) HIPC(hipMemcpy(F2, c->F2, n * 8, hipMemcpyDeviceToHost));
-    if (F3) HIPC(hipMemcpy(F3, c->F3, n * 8, hipMemcpyDeviceToHost));
-    if (F4) HIPC(hipMemcpy(F4, c->F4, n * 8, hipMemcpyDeviceToHost));
+    if (F2 && (rc = download_vol(c, F2, c->F2, DT_F64))) return rc;
+    if (F3 && (rc = download_vol(c, F3, c->F3, DT_F64))) return rc;
+    if (F4 && (rc = download_vol(c, F4, c->F4, DT_F64))) return rc;
     return resolve_profile(c);
 }
 
@@ -871,7 +890,7 @@ namespace {
 // O of the current state into c->Od; with `census`, also the plaquette charges and counts.
 int enqueue_census(hqt_ctx* c, bool census) {
     Census A{};
-    A.nxy = c->nxy; A.nx = c->nx; A.ny = c->ny; A.nz = c->nz;
+    A.nxy = c->nxy; A.ps = c->ps; A.nx = c->nx; A.ny = c->ny; A.nz = c->nz;
     A.R = c->ZR; A.I = c->ZI;
     A.Lzb = c->zo.Lz + (size_t)(c->nz - 1) * c->nz;
     A.O = c->Od; A.charge = c->cmap; A.counts = c->ccount; A.part = c->cpart;
@@ -972,7 +991,7 @@ int hqt_ff(int32_t nx, int32_t ny, int32_t nslices, const double* x_re, const do
         if (hipMemcpy(u, x_re, n * 8, hipMemcpyHostToDevice) != hipSuccess) { rc = fail(HQT_ERR_HIP, "hipMemcpy x"); break; }
         if (x_im && hipMemcpy(v, x_im, n * 8, hipMemcpyHostToDevice) != hipSuccess) { rc = fail(HQT_ERR_HIP, "hipMemcpy x_im"); break; }
         LPass hp[2];
-        hp[0] = base_pass(nx, nx, ny, nslices, 0, ops, flags);
+        hp[0] = base_pass(nx, nx, ny, nslices, 0, ops, flags, 0, 8, (int64_t)nx * ny);
         LGroup& g = add_group(hp[0], u, DT_F64, v, DT_F64);
         if (x_im) {
             // Px (u + i v) = (Pr u - Pi v) + i (Pi u + Pr v)
@@ -983,7 +1002,7 @@ int hqt_ff(int32_t nx, int32_t ny, int32_t nslices, const double* x_re, const do
             add_out(g, a, DT_F64, 0, OP_PRX);
             add_out(g, b, DT_F64, 0, OP_PIX);
         }
-        hp[1] = base_pass(ny, nx, ny, nslices, 1, ops, flags);
+        hp[1] = base_pass(ny, nx, ny, nslices, 1, ops, flags, 0, 8, (int64_t)nx * ny);
         LGroup& h = add_group(hp[1], a, DT_F64, b, DT_F64);
         add_out(h, out, DT_F64, 0, OP_QRY, 1, OP_NQIY);     // Re(Py (a + i b)) = Qr a - Qi b
         if (hipMemcpy(dp, hp, sizeof(hp), hipMemcpyHostToDevice) != hipSuccess) { rc = fail(HQT_ERR_HIP, "hipMemcpy pass"); break; }
@@ -1039,7 +1058,7 @@ int hqt_debug_field(hqt_ctx* c, const char* name, double* out) {
     for (auto& t : tab) {
         if (std::strcmp(t.n, name) == 0) {
             if (int rs_ = hqt_sync_streams(c)) return rs_;
-            return download_as(out, t.p, (size_t)c->npts, t.dt);
+            return download_vol(c, out, t.p, t.dt);
         }
     }
     return fail(HQT_ERR_ARG, std::string("unknown field ") + name);

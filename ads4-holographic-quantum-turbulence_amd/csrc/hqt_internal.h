@@ -40,6 +40,9 @@ struct hqt_ctx {
     hqt_params p{};
     int nx = 0, ny = 0, nz = 0;
     int64_t nxy = 0, npts = 0;
+    // volume fields: planes ps elements apart (nxy rounded up to a multiple of 32, so every z-plane
+    // starts on a 128-byte boundary); nvol = ps * nz elements per field
+    int64_t ps = 0, nvol = 0;
     int fp32 = 0;
     int tdt = DT_F64;
     size_t es = 8;

@@ -112,7 +112,19 @@ struct LPass {
     int32_t Kp, Z;             // folded form: padded folded positions, table offset
     int32_t xcd;               // folded form: block order of the launch (1: XCD-aware), set by the autotuner
     int32_t* flags;            // [0] = NaN step (0: none), [1] = step counter
+    int32_t ps, pad;           // plane stride of the volume fields (elements) and ps - nx*ny
 };
+
+// Volume fields are [nx, ny, nz] with planes ps >= nx*ny elements apart (ps a multiple of 32, so
+// that every z-plane starts on a 128-byte boundary in FP64 and FP32).  Element offset of x-line
+// `line` = y + ny*z:
+__host__ __device__ inline int xline_base(const LPass& P, int line) { return line * P.nx + (line / P.ny) * P.pad; }
+// The same for line = line0 + d (d >= 0, a few tiles at most) given z0 = line0 / ny: no division.
+__host__ __device__ inline int xline_base_at(const LPass& P, int line0, int z0, int d) {
+    int r = line0 + d - z0 * P.ny, z = z0;
+    while (r >= P.ny) { r -= P.ny; ++z; }
+    return (line0 + d) * P.nx + z * P.pad;
+}
 
 // ---------------------------------------------------------------------------------------
 // Radial (z) kernels: one thread per boundary-plane column.
@@ -129,6 +141,7 @@ struct ZOps {
 
 struct ZStage {
     int64_t nxy; int32_t nz; int32_t stage; int32_t mode; // mode 0: RK stage, 1: write F1..F4
+    int64_t ps;                               // plane stride of the volume fields (elements)
     double h;
     ZOps zo;
     const void *sR, *sI, *sAx, *sAy;          // stage state (TS)
@@ -137,7 +150,7 @@ struct ZStage {
     const void *dxR, *dxI, *dyR, *dyI, *lapR, *lapI, *divA, *curl3, *curl4;  // (TS)
     void *zR, *zI, *zAx, *zAy;                // step-start state (TS), updated at stage 4
     void *aR, *aI, *aAx, *aAy;                // RK accumulators (TS)
-    double *rho, *arho;                       // rho [nxy*nz], boundary-flux accumulator [nxy]
+    double *rho, *arho;                       // rho [ps*nz], boundary-flux accumulator [nxy]
     void *nR, *nI, *nAx, *nAy;                // next stage state (TS)
     double *F1R, *F1I, *F2, *F3, *F4;         // mode 1 outputs
     int32_t* flags;
@@ -145,10 +158,11 @@ struct ZStage {
 
 struct ZPost {
     int64_t nxy; int32_t nz; double mu;
+    int64_t ps;                   // plane stride of the volume fields (elements)
     ZOps zo;
     void *R, *I, *Ax, *Ay;        // state (TS): read Psi, reset z=0 rows
     const void *divA, *divB;      // d_x A_x and d_y A_y of the updated state (TS); summed here
-    const double* rho;            // [nxy*nz]
+    const double* rho;            // [ps*nz]
     double* At;                   // out
     const double *axb, *ayb;      // [nxy] boundary values
     double* O;                    // [2*nxy] boundary order parameter (re, im)
@@ -157,12 +171,14 @@ struct ZPost {
 
 struct ZPrep {
     int64_t nxy; int32_t nz;
+    int64_t ps;                     // plane stride of the volume fields (elements)
     const double* Lz; const double* At; double* dzAt;
     int32_t* flags; int32_t bump;   // bump: increment the step counter
 };
 
 struct Wind {
     int32_t nx, ny, nz; int64_t nxy;
+    int64_t ps;                      // plane stride of the volume fields (elements)
     const double* O; double* part;   // [ny] partial sums
     void *R, *I;                     // Psi (TS)
     const double* yph;               // [ny] 2*pi*yy/L
@@ -194,6 +210,7 @@ struct NewtonJob {
 // Boundary condensate and plaquette census of one state.
 struct Census {
     int64_t nxy; int32_t nx, ny, nz;
+    int64_t ps;               // plane stride of the volume fields (elements)
     const void *R, *I;        // Psi (TS), [nx*ny*nz]
     const double* Lzb;        // [nz] last row of Lz (the z=0 row)
     double* O;                // [2*nx*ny] interleaved complex
@@ -214,6 +231,8 @@ hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, 
 size_t line_pass_lds_bytes(const LPass& P, int fp32);
 // Radial kernels: grid.x per member = zblocks(nxy); nz is shared by the members of one launch.
 inline int zblocks(int64_t nxy) { return (int)((nxy + 63) / 64); }
+// Plane stride of the volume fields for nxy columns: a multiple of 32 elements (128 bytes in FP32).
+inline int64_t plane_stride(int64_t nxy) { return (nxy + 31) & ~(int64_t)31; }
 // wpe: 2 or 3 waves per SIMD targeted by the register allocation (see k_zstage).
 hipError_t launch_zstage(const ZStage* d, Members M, int blocks, int nz, int fp32, int wpe, hipStream_t s);
 hipError_t launch_zpost(const ZPost* d, Members M, int blocks, int nz, int fp32, hipStream_t s);

@@ -152,7 +152,7 @@ template <typename T> constexpr int LP1_WPE = sizeof(T) == 8 ? 2 : 4;         //
 // Bytes of one [nx, ny, nz] field of element type S (the buffer bound of every line-pass access).
 template <typename S>
 __device__ __forceinline__ int64_t fbytes(const LPass& P) {
-    return (int64_t)P.nx * P.ny * P.nz * (int64_t)sizeof(S);
+    return (int64_t)P.ps * P.nz * (int64_t)sizeof(S);
 }
 
 // Outputs of a finished line tile: out = acc (+ addend).  The addends are all loaded before the
@@ -164,12 +164,19 @@ __device__ __forceinline__ bool tile_out(const LPass& P, const LOut& O, typename
     const int ln = lane & 15, lk = lane >> 4;
     // element index of accumulator (u, q, r) or Z_OOB outside the grid (32-bit: fields hold
     // fewer than 2^23 elements)
-    const int l0 = (int)line0, nlv = (int)nl, nx = P.nx, pl = P.nx * P.ny * z;
+    const int l0 = (int)line0, nlv = (int)nl, nx = P.nx, pl = P.ps * z;
+    // x-lines: element offsets of the 4 NT lines this lane stores
+    int xb[NT][4];
+    const int z0 = DIR == 0 ? l0 / P.ny : 0;
+#pragma unroll
+    for (int u = 0; u < NT; ++u)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) xb[u][r] = DIR == 0 ? xline_base_at(P, l0, z0, 16 * u + MF<T>::row(lk, r)) : 0;
     auto at = [&](int u, int q, int r) -> uint32_t {
         const int i0 = (wave + 4 * q) * 16;
         if (DIR == 0) {
             const int line = l0 + 16 * u + MF<T>::row(lk, r), i = i0 + ln;
-            return (line < nlv && i < N) ? (uint32_t)(line * nx + i) : Z_OOB;
+            return (line < nlv && i < N) ? (uint32_t)(xb[u][r] + i) : Z_OOB;
         } else {
             const int x = x0 + 16 * u + ln, i = i0 + MF<T>::row(lk, r);
             return (x < nx && i < N) ? (uint32_t)(x + nx * i + pl) : Z_OOB;
@@ -291,13 +298,20 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(LP_W
             // LB lines per tile; with LB = 32 a thread covers lines mrow and mrow + 16
             const int64_t line = (int64_t)lt * LB + mrow;
             okm = line < nl;
-            base = line * P.nx;
+            base = 0;                    // x-line offsets come from xbh below
             step = 0;
         } else {
             const int x = (lt % P.nxc) * LB + mrow, zz = lt / P.nxc;
             okm = x < P.nx;
-            base = (int64_t)x + (int64_t)P.nx * P.ny * zz;
+            base = (int64_t)x + (int64_t)P.ps * zz;
             step = (int64_t)P.nx;
+        }
+        // x-lines: element offsets of this thread's NT lines
+        int xbh[NT];
+        if (DIR == 0) {
+            const int l0 = lt * LB, z0 = l0 / P.ny;
+#pragma unroll
+            for (int u = 0; u < NT; ++u) xbh[u] = xline_base_at(P, l0, z0, mrow + 16 * u);
         }
         // out-of-grid elements read 0 through the buffer bounds check (no branches around the loads)
         auto load = [&](auto tag) {
@@ -310,7 +324,7 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(LP_W
                     const int half = r / (PF / NT), rr = r % (PF / NT);     // NT = 2: second line at +16
                     const int j = jt + JS * rr;
                     const int line = lt * LB + mrow + 16 * half;
-                    e = (line < (int)nl && j < N) ? (uint32_t)(line * P.nx + j) : Z_OOB;
+                    e = (line < (int)nl && j < N) ? (uint32_t)(xbh[half] + j) : Z_OOB;
                 } else {
                     const int j = jt + JS * r;
                     e = (okm && j < N) ? (uint32_t)((int)base + (int)step * j) : Z_OOB;
@@ -488,8 +502,14 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(LP1_
             step = 1;
         } else {
             okm = x0 + mrow < P.nx;
-            base = (int64_t)(x0 + mrow) + (int64_t)P.nx * P.ny * z;
+            base = (int64_t)(x0 + mrow) + (int64_t)P.ps * z;
             step = (int64_t)P.nx;
+        }
+        int xbh[NT];
+        if (DIR == 0) {
+            const int l0 = (int)line0, z0 = l0 / P.ny;
+#pragma unroll
+            for (int u = 0; u < NT; ++u) xbh[u] = xline_base_at(P, l0, z0, mrow + 16 * u);
         }
         auto load = [&](auto tag) {
             using S_ = decltype(tag);
@@ -501,7 +521,7 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(LP1_
                     const int half = r / (PF / NT), rr = r % (PF / NT);
                     const int j = jt + JS * rr;
                     const int line = (int)line0 + mrow + 16 * half;
-                    e = (line < (int)nl && j < N) ? (uint32_t)(line * P.nx + j) : Z_OOB;
+                    e = (line < (int)nl && j < N) ? (uint32_t)(xbh[half] + j) : Z_OOB;
                 } else {
                     const int j = jt + JS * r;
                     e = (okm && j < N) ? (uint32_t)((int)base + (int)step * j) : Z_OOB;

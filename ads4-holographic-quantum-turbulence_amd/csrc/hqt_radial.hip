@@ -46,9 +46,10 @@ __global__ __launch_bounds__(256) void k_wind_apply(const Wind* __restrict__ Ap,
     const double W = round((tot / A.ny) / (2.0 * M_PI));
     if (mb.bx == 0 && threadIdx.x == 0) A.W[0] = W;
     if (W == 0.0) return;
-    const int64_t total = A.nxy * A.nz;
+    const int64_t total = A.ps * A.nz;
     for (int64_t e = (int64_t)mb.bx * 256 + threadIdx.x; e < total; e += (int64_t)mb.nb * 256) {
-        const int64_t c = e % A.nxy;
+        const int64_t c = e % A.ps;
+        if (c >= A.nxy) continue;        // plane padding
         const int y = (int)(c / A.nx);
         double sn, cs;
         sincos(-A.yph[y] * W, &sn, &cs);

@@ -104,7 +104,7 @@ __device__ __forceinline__ void mvl(const double* T, int lane, const double (&in
     }
 }
 
-// Load a [nxy, nz] field into slots (0 outside the grid).
+// Load a field of plane stride n into slots (0 outside the grid).
 template <int NZ, typename TS>
 __device__ __forceinline__ void load_slots(const void* f, int64_t c, bool cok, int g, int64_t n, double (&v)[ZG<NZ>::KS]) {
 #pragma unroll
@@ -187,7 +187,7 @@ __device__ __forceinline__ void zstage_body(const ZStage& A, const ZLds<NZ>& Z, 
                                             const uint32_t (&e)[ZG<NZ>::KS], double (&sR)[ZG<NZ>::KS],
                                             double (&sI)[ZG<NZ>::KS], double (&dv)[ZG<NZ>::KS]) {
     constexpr int KS = ZG<NZ>::KS;
-    const int64_t bt = A.nxy * NZ * (int64_t)sizeof(TS), b8 = A.nxy * NZ * 8;
+    const int64_t bt = A.ps * NZ * (int64_t)sizeof(TS), b8 = A.ps * NZ * 8;
     const int stage = A.stage;
     const double h = A.h;
     double w[KS], u[KS], p1r[KS], p1i[KS], p3[KS], p4[KS];
@@ -378,10 +378,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
         const int k = 4 * s + g;
-        e[s] = (cok && k < NZ) ? (uint32_t)(c + A.nxy * k) : Z_OOB;
+        e[s] = (cok && k < NZ) ? (uint32_t)(c + A.ps * k) : Z_OOB;
     }
     // first operands in flight before the operator staging
-    const int64_t bt = A.nxy * NZ * (int64_t)sizeof(TS);
+    const int64_t bt = A.ps * NZ * (int64_t)sizeof(TS);
     const Rs rsr = rs_make(A.sR, bt), rsi = rs_make(A.sI, bt), rdv = rs_make(A.divA, bt);
     double sR[KS], sI[KS], dv[KS];
 #pragma unroll
@@ -416,7 +416,7 @@ __global__ __launch_bounds__(256) void k_zpost(const ZPost* __restrict__ Ap, con
     const int m = lane & 15, g = lane >> 4;
     const int64_t c = (int64_t)mb.bx * 64 + wave * 16 + m;
     const bool cok = c < A.nxy;
-    const int64_t n = A.nxy;
+    const int64_t n = A.ps;
     double Lz[ZG<NZ>::RT][KS], L1[ZG<NZ>::RT][KS];
     load_op<NZ>(A.zo.Lz, NZ, NZ, Lz, m, g);
     load_op<NZ>(A.zo.Lz1, NZ - 1, NZ - 1, L1, m, g);
@@ -484,12 +484,12 @@ __global__ __launch_bounds__(256) void k_zprep(const ZPrep* __restrict__ Ap, con
     const bool cok = c < A.nxy;
     double Lz[ZG<NZ>::RT][ZG<NZ>::KS], a[ZG<NZ>::KS], d[ZG<NZ>::KS];
     load_op<NZ>(A.Lz, NZ, NZ, Lz, m, g);
-    load_slots<NZ, double>(A.At, c, cok, g, A.nxy, a);
+    load_slots<NZ, double>(A.At, c, cok, g, A.ps, a);
     mv<NZ>(Lz, a, d);
 #pragma unroll
     for (int s = 0; s < ZG<NZ>::KS; ++s) {
         const int k = 4 * s + g;
-        if (cok && k < NZ) A.dzAt[c + A.nxy * k] = d[s];
+        if (cok && k < NZ) A.dzAt[c + A.ps * k] = d[s];
     }
 }
 

@@ -241,8 +241,9 @@ def test_continue_mu_stays_on_superfluid_branch():
 
 def test_footprint_formula_scales_with_points():
     """hqt_ctx_footprint (host only): 42 state-precision arrays plus 4 FP64 arrays per grid
-    point, plus per-plane, per-line and constant terms, plus the filter's cross-term partials (5 fields
-    x one 16-line block of x-lines x 32 doubles); mixed precision halves the first part."""
+    point (each z-plane padded to a multiple of 32 points; 64 x 64 needs none), plus per-plane,
+    per-line and constant terms, plus the filter's cross-term partials (5 fields x one 16-line
+    block of x-lines x 32 doubles); mixed precision halves the first part."""
     import hqt
     a = hqt.footprint(64, 64, 12, "fp64")
     b = hqt.footprint(64, 64, 24, "fp64")

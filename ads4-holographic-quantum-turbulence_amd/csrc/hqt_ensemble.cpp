@@ -3,10 +3,10 @@
 // class (same template instance: precision and Q = Mp/64 for the line passes, nz for the
 // radial kernels) over all members of that class; a member owns a contiguous block range of
 // the launch and reads its own descriptor (buffers, sizes, flags) from a device array.  The
-// whole step is captured once into a hipGraph and replayed per step.  The launches of one step
-// phase (one kernel of the step sequence, one launch per class) touch disjoint members, so they
-// run on up to kLanes streams at once (fork/join through events around each phase), which
-// overlaps one class's tail with the next class's blocks.
+// whole step is captured once into a hipGraph and replayed per step.  The members are split into
+// up to kLanes lanes of about equal point counts; each lane runs its own class launches on its own
+// stream (the lanes share no member, so they meet only at the fork and the join around the step),
+// which overlaps one lane's small or tail launches with another lane's work.
 //
 // The per-member arithmetic is that of hqt_step_n on the same context (same kernels, same
 // operands), so results are identical to stepping each member alone.
@@ -25,7 +25,7 @@ enum LKind { K_LINE, K_ZPREP, K_ZSTAGE, K_ZPOST, K_WPART, K_WAPPLY, K_NAN };
 struct Launch {
     int kind = 0;
     int dir = 0, lkind = 0, Q = 1, nt = 1, ngroups = 1, nz = 0, fp32 = 0, wpe = 2, blocks = 0, nmem = 0, fold = 0, order = 0;
-    int phase = 0;                   // index of the step-sequence kernel this launch belongs to
+    int lane = 0;                    // stream: 0 = the ensemble's own, l > 0 = side stream l - 1
     size_t lds = 0;
     size_t desc_off = 0, boff_off = 0;   // offsets into the device plan buffer
 };
@@ -41,7 +41,7 @@ struct PlanImage {
     }
 };
 
-constexpr int kLanes = 4;            // streams per phase: the ensemble's own + 3 side streams
+constexpr int kLanes = 4;            // the ensemble's own stream + 3 side streams
 
 }  // namespace
 
@@ -50,6 +50,7 @@ struct hqt_ensemble {
     hipStream_t stream = nullptr;
     hipStream_t side[kLanes - 1] = {};
     hipEvent_t fork = nullptr, join[kLanes - 1] = {};
+    int lanes = 1;                   // lanes in use
     std::vector<Launch> plan;
     unsigned char* dplan = nullptr;
     hipGraphExec_t graph = nullptr;
@@ -96,11 +97,9 @@ void add_line(std::vector<Launch>& plan, PlanImage& img, const std::vector<hqt_c
             return std::make_tuple(c->fp32, c->hp[pass].Mp / 64, c->hp[pass].lb / 16, c->hp[pass].fold);
         },
         [&](hqt_ctx* c) { return use(c); });
-    const int phase = plan.empty() ? 0 : plan.back().phase + 1;
     for (auto& kv : cls) {
         Launch L = make_launch<LPass>(img, K_LINE, kv.second, [&](hqt_ctx* c) { return c->hp[pass]; },
                                       [&](hqt_ctx* c) { return line_blocks(c->hp[pass]); });
-        L.phase = phase;
         L.dir = dir;
         L.fp32 = std::get<0>(kv.first);
         L.Q = std::get<1>(kv.first);
@@ -130,10 +129,8 @@ void add_z(std::vector<Launch>& plan, PlanImage& img, const std::vector<hqt_ctx*
     auto cls = classes<std::tuple<int, int, int>>(
         mem, [&](hqt_ctx* c) { return std::make_tuple(c->fp32, by_nz ? c->nz : 0, kind == K_ZSTAGE ? c->zwpe : 2); },
         [&](hqt_ctx* c) { return use(c); });
-    const int phase = plan.empty() ? 0 : plan.back().phase + 1;
     for (auto& kv : cls) {
         Launch L = make_launch<D>(img, kind, kv.second, desc, blocks);
-        L.phase = phase;
         L.fp32 = std::get<0>(kv.first);
         L.nz = std::get<1>(kv.first);
         L.wpe = std::get<2>(kv.first);
@@ -141,10 +138,10 @@ void add_z(std::vector<Launch>& plan, PlanImage& img, const std::vector<hqt_ctx*
     }
 }
 
-// The step sequence of enqueue_step (hqt_host.cpp), class by class.
-void build_plan(hqt_ensemble* e, PlanImage& img) {
+// The step sequence of enqueue_step (hqt_host.cpp), class by class, for the members M of one lane.
+void build_plan(hqt_ensemble* e, PlanImage& img, const std::vector<hqt_ctx*>& M, int lane) {
     auto& P = e->plan;
-    const auto& M = e->mem;
+    const size_t first = P.size();
     auto zb = [](hqt_ctx* c) { return zblocks(c->nxy); };
     add_z<ZPrep>(P, img, M, K_ZPREP, true, all_members, [](hqt_ctx* c) { return c->zpre; }, zb);
     for (int s = 0; s < 4; ++s) {
@@ -163,6 +160,23 @@ void build_plan(hqt_ensemble* e, PlanImage& img) {
     add_line(P, img, M, PYF, 1, KN_LY_FILT, with_filter);
     add_z<NanCheck>(P, img, M, K_NAN, false, without_filter, [](hqt_ctx* c) { return c->nan; },
                     [](hqt_ctx* c) { return nancheck_blocks(c->nvol); });
+    for (size_t i = first; i < P.size(); ++i) P[i].lane = lane;
+}
+
+// Lanes: the members by decreasing point count, each to the lane with the fewest points so far.
+void build_lanes(hqt_ensemble* e, PlanImage& img) {
+    std::vector<hqt_ctx*> order(e->mem);
+    std::stable_sort(order.begin(), order.end(), [](const hqt_ctx* a, const hqt_ctx* b) { return a->npts > b->npts; });
+    const int nl = std::min<int>(kLanes, (int)order.size());
+    std::vector<std::vector<hqt_ctx*>> lane(nl);
+    std::vector<int64_t> load(nl, 0);
+    for (hqt_ctx* c : order) {
+        const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        lane[l].push_back(c);
+        load[l] += c->npts;
+    }
+    e->lanes = nl;
+    for (int l = 0; l < nl; ++l) build_plan(e, img, lane[l], l);
 }
 
 hipError_t issue(const hqt_ensemble* e, const Launch& L, hipStream_t s) {
@@ -183,27 +197,18 @@ hipError_t issue(const hqt_ensemble* e, const Launch& L, hipStream_t s) {
 }
 
 int enqueue(const hqt_ensemble* e) {
-    const auto& P = e->plan;
-    for (size_t i = 0; i < P.size();) {
-        size_t j = i;
-        while (j < P.size() && P[j].phase == P[i].phase) ++j;
-        const int n = (int)(j - i), lanes = std::min(n, kLanes);
-        hipError_t r = hipSuccess;
-        if (lanes > 1) {
-            r = hipEventRecord(e->fork, e->stream);
-            for (int l = 1; l < lanes && r == hipSuccess; ++l) r = hipStreamWaitEvent(e->side[l - 1], e->fork, 0);
-        }
-        for (int k = 0; k < n && r == hipSuccess; ++k) {
-            const int l = k % lanes;
-            r = issue(e, P[i + k], l == 0 ? e->stream : e->side[l - 1]);
-        }
-        for (int l = 1; l < lanes && r == hipSuccess; ++l) {
-            r = hipEventRecord(e->join[l - 1], e->side[l - 1]);
-            if (r == hipSuccess) r = hipStreamWaitEvent(e->stream, e->join[l - 1], 0);
-        }
-        if (r != hipSuccess) return efail(HQT_ERR_HIP, std::string("ensemble launch: ") + hipGetErrorString(r));
-        i = j;
+    auto lane = [&](int l) { return l == 0 ? e->stream : e->side[l - 1]; };
+    hipError_t r = hipSuccess;
+    if (e->lanes > 1) {
+        r = hipEventRecord(e->fork, e->stream);
+        for (int l = 1; l < e->lanes && r == hipSuccess; ++l) r = hipStreamWaitEvent(e->side[l - 1], e->fork, 0);
     }
+    for (size_t k = 0; k < e->plan.size() && r == hipSuccess; ++k) r = issue(e, e->plan[k], lane(e->plan[k].lane));
+    for (int l = 1; l < e->lanes && r == hipSuccess; ++l) {
+        r = hipEventRecord(e->join[l - 1], e->side[l - 1]);
+        if (r == hipSuccess) r = hipStreamWaitEvent(e->stream, e->join[l - 1], 0);
+    }
+    if (r != hipSuccess) return efail(HQT_ERR_HIP, std::string("ensemble launch: ") + hipGetErrorString(r));
     return HQT_OK;
 }
 
@@ -265,7 +270,7 @@ int hqt_ensemble_create(hqt_ctx* const* members, int32_t n, int32_t flags, hqt_e
         e->fork = nullptr; release(e); return efail(HQT_ERR_HIP, "ensemble fork event");
     }
     PlanImage img;
-    build_plan(e, img);
+    build_lanes(e, img);
     r = hipMalloc(&e->dplan, img.bytes.size());
     if (r != hipSuccess) { e->dplan = nullptr; release(e); return efail(HQT_ERR_OOM, "hipMalloc(ensemble plan)"); }
     r = hipMemcpy(e->dplan, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice);

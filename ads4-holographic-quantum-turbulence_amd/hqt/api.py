@@ -406,8 +406,9 @@ class Stepper:
 
 
 class Ensemble:
-    """Independent evolutions stepped together (``hqt_ensemble``): one launch per kernel class
-    per phase over all members, the whole step replayed from one captured hipGraph.
+    """Independent evolutions stepped together (``hqt_ensemble``): the members split into four
+    lanes, each lane one launch per kernel class of the step on its own stream, the whole step
+    replayed from one captured hipGraph.
 
     ``steppers`` are :class:`Stepper` objects with their state set; they may differ in grid,
     nz, mu, drive, switches and precision.  Each member's result equals stepping it alone.

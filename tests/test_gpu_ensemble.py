@@ -57,9 +57,9 @@ def test_ensemble_equals_members_alone(graph):
 
 
 def test_ensemble_many_members_per_class():
-    """Twelve members, two or more per kernel class: the class launches of one kernel run on
-    several streams at once between the phase's fork and join.  Still bitwise equal to each member
-    stepped alone, over repeated graph replays."""
+    """Twelve members, two or more per kernel class and lane: the four lanes run on their own
+    streams between the step's fork and join.  Still bitwise equal to each member stepped alone,
+    over repeated graph replays."""
     import hqt
     specs = MEMBERS + MEMBERS[::-1]
     nsteps = 7

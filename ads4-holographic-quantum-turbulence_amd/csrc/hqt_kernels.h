@@ -117,9 +117,8 @@ struct LPass {
 
 // Volume fields are [nx, ny, nz] with planes ps >= nx*ny elements apart (ps a multiple of 32, so
 // that every z-plane starts on a 128-byte boundary in FP64 and FP32).  Element offset of x-line
-// `line` = y + ny*z:
-__host__ __device__ inline int xline_base(const LPass& P, int line) { return line * P.nx + (line / P.ny) * P.pad; }
-// The same for line = line0 + d (d >= 0, a few tiles at most) given z0 = line0 / ny: no division.
+// line0 + d (line = y + ny*z; d >= 0, a few tiles at most) given z0 = line0 / ny, without a
+// division: line*nx + z*(ps - nx*ny).
 __host__ __device__ inline int xline_base_at(const LPass& P, int line0, int z0, int d) {
     int r = line0 + d - z0 * P.ny, z = z0;
     while (r >= P.ny) { r -= P.ny; ++z; }

@@ -23,6 +23,30 @@ def _ic(nx, ny, nz, seed=3, nv=4):
     return ic.initial_state(nx, ny, nz, L, L, mu=6.0, seed=seed, n_vortices=nv)
 
 
+def test_config1_200_steps_match_golden():
+    """The 64 x 64 x 12 workload at its full length (200 RK4 steps, L = 50, mu = 6, dt = 0.02,
+    4 windings, winding removal on) against the oracle's trajectory committed as a sampled fixture
+    (tests/golden/make_config1.py): every 7th point of every field within 1e-10 of the field's
+    max, and the same winding number removed at the last step."""
+    import hqt
+    from hqt import ic
+    g = np.load(os.path.join(GOLDEN, "config1_200steps_64x64x12.npz"))
+    st0 = ic.initial_state(64, 64, 12, L, L, mu=6.0, seed=1, n_vortices=4, phase_noise=1e-3)
+    print("config-1 200 steps 64x64x12", flush=True)
+    with hqt.Stepper(64, 64, 12, L, L, mu=6.0, dt=0.02, ax_b=0.0, ay_b=0.0) as st:
+        st.set_state(*st0)
+        st.step(int(g["steps"]))
+        z1, z2, z3, z4, rh = st.get_state()
+        W = st.last_winding
+    k = int(g["stride"])
+    for name, v in (("ReZ1", z1.real), ("ImZ1", z1.imag), ("Z2", z2), ("Z3", z3), ("Z4", z4), ("rhoi", rh)):
+        got = np.asarray(v).reshape(-1, order="F")[::k]
+        e = float(np.max(np.abs(got - g[name])) / float(g[name + "_max"]))
+        print(f"  {name}: {e:.3e}", flush=True)
+        assert e < 1e-10, (name, e)
+    assert W == float(g["W"])
+
+
 def test_golden_fixture():
     import hqt
     g = np.load(os.path.join(GOLDEN, "golden_step_12x10x6.npz"))

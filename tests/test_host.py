@@ -192,6 +192,21 @@ def test_zero_flow_profile_is_superfluid_and_stationary():
     assert max(float(np.max(np.abs(f))) for f in F) < 1e-5
 
 
+def test_config1_fixture_is_sampled_oracle_state():
+    """The 200-step fixture's samples are consistent with its own maxima and stride (a quick check that
+    the file holds what make_config1.py writes; the oracle run itself takes about a minute)."""
+    g = np.load(os.path.join(GOLDEN, "config1_200steps_64x64x12.npz"))
+    n = 64 * 64 * 12
+    k = int(g["stride"])
+    for name in ("ReZ1", "ImZ1", "Z2", "Z3", "Z4", "rhoi"):
+        assert g[name].shape == ((n + k - 1) // k,)
+        assert np.all(np.isfinite(g[name])) and np.max(np.abs(g[name])) <= float(g[name + "_max"])
+    # A_t = mu at the boundary row (k = nz) of every column after the constraint solve
+    idx = np.arange(0, n, k)
+    top = idx >= 64 * 64 * 11
+    assert np.max(np.abs(g["Z2"][top] - 6.0)) < 1e-12
+
+
 def test_golden_fixture_reproduced_by_oracle():
     """The committed golden trajectory is what the oracle computes (pins oracle drift)."""
     from oracle import hqt_oracle as O

@@ -102,6 +102,32 @@ def test_twenty_steps_match_oracle():
         assert e < 1e-10, (name, e)
 
 
+def test_sixty_steps_match_oracle_padded_planes():
+    """60 steps of a 36 x 28 x 10 grid under a sinusoidal A_x drive: nx*ny = 1008 is not a multiple
+    of 32, so the device planes carry 16 padding points each and 16-line x-tiles straddle slices.
+    Winding removal is off: this seeded state's mean winding sits at -0.5 every other step (the
+    oracle removes -1, +1, -1, ...), a tie that rounding decides, not the kernels."""
+    print("60-step parity 36x28x10", flush=True)
+    hqt, O = _hqt(), _oracle()
+    from hqt import ic
+    nx, ny, nz = 36, 28, 10
+    z1, z2, z3, z4, rh = _state(nx, ny, nz, seed=13, nv=4)
+    axb = ic.sinusoidal_drive(nx, ny, L, kk=3, amp=0.1)
+    prm = O.Params(mu=6.0, h=0.02, ax_b=axb, ay_b=0.0, winding=False)
+    with hqt.Stepper(nx, ny, nz, L, L, mu=6.0, dt=0.02, ax_b=axb, ay_b=0.0, winding=False) as st:
+        st.set_state(z1, z2, z3, z4, rh)
+        st.step(60)
+        G = st.get_state()
+    ops = O.Operators(nx, ny, nz, L, L)
+    S = [z1, z2, z3, z4, rh]
+    for _ in range(60):
+        *S, _w = O.step(ops, prm, *S)
+    for name, a, b in zip(("Z1", "Z2", "Z3", "Z4", "rhoi"), G, S):
+        e = rel_err(a, b)
+        print(f"  {name}: {e:.3e}", flush=True)
+        assert e < 1e-10, (name, e)
+
+
 @pytest.mark.parametrize("nx,ny,ns,cplx", [(17, 19, 3, True), (64, 64, 2, False), (301, 301, 2, False), (33, 16, 4, True)])
 def test_ff_matches_oracle(nx, ny, ns, cplx):
     print(f"ff parity {nx}x{ny}x{ns} complex={cplx}", flush=True)

@@ -143,7 +143,6 @@ void build_plan(hqt_ensemble* e, PlanImage& img, const std::vector<hqt_ctx*>& M,
     auto& P = e->plan;
     const size_t first = P.size();
     auto zb = [](hqt_ctx* c) { return zblocks(c->nxy); };
-    add_z<ZPrep>(P, img, M, K_ZPREP, true, all_members, [](hqt_ctx* c) { return c->zpre; }, zb);
     for (int s = 0; s < 4; ++s) {
         add_line(P, img, M, PX1 + s, 0, KN_LX_STAGE, all_members);
         add_line(P, img, M, PY1 + s, 1, KN_LY_STAGE, all_members);

@@ -46,7 +46,7 @@ int64_t ctx_footprint(const hqt_params& p) {
     auto r8 = [](int64_t b) { return b == 0 ? 8 : b; };
     int64_t b = 0;
     b += 42 * np * es;                                   // state, stage, derivative and filter buffers (TS)
-    b += 4 * np * 8;                                     // At, rho, dzAt, AtR
+    b += 3 * np * 8;                                     // At, rho, AtR
     b += nxy * 8 + 2 * nxy * 8 + r8(p.ny * 8) + 8;       // arho, O, part, W
     b += 2 * nxy * 8 + r8(p.ny * 8) + 16;                // axb, ayb, yph, flags
     b += (int64_t)NPASS * (int64_t)sizeof(LPass) + (int64_t)sizeof(hqt_ctx::DevDesc);
@@ -408,7 +408,6 @@ int enqueue_step(hqt_ctx* c) {
     int rc;
     const int zb = zblocks(c->nxy), nz = c->nz, f32 = c->fp32;
     hipStream_t st = c->stream;
-    if ((rc = run(c, KN_ZPREP, [&] { return launch_zprep(&c->dd->zpre[0], kOne, zb, nz, st); }))) return rc;
     for (int s = 0; s < 4; ++s) {
         if ((rc = line(c, PX1 + s, 0, KN_LX_STAGE))) return rc;
         if ((rc = line(c, PY1 + s, 1, KN_LY_STAGE))) return rc;
@@ -459,7 +458,7 @@ int build_context(hqt_ctx* c) {
     ALLOC(c->divA, np * es); ALLOC(c->curl3, np * es); ALLOC(c->curl4, np * es);
     ALLOC(c->dxAt, np * es); ALLOC(c->dyAt, np * es);
     for (int i = 0; i < 5; ++i) { ALLOC(c->fa[i], np * es); ALLOC(c->fb[i], np * es); }
-    ALLOC(c->At, np * 8); ALLOC(c->rho, np * 8); ALLOC(c->dzAt, np * 8); ALLOC(c->AtR, np * 8);
+    ALLOC(c->At, np * 8); ALLOC(c->rho, np * 8); ALLOC(c->AtR, np * 8);
     ALLOC(c->arho, nxy * 8); ALLOC(c->O, 2 * nxy * 8); ALLOC(c->part, (size_t)ny * 8); ALLOC(c->W, 8);
     ALLOC(c->axb, nxy * 8); ALLOC(c->ayb, nxy * 8); ALLOC(c->yph, (size_t)ny * 8);
     ALLOC(c->flags, 4 * sizeof(int32_t));
@@ -868,7 +867,6 @@ int hqt_rhs(hqt_ctx* c, const double* Z1, const double* Z2, const double* Z3, co
     HIPC(hipMemcpy(saved, c->flags, sizeof(saved), hipMemcpyDeviceToHost));
     HIPC(hipMemset(c->flags, 0, sizeof(saved)));
     const int zb = zblocks(c->nxy);
-    if ((rc = run(c, KN_ZPREP, [&] { return launch_zprep(&c->dd->zpre[1], kOne, zb, c->nz, c->stream); }))) return rc;
     if ((rc = line(c, PXR, 0, KN_LX_RHS))) return rc;
     if ((rc = line(c, PYR, 1, KN_LY_RHS))) return rc;
     if ((rc = run(c, KN_ZRHS, [&] { return launch_zstage(&c->dd->zs[4], kOne, zb, c->nz, c->fp32, c->zwpe, c->stream); }))) return rc;
@@ -1054,7 +1052,7 @@ int hqt_debug_field(hqt_ctx* c, const char* name, double* out) {
         {"dxAx", c->dxAx, c->tdt}, {"dxAy", c->dxAy, c->tdt}, {"dxxAy", c->dxxAy, c->tdt},
         {"dyR", c->dyR, c->tdt}, {"dyI", c->dyI, c->tdt}, {"lapR", c->lapR, c->tdt}, {"lapI", c->lapI, c->tdt},
         {"divA", c->divA, c->tdt}, {"curl3", c->curl3, c->tdt}, {"curl4", c->curl4, c->tdt},
-        {"dxAt", c->dxAt, c->tdt}, {"dyAt", c->dyAt, c->tdt}, {"dzAt", c->dzAt, DT_F64}};
+        {"dxAt", c->dxAt, c->tdt}, {"dyAt", c->dyAt, c->tdt}};
     for (auto& t : tab) {
         if (std::strcmp(t.n, name) == 0) {
             if (int rs_ = hqt_sync_streams(c)) return rs_;

@@ -144,7 +144,7 @@ struct ZStage {
     double h;
     ZOps zo;
     const void *sR, *sI, *sAx, *sAy;          // stage state (TS)
-    const double *At, *dzAt;                  // frozen A_t and d_z A_t
+    const double *At, *dzAt;                  // frozen A_t (d_z A_t is formed in the kernel; dzAt unused)
     const void *dxAt, *dyAt;                  // frozen x/y derivatives of A_t (TS)
     const void *dxR, *dxI, *dyR, *dyI, *lapR, *lapI, *divA, *curl3, *curl4;  // (TS)
     void *zR, *zI, *zAx, *zAy;                // step-start state (TS), updated at stage 4

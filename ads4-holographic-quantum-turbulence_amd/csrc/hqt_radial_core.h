@@ -185,7 +185,8 @@ template <int NZ> struct ZLds {
 template <int NZ, typename TS, int MODE>
 __device__ __forceinline__ void zstage_body(const ZStage& A, const ZLds<NZ>& Z, int lane, int g, int64_t c, bool cok,
                                             const uint32_t (&e)[ZG<NZ>::KS], double (&sR)[ZG<NZ>::KS],
-                                            double (&sI)[ZG<NZ>::KS], double (&dv)[ZG<NZ>::KS]) {
+                                            double (&sI)[ZG<NZ>::KS], double (&dv)[ZG<NZ>::KS],
+                                            const double (&at0)[ZG<NZ>::KS]) {
     constexpr int KS = ZG<NZ>::KS;
     const int64_t bt = A.ps * NZ * (int64_t)sizeof(TS), b8 = A.ps * NZ * 8;
     const int stage = A.stage;
@@ -195,13 +196,12 @@ __device__ __forceinline__ void zstage_body(const ZStage& A, const ZLds<NZ>& Z, 
     TS ax[KS], ay[KS], lr[KS], li[KS], xr[KS], xi[KS], yr[KS], yi[KS], c3[KS], c4[KS];
     double dat[KS];
     {
-        const Rs rdat = rs_make(A.dzAt, b8), rax = rs_make(A.sAx, bt), ray = rs_make(A.sAy, bt);
+        const Rs rax = rs_make(A.sAx, bt), ray = rs_make(A.sAy, bt);
         const Rs rlr = rs_make(A.lapR, bt), rli = rs_make(A.lapI, bt);
         const Rs rxr = rs_make(A.dxR, bt), rxi = rs_make(A.dxI, bt), ryr = rs_make(A.dyR, bt), ryi = rs_make(A.dyI, bt);
         const Rs rc3 = rs_make(A.curl3, bt), rc4 = rs_make(A.curl4, bt);
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
-            dat[s] = bldr<double>(rdat, e[s]);
             ax[s] = bldr<TS>(rax, e[s]); ay[s] = bldr<TS>(ray, e[s]);
             lr[s] = bldr<TS>(rlr, e[s]); li[s] = bldr<TS>(rli, e[s]);
             xr[s] = bldr<TS>(rxr, e[s]); xi[s] = bldr<TS>(rxi, e[s]);
@@ -231,6 +231,7 @@ __device__ __forceinline__ void zstage_body(const ZStage& A, const ZLds<NZ>& Z, 
         for (int s = 0; s < KS; ++s) bst<double>(rr, e[s], bld<double>(rr, e[s]) - fin);
     }
     ZSB();
+    mvl<NZ>(Z.lz, lane, at0, dat);   // d_z A_t of the frozen A_t
     // F3 operands in flight during the point-wise pass
     TS fa[KS], fd[KS];
     {
@@ -304,13 +305,12 @@ __device__ __forceinline__ void zstage_body(const ZStage& A, const ZLds<NZ>& Z, 
     }
     // ---- F1 ----
     {
-        const Rs rat = rs_make(A.At, b8);
         const RkRs RR = rk_rs<TS>(stage, A.aR, A.zR, A.nR, bt), RI = rk_rs<TS>(stage, A.aI, A.zI, A.nI, bt);
         double atv[KS];
         TS aR0[KS], zR0[KS], aI0[KS], zI0[KS];
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
-            atv[s] = bldr<double>(rat, e[s]);
+            atv[s] = at0[s];
             if (MODE == 0) {
                 aR0[s] = bldr<TS>(RR.acc_ld, e[s]); zR0[s] = bldr<TS>(RR.z_ld, e[s]);
                 aI0[s] = bldr<TS>(RI.acc_ld, e[s]); zI0[s] = bldr<TS>(RI.z_ld, e[s]);
@@ -383,9 +383,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     // first operands in flight before the operator staging
     const int64_t bt = A.ps * NZ * (int64_t)sizeof(TS);
     const Rs rsr = rs_make(A.sR, bt), rsi = rs_make(A.sI, bt), rdv = rs_make(A.divA, bt);
-    double sR[KS], sI[KS], dv[KS];
+    double sR[KS], sI[KS], dv[KS], at0[KS];
+    const Rs rat0 = rs_make(A.At, A.ps * NZ * 8);
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
+        at0[s] = bld<double>(rat0, e[s]);
         dv[s] = bld<TS>(rdv, e[s]);
         sR[s] = bld<TS>(rsr, e[s]);
         sI[s] = bld<TS>(rsi, e[s]);
@@ -401,8 +403,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     }
     __syncthreads();
     if (stop != 0) return;           // same value for the whole grid: the block returns together
-    if (A.mode == 1) zstage_body<NZ, TS, 1>(A, Z, lane, g, c, cok, e, sR, sI, dv);
-    else zstage_body<NZ, TS, 0>(A, Z, lane, g, c, cok, e, sR, sI, dv);
+    // the step counter advances at the step's first radial stage (no kernel before it in the step
+    // reports a NaN step: the previous step's guard has already checked the state it reads)
+    if (A.mode == 0 && A.stage == 1 && mb.bx == 0 && threadIdx.x == 0) A.flags[1] = A.flags[1] + 1;
+    if (A.mode == 1) zstage_body<NZ, TS, 1>(A, Z, lane, g, c, cok, e, sR, sI, dv, at0);
+    else zstage_body<NZ, TS, 0>(A, Z, lane, g, c, cok, e, sR, sI, dv, at0);
 }
 
 // ---- post-step: A_t constraint solve and boundary reset (:129-133), O for the winding (:136)

@@ -20,7 +20,7 @@
 
 namespace {
 
-enum LKind { K_LINE, K_ZPREP, K_ZSTAGE, K_ZPOST, K_WPART, K_WAPPLY, K_NAN };
+enum LKind { K_LINE, K_ZSTAGE, K_ZPOST, K_WPART, K_WAPPLY, K_NAN };
 
 struct Launch {
     int kind = 0;
@@ -185,7 +185,6 @@ hipError_t issue(const hqt_ensemble* e, const Launch& L, hipStream_t s) {
         case K_LINE:
             return launch_line_pass(static_cast<const LPass*>(d), M, L.blocks, L.ngroups, L.lds, L.Q, L.nt, L.dir, L.fp32,
                                     L.lkind, s, L.fold, L.order);
-        case K_ZPREP: return launch_zprep(static_cast<const ZPrep*>(d), M, L.blocks, L.nz, s);
         case K_ZSTAGE: return launch_zstage(static_cast<const ZStage*>(d), M, L.blocks, L.nz, L.fp32, L.wpe, s);
         case K_ZPOST: return launch_zpost(static_cast<const ZPost*>(d), M, L.blocks, L.nz, L.fp32, s);
         case K_WPART: return launch_wind_partial(static_cast<const Wind*>(d), M, L.blocks, s);

@@ -2,9 +2,9 @@
 // sequence of one RK4 step, and the stand-alone operators.
 //
 // One step (tevol_driven_cluster.m:120-142) is queued as:
-//   zprep                          d_z A_t of the frozen A_t (A_t is held fixed in all 4 stages)
-//   4 x { line_x, line_y, zstage } RHS F1..F4 at the stage state, RK bookkeeping (:120-128)
-//   line_x_cons, line_y_cons       d_x A_x + d_y A_y of the updated state (:129)
+//   4 x { line_x, line_y, zstage } RHS F1..F4 at the stage state (A_t held fixed in all 4 stages,
+//                                  its d_z formed in zstage), RK bookkeeping (:120-128)
+//   line_x_cons, line_y_cons       d_x A_x and d_y A_y of the updated state (:129)
 //   zpost                          A_t = Lzr(Lzr(cons) - rhoi) + mu, boundary reset, O (:129-136)
 //   winding                        removal of the global winding along y (:136-139)
 //   line_x_filter, line_y_filter   ff on Re Psi, Im Psi, A_t, A_x, A_y (:142), with the NaN guard (:182-185)
@@ -313,8 +313,6 @@ int upload_desc(hqt_ctx* c) {
     for (int s = 0; s < 4; ++s) h.zs[s] = c->zs[s];
     h.zs[4] = c->zr;
     h.zp = c->zp;
-    h.zpre[0] = c->zpre;
-    h.zpre[1] = c->zpreR;
     h.wind = c->wind;
     h.nan = c->nan;
     HIPC(hipMemcpy(c->dd, &h, sizeof(h), hipMemcpyHostToDevice));
@@ -645,7 +643,7 @@ int build_context(hqt_ctx* c) {
         std::memset(&Z, 0, sizeof(Z));
         Z.nxy = c->nxy; Z.ps = c->ps; Z.nz = nz; Z.stage = s + 1; Z.mode = 0; Z.h = c->p.dt; Z.zo = c->zo;
         Z.sR = Sin[s][0]; Z.sI = Sin[s][1]; Z.sAx = Sin[s][2]; Z.sAy = Sin[s][3];
-        Z.At = c->At; Z.dzAt = c->dzAt; Z.dxAt = c->dxAt; Z.dyAt = c->dyAt;
+        Z.At = c->At; Z.dxAt = c->dxAt; Z.dyAt = c->dyAt;
         Z.dxR = c->dxR; Z.dxI = c->dxI; Z.dyR = c->dyR; Z.dyI = c->dyI; Z.lapR = c->lapR; Z.lapI = c->lapI;
         Z.divA = c->divA; Z.curl3 = c->curl3; Z.curl4 = c->curl4;
         Z.zR = c->ZR; Z.zI = c->ZI; Z.zAx = c->ZAx; Z.zAy = c->ZAy;
@@ -665,9 +663,6 @@ int build_context(hqt_ctx* c) {
     c->zp.divA = c->dxAx; c->zp.divB = c->curl3;
     c->zp.rho = c->rho; c->zp.At = c->At; c->zp.axb = c->axb; c->zp.ayb = c->ayb; c->zp.O = c->O;
     c->zp.flags = c->flags;
-    c->zpre.nxy = c->nxy; c->zpre.ps = c->ps; c->zpre.nz = nz; c->zpre.Lz = c->zo.Lz; c->zpre.At = c->At; c->zpre.dzAt = c->dzAt;
-    c->zpre.flags = c->flags; c->zpre.bump = 1;
-    c->zpreR = c->zpre; c->zpreR.At = c->AtR; c->zpreR.bump = 0;
     std::memset(&c->wind, 0, sizeof(c->wind));
     c->wind.nx = nx; c->wind.ny = ny; c->wind.nz = nz; c->wind.nxy = c->nxy; c->wind.ps = c->ps;
     c->wind.O = c->O; c->wind.part = c->part; c->wind.R = c->ZR; c->wind.I = c->ZI;

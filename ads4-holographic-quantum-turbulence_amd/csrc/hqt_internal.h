@@ -21,11 +21,11 @@ int hqt_fail(int code, const std::string& msg);
     } while (0)
 
 enum KName {
-    KN_ZPREP, KN_LX_STAGE, KN_LY_STAGE, KN_ZSTAGE, KN_LX_CONS, KN_LY_CONS, KN_ZPOST, KN_WIND,
+    KN_LX_STAGE, KN_LY_STAGE, KN_ZSTAGE, KN_LX_CONS, KN_LY_CONS, KN_ZPOST, KN_WIND,
     KN_LX_FILT, KN_LY_FILT, KN_NAN, KN_LX_RHS, KN_LY_RHS, KN_ZRHS, KN_COUNT
 };
 inline constexpr const char* kNames[KN_COUNT] = {
-    "zprep", "line_x_stage", "line_y_stage", "zstage", "line_x_cons", "line_y_cons", "zpost", "winding",
+    "line_x_stage", "line_y_stage", "zstage", "line_x_cons", "line_y_cons", "zpost", "winding",
     "line_x_filter", "line_y_filter", "nancheck", "line_x_rhs", "line_y_rhs", "zrhs"};
 
 enum Op {
@@ -58,7 +58,7 @@ struct hqt_ctx {
     void *dyR, *dyI, *lapR, *lapI, *divA, *curl3, *curl4;
     void *dxAt, *dyAt;
     void *fa[5], *fb[5];
-    double *At, *rho, *dzAt, *arho, *O, *part, *W, *axb, *ayb, *yph, *zops, *AtR;
+    double *At, *rho, *arho, *O, *part, *W, *axb, *ayb, *yph, *zops, *AtR;
     double *F1R = nullptr, *F1I = nullptr, *F2 = nullptr, *F3 = nullptr, *F4 = nullptr;
     int32_t* flags = nullptr;
     void* ops[OP_COUNT];
@@ -73,11 +73,10 @@ struct hqt_ctx {
     ZStage zs[4];
     ZStage zr;
     ZPost zp;
-    ZPrep zpre, zpreR;
     Wind wind;
     NanCheck nan{};
     // device copies of the radial/winding/guard descriptors (kernels read them from memory)
-    struct DevDesc { ZStage zs[5]; ZPost zp; ZPrep zpre[2]; Wind wind; NanCheck nan; };
+    struct DevDesc { ZStage zs[5]; ZPost zp; Wind wind; NanCheck nan; };
     DevDesc* dd = nullptr;
     hipStream_t owner = nullptr;     // stream of an ensemble that steps this context, if any
     CheckpointWriter* ckpt = nullptr;  // asynchronous checkpoint writer (created on first use)

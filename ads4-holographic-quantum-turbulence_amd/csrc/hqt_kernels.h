@@ -144,7 +144,7 @@ struct ZStage {
     double h;
     ZOps zo;
     const void *sR, *sI, *sAx, *sAy;          // stage state (TS)
-    const double *At, *dzAt;                  // frozen A_t (d_z A_t is formed in the kernel; dzAt unused)
+    const double* At;                         // frozen A_t (its d_z is formed in the kernel)
     const void *dxAt, *dyAt;                  // frozen x/y derivatives of A_t (TS)
     const void *dxR, *dxI, *dyR, *dyI, *lapR, *lapI, *divA, *curl3, *curl4;  // (TS)
     void *zR, *zI, *zAx, *zAy;                // step-start state (TS), updated at stage 4
@@ -166,13 +166,6 @@ struct ZPost {
     const double *axb, *ayb;      // [nxy] boundary values
     double* O;                    // [2*nxy] boundary order parameter (re, im)
     int32_t* flags;
-};
-
-struct ZPrep {
-    int64_t nxy; int32_t nz;
-    int64_t ps;                     // plane stride of the volume fields (elements)
-    const double* Lz; const double* At; double* dzAt;
-    int32_t* flags; int32_t bump;   // bump: increment the step counter
 };
 
 struct Wind {
@@ -235,7 +228,6 @@ inline int64_t plane_stride(int64_t nxy) { return (nxy + 31) & ~(int64_t)31; }
 // wpe: 2 or 3 waves per SIMD targeted by the register allocation (see k_zstage).
 hipError_t launch_zstage(const ZStage* d, Members M, int blocks, int nz, int fp32, int wpe, hipStream_t s);
 hipError_t launch_zpost(const ZPost* d, Members M, int blocks, int nz, int fp32, hipStream_t s);
-hipError_t launch_zprep(const ZPrep* d, Members M, int blocks, int nz, hipStream_t s);
 // Winding: partial sums (grid.x per member = ny), then the removal (wind_apply_blocks per member).
 int wind_apply_blocks(int64_t nxy, int nz);
 hipError_t launch_wind_partial(const Wind* d, Members M, int blocks, hipStream_t s);

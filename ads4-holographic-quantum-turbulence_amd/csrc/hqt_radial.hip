@@ -66,8 +66,7 @@ __global__ __launch_bounds__(256) void k_wind_apply(const Wind* __restrict__ Ap,
 bool zkernels_support_nz(int nz) { return nz >= 4 && nz <= 32; }
 
 #define HQT_DECL(n) extern template hipError_t zstage_launch_nz<n>(const ZStage*, Members, int, int, int, hipStream_t); \
-                    extern template hipError_t zpost_launch_nz<n>(const ZPost*, Members, int, int, hipStream_t); \
-                    extern template hipError_t zprep_launch_nz<n>(const ZPrep*, Members, int, hipStream_t);
+                    extern template hipError_t zpost_launch_nz<n>(const ZPost*, Members, int, int, hipStream_t);
 HQT_NZ_LIST(HQT_DECL)
 #undef HQT_DECL
 
@@ -88,14 +87,6 @@ hipError_t launch_zpost(const ZPost* d, Members M, int blocks, int nz, int fp32,
     }
 }
 
-hipError_t launch_zprep(const ZPrep* d, Members M, int blocks, int nz, hipStream_t s) {
-    switch (nz) {
-#define HQT_ZR(n) case n: return zprep_launch_nz<n>(d, M, blocks, s);
-        HQT_NZ_LIST(HQT_ZR)
-#undef HQT_ZR
-        default: return hipErrorInvalidValue;
-    }
-}
 
 int wind_apply_blocks(int64_t nxy, int nz) {
     const int64_t b = (nxy * nz + 255) / 256;

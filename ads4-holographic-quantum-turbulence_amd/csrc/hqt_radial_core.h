@@ -476,28 +476,6 @@ __global__ __launch_bounds__(256) void k_zpost(const ZPost* __restrict__ Ap, con
     }
 }
 
-// ---- d_z A_t of the frozen A_t; also advances the device step counter ----------------------
-template <int NZ>
-__global__ __launch_bounds__(256) void k_zprep(const ZPrep* __restrict__ Ap, const Members M) {
-    const MemberBlock mb = member_block(M);
-    const ZPrep& A = Ap[mb.m];
-    if (stopped(A.flags)) return;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int m = lane & 15, g = lane >> 4;
-    const int64_t c = (int64_t)mb.bx * 64 + wave * 16 + m;
-    if (A.bump && mb.bx == 0 && threadIdx.x == 0) A.flags[1] = A.flags[1] + 1;
-    const bool cok = c < A.nxy;
-    double Lz[ZG<NZ>::RT][ZG<NZ>::KS], a[ZG<NZ>::KS], d[ZG<NZ>::KS];
-    load_op<NZ>(A.Lz, NZ, NZ, Lz, m, g);
-    load_slots<NZ, double>(A.At, c, cok, g, A.ps, a);
-    mv<NZ>(Lz, a, d);
-#pragma unroll
-    for (int s = 0; s < ZG<NZ>::KS; ++s) {
-        const int k = 4 * s + g;
-        if (cok && k < NZ) A.dzAt[c + A.ps * k] = d[s];
-    }
-}
-
 // Per-nz launchers; each hqt_radial_nz_*.hip translation unit instantiates a range of nz.
 template <int NZ>
 hipError_t zstage_launch_nz(const ZStage* d, Members M, int blocks, int fp32, int wpe, hipStream_t s) {
@@ -514,11 +492,6 @@ template <int NZ>
 hipError_t zpost_launch_nz(const ZPost* d, Members M, int blocks, int fp32, hipStream_t s) {
     if (fp32) hipLaunchKernelGGL((k_zpost<NZ, float>), dim3(blocks), dim3(256), 0, s, d, M);
     else hipLaunchKernelGGL((k_zpost<NZ, double>), dim3(blocks), dim3(256), 0, s, d, M);
-    return hipGetLastError();
-}
-template <int NZ>
-hipError_t zprep_launch_nz(const ZPrep* d, Members M, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL((k_zprep<NZ>), dim3(blocks), dim3(256), 0, s, d, M);
     return hipGetLastError();
 }
 

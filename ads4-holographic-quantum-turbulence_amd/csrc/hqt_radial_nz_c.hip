@@ -4,11 +4,8 @@
 namespace hqt {
 template hipError_t zstage_launch_nz<18>(const ZStage*, Members, int, int, int, hipStream_t);
 template hipError_t zpost_launch_nz<18>(const ZPost*, Members, int, int, hipStream_t);
-template hipError_t zprep_launch_nz<18>(const ZPrep*, Members, int, hipStream_t);
 template hipError_t zstage_launch_nz<19>(const ZStage*, Members, int, int, int, hipStream_t);
 template hipError_t zpost_launch_nz<19>(const ZPost*, Members, int, int, hipStream_t);
-template hipError_t zprep_launch_nz<19>(const ZPrep*, Members, int, hipStream_t);
 template hipError_t zstage_launch_nz<20>(const ZStage*, Members, int, int, int, hipStream_t);
 template hipError_t zpost_launch_nz<20>(const ZPost*, Members, int, int, hipStream_t);
-template hipError_t zprep_launch_nz<20>(const ZPrep*, Members, int, hipStream_t);
 }  // namespace hqt

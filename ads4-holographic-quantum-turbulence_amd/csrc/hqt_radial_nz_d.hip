@@ -4,11 +4,8 @@
 namespace hqt {
 template hipError_t zstage_launch_nz<21>(const ZStage*, Members, int, int, int, hipStream_t);
 template hipError_t zpost_launch_nz<21>(const ZPost*, Members, int, int, hipStream_t);
-template hipError_t zprep_launch_nz<21>(const ZPrep*, Members, int, hipStream_t);
 template hipError_t zstage_launch_nz<22>(const ZStage*, Members, int, int, int, hipStream_t);
 template hipError_t zpost_launch_nz<22>(const ZPost*, Members, int, int, hipStream_t);
-template hipError_t zprep_launch_nz<22>(const ZPrep*, Members, int, hipStream_t);
 template hipError_t zstage_launch_nz<23>(const ZStage*, Members, int, int, int, hipStream_t);
 template hipError_t zpost_launch_nz<23>(const ZPost*, Members, int, int, hipStream_t);
-template hipError_t zprep_launch_nz<23>(const ZPrep*, Members, int, hipStream_t);
 }  // namespace hqt

@@ -4,11 +4,8 @@
 namespace hqt {
 template hipError_t zstage_launch_nz<24>(const ZStage*, Members, int, int, int, hipStream_t);
 template hipError_t zpost_launch_nz<24>(const ZPost*, Members, int, int, hipStream_t);
-template hipError_t zprep_launch_nz<24>(const ZPrep*, Members, int, hipStream_t);
 template hipError_t zstage_launch_nz<25>(const ZStage*, Members, int, int, int, hipStream_t);
 template hipError_t zpost_launch_nz<25>(const ZPost*, Members, int, int, hipStream_t);
-template hipError_t zprep_launch_nz<25>(const ZPrep*, Members, int, hipStream_t);
 template hipError_t zstage_launch_nz<26>(const ZStage*, Members, int, int, int, hipStream_t);
 template hipError_t zpost_launch_nz<26>(const ZPost*, Members, int, int, hipStream_t);
-template hipError_t zprep_launch_nz<26>(const ZPrep*, Members, int, hipStream_t);
 }  // namespace hqt

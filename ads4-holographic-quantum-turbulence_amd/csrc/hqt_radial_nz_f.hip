@@ -4,8 +4,6 @@
 namespace hqt {
 template hipError_t zstage_launch_nz<27>(const ZStage*, Members, int, int, int, hipStream_t);
 template hipError_t zpost_launch_nz<27>(const ZPost*, Members, int, int, hipStream_t);
-template hipError_t zprep_launch_nz<27>(const ZPrep*, Members, int, hipStream_t);
 template hipError_t zstage_launch_nz<28>(const ZStage*, Members, int, int, int, hipStream_t);
 template hipError_t zpost_launch_nz<28>(const ZPost*, Members, int, int, hipStream_t);
-template hipError_t zprep_launch_nz<28>(const ZPrep*, Members, int, hipStream_t);
 }  // namespace hqt

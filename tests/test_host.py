@@ -255,7 +255,7 @@ def test_continue_mu_stays_on_superfluid_branch():
 
 
 def test_footprint_formula_scales_with_points():
-    """hqt_ctx_footprint (host only): 42 state-precision arrays plus 4 FP64 arrays per grid
+    """hqt_ctx_footprint (host only): 42 state-precision arrays plus 3 FP64 arrays per grid
     point (each z-plane padded to a multiple of 32 points; 64 x 64 needs none), plus per-plane,
     per-line and constant terms, plus the filter's cross-term partials (5 fields x one 16-line
     block of x-lines x 32 doubles); mixed precision halves the first part."""
@@ -264,7 +264,7 @@ def test_footprint_formula_scales_with_points():
     b = hqt.footprint(64, 64, 24, "fp64")
     per_z = b - a
     blocks = (64 * 24 + 15) // 16 - (64 * 12 + 15) // 16
-    assert per_z == (12 * 64 * 64 * (42 * 8 + 32) + ((24 * 24 + 23 * 23 + 120) - (12 * 12 + 11 * 11 + 60)) * 8
+    assert per_z == (12 * 64 * 64 * (42 * 8 + 24) + ((24 * 24 + 23 * 23 + 120) - (12 * 12 + 11 * 11 + 60)) * 8
                      + 5 * blocks * 32 * 8)
     m = hqt.footprint(64, 64, 12, "mixed")
     assert a - m > 64 * 64 * 12 * 42 * 4

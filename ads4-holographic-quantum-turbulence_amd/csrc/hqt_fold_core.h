@@ -263,8 +263,10 @@ __device__ __forceinline__ bool fold_out(const LPass& P, const LOut& O, typename
 // folds groups with one source, or two sources and one output, so one output's accumulators are
 // live at a time.  The global loads of the next unit are issued right after a unit is staged and
 // stay in flight in registers during the unit's MFMAs.
+// The body of one folded pass for work-group (bx, by) of an nbx x nby grid.
 template <typename T, int DIR, int QF, int KIND, int NT, typename R>
-__global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_WPE<T, R>))) void k_fold_pass(const LPass* __restrict__ Pp, const Members M, const int xcd_order) {
+__device__ __forceinline__ void fold_pass_body(const LPass* __restrict__ Pp, const Members M, const int xcd_order,
+                                               const int bx, const int by, const int nbx, const int nby) {
     constexpr int LB = 16 * NT;
     constexpr int JS = DIR == 0 ? 16 : LP_THREADS / LB;            // j step between a thread's elements
     constexpr int PFJ = ((64 * QF + 8) + JS - 1) / JS;             // folded positions per thread and line half
@@ -273,7 +275,7 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
     // the launch order, group-major.  xcd_order = 1: XCD-aware over the whole (tile, group) grid,
     // tile-major, so that one XCD's L2 serves a run of consecutive tiles (their lines share cache
     // lines at the edges, which matters for the strided y-lines) with all of their groups.
-    const int nbx = (int)gridDim.x, nby = (int)gridDim.y, lin0 = (int)blockIdx.x + nbx * (int)blockIdx.y;
+    const int lin0 = bx + nbx * by;
     int lin, bxl, gy;
     if (xcd_order) {
         lin = xcd_block(lin0, nbx * nby);
@@ -281,8 +283,8 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
         gy = lin - bxl * nby;
     } else {
         lin = lin0;
-        bxl = (int)blockIdx.x;
-        gy = (int)blockIdx.y;
+        bxl = bx;
+        gy = by;
     }
     const MemberBlock mb = member_block_at(M, bxl, nbx);
     const LPass& P = Pp[mb.m];
@@ -562,6 +564,56 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
                 }
             }
         }
+    }
+}
+
+template <typename T, int DIR, int QF, int KIND, int NT, typename R>
+__global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_WPE<T, R>))) void k_fold_pass(const LPass* __restrict__ Pp, const Members M, const int xcd_order) {
+    fold_pass_body<T, DIR, QF, KIND, NT, R>(Pp, M, xcd_order, (int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x,
+                                            (int)gridDim.y);
+}
+
+// An x-pass and a y-pass of one kind and shape in one launch (one context, one group each, 16 lines
+// per work-group): work-groups [0, nb0) run the x-pass P0, the next nb1 the y-pass P1.  For two
+// passes that are independent and each fill only part of the GPU (the constraint's).
+template <typename T, int QF, int KIND, typename R>
+__global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_WPE<T, R>))) void k_fold_pair(
+    const LPass* __restrict__ P0, const LPass* __restrict__ P1, const int nb0, const int nb1, const int order0,
+    const int order1) {
+    const Members one{nullptr, 1};
+    const int b = (int)blockIdx.x;
+    if (b < nb0) fold_pass_body<T, 0, QF, KIND, 1, R>(P0, one, order0, b, 0, nb0, 1);
+    else fold_pass_body<T, 1, QF, KIND, 1, R>(P1, one, order1, b - nb0, 0, nb1, 1);
+}
+
+template <typename T, int QF, int KIND, typename R>
+hipError_t fold_pair_launch_one(const LPass* p0, const LPass* p1, int nb0, int nb1, size_t lds, hipStream_t s,
+                                int order0, int order1) {
+    const void* fn = reinterpret_cast<const void*>(&k_fold_pair<T, QF, KIND, R>);
+    if (lds > 64 * 1024) {
+        static bool done = false;
+        if (!done) {
+            const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            if (e != hipSuccess) return e;
+            done = true;
+        }
+    }
+    hipLaunchKernelGGL((k_fold_pair<T, QF, KIND, R>), dim3(nb0 + nb1), dim3(LP_THREADS), lds, s, p0, p1, nb0, nb1,
+                       order0, order1);
+    return hipGetLastError();
+}
+
+// The constraint pair (kind 1) of one precision, by QF (both passes of one QF).
+template <typename T>
+hipError_t fold_pair_launch_cons(const LPass* p0, const LPass* p1, int nb0, int nb1, int QF, size_t lds, hipStream_t s,
+                                 int order0, int order1) {
+    switch (QF) {
+        case 1: return fold_pair_launch_one<T, 1, 1, T>(p0, p1, nb0, nb1, lds, s, order0, order1);
+        case 2: return fold_pair_launch_one<T, 2, 1, T>(p0, p1, nb0, nb1, lds, s, order0, order1);
+        case 3: return fold_pair_launch_one<T, 3, 1, T>(p0, p1, nb0, nb1, lds, s, order0, order1);
+        case 4: return fold_pair_launch_one<T, 4, 1, T>(p0, p1, nb0, nb1, lds, s, order0, order1);
+        case 5: return fold_pair_launch_one<T, 5, 1, T>(p0, p1, nb0, nb1, lds, s, order0, order1);
+        default: return hipErrorInvalidValue;
     }
 }
 

@@ -411,8 +411,22 @@ int enqueue_step(hqt_ctx* c) {
         if ((rc = line(c, PY1 + s, 1, KN_LY_STAGE))) return rc;
         if ((rc = run(c, KN_ZSTAGE, [&] { return launch_zstage(&c->dd->zs[s], kOne, zb, nz, f32, c->zwpe, st); }))) return rc;
     }
-    if ((rc = line(c, PXC, 0, KN_LX_CONS))) return rc;
-    if ((rc = line(c, PYC, 1, KN_LY_CONS))) return rc;
+    {
+        // the constraint's two passes are independent and each fills only part of the GPU: one
+        // launch for both when they have the same shape
+        const LPass &X = c->hp[PXC], &Y = c->hp[PYC];
+        if (X.fold == 1 && Y.fold == 1 && X.lb == 16 && Y.lb == 16 && X.Mp == Y.Mp && X.ngroups == 1 && Y.ngroups == 1) {
+            const size_t lds = std::max(line_pass_lds_bytes(X, f32), line_pass_lds_bytes(Y, f32));
+            if ((rc = run(c, KN_CONS, [&] {
+                     return launch_cons_pair(c->dp + PXC, c->dp + PYC, line_blocks(X), line_blocks(Y), lds, X.Mp / 64, f32,
+                                             st, X.xcd, Y.xcd);
+                 })))
+                return rc;
+        } else {
+            if ((rc = line(c, PXC, 0, KN_LX_CONS))) return rc;
+            if ((rc = line(c, PYC, 1, KN_LY_CONS))) return rc;
+        }
+    }
     if ((rc = run(c, KN_ZPOST, [&] { return launch_zpost(&c->dd->zp, kOne, zb, nz, f32, st); }))) return rc;
     if (c->p.winding)
         if ((rc = run(c, KN_WIND, [&] {

@@ -22,11 +22,11 @@ int hqt_fail(int code, const std::string& msg);
 
 enum KName {
     KN_LX_STAGE, KN_LY_STAGE, KN_ZSTAGE, KN_LX_CONS, KN_LY_CONS, KN_ZPOST, KN_WIND,
-    KN_LX_FILT, KN_LY_FILT, KN_NAN, KN_LX_RHS, KN_LY_RHS, KN_ZRHS, KN_COUNT
+    KN_LX_FILT, KN_LY_FILT, KN_NAN, KN_LX_RHS, KN_LY_RHS, KN_ZRHS, KN_CONS, KN_COUNT
 };
 inline constexpr const char* kNames[KN_COUNT] = {
     "line_x_stage", "line_y_stage", "zstage", "line_x_cons", "line_y_cons", "zpost", "winding",
-    "line_x_filter", "line_y_filter", "nancheck", "line_x_rhs", "line_y_rhs", "zrhs"};
+    "line_x_filter", "line_y_filter", "nancheck", "line_x_rhs", "line_y_rhs", "zrhs", "line_cons"};
 
 enum Op {
     OP_D1X, OP_D2X, OP_ND1X, OP_PRX, OP_PIX, OP_NPIX, OP_D1Y, OP_D2Y, OP_ND1Y, OP_QRY, OP_QIY, OP_NQIY,

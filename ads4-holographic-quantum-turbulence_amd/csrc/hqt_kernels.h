@@ -218,6 +218,10 @@ struct Census {
 // kind: 0 RHS stage, 1 constraint, 2 filter (separate instantiations for profiling).
 // fold: the even/odd folded kernels (descriptors built with fold = 1 or 2; Q is then QF).
 // order (folded kernels only): 0 launch order, 1 XCD-aware block order; results are the same.
+// The constraint's folded x- and y-passes (one group each, 16 lines per work-group, the same Q)
+// in one launch of nb0 + nb1 work-groups; one context.
+hipError_t launch_cons_pair(const LPass* d0, const LPass* d1, int nb0, int nb1, size_t lds, int Q, int fp32,
+                            hipStream_t s, int order0, int order1);
 hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, size_t lds, int Q, int nt, int dir,
                             int fp32, int kind, hipStream_t s, int fold = 0, int order = 0);
 size_t line_pass_lds_bytes(const LPass& P, int fp32);

@@ -12,6 +12,8 @@ extern template hipError_t fold_launch_td<double, 0>(const LPass*, Members, dim3
 extern template hipError_t fold_launch_td<double, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int, int);
 extern template hipError_t fold_launch_td<float, 0>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int, int);
 extern template hipError_t fold_launch_td<float, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int, int);
+extern template hipError_t fold_pair_launch_cons<double>(const LPass*, const LPass*, int, int, int, size_t, hipStream_t, int, int);
+extern template hipError_t fold_pair_launch_cons<float>(const LPass*, const LPass*, int, int, int, size_t, hipStream_t, int, int);
 
 size_t line_pass_lds_bytes(const LPass& P, int fp32) {
     int maxslots = 0;
@@ -40,6 +42,13 @@ hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, 
                          : line_launch_td<float, 0>(d, M, grid, Q, nt, kind, lds, s);
     return dir ? line_launch_td<double, 1>(d, M, grid, Q, nt, kind, lds, s)
                : line_launch_td<double, 0>(d, M, grid, Q, nt, kind, lds, s);
+}
+
+hipError_t launch_cons_pair(const LPass* d0, const LPass* d1, int nb0, int nb1, size_t lds, int Q, int fp32,
+                            hipStream_t s, int order0, int order1) {
+    if (Q < 1 || Q > 5 || nb0 <= 0 || nb1 <= 0 || lds > 160 * 1024) return hipErrorInvalidValue;
+    return fp32 ? fold_pair_launch_cons<float>(d0, d1, nb0, nb1, Q, lds, s, order0, order1)
+                : fold_pair_launch_cons<double>(d0, d1, nb0, nb1, Q, lds, s, order0, order1);
 }
 
 }  // namespace hqt

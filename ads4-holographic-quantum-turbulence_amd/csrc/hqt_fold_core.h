@@ -582,8 +582,11 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(FP_W
     const int order1) {
     const Members one{nullptr, 1};
     const int b = (int)blockIdx.x;
+    // the y-pass starts at a multiple of 8, so its work-group i runs on XCD i mod 8 as in a launch of its
+    // own and its XCD-aware order groups the same tiles on one L2; the padding work-groups do nothing
+    const int nb0p = (nb0 + 7) & ~7;
     if (b < nb0) fold_pass_body<T, 0, QF, KIND, 1, R>(P0, one, order0, b, 0, nb0, 1);
-    else fold_pass_body<T, 1, QF, KIND, 1, R>(P1, one, order1, b - nb0, 0, nb1, 1);
+    else if (b >= nb0p) fold_pass_body<T, 1, QF, KIND, 1, R>(P1, one, order1, b - nb0p, 0, nb1, 1);
 }
 
 template <typename T, int QF, int KIND, typename R>
@@ -598,7 +601,7 @@ hipError_t fold_pair_launch_one(const LPass* p0, const LPass* p1, int nb0, int n
             done = true;
         }
     }
-    hipLaunchKernelGGL((k_fold_pair<T, QF, KIND, R>), dim3(nb0 + nb1), dim3(LP_THREADS), lds, s, p0, p1, nb0, nb1,
+    hipLaunchKernelGGL((k_fold_pair<T, QF, KIND, R>), dim3(((nb0 + 7) & ~7) + nb1), dim3(LP_THREADS), lds, s, p0, p1, nb0, nb1,
                        order0, order1);
     return hipGetLastError();
 }

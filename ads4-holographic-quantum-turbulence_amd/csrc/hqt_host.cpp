@@ -896,6 +896,8 @@ int hqt_rhs(hqt_ctx* c, const double* Z1, const double* Z2, const double* Z3, co
 namespace {
 // O of the current state into c->Od; with `census`, also the plaquette charges and counts.
 int enqueue_census(hqt_ctx* c, bool census) {
+    // the state may still be written by an ensemble's queued steps on its own stream
+    if (c->owner) HIPC(hipStreamSynchronize(c->owner));
     Census A{};
     A.nxy = c->nxy; A.ps = c->ps; A.nx = c->nx; A.ny = c->ny; A.nz = c->nz;
     A.R = c->ZR; A.I = c->ZI;

@@ -111,6 +111,20 @@ int newton_batch_run(hipStream_t s, NewtonWork& w, int nz, double zh, double sma
                      const double* Z1, const double* Z2, const double* vy1, const double* vy, int it_max, double tol,
                      double* z1, double* z2, double* VY1, int32_t* it, double* err, uint8_t* conv);
 
+// Device bytes newton_batch_run needs for nb problems of radial size nz, and the same run in a caller-given
+// device buffer of at least that many bytes (no allocation).
+size_t newton_batch_bytes(int nz, int nb);
+int newton_batch_run_in(hipStream_t s, unsigned char* buf, int nz, double zh, double small, double D, int nb,
+                        const double* Z1, const double* Z2, const double* vy1, const double* vy, int it_max,
+                        double tol, double* z1, double* z2, double* VY1, int32_t* it, double* err, uint8_t* conv);
+
+// Argument checks of the batched Newton entry points (it_max <= 0 and tol <= 0 select the reference's
+// 1000 and 1e-6), and their return status from the per-problem convergence flags.
+int newton_batch_args(const char* who, int nz, int nb, const void* Z1, const void* Z2, const void* vy1, const void* vy,
+                      const void* z1, const void* z2, const void* VY1, const void* it, const void* err,
+                      const void* conv, int32_t& it_max, double& tol);
+int newton_batch_status(const char* who, int nb, const uint8_t* conv);
+
 // Device bytes a context with these parameters allocates (state, stage buffers, operator
 // tables, descriptors), as hqt_ctx_create would; used for admission by HBM budget.
 int64_t ctx_footprint(const hqt_params& p);

@@ -12,11 +12,48 @@ NewtonWork::~NewtonWork() {
 
 namespace {
 size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+
+// Byte offsets of the device layout: host image (operators, inputs) | results 3*vec | out 3*nb | jobs |
+// per-problem Jacobians (when they do not fit in LDS).
+struct NewtonLayout { size_t img, res, out, jobs, work, total; };
+
+NewtonLayout newton_layout(int nz, int nb) {
+    const size_t nn = (size_t)nz * nz, n3 = 3 * (size_t)nz, vec = (size_t)nz * nb;
+    NewtonLayout L;
+    L.img = 0;
+    L.res = align256((2 * nn + nz + 3 * vec) * 8);
+    L.out = L.res + align256(3 * vec * 8);
+    L.jobs = L.out + align256(3 * (size_t)nb * 8);
+    L.work = L.jobs + align256((size_t)nb * sizeof(NewtonJob));
+    L.total = L.work + (newton_lds_resident(nz) ? 0 : (size_t)nb * n3 * n3 * 8);
+    return L;
+}
 }  // namespace
+
+size_t newton_batch_bytes(int nz, int nb) { return newton_layout(nz, nb).total; }
 
 int newton_batch_run(hipStream_t s, NewtonWork& w, int nz, double zh, double small, double D, int nb,
                      const double* Z1, const double* Z2, const double* vy1, const double* vy, int it_max, double tol,
                      double* z1, double* z2, double* VY1, int32_t* it, double* err, uint8_t* conv) {
+    const size_t total = newton_batch_bytes(nz, nb);
+    if (total > w.cap) {
+        if (w.buf) {
+            HIPC(hipStreamSynchronize(s));
+            (void)hipFree(w.buf);
+            w.buf = nullptr;
+            w.cap = 0;
+        }
+        const hipError_t e = hipMalloc(&w.buf, total);
+        if (e != hipSuccess) { w.buf = nullptr; return hqt_fail(HQT_ERR_OOM, "hipMalloc(Newton workspace)"); }
+        w.cap = total;
+    }
+    return newton_batch_run_in(s, w.buf, nz, zh, small, D, nb, Z1, Z2, vy1, vy, it_max, tol, z1, z2, VY1, it, err,
+                               conv);
+}
+
+int newton_batch_run_in(hipStream_t s, unsigned char* buf, int nz, double zh, double small, double D, int nb,
+                        const double* Z1, const double* Z2, const double* vy1, const double* vy, int it_max,
+                        double tol, double* z1, double* z2, double* VY1, int32_t* it, double* err, uint8_t* conv) {
     // radial grid and operators (NewtonRaphson_vy.m:12-21), as in the host solver
     std::vector<double> zc, DM;
     chebdif(nz, 1, zc, DM);
@@ -40,27 +77,14 @@ int newton_batch_run(hipStream_t s, NewtonWork& w, int nz, double zh, double sma
     std::memcpy(hin, Z1, vec * 8);
     std::memcpy(hin + vec, Z2, vec * 8);
     std::memcpy(hin + 2 * vec, vy1, vec * 8);
-    // device layout: host image | results 3*vec | out 3*nb | jobs | per-problem Jacobians (when not in LDS)
     const bool lds = newton_lds_resident(nz);
-    const size_t o_img = 0, o_res = align256(host.size() * 8), o_out = o_res + align256(3 * vec * 8),
-                 o_jobs = o_out + align256(3 * (size_t)nb * 8), o_work = o_jobs + align256((size_t)nb * sizeof(NewtonJob)),
-                 total = o_work + (lds ? 0 : (size_t)nb * n3 * n3 * 8);
-    if (total > w.cap) {
-        if (w.buf) {
-            HIPC(hipStreamSynchronize(s));
-            (void)hipFree(w.buf);
-            w.buf = nullptr;
-            w.cap = 0;
-        }
-        const hipError_t e = hipMalloc(&w.buf, total);
-        if (e != hipSuccess) { w.buf = nullptr; return hqt_fail(HQT_ERR_OOM, "hipMalloc(Newton workspace)"); }
-        w.cap = total;
-    }
-    double* dimg = reinterpret_cast<double*>(w.buf + o_img);
-    double* dres = reinterpret_cast<double*>(w.buf + o_res);
-    double* dout = reinterpret_cast<double*>(w.buf + o_out);
-    NewtonJob* djobs = reinterpret_cast<NewtonJob*>(w.buf + o_jobs);
-    double* dwork = reinterpret_cast<double*>(w.buf + o_work);
+    const NewtonLayout lay = newton_layout(nz, nb);
+    const size_t o_img = lay.img, o_res = lay.res, o_out = lay.out, o_jobs = lay.jobs, o_work = lay.work;
+    double* dimg = reinterpret_cast<double*>(buf + o_img);
+    double* dres = reinterpret_cast<double*>(buf + o_res);
+    double* dout = reinterpret_cast<double*>(buf + o_out);
+    NewtonJob* djobs = reinterpret_cast<NewtonJob*>(buf + o_jobs);
+    double* dwork = reinterpret_cast<double*>(buf + o_work);
     const double* din = dimg + 2 * nn + nz;
     std::vector<NewtonJob> jobs(nb);
     for (int b = 0; b < nb; ++b) {
@@ -101,15 +125,32 @@ int newton_batch_run(hipStream_t s, NewtonWork& w, int nz, double zh, double sma
     return HQT_OK;
 }
 
+int newton_batch_args(const char* who, int nz, int nb, const void* Z1, const void* Z2, const void* vy1, const void* vy,
+                      const void* z1, const void* z2, const void* VY1, const void* it, const void* err,
+                      const void* conv, int32_t& it_max, double& tol) {
+    if (nz < 3 || nz > 64) return hqt_fail(HQT_ERR_ARG, std::string(who) + ": nz must lie in [3, 64]");
+    if (nb < 1) return hqt_fail(HQT_ERR_ARG, std::string(who) + ": nb must be positive");
+    if (!Z1 || !Z2 || !vy1 || !vy || !z1 || !z2 || !VY1 || !it || !err || !conv)
+        return hqt_fail(HQT_ERR_ARG, std::string(who) + ": NULL argument");
+    if (it_max <= 0) it_max = 1000;
+    if (!(tol > 0)) tol = 1e-6;
+    return HQT_OK;
+}
+
+int newton_batch_status(const char* who, int nb, const uint8_t* conv) {
+    int nconv = 0;
+    for (int b = 0; b < nb; ++b) nconv += conv[b];
+    if (nconv != nb)
+        return hqt_fail(HQT_ERR_NOCONV, std::string(who) + ": " + std::to_string(nb - nconv) + " of " +
+                                            std::to_string(nb) + " problems did not converge");
+    return HQT_OK;
+}
+
 extern "C" int hqt_newton_vy_batch(int32_t nz, double zh, double small, double D, int32_t nb, const double* Z1,
                                    const double* Z2, const double* vy1, const double* vy, int32_t it_max, double tol,
                                    double* z1, double* z2, double* VY1, int32_t* it, double* err, uint8_t* conv) {
-    if (nz < 3 || nz > 64) return hqt_fail(HQT_ERR_ARG, "hqt_newton_vy_batch: nz must lie in [3, 64]");
-    if (nb < 1) return hqt_fail(HQT_ERR_ARG, "hqt_newton_vy_batch: nb must be positive");
-    if (!Z1 || !Z2 || !vy1 || !vy || !z1 || !z2 || !VY1 || !it || !err || !conv)
-        return hqt_fail(HQT_ERR_ARG, "hqt_newton_vy_batch: NULL argument");
-    if (it_max <= 0) it_max = 1000;
-    if (!(tol > 0)) tol = 1e-6;
+    static const char* who = "hqt_newton_vy_batch";
+    if (int rc = newton_batch_args(who, nz, nb, Z1, Z2, vy1, vy, z1, z2, VY1, it, err, conv, it_max, tol)) return rc;
     hipStream_t s = nullptr;
     HIPC(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     int rc;
@@ -119,10 +160,5 @@ extern "C" int hqt_newton_vy_batch(int32_t nz, double zh, double small, double D
     }
     (void)hipStreamDestroy(s);
     if (rc) return rc;
-    int nconv = 0;
-    for (int b = 0; b < nb; ++b) nconv += conv[b];
-    if (nconv != nb)
-        return hqt_fail(HQT_ERR_NOCONV, "hqt_newton_vy_batch: " + std::to_string(nb - nconv) + " of " + std::to_string(nb) +
-                                            " problems did not converge");
-    return HQT_OK;
+    return newton_batch_status(who, nb, conv);
 }

@@ -109,23 +109,41 @@ def run_gpu(nx, ny, nz, L, precision, steps, warmup, state, axb, profile_steps, 
     return (t1 - t0), final, stats
 
 
-def cpu_baseline(nx, ny, nz, L, state, axb, budget_s=20.0):
-    """The oracle (reference dense form, OpenBLAS/NumPy) on this host: 1 warm-up step, then as
-    many steps as fit in ``budget_s`` (at least 2)."""
+def cpu_info():
+    """CPU model, logical CPUs visible to this process and the BLAS/OpenMP thread settings."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count()
+    env = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")}
+    return {"model": model, "logical_cpus": os.cpu_count(), "affinity_cpus": affinity, "env": env}
+
+
+def cpu_baseline(nx, ny, nz, L, state, axb, warmup=2, timed=5):
+    """The oracle (the reference's dense-matrix form, NumPy/OpenBLAS) on this host: ``warmup``
+    untimed steps, then ``timed`` steps timed one by one; returns the median step time's rate and
+    the per-step times."""
     from oracle import hqt_oracle as O
     ops = O.Operators(nx, ny, nz, L, L)
     prm = O.Params(mu=6.0, h=0.02, ax_b=axb, ay_b=0.0, winding=True)
     S = list(state)
-    *S, _w = O.step(ops, prm, *S)
-    n = 0
-    t0 = time.perf_counter()
-    while True:
+    for _ in range(warmup):
         *S, _w = O.step(ops, prm, *S)
-        n += 1
-        if n >= 2 and time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
-    return n / dt, n
+    times = []
+    for _ in range(timed):
+        t0 = time.perf_counter()
+        *S, _w = O.step(ops, prm, *S)
+        times.append(time.perf_counter() - t0)
+    return 1.0 / float(np.median(times)), times
 
 
 def ensemble_leg(members, steps, seed=4):
@@ -271,6 +289,8 @@ def main():
     ap.add_argument("--n", type=int, default=301)
     ap.add_argument("--nz", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-warmup", type=int, default=2, help="untimed oracle steps before the CPU timing")
+    ap.add_argument("--cpu-steps", type=int, default=5, help="timed oracle steps (their median is reported)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-accuracy", action="store_true",
@@ -389,11 +409,15 @@ def main():
         out["serving"] = loadgen.run(rate=args.serving_rate, duration=args.serving_seconds, seed=0)
 
     if not args.no_cpu:
-        cps, ncs = cpu_baseline(nx, ny, nz, L, state, axb)
-        out["cpu_baseline"] = {"value": round(cps, 5), "unit": "steps/s", "cores": os.cpu_count(),
+        cps, times = cpu_baseline(nx, ny, nz, L, state, axb, warmup=args.cpu_warmup, timed=args.cpu_steps)
+        info = cpu_info()
+        out["cpu_baseline"] = {"value": round(cps, 5), "unit": "steps/s", "cores": info["affinity_cpus"],
                                "kind": "port",
-                               "sample": f"{ncs} steps of the same workload after 1 warm-up step, NumPy/OpenBLAS "
-                                         f"dense-matrix restatement (oracle), all host threads"}
+                               "sample": f"median of {len(times)} timed steps after {args.cpu_warmup} warm-up steps of "
+                                         f"the same workload, NumPy/OpenBLAS dense-matrix restatement (oracle), "
+                                         f"all threads of this process's CPU set",
+                               "step_s": [round(t, 3) for t in times],
+                               "projected_5000_steps_s": round(5000 / cps, 1), "cpu": info}
     print(json.dumps(out), flush=True)
 
 

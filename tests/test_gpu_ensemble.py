@@ -133,3 +133,24 @@ def test_ensemble_rejects_bad_membership():
         hqt.Ensemble([a, a])
     assert ei.value.code == hqt.HQT_ERR_ARG
     a.close()
+
+
+def test_member_diagnostics_wait_for_ensemble_steps():
+    """A member's order parameter and census read right after queueing ensemble steps see the
+    state after those steps (the same values as read after the ensemble's sync)."""
+    import hqt
+    a = _make(MEMBERS[1], 100)
+    b = _make(MEMBERS[4], 101)
+    ens = hqt.Ensemble([a, b])
+    ens.step_async(15)
+    O_early = a.order_parameter()
+    c_early = b.vortex_census()
+    ens.sync()
+    assert np.array_equal(O_early, a.order_parameter())
+    assert c_early == b.vortex_census()
+    with _make(MEMBERS[1], 100) as alone:
+        alone.step(15)
+        assert np.array_equal(O_early, alone.order_parameter())
+    ens.close()
+    a.close()
+    b.close()

@@ -1,0 +1,179 @@
+"""The benchmark workloads at their own sizes, on the GPU through the C ABI.
+
+* The reference's own driven run at full length: 301 x 301 x 20 on L = (nx-1)/2 = 150, the kk = 7
+  profile with uniform A_y(z=0) = -vy, 5000 steps (``scripts/simulation/tevol_driven_cluster.m``).
+  Checked by finiteness, charge conservation and the vortex census against the oracle's countN.
+* The fine-grid time-step limit: the oracle runs of ``tests/golden/make_instability.py`` (32^2 x 12
+  at spacing 0.17 turns non-finite, at spacing 0.48 stays finite), repeated on the GPU.
+* The batched ensemble of 48 independent evolutions drawn from the ensemble workload's ranges
+  (nx = ny in {64, 96, 128, 192, 256}, nz in {12, 16, 20, 24}, mu ~ U[4, 8], kk in 3..9,
+  8-64 windings, FP64 and mixed): bitwise equal to each member stepped alone.
+* A serving stream drawn from the serving workload's ranges (Newton nz in [16, 48],
+  mu ~ U[3.5, 10]; evolutions nx = ny in [64, 301], nz in [12, 24], mu ~ U[4, 8]): every evolution
+  bitwise equal to running alone, every Newton solve equal to the oracle's NewtonRaphson_vy.
+"""
+
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def test_reference_box_5000_steps():
+    import hqt
+    from hqt import ic
+    from oracle import hqt_oracle as O
+    n, nz = 301, 20
+    L = (n - 1) / 2
+    print("reference box 301^2 x 20, L = 150, 5000 steps", flush=True)
+    with hqt.Stepper(n, n, nz, L, L, mu=6.0, dt=0.02, winding=True, ax_b=0.0, ay_b=2 * math.pi * 7 / 20) as st:
+        st.init_state(ic.load_profile(), amp_noise=1e-2, phase_noise=0.0, seed=5)
+        q0 = float(np.sum(st.get_state()[4][:, :, -1]))
+        st.step(5000)
+        z1, z2, z3, z4, rh = st.get_state()
+        for f in (z1, z2, z3, z4, rh):
+            assert np.all(np.isfinite(f))
+        q = float(np.sum(rh[:, :, -1]))
+        assert abs(q - q0) <= 1e-9 * abs(q0), (q, q0)
+        counts = st.vortex_census()[:3]
+        O_dev = st.order_parameter()
+    ops = O.Operators(n, n, nz, L, L)
+    O_host = O.boundary_order_parameter(ops, z1)
+    assert np.max(np.abs(O_dev - O_host)) <= 1e-11 * np.max(np.abs(O_host))
+    assert counts == O.countN(O_host)
+    assert counts == O.countN(O_dev)
+
+
+@pytest.mark.parametrize("case", ["fine", "coarse"])
+def test_fine_grid_time_step_limit(case):
+    import hqt
+    from hqt import ic
+    ref = np.load(os.path.join(GOLDEN, "instability_32x32x12.npz"))
+    L = float(ref[f"{case}_L"])
+    nan_ref = int(ref[f"{case}_nan_step"])
+    rows = ref[f"{case}_rows"]
+    print(f"time-step limit {case}: L = {L}, oracle NaN step {nan_ref}", flush=True)
+    with hqt.Stepper(32, 32, 12, L, L, mu=6.0, dt=0.02, winding=False) as st:
+        st.set_state(*ic.initial_state(32, 32, 12, L, L, mu=6.0, seed=1, n_vortices=4, phase_noise=1e-3))
+        done = 0
+        for row in rows:
+            stp = int(row[0])
+            st.step_async(stp - done)
+            done = stp
+            if st.nan_step():
+                break
+            z1, z2, z3, z4, rh = st.get_state()
+            got = [np.max(np.abs(z1)), np.max(np.abs(z2)), np.max(np.abs(z3)), np.max(np.abs(z4)),
+                   np.sum(rh[:, :, -1])]
+            if stp <= 200:       # before the growth takes over, the diagnostics agree closely
+                assert np.allclose(got, row[1:], rtol=1e-9, atol=1e-12), (stp, got, row[1:])
+        if nan_ref:
+            st.step_async(nan_ref + 100 - done)
+            ns = st.nan_step()
+            assert ns > 0 and abs(ns - nan_ref) <= 50, (ns, nan_ref)
+        else:
+            assert st.nan_step() == 0
+            assert np.allclose(got, rows[-1][1:], rtol=1e-6, atol=1e-9), (got, rows[-1])
+
+
+def _config4_specs(nmem=48, seed=4):
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(nmem):
+        n = int(rng.choice([64, 96, 128, 192, 256]))
+        nz = int(rng.choice([12, 16, 20, 24]))
+        mu = float(rng.uniform(4.1, 8.0))       # the superfluid branch ends near mu = 4.07
+        kk = int(rng.integers(3, 10))
+        nv = int(rng.integers(4, 33)) * 2
+        specs.append((n, nz, mu, kk, nv, int(rng.integers(1 << 30)), "fp64" if i % 2 == 0 else "mixed"))
+    return specs
+
+
+def _config4_member(spec):
+    import hqt
+    from hqt import ic
+    n, nz, mu, kk, nv, s, prec = spec
+    L = 50.0
+    st = hqt.Stepper(n, n, nz, L, L, mu=mu, dt=0.02, precision=prec, ax_b=ic.sinusoidal_drive(n, n, L, kk, 0.1))
+    r = np.random.default_rng(s)
+    vort = (r.uniform(0, L, nv), r.uniform(0, L, nv), np.where(np.arange(nv) % 2 == 0, 1.0, -1.0))
+    st.init_state(ic.zero_flow_profile(mu), phase_noise=1e-3, seed=s, vortices=vort)
+    return st
+
+
+def test_config4_ensemble_48_members():
+    import hqt
+    specs = _config4_specs()
+    nsteps = 5
+    print("ensemble of 48 members", flush=True)
+    ref = []
+    for spec in specs:
+        with _config4_member(spec) as st:
+            st.step(nsteps)
+            ref.append(st.get_state())
+    sts = [_config4_member(spec) for spec in specs]
+    try:
+        with hqt.Ensemble(sts) as ens:
+            ens.step(nsteps)
+            for i, st in enumerate(sts):
+                for a, b in zip(st.get_state(), ref[i]):
+                    assert np.array_equal(a, b), f"member {i} {specs[i]}"
+    finally:
+        for st in sts:
+            st.close()
+
+
+def test_config5_serving_stream():
+    import hqt
+    from hqt import ic, loadgen
+    from oracle import hqt_oracle as O
+    rng = np.random.default_rng(8)
+    newton = []
+    for _ in range(6):
+        nz = int(rng.integers(16, 49))
+        mu = float(rng.uniform(3.5, 10.0))
+        newton.append((nz, mu, loadgen.newton_guess(nz, mu)))
+    evo = []
+    for _ in range(4):
+        n = int(rng.integers(64, 302))
+        nz = int(rng.integers(12, 25))
+        mu = float(rng.uniform(4.1, 8.0))
+        steps = int(rng.integers(50, 121))
+        kk = int(rng.integers(3, 10))
+        s = int(rng.integers(1 << 30))
+        box = (n - 1) / 2
+        evo.append((n, nz, mu, steps, kk, box, ic.initial_state(n, n, nz, box, box, mu=mu, seed=s, n_vortices=4,
+                                                                  phase_noise=1e-3)))
+    print("serving stream: Newton nz", [x[0] for x in newton], "evolutions n", [x[0] for x in evo], flush=True)
+    with hqt.Server(64 << 30, quantum=25) as srv:
+        ids = []
+        for k in range(max(len(newton), len(evo))):
+            if k < len(evo):
+                n, nz, mu, steps, kk, box, state = evo[k]
+                ids.append(("evo", k, srv.submit_evolution(state, steps, n, n, nz, box, box, mu=mu, dt=0.02,
+                                                           ax_b=ic.sinusoidal_drive(n, n, box, kk, 0.1))))
+            if k < len(newton):
+                nz, mu, (Z1, Z2, V) = newton[k]
+                ids.append(("new", k, srv.submit_newton(nz, 1, 0, 0, Z1, Z2, V, loadgen.VY_REF)))
+        results = {(kind, k): srv.result(rid, timeout=120.0) for kind, k, rid in ids}
+    for k, (n, nz, mu, steps, kk, box, state) in enumerate(evo):
+        st_, lat, out = results[("evo", k)]
+        assert st_ == 0
+        with hqt.Stepper(n, n, nz, box, box, mu=mu, dt=0.02, ax_b=ic.sinusoidal_drive(n, n, box, kk, 0.1)) as st:
+            st.set_state(*state)
+            st.step(steps)
+            for a, b in zip(out, st.get_state()):
+                assert np.array_equal(a, b), f"evolution {k}"
+    for k, (nz, mu, (Z1, Z2, V)) in enumerate(newton):
+        st_, lat, (z1, z2, v, it, err) = results[("new", k)]
+        r1, r2, rv, rit, rerr = O.NewtonRaphson_vy(nz, 1, 0, 0, Z1, Z2, V, loadgen.VY_REF)
+        assert (st_ == 0) == (rit < 1000 and np.isfinite(rerr)), (k, nz, mu, st_, rit)
+        if st_ == 0:
+            assert it == rit, (k, it, rit)
+            for a, b in ((z1, r1), (z2, r2), (v, rv)):
+                assert np.max(np.abs(a - b)) <= 1e-9 * max(np.max(np.abs(b)), 1.0), (k, nz, mu)

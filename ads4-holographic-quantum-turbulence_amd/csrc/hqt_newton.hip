@@ -11,8 +11,10 @@
 // The 3nz x 3nz Jacobian lives in LDS for nz <= 41 (the reference's nz = 30 takes 65 KB) and
 // in a per-problem global workspace above that.  The radial operators Lz, Lz^2 and all
 // vectors are always in LDS.
-#include "hqt_kernels.h"
+#include "hqt_internal.h"
 #include <math.h>
+
+#include <mutex>
 
 namespace hqt {
 
@@ -305,3 +307,47 @@ hipError_t launch_newton(const NewtonJob* jobs, int njobs, int nz, hipStream_t s
 }
 
 }  // namespace hqt
+
+// ---- allocation-free batched Newton: hqt_newton_vy_batch_noalloc --------------------------------------
+// The device workspace of hqt_newton_vy_batch (operators, inputs, results, job descriptors and, for
+// nz > 41, the per-problem Jacobians) placed in a fixed array instead of a hipMalloc'd buffer.  Each GPU
+// holds its own copy of the array for the life of the process.  Its host-side name is not a device
+// address: the address is looked up once per device with hipGetSymbolAddress and kept in g_scratch.
+static __device__ __align__(256) unsigned char scratch[HQT_NEWTON_NOALLOC_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu[kMaxDevices];     // one call at a time per device: the calls share its array
+void* g_scratch[kMaxDevices] = {};        // device address of `scratch` on each device (0 until set up)
+}  // namespace
+
+extern "C" int hqt_newton_vy_batch_noalloc(int32_t nz, double zh, double small, double D, int32_t nb,
+                                           const double* Z1, const double* Z2, const double* vy1, const double* vy,
+                                           int32_t it_max, double tol, double* z1, double* z2, double* VY1,
+                                           int32_t* it, double* err, uint8_t* conv) {
+    static const char* who = "hqt_newton_vy_batch_noalloc";
+    if (int rc = newton_batch_args(who, nz, nb, Z1, Z2, vy1, vy, z1, z2, VY1, it, err, conv, it_max, tol)) return rc;
+    const size_t need = newton_batch_bytes(nz, nb);
+    if (need > sizeof(scratch))
+        return hqt_fail(HQT_ERR_ARG, std::string(who) + ": " + std::to_string(nb) + " problems of nz = " +
+                                         std::to_string(nz) + " need " + std::to_string(need) +
+                                         " bytes of device workspace, more than the " + std::to_string(sizeof(scratch)) +
+                                         "-byte allocation-free workspace (use hqt_newton_vy_batch or smaller batches)");
+    int dev = -1;
+    HIPC(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return hqt_fail(HQT_ERR_ARG, std::string(who) + ": device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu[dev]);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return hqt_fail(HQT_ERR_HIP, std::string(who) + ": hipGetSymbolAddress(scratch): " + hipGetErrorString(e));
+        g_scratch[dev] = p;
+    }
+    // the null stream: no stream is created; the library's own streams are non-blocking and do not wait on it
+    const int rc = newton_batch_run_in(nullptr, static_cast<unsigned char*>(g_scratch[dev]), nz, zh, small, D, nb, Z1,
+                                       Z2, vy1, vy, it_max, tol, z1, z2, VY1, it, err, conv);
+    if (rc) return rc;
+    return newton_batch_status(who, nb, conv);
+}
+

@@ -28,7 +28,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "csrc", "libhqt.so")
 EXPORTS = (
     "hqt_ctx_create", "hqt_ctx_destroy", "hqt_set_state", "hqt_get_state", "hqt_step_n", "hqt_sync",
     "hqt_last_winding", "hqt_rhs", "hqt_order_parameter", "hqt_ff", "hqt_fourdif", "hqt_chebdif",
-    "hqt_newton_vy", "hqt_newton_vy_batch", "hqt_vortex_census", "hqt_ensemble_create", "hqt_ensemble_step_n", "hqt_ensemble_sync",
+    "hqt_newton_vy", "hqt_newton_vy_batch", "hqt_newton_vy_batch_noalloc", "hqt_vortex_census", "hqt_ensemble_create", "hqt_ensemble_step_n", "hqt_ensemble_sync",
     "hqt_ensemble_launches", "hqt_ensemble_destroy", "hqt_server_create", "hqt_server_destroy",
     "hqt_server_submit_evolution", "hqt_server_submit_newton", "hqt_server_wait", "hqt_server_evolution_result",
     "hqt_server_newton_result", "hqt_server_stats", "hqt_server_counters", "hqt_ctx_footprint", "hqt_ctx_device_bytes",
@@ -127,6 +127,7 @@ def lib():
                                 C.POINTER(i32), dp]
     L.hqt_newton_vy_batch.argtypes = [i32, C.c_double, C.c_double, C.c_double, i32, dp, dp, dp, dp, i32,
                                       C.c_double, dp, dp, dp, C.POINTER(i32), dp, C.POINTER(C.c_uint8)]
+    L.hqt_newton_vy_batch_noalloc.argtypes = L.hqt_newton_vy_batch.argtypes
     L.hqt_debug_field.argtypes = [vp, C.c_char_p, dp]
     L.hqt_profile.argtypes = [vp, i32]
     L.hqt_calibration_copy.argtypes = [C.c_int64, dp]

@@ -101,12 +101,15 @@ def NewtonRaphson_vy(nz, zh, small, D, Z1, Z2, vy1, vy, raise_on_failure=False):
     return z1, z2, VY1, int(it.value), float(err.value)
 
 
-def NewtonRaphson_vy_batch(nz, zh, small, D, Z1, Z2, vy1, vy, it_max=1000, tol=1e-6, raise_on_failure=False):
+def NewtonRaphson_vy_batch(nz, zh, small, D, Z1, Z2, vy1, vy, it_max=1000, tol=1e-6, raise_on_failure=False,
+                           allocation_free=False):
     """NewtonRaphson_vy over a batch on the GPU (one work-group per problem).
 
     ``Z1, Z2, vy1`` are ``[nz, nb]`` (problem ``b`` in column ``b``; an nz-vector is broadcast
     over the batch), ``vy`` is ``[nb]``.  Returns ``z1, z2, VY1`` (``[nz, nb]``), ``it``,
-    ``err`` and ``conv`` (``[nb]``), each problem iterated exactly as ``NewtonRaphson_vy``."""
+    ``err`` and ``conv`` (``[nb]``), each problem iterated exactly as ``NewtonRaphson_vy``.
+    ``allocation_free=True`` runs ``hqt_newton_vy_batch_noalloc`` (no device allocation; the
+    workspace is a fixed per-device array of ``HQT_NEWTON_NOALLOC_BYTES``)."""
     nz = int(nz)
     vy = np.atleast_1d(np.asarray(vy, dtype=np.float64))
     nb = vy.size
@@ -125,7 +128,8 @@ def NewtonRaphson_vy_batch(nz, zh, small, D, Z1, Z2, vy1, vy, it_max=1000, tol=1
     it = np.zeros(nb, dtype=np.int32)
     err = np.zeros(nb)
     conv = np.zeros(nb, dtype=np.uint8)
-    rc = lib().hqt_newton_vy_batch(nz, float(zh), float(small), float(D), nb, dptr(a), dptr(b), dptr(v), dptr(vy),
+    fn = lib().hqt_newton_vy_batch_noalloc if allocation_free else lib().hqt_newton_vy_batch
+    rc = fn(nz, float(zh), float(small), float(D), nb, dptr(a), dptr(b), dptr(v), dptr(vy),
                                    int(it_max), float(tol), dptr(z1), dptr(z2), dptr(VY1),
                                    it.ctypes.data_as(C.POINTER(C.c_int32)), dptr(err),
                                    conv.ctypes.data_as(C.POINTER(C.c_uint8)))

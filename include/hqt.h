@@ -251,6 +251,23 @@ int hqt_newton_vy_batch(int32_t nz, double zh, double small, double D, int32_t n
                         int32_t it_max, double tol, double* z1, double* z2, double* VY1,
                         int32_t* it, double* err, uint8_t* conv);
 
+/* hqt_newton_vy_batch without any device allocation, for callers that cannot call hipMalloc
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid allocation).  No
+ * counterpart in the reference.  Same arguments, results and status as hqt_newton_vy_batch (the same
+ * kernel on the same inputs: bitwise the same outputs); the device workspace (operators, inputs,
+ * results, job descriptors and, for nz > 41, the 3nz x 3nz Jacobians) lives in a fixed per-device
+ * array of HQT_NEWTON_NOALLOC_BYTES bytes that each GPU the library runs on holds until the process
+ * ends.  A batch needs 16 nz^2 + 48 nz nb + 144 nb bytes, plus 72 nz^2 nb for nz > 41 (about 8000
+ * problems fit at nz <= 41, 100 at nz = 48, 56 at nz = 64); a larger one is rejected with
+ * HQT_ERR_ARG before any GPU work.  Calls on one GPU share its array: the library
+ * runs them one at a time (a per-device lock), so calls from several host threads wait for one
+ * another.  Runs on the device's null stream and returns when the results are on the host. */
+#define HQT_NEWTON_NOALLOC_BYTES (16u << 20)
+int hqt_newton_vy_batch_noalloc(int32_t nz, double zh, double small, double D, int32_t nb,
+                                const double* Z1, const double* Z2, const double* vy1, const double* vy,
+                                int32_t it_max, double tol, double* z1, double* z2, double* VY1,
+                                int32_t* it, double* err, uint8_t* conv);
+
 /* Copy one internal device field of the last hqt_rhs / hqt_step_n into out ([nx,ny,nz], FP64
  * values).  Names: "dxR" "dxI" "dxxR" "dxxI" "dxAx" "dxAy" "dxxAy" (x-pass outputs), "dyR" "dyI"
  * "lapR" "lapI" "divA" "curl3" "curl4" (y-pass outputs), "dxAt" "dyAt" "dzAt" (frozen A_t

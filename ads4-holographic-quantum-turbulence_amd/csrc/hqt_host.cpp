@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "hqt_internal.h"
+#include "hqt_spec_core.h"
 
 namespace {
 thread_local std::string g_err;
@@ -55,6 +56,10 @@ int64_t ctx_footprint(const hqt_params& p) {
     b += 6 * ((int64_t)line_geom(p.nx).TL + (int64_t)line_geom(p.ny).TL) * es;   // x/y operator tables
     const bool fold = !(p.flags & HQT_DENSE_LINES);
     if (fold) b += 6 * ((int64_t)fold_geom(p.nx).TL + (int64_t)fold_geom(p.ny).TL) * es;   // folded tables
+    const bool factored = p.precision == HQT_MIXED ? (p.flags & HQT_FACTORED_STAGES) != 0 : !(p.flags & HQT_FOLDED_STAGES);
+    if (fold && factored)                                 // trig tables of the factored-DFT stage passes
+        for (int n : {p.nx, p.ny})
+            if (spec_fact_id(n) >= 0) b += (int64_t)spec_table_len(spec_fact_id(n)) * es;
     bool lr[2] = {false, false};
     for (int d = 0; d < 2; ++d) {                         // rank-4 filter tables, when used
         if (p.flags & HQT_DENSE_FILTER) break;
@@ -276,6 +281,63 @@ int build_tables(hqt_ctx* c, int N, double L, const uint8_t* mask, int base) {
     return HQT_OK;
 }
 
+// cos and sin of 2 pi a / b in long double, exact at the quarter turns.
+void cos_sin_2pi(long long a, long long b, double& c, double& sn) {
+    a %= b;
+    if (a < 0) a += b;
+    if (4 * a % b == 0) {
+        const int q = (int)(4 * a / b);
+        c = q == 0 ? 1.0 : q == 2 ? -1.0 : 0.0;
+        sn = q == 1 ? 1.0 : q == 3 ? -1.0 : 0.0;
+        return;
+    }
+    const long double t = 2.0L * 3.14159265358979323846264338327950288L * (long double)a / (long double)b;
+    c = (double)cosl(t);
+    sn = (double)sinl(t);
+}
+
+// Trig table of the factored-DFT passes for line length N (SpecGeom layout), in the context precision;
+// none when N has no factored form.
+int build_spec_table(hqt_ctx* c, int N, int dir) {
+    const int fid = spec_fact_id(N);
+    if (fid < 0) return HQT_OK;
+    const int N1 = kSpecFact[fid][0], N2 = kSpecFact[fid][1], H = N2 / 2;
+    std::vector<double> t((size_t)spec_table_len(fid));
+    for (int m = 0; m <= H; ++m) cos_sin_2pi(m, N2, t[m], t[H + 1 + m]);
+    for (int m = 0; m < N1; ++m) cos_sin_2pi(m, N1, t[2 * (H + 1) + m], t[2 * (H + 1) + N1 + m]);
+    const size_t tw = 2 * (H + 1) + 2 * (size_t)N1;
+    for (int k2 = 0; k2 <= H; ++k2)
+        for (int n1 = 0; n1 < N1; ++n1)
+            cos_sin_2pi((long long)n1 * k2, N, t[tw + 2 * ((size_t)k2 * N1 + n1)], t[tw + 2 * ((size_t)k2 * N1 + n1) + 1]);
+    int rc = dalloc(c, t.size() * c->es, &c->stab[dir]);
+    if (rc) return rc;
+    return upload_as(c, c->stab[dir], t.data(), t.size(), c->tdt);
+}
+
+// An RK-stage pass (groups of d/dx, d2/dx2, -d/dx terms only) in the factored-DFT form, when its line
+// length has one; otherwise unchanged.
+int make_spec_pass(hqt_ctx* c, LPass& P, int dir) {
+    const int N = P.N, fid = spec_fact_id(N);
+    if (fid < 0 || !c->stab[dir]) return HQT_OK;
+    for (int g = 0; g < P.ngroups; ++g)
+        for (int o = 0; o < P.g[g].nout; ++o)
+            for (int e = 0; e < P.g[g].out[o].nterms; ++e) {
+                LTerm& tm = P.g[g].out[o].term[e];
+                const int op = tm.lr ? -1 : P.g[g].slot_op[tm.slot];
+                if (op == OP_D1X || op == OP_D1Y) tm.sk = 1;
+                else if (op == OP_D2X || op == OP_D2Y) tm.sk = 2;
+                else if (op == OP_ND1X || op == OP_ND1Y) tm.sk = 3;
+                else return fail(HQT_ERR_ARG, "internal: factored-DFT pass with a term that is not a derivative");
+            }
+    P.fold = 3;
+    P.Mp = 64 * (fid + 1);
+    P.S = N | 1;
+    P.sp = 2.0 * M_PI / (dir ? c->p.Ly : c->p.Lx);
+    P.st = c->stab[dir];
+    set_lb(P, 16, dir, 1);
+    return HQT_OK;
+}
+
 template <class F>
 int run(hqt_ctx* c, int name, F&& f) {
     hipEvent_t a = nullptr, b = nullptr;
@@ -357,6 +419,7 @@ int tune_lines(hqt_ctx* c) {
             for (int order : {0, 1}) {
               if (order && !H.fold) continue;                      // the dense kernels keep the launch order
               for (int lb : {16, 32}) {
+                if (lb == 32 && H.fold >= 3) continue;              // the factored-DFT kernel takes 16 lines
                 for (int tpb : {1, 2, 4}) {
                     if (dir == 1 && tpb != 1 && !H.fold) continue;   // dense y-lines: one tile per work-group
                     set_lb(H, lb, dir, tpb);
@@ -386,7 +449,7 @@ int tune_lines(hqt_ctx* c) {
         }
         // choice = 10000 order + 100 lines + tiles per work-group; the dense y-line kernel processes one
         // tile per work-group and keeps the launch order
-        set_lb(H, (choice % 10000) / 100, dir, (dir == 1 && !H.fold) ? 1 : choice % 100);
+        set_lb(H, H.fold >= 3 ? 16 : (choice % 10000) / 100, dir, (dir == 1 && !H.fold) ? 1 : choice % 100);
         H.xcd = H.fold ? choice / 10000 : 0;
         if (line_pass_lds_bytes(H, c->fp32) > kLdsCap) set_lb(H, 16, dir, 1);
     }
@@ -584,6 +647,19 @@ int build_context(hqt_ctx* c) {
     void* SR_[4] = {c->SB[0], c->SB[1], c->SB[2], c->SB[3]};
     stage_x(c->hp[PXR], SR_, c->AtR);
     stage_y(c->hp[PYR], SR_, c->AtR);
+    // the RK stages' (and the RHS entry's) derivative passes in the factored-DFT form where it exists: by
+    // default in FP64, where the VALU transforms beat the FP64 matrix products; FP32 matrix products run at
+    // twice the FP64 rate while the FP32 vector FMA does not, so the mixed form keeps the folded passes
+    // unless HQT_FACTORED_STAGES asks for them
+    const bool factored = c->fp32 ? (c->p.flags & HQT_FACTORED_STAGES) != 0 : !(c->p.flags & HQT_FOLDED_STAGES);
+    if (c->fold && factored) {
+        if ((rc = build_spec_table(c, nx, 0))) return rc;
+        if ((rc = build_spec_table(c, ny, 1))) return rc;
+        for (int p : {PX1, PX2, PX3, PX4, PXR})
+            if ((rc = make_spec_pass(c, c->hp[p], 0))) return rc;
+        for (int p : {PY1, PY2, PY3, PY4, PYR})
+            if ((rc = make_spec_pass(c, c->hp[p], 1))) return rc;
+    }
     {
         LPass& P = c->hp[PXC];
         P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo, (int)c->es, c->ps);
@@ -647,7 +723,10 @@ int build_context(hqt_ctx* c) {
             if (H.fold && H.g[g].ntiles > 1 && H.g[g].nout > 1)
                 return fail(HQT_ERR_ARG, "internal: folded line pass group with two sources and two outputs");
             for (int t = 0; t < H.g[g].ntiles; ++t)
-                if (H.fold && c->fp32 && H.g[g].src_dt[t] == DT_F64) H.fold = 2;
+                if (c->fp32 && H.g[g].src_dt[t] == DT_F64) {
+                    if (H.fold == 1) H.fold = 2;          // folded, double prefetch registers
+                    else if (H.fold == 3) H.fold = 4;     // factored-DFT, double prefetch registers
+                }
         }
     HIPC(hipMemcpy(c->dp, c->hp, sizeof(c->hp), hipMemcpyHostToDevice));
 

@@ -64,6 +64,7 @@ struct hqt_ctx {
     void* ops[OP_COUNT];
     void* fops[OP_COUNT];            // folded tables of the same operators (NULL where unused)
     int fold = 1;                    // line passes in the even/odd folded form
+    void* stab[2] = {nullptr, nullptr};   // trig tables of the factored-DFT passes (x, y), or NULL
     int gfilt = 0;                   // folded filter with the cross term through the 4 x 4 slice matrices
     int64_t gblk = 0;                // 16-line blocks of the x-lines (partials of the cross term)
     double* Gp = nullptr;            // [5 fields][gblk][2][16] cross-term partials

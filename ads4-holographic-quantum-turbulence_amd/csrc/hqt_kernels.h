@@ -77,7 +77,8 @@ constexpr int LP_THREADS = 256;
 // lr = 1: rank-4 operator U V^T with the [V (Np x 4) | U (Mp x 4)] row-major table at P.ops[slot].
 // pad: parity of a dense operator's column (+1 even, -1 odd); rsum: the operator's row sum
 // (C 1 = rsum 1: 1 for the filter's real part, 0 for the derivatives).  Both used by the folded form.
-struct LTerm { int32_t tile; int32_t slot; int32_t lr; int32_t pad; int32_t rsum; };
+// sk (factored-DFT passes): the term's spectral multiplier, 1 d/dx, 2 d2/dx2, 3 -d/dx.
+struct LTerm { int32_t tile; int32_t slot; int32_t lr; int32_t pad; int32_t rsum; int32_t sk; };
 // gmode (folded filter, the rank-4 x rank-4 cross term of Re((P_x (x) P_y) x) as U_x G U_y^T per z-slice):
 //   1 (x-pass): instead of storing the output, write per-16-line-block partials of G = D_x V_y, D_x the
 //     output's rank-4 dots, to gp ([block][2 slices][4 x 4] doubles; gtab: V_y, ny x 4);
@@ -108,11 +109,16 @@ struct LPass {
     const void* ops[LP_MAXOPS]; // Toeplitz tables: tab[i - j + Np + 4] = T[i][j], TL entries
                                 // (folded form: tab[d + Z] = c[d mod N] / 2, TL entries)
     int32_t fold;              // even/odd folded form (hqt_fold_core.h; Mp = 64 QF, S its LDS row stride):
-                               // 1, or 2 when a source is double in an FP32 pass (wide prefetch registers)
+                               // 1, or 2 when a source is double in an FP32 pass (wide prefetch registers);
+                               // 3: the factored-DFT form (hqt_spec_core.h), 4 the same with double prefetch
     int32_t Kp, Z;             // folded form: padded folded positions, table offset
     int32_t xcd;               // folded form: block order of the launch (1: XCD-aware), set by the autotuner
     int32_t* flags;            // [0] = NaN step (0: none), [1] = step counter
     int32_t ps, pad;           // plane stride of the volume fields (elements) and ps - nx*ny
+    // factored-DFT form (fold = 3 or 4, hqt_spec_core.h): Mp = 64 (factorization index + 1), S the LDS row
+    // stride, sp = 2 pi / L (the derivative scale of the direction), st its trig table (pass precision)
+    double sp;
+    const void* st;
 };
 
 // Volume fields are [nx, ny, nz] with planes ps >= nx*ny elements apart (ps a multiple of 32, so

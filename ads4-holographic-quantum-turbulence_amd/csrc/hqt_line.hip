@@ -1,6 +1,7 @@
 // Line-operator passes: dispatch over the (precision, direction) families whose kernels are
 // compiled in hqt_line_*.hip (the kernel itself is in hqt_line_core.h).
 #include "hqt_fold_core.h"
+#include "hqt_spec_core.h"
 
 namespace hqt {
 
@@ -12,10 +13,19 @@ extern template hipError_t fold_launch_td<double, 0>(const LPass*, Members, dim3
 extern template hipError_t fold_launch_td<double, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int, int);
 extern template hipError_t fold_launch_td<float, 0>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int, int);
 extern template hipError_t fold_launch_td<float, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int, int);
+extern template hipError_t spec_launch_td<double, 0>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
+extern template hipError_t spec_launch_td<double, 1>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
+extern template hipError_t spec_launch_td<float, 0>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
+extern template hipError_t spec_launch_td<float, 1>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
 extern template hipError_t fold_pair_launch_cons<double>(const LPass*, const LPass*, int, int, int, size_t, hipStream_t, int, int);
 extern template hipError_t fold_pair_launch_cons<float>(const LPass*, const LPass*, int, int, int, size_t, hipStream_t, int, int);
 
 size_t line_pass_lds_bytes(const LPass& P, int fp32) {
+    if (P.fold == 3 || P.fold == 4) {
+        // factored-DFT form: the line region and the inverse region (lb x S each) and the trig table
+        const int fid = P.Mp / 64 - 1;
+        return (fp32 ? sizeof(float) : sizeof(double)) * (2 * (size_t)P.lb * P.S + (size_t)spec_table_len(fid));
+    }
     int maxslots = 0;
     for (int g = 0; g < P.ngroups; ++g) maxslots = P.g[g].nslots > maxslots ? P.g[g].nslots : maxslots;
     const size_t es = fp32 ? sizeof(float) : sizeof(double);
@@ -30,6 +40,14 @@ hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, 
                             int fp32, int kind, hipStream_t s, int fold, int order) {
     if (Q < 1 || Q > 8 || blocks <= 0 || (nt != 1 && nt != 2) || lds > 160 * 1024) return hipErrorInvalidValue;
     const dim3 grid((unsigned)blocks, (unsigned)ngroups);
+    if (fold == 3 || fold == 4) {
+        if (nt != 1 || Q > kSpecNF) return hipErrorInvalidValue;
+        const int wide = fold == 4;
+        if (fp32) return dir ? spec_launch_td<float, 1>(d, M, grid, Q - 1, lds, s, order, wide)
+                             : spec_launch_td<float, 0>(d, M, grid, Q - 1, lds, s, order, wide);
+        return dir ? spec_launch_td<double, 1>(d, M, grid, Q - 1, lds, s, order, wide)
+                   : spec_launch_td<double, 0>(d, M, grid, Q - 1, lds, s, order, wide);
+    }
     if (fold) {
         if (Q > 5) return hipErrorInvalidValue;
         const int wide = fold == 2;

@@ -21,6 +21,8 @@ HQT_RADIAL_W3 = 16
 HQT_DENSE_FILTER = 32
 HQT_DENSE_LINES = 64
 HQT_ORDER_XCD = 128
+HQT_FOLDED_STAGES = 256
+HQT_FACTORED_STAGES = 512
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "csrc", "libhqt.so")

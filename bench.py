@@ -66,6 +66,38 @@ def line_pass_flops(name, nx, ny, nz, folded=True):
     return dense * per * S + lr * lr_flops * S
 
 
+def line_pass_fields(name, stage_mix=(1, 3)):
+    """Fields one launch of a line pass reads (sources and addends) and writes, averaged over the RK stages
+    for the stage passes (stage 1 adds the derivatives of A_t)."""
+    if name == "line_x_stage":      # reads Re/Im Psi, A_x, A_y (+ A_t); writes 7 (+ dx A_t)
+        return (13 * stage_mix[0] + 11 * stage_mix[1]) / sum(stage_mix)
+    if name == "line_y_stage":      # reads 5 sources + 4 addends (+ A_t); writes 7 (+ dy A_t)
+        return (18 * stage_mix[0] + 16 * stage_mix[1]) / sum(stage_mix)
+    if name in ("line_x_filter", "line_y_filter"):
+        return 10
+    return None
+
+
+def factored_transforms(name, stage_mix=(1, 3)):
+    """DFTs (forward + inverse) of one launch of a factored-DFT stage pass, per line."""
+    if name == "line_x_stage":      # R, I: 1 + 2 each; A_x: 1 + 1; A_y: 1 + 2 (+ A_t: 1 + 1)
+        return (13 * stage_mix[0] + 11 * stage_mix[1]) / sum(stage_mix)
+    if name == "line_y_stage":      # R, I: 1 + 2 each; A_y: 1 + 1; (A_x, dx A_y): 2 + 1; dx A_x: 1 + 1 (+ A_t: 1 + 1)
+        return (15 * stage_mix[0] + 13 * stage_mix[1]) / sum(stage_mix)
+    return None
+
+
+def factored_flops_per_point(N1, N2):
+    """FLOPs per point of one factored real DFT of length N1 N2 (forward or inverse): stage A, the even and
+    odd halves of N1 real DFTs of length N2, (H+1) H + HI^2 multiply-adds each; stage B, per kept column
+    the twiddle (4 flops per point) and the folded complex DFT of length N1 (4 multiply-adds per term and
+    output pair)."""
+    H, HI, H1, HI1 = N2 // 2, (N2 - 1) // 2, N1 // 2, (N1 - 1) // 2
+    stage_a = 2 * N1 * ((H + 1) * H + HI * HI)
+    stage_b = (H + 1) * (4 * N1 + 2 * 4 * HI1 * (H1 + 1))
+    return (stage_a + stage_b) / (N1 * N2)
+
+
 def profiled_traffic(kernel):
     """HBM bytes per launch of ``kernel`` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
     (calibrated against a plain copy kernel), or None when no summary covers it."""
@@ -87,10 +119,10 @@ def step_flops(nx, ny, nz, folded):
     return tot
 
 
-def run_gpu(nx, ny, nz, L, precision, steps, warmup, state, axb, profile_steps, dense_lines=False):
+def run_gpu(nx, ny, nz, L, precision, steps, warmup, state, axb, profile_steps, dense_lines=False, folded=False):
     import hqt
     st = hqt.Stepper(nx, ny, nz, L, L, mu=6.0, dt=0.02, winding=True, filter=True, precision=precision,
-                     ax_b=axb, ay_b=0.0, dense_lines=dense_lines)
+                     ax_b=axb, ay_b=0.0, dense_lines=dense_lines, factored=False if folded else None)
     st.set_state(*state)
     st.step(warmup)
     st.sync()
@@ -300,6 +332,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--serving-rate", type=float, default=2.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serving-seconds", type=float, default=15.0)
+    ap.add_argument("--folded", action="store_true",
+                    help="RK-stage x/y derivatives as folded matrix products instead of the factored-DFT form")
     ap.add_argument("--dense-lines", action="store_true",
                     help="x/y operators as dense Toeplitz products (the reference's form) instead of the folded form")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
@@ -327,7 +361,8 @@ def main():
     S = nx * ny * nz
 
     prof_steps = max(5, min(args.steps, 50))
-    t64, fin64, stats = run_gpu(nx, ny, nz, L, "fp64", args.steps, args.warmup, state, axb, prof_steps, args.dense_lines)
+    t64, fin64, stats = run_gpu(nx, ny, nz, L, "fp64", args.steps, args.warmup, state, axb, prof_steps, args.dense_lines,
+                                args.folded)
     sps = args.steps / t64
 
     # dominant kernel roofline (HIP events on the library's stream)
@@ -336,20 +371,28 @@ def main():
         kernels[k] = {"launches": cnt, "avg_us": 1e3 * ms / cnt, "ms_per_step": ms / prof_steps}
     dom = max(stats.items(), key=lambda kv: kv[1][1])[0] if stats else None
     roof = None
-    if dom is not None:
+    fact = {301: (7, 43), 32: (4, 8), 64: (8, 8), 96: (8, 12), 128: (8, 16), 192: (8, 24), 256: (8, 32)}
+    factored_form = not args.dense_lines and not args.folded and nx == ny and nx in fact
+    if dom is not None and line_pass_fields(dom) is not None:
         cnt, ms = stats[dom]
-        fl = line_pass_flops(dom, nx, ny, nz, folded=not args.dense_lines)
         avg_s = ms / cnt / 1e3
+        alg_bytes = line_pass_fields(dom) * S * 8
+        ach = alg_bytes / avg_s / 1e9
+        tr = profiled_traffic(dom + ("_dense" if args.dense_lines else "")) if (nx, ny, nz) == (301, 301, 20) else None
+        roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": tr, "avg_launch_us": round(avg_s * 1e6, 2),
+                "algorithmic_bytes_per_launch": alg_bytes,
+                "bytes_form": "fields read (sources, addends) + fields written, 8 B per point each"}
+        if factored_form and factored_transforms(dom) is not None:
+            fl = factored_transforms(dom) * factored_flops_per_point(*fact[nx]) * S
+            form = "factored DFT (%d x %d): each transform %.1f flops per point" % (fact[nx] + (factored_flops_per_point(*fact[nx]),))
+        else:
+            fl = line_pass_flops(dom, nx, ny, nz, folded=not args.dense_lines)
+            form = ("dense operator, 2 N flops per point per application" if args.dense_lines else
+                    "folded (even/odd) operator, 2 (N/2+1) flops per point per application")
         if fl is not None:
-            ach = fl / avg_s / 1e12
-            tr = profiled_traffic(dom + ("_dense" if args.dense_lines else "")) if (nx, ny, nz) == (301, 301, 20) else None
-            roof = {"kernel": dom, "bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS,
-                    "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": tr,
-                    "avg_launch_us": round(avg_s * 1e6, 2),
-                    "flops_per_launch": fl,
-                    "flops_form": ("dense operator, 2 N flops per point per application" if args.dense_lines else
-                                   "folded (even/odd) operator, 2 (N/2+1) flops per point per application"),
-                    "dense_form_equivalent_TFLOPs": round(line_pass_flops(dom, nx, ny, nz, folded=False) / avg_s / 1e12, 3)}
+            roof["flops"] = {"achieved": round(fl / avg_s / 1e12, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                             "frac": round(fl / avg_s / 1e12 / PEAK_FP64_TFLOPS, 4), "flops_per_launch": fl, "form": form}
 
     step_bytes = 368.0 * S          # compulsory FP64 bytes per step (4 stages + post-step)
     out = {
@@ -371,7 +414,12 @@ def main():
         "step_hbm": {"algorithmic_bytes_per_step": step_bytes,
                      "achieved_GBps": round(step_bytes * sps / 1e9, 2),
                      "frac_of_peak": round(step_bytes * sps / 1e9 / PEAK_HBM_GBS, 4)},
-        "step_flops": {"folded_form": step_flops(nx, ny, nz, True), "dense_form": step_flops(nx, ny, nz, False)},
+        "step_flops": {"folded_form": step_flops(nx, ny, nz, True), "dense_form": step_flops(nx, ny, nz, False),
+                       "factored_stages": (4 * (factored_transforms("line_x_stage") + factored_transforms("line_y_stage"))
+                                           * factored_flops_per_point(*fact[nx]) * S
+                                           + step_flops(nx, ny, nz, True)
+                                           - 4 * (line_pass_flops("line_x_stage", nx, ny, nz) + line_pass_flops("line_y_stage", nx, ny, nz)))
+                       if nx == ny and nx in fact else None},
         "roofline": roof,
         "kernels": kernels,
     }

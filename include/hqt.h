@@ -49,6 +49,12 @@ extern "C" {
 #define HQT_DENSE_LINES 64  /* x/y operators and filter as dense Toeplitz products (func_Lx's own form)
                                instead of their even/odd folded form (the default: half the
                                multiply-adds; results agree to rounding) */
+#define HQT_FOLDED_STAGES 256 /* FP64: the RK stages' x/y derivatives as even/odd folded matrix products even
+                               where the line length has a factored-DFT form (the FP64 default there,
+                               N = 32, 64, 96, 128, 192, 256, 301: each line's DFT once, the operators'
+                               eigenvalues, one inverse per output; results agree to rounding) */
+#define HQT_FACTORED_STAGES 512 /* HQT_MIXED: the factored-DFT form of the RK stages' x/y derivatives where it
+                               exists (the mixed form's default is the folded FP32 matrix products) */
 
 #define HQT_FP64   0        /* plain form: FP64 end to end */
 #define HQT_MIXED  1        /* x/y operators, filter, stage arithmetic FP32; z operators FP64 */

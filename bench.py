@@ -378,7 +378,8 @@ def main():
         avg_s = ms / cnt / 1e3
         alg_bytes = line_pass_fields(dom) * S * 8
         ach = alg_bytes / avg_s / 1e9
-        tr = profiled_traffic(dom + ("_dense" if args.dense_lines else "")) if (nx, ny, nz) == (301, 301, 20) else None
+        suffix = "_dense" if args.dense_lines else "_folded" if args.folded else ""
+        tr = profiled_traffic(dom + suffix) if (nx, ny, nz) == (301, 301, 20) else None
         roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": tr, "avg_launch_us": round(avg_s * 1e6, 2),
                 "algorithmic_bytes_per_launch": alg_bytes,

@@ -194,7 +194,8 @@ int hqt_checkpoint_save_async(hqt_ctx* c, const char* path) {
     if (!c->have_state) return hqt_fail(HQT_ERR_STATE, "no state uploaded");
     if (int rc = wait_writer(c)) return rc;
     // snapshots keep the device layout (planes ps apart); the copy to the host drops the padding
-    const size_t n = (size_t)c->npts, es = c->es, nv = (size_t)c->nvol, nxy = (size_t)c->nxy, ps = (size_t)c->ps;
+    const size_t n = (size_t)c->npts, es = c->es, nv = (size_t)c->nvol, nxy = (size_t)c->nxy, ps = (size_t)c->ps,
+                 rp = (size_t)c->rp;
     if (!c->ckpt) {
         CheckpointWriter* w = new (std::nothrow) CheckpointWriter();
         if (!w) return hqt_fail(HQT_ERR_OOM, "host allocation failed");
@@ -220,16 +221,18 @@ int hqt_checkpoint_save_async(hqt_ctx* c, const char* path) {
     w->msg.clear();
     const int nx = c->nx, ny = c->ny, nz = c->nz, fp32 = c->fp32;
     const std::string file(path);
-    w->th = std::thread([w, n, es, nxy, ps, nx, ny, nz, fp32, file] {
+    w->th = std::thread([w, n, es, nxy, ps, rp, nx, ny, nz, fp32, file] {
         double* h = static_cast<double*>(w->host);
         auto fail_w = [w](int code, const std::string& m) { w->status = code; w->msg = m; };
         if (hipStreamWaitEvent(w->copy, w->ready, 0) != hipSuccess) return fail_w(HQT_ERR_HIP, "checkpoint: wait");
         // TS fields land in the first half of each double slot region, FP64 fields directly
         for (int k = 0; k < 6; ++k) {
             const size_t e = k < 4 ? es : 8;
-            if (hipMemcpy2DAsync(h + k * n, nxy * e, w->snap[k], ps * e, nxy * e, (size_t)nz, hipMemcpyDeviceToHost,
-                                 w->copy) != hipSuccess)
-                return fail_w(HQT_ERR_HIP, "checkpoint: copy");
+            for (size_t z = 0; z < (size_t)nz; ++z)     // one pitched copy per plane (rows rp, planes ps apart)
+                if (hipMemcpy2DAsync(reinterpret_cast<char*>(h + k * n) + z * nxy * e, (size_t)nx * e,
+                                     static_cast<const char*>(w->snap[k]) + z * ps * e, rp * e, (size_t)nx * e,
+                                     (size_t)ny, hipMemcpyDeviceToHost, w->copy) != hipSuccess)
+                    return fail_w(HQT_ERR_HIP, "checkpoint: copy");
         }
         if (hipStreamSynchronize(w->copy) != hipSuccess) return fail_w(HQT_ERR_HIP, "checkpoint: sync");
         if (fp32) {

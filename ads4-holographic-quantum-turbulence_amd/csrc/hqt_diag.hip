@@ -12,8 +12,9 @@ constexpr int DG_T = 256;
 // O(i,j) = sum_m Lz(nz,m) Psi(i,j,m): one thread per column, coalesced over the plane.
 template <typename TS>
 __global__ __launch_bounds__(DG_T) void k_order_param(const Census A) {
-    const int64_t col = (int64_t)blockIdx.x * DG_T + threadIdx.x;
-    if (col >= A.nxy) return;
+    const int64_t p = (int64_t)blockIdx.x * DG_T + threadIdx.x;     // point (i, j) of the plane, i + nx j
+    if (p >= A.nxy) return;
+    const int64_t col = p % A.nx + A.rp * (p / A.nx);                // its column in the padded layout
     const TS* R = (const TS*)A.R;
     const TS* I = (const TS*)A.I;
     double a = 0.0, b = 0.0;
@@ -22,8 +23,8 @@ __global__ __launch_bounds__(DG_T) void k_order_param(const Census A) {
         a += l * (double)R[col + A.ps * m];
         b += l * (double)I[col + A.ps * m];
     }
-    A.O[2 * col] = a;
-    A.O[2 * col + 1] = b;
+    A.O[2 * p] = a;
+    A.O[2 * p + 1] = b;
 }
 
 // Plaquette (i,j) with corners x1=O(i,j), x2=O(i+1,j), x3=O(i+1,j+1), x4=O(i,j+1) (periodic):

@@ -142,7 +142,7 @@ void add_z(std::vector<Launch>& plan, PlanImage& img, const std::vector<hqt_ctx*
 void build_plan(hqt_ensemble* e, PlanImage& img, const std::vector<hqt_ctx*>& M, int lane) {
     auto& P = e->plan;
     const size_t first = P.size();
-    auto zb = [](hqt_ctx* c) { return zblocks(c->nxy); };
+    auto zb = [](hqt_ctx* c) { return zblocks(c->ncol); };
     for (int s = 0; s < 4; ++s) {
         add_line(P, img, M, PX1 + s, 0, KN_LX_STAGE, all_members);
         add_line(P, img, M, PY1 + s, 1, KN_LY_STAGE, all_members);
@@ -154,7 +154,7 @@ void build_plan(hqt_ensemble* e, PlanImage& img, const std::vector<hqt_ctx*>& M,
     add_z<Wind>(P, img, M, K_WPART, false, with_winding, [](hqt_ctx* c) { return c->wind; },
                 [](hqt_ctx* c) { return c->ny; });
     add_z<Wind>(P, img, M, K_WAPPLY, false, with_winding, [](hqt_ctx* c) { return c->wind; },
-                [](hqt_ctx* c) { return wind_apply_blocks(c->nxy, c->nz); });
+                [](hqt_ctx* c) { return wind_apply_blocks(c->ncol, c->nz); });
     add_line(P, img, M, PXF, 0, KN_LX_FILT, with_filter);
     add_line(P, img, M, PYF, 1, KN_LY_FILT, with_filter);
     add_z<NanCheck>(P, img, M, K_NAN, false, without_filter, [](hqt_ctx* c) { return c->nan; },

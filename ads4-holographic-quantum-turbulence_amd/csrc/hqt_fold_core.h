@@ -179,7 +179,7 @@ __device__ __forceinline__ bool fold_out(const LPass& P, const LOut& O, typename
                                          typename MF<T>::v4 (&aO)[NT][QF], const T (&cst)[NT][4], int line0, int x0,
                                          int z, int nl, int N, int role, int lane) {
     const int ln = lane & 15, lk = lane >> 4;
-    const int nx = P.nx, pl = P.ps * z, Mh = N >> 1;
+    const int nx = P.nx, rp = P.rp, pl = P.ps * z, Mh = N >> 1;
     // x-lines: element offsets of the 4 NT lines this lane stores (z = the z-slice of line0)
     int xb[NT][4];
 #pragma unroll
@@ -201,7 +201,7 @@ __device__ __forceinline__ bool fold_out(const LPass& P, const LOut& O, typename
         const bool ok = mirror ? (okl && i >= 1 && 2 * i < N) : (okl && i <= Mh);
         const int pos = mirror ? N - i : i;
         if (!ok) return Z_OOB;
-        return DIR == 0 ? (uint32_t)(xb[u][r] + pos) : (uint32_t)(line + nx * pos + pl);
+        return DIR == 0 ? (uint32_t)(xb[u][r] + pos) : (uint32_t)(line + rp * pos + pl);
     };
     bool bad = false;
 #pragma unroll
@@ -379,8 +379,8 @@ __device__ __forceinline__ void fold_pass_body(const LPass* __restrict__ Pp, con
                         const int x = x0 + mrow;
                         const bool okl = x < P.nx;
                         const int base = x + P.ps * z;
-                        e0 = (okl && j <= Mh) ? (uint32_t)(base + P.nx * j) : Z_OOB;
-                        e1 = (okl && j >= 1 && 2 * j < N) ? (uint32_t)(base + P.nx * (N - j)) : Z_OOB;
+                        e0 = (okl && j <= Mh) ? (uint32_t)(base + P.rp * j) : Z_OOB;
+                        e1 = (okl && j >= 1 && 2 * j < N) ? (uint32_t)(base + P.rp * (N - j)) : Z_OOB;
                     }
                     pa[h][r] = (R)bldr<S_>(rs, e0);
                     pb[h][r] = (R)bldr<S_>(rs, e1);

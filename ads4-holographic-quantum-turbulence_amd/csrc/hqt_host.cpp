@@ -42,14 +42,15 @@ bool gfilt_on(const hqt_params& p, bool lrx, bool lry) {
 }
 
 int64_t ctx_footprint(const hqt_params& p) {
-    const int64_t nxy = (int64_t)p.nx * p.ny, np = plane_stride(nxy) * p.nz, nz = p.nz;
+    const int64_t nxy = (int64_t)p.nx * p.ny, ncol = row_stride(p.nx) * p.ny, np = plane_stride(ncol) * p.nz,
+                  nz = p.nz;
     const int64_t es = p.precision == HQT_MIXED ? 4 : 8;
     auto r8 = [](int64_t b) { return b == 0 ? 8 : b; };
     int64_t b = 0;
     b += 42 * np * es;                                   // state, stage, derivative and filter buffers (TS)
     b += 3 * np * 8;                                     // At, rho, AtR
-    b += nxy * 8 + 2 * nxy * 8 + r8(p.ny * 8) + 8;       // arho, O, part, W
-    b += 2 * nxy * 8 + r8(p.ny * 8) + 16;                // axb, ayb, yph, flags
+    b += ncol * 8 + 2 * ncol * 8 + r8(p.ny * 8) + 8;     // arho, O, part, W
+    b += 2 * ncol * 8 + r8(p.ny * 8) + 16;               // axb, ayb, yph, flags
     b += (int64_t)NPASS * (int64_t)sizeof(LPass) + (int64_t)sizeof(hqt_ctx::DevDesc);
     b += 2 * nxy * 8 + nxy + 8 + 2 * (int64_t)census_blocks(nxy) * 8;   // diagnostics
     b += (nz * nz + (nz - 1) * (nz - 1) + 5 * nz) * 8;   // radial operators
@@ -99,27 +100,30 @@ int dalloc(hqt_ctx* c, size_t bytes, void** out) {
 }
 
 // Volume fields: host arrays are dense [nx, ny, nz]; on the device the planes are c->ps elements apart.
-int upload_vol(hqt_ctx* c, void* dst, const double* src, int dt) {
-    const size_t nxy = (size_t)c->nxy, nz = (size_t)c->nz, es = dt == DT_F64 ? 8 : 4, dp = (size_t)c->ps * es;
-    if (dt == DT_F64) {
-        HIPC(hipMemcpy2D(dst, dp, src, nxy * es, nxy * es, nz, hipMemcpyHostToDevice));
-    } else {
-        std::vector<float> tmp(nxy * nz);
-        for (size_t i = 0; i < tmp.size(); ++i) tmp[i] = (float)src[i];
-        HIPC(hipMemcpy2D(dst, dp, tmp.data(), nxy * es, nxy * es, nz, hipMemcpyHostToDevice));
+// Host array [nx, ny, nz] (dense) <-> device field (rows rp, planes ps elements apart), one pitched copy per plane.
+int copy_planes(const hqt_ctx* c, void* dst, const void* src, size_t es, hipMemcpyKind kind) {
+    const size_t nx = (size_t)c->nx, ny = (size_t)c->ny, nxy = (size_t)c->nxy, rp = (size_t)c->rp, ps = (size_t)c->ps;
+    const bool up = kind == hipMemcpyHostToDevice;
+    for (size_t z = 0; z < (size_t)c->nz; ++z) {
+        char* d = static_cast<char*>(dst) + z * (up ? ps : nxy) * es;
+        const char* s = static_cast<const char*>(src) + z * (up ? nxy : ps) * es;
+        HIPC(hipMemcpy2D(d, (up ? rp : nx) * es, s, (up ? nx : rp) * es, nx * es, ny, kind));
     }
     return HQT_OK;
 }
 
+int upload_vol(hqt_ctx* c, void* dst, const double* src, int dt) {
+    if (dt == DT_F64) return copy_planes(c, dst, src, 8, hipMemcpyHostToDevice);
+    std::vector<float> tmp((size_t)c->npts);
+    for (size_t i = 0; i < tmp.size(); ++i) tmp[i] = (float)src[i];
+    return copy_planes(c, dst, tmp.data(), 4, hipMemcpyHostToDevice);
+}
+
 int download_vol(const hqt_ctx* c, double* dst, const void* src, int dt) {
-    const size_t nxy = (size_t)c->nxy, nz = (size_t)c->nz, es = dt == DT_F64 ? 8 : 4, dp = (size_t)c->ps * es;
-    if (dt == DT_F64) {
-        HIPC(hipMemcpy2D(dst, nxy * es, src, dp, nxy * es, nz, hipMemcpyDeviceToHost));
-    } else {
-        std::vector<float> tmp(nxy * nz);
-        HIPC(hipMemcpy2D(tmp.data(), nxy * es, src, dp, nxy * es, nz, hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < tmp.size(); ++i) dst[i] = tmp[i];
-    }
+    if (dt == DT_F64) return copy_planes(c, dst, src, 8, hipMemcpyDeviceToHost);
+    std::vector<float> tmp((size_t)c->npts);
+    if (int rc = copy_planes(c, tmp.data(), src, 4, hipMemcpyDeviceToHost)) return rc;
+    for (size_t i = 0; i < tmp.size(); ++i) dst[i] = tmp[i];
     return HQT_OK;
 }
 
@@ -145,7 +149,8 @@ void set_lb(LPass& P, int lb, int dir, int tpb = 1) {
     else P.ntiles = P.nxc * P.nz;
 }
 
-LPass base_pass(int N, int nx, int ny, int nz, int dir, void* const* ops, int32_t* flags, int fold, int es, int64_t ps) {
+LPass base_pass(int N, int nx, int ny, int nz, int dir, void* const* ops, int32_t* flags, int fold, int es, int64_t ps,
+                int64_t rp) {
     LPass P;
     std::memset(&P, 0, sizeof(P));
     const LineGeom lg = line_geom(N);
@@ -165,7 +170,8 @@ LPass base_pass(int N, int nx, int ny, int nz, int dir, void* const* ops, int32_
     }
     P.nx = nx; P.ny = ny; P.nz = nz;
     P.ps = (int32_t)ps;
-    P.pad = (int32_t)(ps - (int64_t)nx * ny);
+    P.rp = (int32_t)rp;
+    P.pad = (int32_t)(ps - rp * ny);
     set_lb(P, 16, dir);
     for (int i = 0; i < OP_COUNT; ++i) P.ops[i] = ops[i];
     P.flags = flags;
@@ -467,7 +473,7 @@ int tune_lines(hqt_ctx* c) {
 
 int enqueue_step(hqt_ctx* c) {
     int rc;
-    const int zb = zblocks(c->nxy), nz = c->nz, f32 = c->fp32;
+    const int zb = zblocks(c->ncol), nz = c->nz, f32 = c->fp32;
     hipStream_t st = c->stream;
     for (int s = 0; s < 4; ++s) {
         if ((rc = line(c, PX1 + s, 0, KN_LX_STAGE))) return rc;
@@ -494,7 +500,7 @@ int enqueue_step(hqt_ctx* c) {
     if (c->p.winding)
         if ((rc = run(c, KN_WIND, [&] {
                  hipError_t e = launch_wind_partial(&c->dd->wind, kOne, c->ny, st);
-                 return e != hipSuccess ? e : launch_wind_apply(&c->dd->wind, kOne, wind_apply_blocks(c->nxy, nz), f32, st);
+                 return e != hipSuccess ? e : launch_wind_apply(&c->dd->wind, kOne, wind_apply_blocks(c->ncol, nz), f32, st);
              })))
             return rc;
     if (c->p.filter) {
@@ -522,7 +528,7 @@ int resolve_profile(hqt_ctx* c) {
 
 int build_context(hqt_ctx* c) {
     const int nx = c->nx, ny = c->ny, nz = c->nz;
-    const size_t es = c->es, np = (size_t)c->nvol, nxy = (size_t)c->nxy;
+    const size_t es = c->es, np = (size_t)c->nvol, nxy = (size_t)c->nxy, ncol = (size_t)c->ncol;
     int rc;
 #define ALLOC(ptr, bytes) do { void* q_; if ((rc = dalloc(c, (bytes), &q_))) return rc; ptr = (decltype(ptr))q_; } while (0)
     ALLOC(c->ZR, np * es); ALLOC(c->ZI, np * es); ALLOC(c->ZAx, np * es); ALLOC(c->ZAy, np * es);
@@ -534,8 +540,9 @@ int build_context(hqt_ctx* c) {
     ALLOC(c->dxAt, np * es); ALLOC(c->dyAt, np * es);
     for (int i = 0; i < 5; ++i) { ALLOC(c->fa[i], np * es); ALLOC(c->fb[i], np * es); }
     ALLOC(c->At, np * 8); ALLOC(c->rho, np * 8); ALLOC(c->AtR, np * 8);
-    ALLOC(c->arho, nxy * 8); ALLOC(c->O, 2 * nxy * 8); ALLOC(c->part, (size_t)ny * 8); ALLOC(c->W, 8);
-    ALLOC(c->axb, nxy * 8); ALLOC(c->ayb, nxy * 8); ALLOC(c->yph, (size_t)ny * 8);
+    // per-column arrays of the radial kernels: one entry per column of the padded plane
+    ALLOC(c->arho, ncol * 8); ALLOC(c->O, 2 * ncol * 8); ALLOC(c->part, (size_t)ny * 8); ALLOC(c->W, 8);
+    ALLOC(c->axb, ncol * 8); ALLOC(c->ayb, ncol * 8); ALLOC(c->yph, (size_t)ny * 8);
     ALLOC(c->flags, 4 * sizeof(int32_t));
     ALLOC(c->dp, NPASS * sizeof(LPass));
     ALLOC(c->dd, sizeof(hqt_ctx::DevDesc));
@@ -580,18 +587,19 @@ int build_context(hqt_ctx* c) {
     c->zo.f2 = c->zops + off_f2;
 
     // ---- boundary arrays and the winding phase (:43, :133, :139) ----
-    std::vector<double> bx(nxy), by(nxy), yph(ny);
+    std::vector<double> bx(ncol, 0.0), by(ncol, 0.0), yph(ny);
     for (size_t i = 0; i < nxy; ++i) {
-        bx[i] = c->p.ax_b ? c->p.ax_b[i] : c->p.ax_b_const;
-        by[i] = c->p.ay_b ? c->p.ay_b[i] : c->p.ay_b_const;
+        const size_t col = i % nx + (size_t)c->rp * (i / nx);
+        bx[col] = c->p.ax_b ? c->p.ax_b[i] : c->p.ax_b_const;
+        by[col] = c->p.ay_b ? c->p.ay_b[i] : c->p.ay_b_const;
     }
     for (int j = 0; j < ny; ++j) {
         const double ycheb = 2 * M_PI * j / ny;
         const double yy = (ycheb * c->p.Ly) / 2 / M_PI;
         yph[j] = 2 * M_PI * yy / c->p.Ly;
     }
-    HIPC(hipMemcpy(c->axb, bx.data(), nxy * 8, hipMemcpyHostToDevice));
-    HIPC(hipMemcpy(c->ayb, by.data(), nxy * 8, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->axb, bx.data(), ncol * 8, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->ayb, by.data(), ncol * 8, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(c->yph, yph.data(), (size_t)ny * 8, hipMemcpyHostToDevice));
 
     // ---- x/y operator tables (:13-14, :27-31, :38-39, :47-48) ----
@@ -615,7 +623,7 @@ int build_context(hqt_ctx* c) {
                        {c->SB[0], c->SB[1], c->SB[2], c->SB[3]},
                        {c->SA[0], c->SA[1], c->SA[2], c->SA[3]}};
     auto stage_x = [&](LPass& P, void* const* s, const double* at) {
-        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo, (int)c->es, c->ps);
+        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo, (int)c->es, c->ps, c->rp);
         LGroup* g = &add_group(P, s[0], T);
         add_out(*g, c->dxR, T, 0, OP_D1X); add_out(*g, c->dxxR, T, 0, OP_D2X);
         g = &add_group(P, s[1], T);
@@ -627,7 +635,7 @@ int build_context(hqt_ctx* c) {
         if (at) { g = &add_group(P, at, DT_F64); add_out(*g, c->dxAt, T, 0, OP_D1X); }
     };
     auto stage_y = [&](LPass& P, void* const* s, const double* at) {
-        P = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo, (int)c->es, c->ps);
+        P = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo, (int)c->es, c->ps, c->rp);
         LGroup* g = &add_group(P, s[0], T);
         add_out(*g, c->dyR, T, 0, OP_D1Y); add_out(*g, c->lapR, T, 0, OP_D2Y, -1, -1, c->dxxR, T);
         g = &add_group(P, s[1], T);
@@ -662,11 +670,11 @@ int build_context(hqt_ctx* c) {
     }
     {
         LPass& P = c->hp[PXC];
-        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo, (int)c->es, c->ps);
+        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo, (int)c->es, c->ps, c->rp);
         LGroup& g = add_group(P, c->ZAx, T);
         add_out(g, c->dxAx, T, 0, OP_D1X);
         LPass& Q = c->hp[PYC];
-        Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo, (int)c->es, c->ps);
+        Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo, (int)c->es, c->ps, c->rp);
         LGroup& h = add_group(Q, c->ZAy, T);
         // d_y A_y into curl3 (a stage temporary, free after the last stage); the post-step kernel
         // adds d_x A_x, so the two passes are independent
@@ -676,9 +684,9 @@ int build_context(hqt_ctx* c) {
         void* f[5] = {c->ZR, c->ZI, c->At, c->ZAx, c->ZAy};
         int fdt[5] = {T, T, DT_F64, T, T};
         LPass& P = c->hp[PXF];
-        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo, (int)c->es, c->ps);
+        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo, (int)c->es, c->ps, c->rp);
         LPass& Q = c->hp[PYF];
-        Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo, (int)c->es, c->ps);
+        Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo, (int)c->es, c->ps, c->rp);
         // Im(P) has rank 4 for the reference's masks: applied as U (V^T x) instead of a dense product.
         // Folded with both directions rank 4 (and ny >= 16): the cross term (Im P_x (x) Im P_y) x is
         // U_x G U_y^T per z-slice, so the x-pass writes partials of G (4 x 4 per slice) instead of the
@@ -734,7 +742,7 @@ int build_context(hqt_ctx* c) {
     for (int s = 0; s < 4; ++s) {
         ZStage& Z = c->zs[s];
         std::memset(&Z, 0, sizeof(Z));
-        Z.nxy = c->nxy; Z.ps = c->ps; Z.nz = nz; Z.stage = s + 1; Z.mode = 0; Z.h = c->p.dt; Z.zo = c->zo;
+        Z.nxy = c->ncol; Z.ps = c->ps; Z.nz = nz; Z.stage = s + 1; Z.mode = 0; Z.h = c->p.dt; Z.zo = c->zo;
         Z.sR = Sin[s][0]; Z.sI = Sin[s][1]; Z.sAx = Sin[s][2]; Z.sAy = Sin[s][3];
         Z.At = c->At; Z.dxAt = c->dxAt; Z.dyAt = c->dyAt;
         Z.dxR = c->dxR; Z.dxI = c->dxI; Z.dyR = c->dyR; Z.dyI = c->dyI; Z.lapR = c->lapR; Z.lapI = c->lapI;
@@ -751,13 +759,13 @@ int build_context(hqt_ctx* c) {
     c->zr.sR = c->SB[0]; c->zr.sI = c->SB[1]; c->zr.sAx = c->SB[2]; c->zr.sAy = c->SB[3];
     c->zr.At = c->AtR;
     std::memset(&c->zp, 0, sizeof(c->zp));
-    c->zp.nxy = c->nxy; c->zp.ps = c->ps; c->zp.nz = nz; c->zp.mu = c->p.mu; c->zp.zo = c->zo;
+    c->zp.nxy = c->ncol; c->zp.ps = c->ps; c->zp.nz = nz; c->zp.mu = c->p.mu; c->zp.zo = c->zo;
     c->zp.R = c->ZR; c->zp.I = c->ZI; c->zp.Ax = c->ZAx; c->zp.Ay = c->ZAy;
     c->zp.divA = c->dxAx; c->zp.divB = c->curl3;
     c->zp.rho = c->rho; c->zp.At = c->At; c->zp.axb = c->axb; c->zp.ayb = c->ayb; c->zp.O = c->O;
     c->zp.flags = c->flags;
     std::memset(&c->wind, 0, sizeof(c->wind));
-    c->wind.nx = nx; c->wind.ny = ny; c->wind.nz = nz; c->wind.nxy = c->nxy; c->wind.ps = c->ps;
+    c->wind.nx = nx; c->wind.ny = ny; c->wind.nz = nz; c->wind.nxy = c->ncol; c->wind.ps = c->ps; c->wind.rp = c->rp;
     c->wind.O = c->O; c->wind.part = c->part; c->wind.R = c->ZR; c->wind.I = c->ZI;
     c->wind.yph = c->yph; c->wind.W = c->W; c->wind.flags = c->flags;
     c->nan.R = c->ZR; c->nan.I = c->ZI; c->nan.Ax = c->ZAx; c->nan.Ay = c->ZAy; c->nan.At = c->At;
@@ -809,7 +817,9 @@ int hqt_ctx_create(const hqt_params* p, hqt_ctx** out) {
     c->nx = p->nx; c->ny = p->ny; c->nz = p->nz;
     c->nxy = (int64_t)p->nx * p->ny;
     c->npts = c->nxy * p->nz;
-    c->ps = plane_stride(c->nxy);
+    c->rp = row_stride(p->nx);
+    c->ncol = c->rp * p->ny;
+    c->ps = plane_stride(c->ncol);
     c->nvol = c->ps * p->nz;
     c->fp32 = p->precision == HQT_MIXED;
     c->tdt = c->fp32 ? DT_F32 : DT_F64;
@@ -954,7 +964,7 @@ int hqt_rhs(hqt_ctx* c, const double* Z1, const double* Z2, const double* Z3, co
     int32_t saved[2];
     HIPC(hipMemcpy(saved, c->flags, sizeof(saved), hipMemcpyDeviceToHost));
     HIPC(hipMemset(c->flags, 0, sizeof(saved)));
-    const int zb = zblocks(c->nxy);
+    const int zb = zblocks(c->ncol);
     if ((rc = line(c, PXR, 0, KN_LX_RHS))) return rc;
     if ((rc = line(c, PYR, 1, KN_LY_RHS))) return rc;
     if ((rc = run(c, KN_ZRHS, [&] { return launch_zstage(&c->dd->zs[4], kOne, zb, c->nz, c->fp32, c->zwpe, c->stream); }))) return rc;
@@ -978,7 +988,7 @@ int enqueue_census(hqt_ctx* c, bool census) {
     // the state may still be written by an ensemble's queued steps on its own stream
     if (c->owner) HIPC(hipStreamSynchronize(c->owner));
     Census A{};
-    A.nxy = c->nxy; A.ps = c->ps; A.nx = c->nx; A.ny = c->ny; A.nz = c->nz;
+    A.nxy = c->nxy; A.ps = c->ps; A.rp = c->rp; A.nx = c->nx; A.ny = c->ny; A.nz = c->nz;
     A.R = c->ZR; A.I = c->ZI;
     A.Lzb = c->zo.Lz + (size_t)(c->nz - 1) * c->nz;
     A.O = c->Od; A.charge = c->cmap; A.counts = c->ccount; A.part = c->cpart;
@@ -1079,7 +1089,7 @@ int hqt_ff(int32_t nx, int32_t ny, int32_t nslices, const double* x_re, const do
         if (hipMemcpy(u, x_re, n * 8, hipMemcpyHostToDevice) != hipSuccess) { rc = fail(HQT_ERR_HIP, "hipMemcpy x"); break; }
         if (x_im && hipMemcpy(v, x_im, n * 8, hipMemcpyHostToDevice) != hipSuccess) { rc = fail(HQT_ERR_HIP, "hipMemcpy x_im"); break; }
         LPass hp[2];
-        hp[0] = base_pass(nx, nx, ny, nslices, 0, ops, flags, 0, 8, (int64_t)nx * ny);
+        hp[0] = base_pass(nx, nx, ny, nslices, 0, ops, flags, 0, 8, (int64_t)nx * ny, nx);
         LGroup& g = add_group(hp[0], u, DT_F64, v, DT_F64);
         if (x_im) {
             // Px (u + i v) = (Pr u - Pi v) + i (Pi u + Pr v)
@@ -1090,7 +1100,7 @@ int hqt_ff(int32_t nx, int32_t ny, int32_t nslices, const double* x_re, const do
             add_out(g, a, DT_F64, 0, OP_PRX);
             add_out(g, b, DT_F64, 0, OP_PIX);
         }
-        hp[1] = base_pass(ny, nx, ny, nslices, 1, ops, flags, 0, 8, (int64_t)nx * ny);
+        hp[1] = base_pass(ny, nx, ny, nslices, 1, ops, flags, 0, 8, (int64_t)nx * ny, nx);
         LGroup& h = add_group(hp[1], a, DT_F64, b, DT_F64);
         add_out(h, out, DT_F64, 0, OP_QRY, 1, OP_NQIY);     // Re(Py (a + i b)) = Qr a - Qi b
         if (hipMemcpy(dp, hp, sizeof(hp), hipMemcpyHostToDevice) != hipSuccess) { rc = fail(HQT_ERR_HIP, "hipMemcpy pass"); break; }

@@ -21,7 +21,7 @@ namespace hqt {
 
 struct ICDev {
     int32_t nx, ny, nz, nv;
-    int64_t nxy, ps;      // columns; plane stride of the volume fields
+    int64_t nxy, ps, rp;  // points of a plane (nx ny); plane and row stride of the volume fields
     double Lx, Ly, amp_noise, phase_noise, inv_sigma2;
     uint32_t key0, key1;
     const double* prof;   // [4*nz]: Re(Z1 e^{i theta}), Im(Z1 e^{i theta}), Z2, -Vy1
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(256) void k_init_state(const ICDev A) {
     TS* X = (TS*)A.Ax;
     TS* Y = (TS*)A.Ay;
     for (int k = 0; k < A.nz; ++k) {
-        const int64_t p = col + A.ps * k;
+        const int64_t p = i + A.rp * j + A.ps * k;
         const double zr = A.prof[k], zi = A.prof[A.nz + k];
         R[p] = (TS)(zr * ar - zi * ai);
         I[p] = (TS)(zr * ai + zi * ar);
@@ -195,7 +195,7 @@ int hqt_init_state(hqt_ctx* c, const hqt_init* in) {
     HIPC(hipMalloc(&dbuf, prof.size() * 8));
     hipError_t e = hipMemcpyAsync(dbuf, prof.data(), prof.size() * 8, hipMemcpyHostToDevice, c->stream);
     ICDev D{};
-    D.nx = c->nx; D.ny = c->ny; D.nz = nz; D.nv = in->nv; D.nxy = c->nxy; D.ps = c->ps;
+    D.nx = c->nx; D.ny = c->ny; D.nz = nz; D.nv = in->nv; D.nxy = c->nxy; D.ps = c->ps; D.rp = c->rp;
     D.Lx = c->p.Lx; D.Ly = c->p.Ly;
     D.amp_noise = in->amp_noise; D.phase_noise = in->phase_noise;
     D.inv_sigma2 = in->sigma > 0 ? 1.0 / (in->sigma * in->sigma) : 0.0;

@@ -40,6 +40,9 @@ struct hqt_ctx {
     hqt_params p{};
     int nx = 0, ny = 0, nz = 0;
     int64_t nxy = 0, npts = 0;
+    // rows rp elements apart (nx rounded up to 16: every 16-point row tile is one aligned 128-byte segment in
+    // FP64); ncol = rp * ny columns per plane, the padding columns included (radial kernels run over them)
+    int64_t rp = 0, ncol = 0;
     // volume fields: planes ps elements apart (nxy rounded up to a multiple of 32, so every z-plane
     // starts on a 128-byte boundary); nvol = ps * nz elements per field
     int64_t ps = 0, nvol = 0;

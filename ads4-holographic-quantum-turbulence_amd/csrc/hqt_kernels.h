@@ -114,21 +114,22 @@ struct LPass {
     int32_t Kp, Z;             // folded form: padded folded positions, table offset
     int32_t xcd;               // folded form: block order of the launch (1: XCD-aware), set by the autotuner
     int32_t* flags;            // [0] = NaN step (0: none), [1] = step counter
-    int32_t ps, pad;           // plane stride of the volume fields (elements) and ps - nx*ny
+    int32_t ps, pad;           // plane stride of the volume fields (elements) and ps - rp*ny
+    int32_t rp;                // row stride of the volume fields (elements): nx rounded up to 16
     // factored-DFT form (fold = 3 or 4, hqt_spec_core.h): Mp = 64 (factorization index + 1), S the LDS row
     // stride, sp = 2 pi / L (the derivative scale of the direction), st its trig table (pass precision)
     double sp;
     const void* st;
 };
 
-// Volume fields are [nx, ny, nz] with planes ps >= nx*ny elements apart (ps a multiple of 32, so
-// that every z-plane starts on a 128-byte boundary in FP64 and FP32).  Element offset of x-line
-// line0 + d (line = y + ny*z; d >= 0, a few tiles at most) given z0 = line0 / ny, without a
-// division: line*nx + z*(ps - nx*ny).
+// Volume fields are [nx, ny, nz] with rows rp >= nx elements apart (rp a multiple of 16, so that a
+// 16-point tile of a row is one aligned 128-byte segment in FP64 and half of one in FP32) and planes
+// ps >= rp*ny elements apart (a multiple of 32).  Element offset of x-line line0 + d (line = y + ny*z;
+// d >= 0, a few tiles at most) given z0 = line0 / ny, without a division: line*rp + z*(ps - rp*ny).
 __host__ __device__ inline int xline_base_at(const LPass& P, int line0, int z0, int d) {
     int r = line0 + d - z0 * P.ny, z = z0;
     while (r >= P.ny) { r -= P.ny; ++z; }
-    return (line0 + d) * P.nx + z * P.pad;
+    return (line0 + d) * P.rp + z * P.pad;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -175,8 +176,9 @@ struct ZPost {
 };
 
 struct Wind {
-    int32_t nx, ny, nz; int64_t nxy;
+    int32_t nx, ny, nz; int64_t nxy; // nxy: columns of a plane including the row padding (rp * ny)
     int64_t ps;                      // plane stride of the volume fields (elements)
+    int64_t rp;                      // row stride (elements); O is [2 * rp * ny] (column (x, y) at x + rp y)
     const double* O; double* part;   // [ny] partial sums
     void *R, *I;                     // Psi (TS)
     const double* yph;               // [ny] 2*pi*yy/L
@@ -207,8 +209,9 @@ struct NewtonJob {
 
 // Boundary condensate and plaquette census of one state.
 struct Census {
-    int64_t nxy; int32_t nx, ny, nz;
+    int64_t nxy; int32_t nx, ny, nz;   // nxy = nx * ny points (O, charge: [nx, ny] without row padding)
     int64_t ps;               // plane stride of the volume fields (elements)
+    int64_t rp;               // row stride of the volume fields (elements)
     const void *R, *I;        // Psi (TS), [nx*ny*nz]
     const double* Lzb;        // [nz] last row of Lz (the z=0 row)
     double* O;                // [2*nx*ny] interleaved complex
@@ -235,6 +238,8 @@ size_t line_pass_lds_bytes(const LPass& P, int fp32);
 inline int zblocks(int64_t nxy) { return (int)((nxy + 63) / 64); }
 // Plane stride of the volume fields for nxy columns: a multiple of 32 elements (128 bytes in FP32).
 inline int64_t plane_stride(int64_t nxy) { return (nxy + 31) & ~(int64_t)31; }
+// Row stride of the volume fields for rows of nx points: a multiple of 16 elements.
+inline int64_t row_stride(int64_t nx) { return (nx + 15) & ~(int64_t)15; }
 // wpe: 2 or 3 waves per SIMD targeted by the register allocation (see k_zstage).
 hipError_t launch_zstage(const ZStage* d, Members M, int blocks, int nz, int fp32, int wpe, hipStream_t s);
 hipError_t launch_zpost(const ZPost* d, Members M, int blocks, int nz, int fp32, hipStream_t s);

@@ -164,7 +164,7 @@ __device__ __forceinline__ bool tile_out(const LPass& P, const LOut& O, typename
     const int ln = lane & 15, lk = lane >> 4;
     // element index of accumulator (u, q, r) or Z_OOB outside the grid (32-bit: fields hold
     // fewer than 2^23 elements)
-    const int l0 = (int)line0, nlv = (int)nl, nx = P.nx, pl = P.ps * z;
+    const int l0 = (int)line0, nlv = (int)nl, nx = P.nx, rp = P.rp, pl = P.ps * z;
     // x-lines: element offsets of the 4 NT lines this lane stores
     int xb[NT][4];
     const int z0 = DIR == 0 ? l0 / P.ny : 0;
@@ -179,7 +179,7 @@ __device__ __forceinline__ bool tile_out(const LPass& P, const LOut& O, typename
             return (line < nlv && i < N) ? (uint32_t)(xb[u][r] + i) : Z_OOB;
         } else {
             const int x = x0 + 16 * u + ln, i = i0 + MF<T>::row(lk, r);
-            return (x < nx && i < N) ? (uint32_t)(x + nx * i + pl) : Z_OOB;
+            return (x < nx && i < N) ? (uint32_t)(x + rp * i + pl) : Z_OOB;
         }
     };
     T add[NT][Q][4];
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(LP_W
             const int x = (lt % P.nxc) * LB + mrow, zz = lt / P.nxc;
             okm = x < P.nx;
             base = (int64_t)x + (int64_t)P.ps * zz;
-            step = (int64_t)P.nx;
+            step = (int64_t)P.rp;
         }
         // x-lines: element offsets of this thread's NT lines
         int xbh[NT];
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(LP1_
         } else {
             okm = x0 + mrow < P.nx;
             base = (int64_t)(x0 + mrow) + (int64_t)P.ps * z;
-            step = (int64_t)P.nx;
+            step = (int64_t)P.rp;
         }
         int xbh[NT];
         if (DIR == 0) {

@@ -27,7 +27,7 @@ __global__ __launch_bounds__(256) void k_wind_partial(const Wind* __restrict__ A
     const int j = mb.bx, jn = (j + 1) % A.ny;
     double s = 0.0;
     for (int x = threadIdx.x; x < A.nx; x += 256) {
-        const int64_t a = (int64_t)x + (int64_t)A.nx * jn, b = (int64_t)x + (int64_t)A.nx * j;
+        const int64_t a = (int64_t)x + A.rp * jn, b = (int64_t)x + A.rp * j;
         s += angle_ratio(A.O[2 * a], A.O[2 * a + 1], A.O[2 * b], A.O[2 * b + 1]);
     }
     const double t = block_sum_256(s, sh);
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void k_wind_apply(const Wind* __restrict__ Ap,
     for (int64_t e = (int64_t)mb.bx * 256 + threadIdx.x; e < total; e += (int64_t)mb.nb * 256) {
         const int64_t c = e % A.ps;
         if (c >= A.nxy) continue;        // plane padding
-        const int y = (int)(c / A.nx);
+        const int y = (int)(c / A.rp);
         double sn, cs;
         sincos(-A.yph[y] * W, &sn, &cs);
         const double r = ld<TS>(A.R, e), i = ld<TS>(A.I, e);

@@ -226,7 +226,7 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
     auto elem = [&](int line0, int z, int x0, int xb, int j) -> uint32_t {
         if (DIR == 0) return (line0 + mrow < nl && j < N) ? (uint32_t)(xb + j) : Z_OOB;
         const int x = x0 + mrow;
-        return (x < P.nx && j < N) ? (uint32_t)(x + P.nx * j + P.ps * z) : Z_OOB;
+        return (x < P.nx && j < N) ? (uint32_t)(x + P.rp * j + P.ps * z) : Z_OOB;
     };
     auto fetch = [&](int unit) {
         const int lt = tile0 + unit / Gr.ntiles, t = unit % Gr.ntiles;

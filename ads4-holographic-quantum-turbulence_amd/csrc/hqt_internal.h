@@ -111,16 +111,46 @@ struct NewtonWork {
     size_t cap = 0;
     ~NewtonWork();
 };
+// mu-continuation of a batch (all three arrays [nb]): problem b starts from its guess rescaled to
+// A_t(z=0) = mu_seed[b] (NaN: one plain solve) and steps mu by at most dmu[b]; steps[b] receives the
+// number of solves (0 for a plain solve).
+struct NewtonCont {
+    const double* mu_seed;
+    const double* dmu;
+    int32_t* steps;
+};
 int newton_batch_run(hipStream_t s, NewtonWork& w, int nz, double zh, double small, double D, int nb,
                      const double* Z1, const double* Z2, const double* vy1, const double* vy, int it_max, double tol,
-                     double* z1, double* z2, double* VY1, int32_t* it, double* err, uint8_t* conv);
+                     double* z1, double* z2, double* VY1, int32_t* it, double* err, uint8_t* conv,
+                     const NewtonCont* cont = nullptr);
+
+// A Newton launch over problems of any radial sizes and operator parameters: per problem nz, zh, small, D,
+// vy, continuation seed and step, and host pointers to its [nz] guesses and results; it, err, conv (and
+// steps, if not NULL) are [nb].  One work-group per problem, in at most two kernel launches (problems with
+// the Jacobian in LDS, then the rest).
+struct NewtonProblems {
+    int nb = 0;
+    std::vector<int> nz;
+    std::vector<double> zh, small, D, vy, mu_seed, dmu;
+    std::vector<const double*> Z1, Z2, vy1;
+    std::vector<double*> z1, z2, VY1;
+    int32_t* it = nullptr;
+    double* err = nullptr;
+    uint8_t* conv = nullptr;
+    int32_t* steps = nullptr;
+    void resize(int n);   // n problems with zh = 1, small = D = 0, no continuation, NULL pointers
+};
+size_t newton_problems_bytes(const NewtonProblems& P);
+int newton_problems_run(hipStream_t s, NewtonWork& w, const NewtonProblems& P, int it_max, double tol);
+int newton_problems_run_in(hipStream_t s, unsigned char* buf, const NewtonProblems& P, int it_max, double tol);
 
 // Device bytes newton_batch_run needs for nb problems of radial size nz, and the same run in a caller-given
 // device buffer of at least that many bytes (no allocation).
 size_t newton_batch_bytes(int nz, int nb);
 int newton_batch_run_in(hipStream_t s, unsigned char* buf, int nz, double zh, double small, double D, int nb,
                         const double* Z1, const double* Z2, const double* vy1, const double* vy, int it_max,
-                        double tol, double* z1, double* z2, double* VY1, int32_t* it, double* err, uint8_t* conv);
+                        double tol, double* z1, double* z2, double* VY1, int32_t* it, double* err, uint8_t* conv,
+                        const NewtonCont* cont = nullptr);
 
 // Argument checks of the batched Newton entry points (it_max <= 0 and tol <= 0 select the reference's
 // 1000 and 1e-6), and their return status from the per-problem convergence flags.
@@ -128,6 +158,9 @@ int newton_batch_args(const char* who, int nz, int nb, const void* Z1, const voi
                       const void* z1, const void* z2, const void* VY1, const void* it, const void* err,
                       const void* conv, int32_t& it_max, double& tol);
 int newton_batch_status(const char* who, int nb, const uint8_t* conv);
+// Checks the continuation arguments of nb problems: each non-NaN seed finite, nonzero and of the sign of
+// the problem's target A_t(z=0) (Z2's last entry), its step finite and positive.
+int newton_cont_args(const char* who, int nz, int nb, const double* Z2, const double* mu_seed, const double* dmu);
 
 // Device bytes a context with these parameters allocates (state, stage buffers, operator
 // tables, descriptors), as hqt_ctx_create would; used for admission by HBM budget.

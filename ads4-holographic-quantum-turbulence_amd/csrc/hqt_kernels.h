@@ -203,8 +203,9 @@ struct NewtonJob {
     const double* zz;    // [nz] radial nodes
     const double *Z1, *Z2, *vy1;   // [nz] initial guesses
     double *z1, *z2, *VY1;         // [nz] results
+    double mu_seed, dmu;           // mu-continuation seed and largest step (NaN seed: one solve)
     double* work;                  // [(3nz)^2] Jacobian / LU factors when they do not fit in LDS
-    double* out;                   // [3]: it, err, converged
+    double* out;                   // [4]: it, err, converged, solves of the continuation
 };
 
 // Boundary condensate and plaquette census of one state.
@@ -253,7 +254,9 @@ bool zkernels_support_nz(int nz);
 // Plain streaming copy of `bytes` (multiple of 32) used to calibrate traffic counters.
 hipError_t launch_copy_calib(const void* src, void* dst, int64_t bytes, hipStream_t s);
 // Batched Newton (one work-group per problem); every job of one launch has the same nz <= 64.
-hipError_t launch_newton(const NewtonJob* jobs, int njobs, int nz, hipStream_t s);
+// njobs problems of radial size at most maxnz; lds: every Jacobian in LDS (each nz resident), else in
+// NewtonJob::work.
+hipError_t launch_newton(const NewtonJob* jobs, int njobs, int maxnz, bool lds, hipStream_t s);
 bool newton_lds_resident(int nz);   // Jacobian held in LDS (else in NewtonJob::work)
 // O from the state, then the census of O.
 hipError_t launch_order_param(const Census& A, int fp32, hipStream_t s);

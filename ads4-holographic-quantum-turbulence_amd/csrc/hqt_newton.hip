@@ -8,6 +8,9 @@
 //     (the LAPACK getrf/getrs order);
 //   - the update and the new residual (:73-83);
 //   - the stop tests: NaN update norm (:88-90), max|Fun_next| < tol (:93).
+// A problem with a continuation seed walks mu from the seed to its target inside the same
+// work-group, one solve per mu step, as the reference's stationary-state notebook does by hand
+// (scripts/legacy/Statinary_state_with_v.m:11-17,50-52).
 // The 3nz x 3nz Jacobian lives in LDS for nz <= 41 (the reference's nz = 30 takes 65 KB) and
 // in a per-problem global workspace above that.  The radial operators Lz, Lz^2 and all
 // vectors are always in LDS.
@@ -22,11 +25,14 @@ constexpr int NW_T = 512;          // threads per problem
 constexpr int NW_W = NW_T / 64;    // waves per problem
 constexpr int NW_MAXNZ = 64;
 constexpr size_t NW_LDS_LIMIT = 160 * 1024;
+constexpr int NW_CONT_IT = 100;          // iteration limit of one continuation solve (it_max if smaller)
+constexpr int NW_CONT_MAX_STEPS = 1024;  // solves per continuation, the seed's included
 
 struct NwVec {
     double Z1[NW_MAXNZ], Z2[NW_MAXNZ], V[NW_MAXNZ], Z2f[NW_MAXNZ];
     double T1[NW_MAXNZ], T2[NW_MAXNZ], TV[NW_MAXNZ], T2f[NW_MAXNZ];
     double zz[NW_MAXNZ], f[NW_MAXNZ], df[NW_MAXNZ];
+    double C1[NW_MAXNZ], C2[NW_MAXNZ], CV[NW_MAXNZ];   // last converged continuation point
     double F[3 * NW_MAXNZ], Fn[3 * NW_MAXNZ], y[3 * NW_MAXNZ], x[3 * NW_MAXNZ];
     double pv[NW_W];
     int pi[NW_W];
@@ -105,29 +111,14 @@ __device__ void nw_z2f(int nz, const double* Lz, const NwVec& s, const double* b
     }
 }
 
-template <bool MLDS>
-__global__ __launch_bounds__(NW_T) void k_newton(const NewtonJob* __restrict__ jobs) {
-    extern __shared__ double dyn[];
-    __shared__ NwVec s;
-    const NewtonJob J = jobs[blockIdx.x];
-    const int nz = J.nz, n = 3 * nz;
+// One NewtonRaphson_vy solve from the iterate in s.Z1, s.Z2, s.V (V = Vy1, already shifted as :28-29):
+// Z2_f and the first residual (:26-27, :62), then the iteration (:60-96).  On return s.Z1, s.Z2, s.V
+// hold the last iterate and K, err the reference's `it` and `error`; the result is 1 when
+// max|Fun_next| < tol.  Every branch is taken on block-uniform values, so all threads return together.
+__device__ int nw_solve(int nz, const double* Lz, const double* L2, double* M, NwVec& s, int it_max, double tol,
+                        int& K_out, double& err_out) {
+    const int n = 3 * nz;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    double* Lz = dyn;
-    double* L2 = dyn + nz * nz;
-    double* M = MLDS ? dyn + 2 * nz * nz : J.work;   // row-major n x n
-
-    for (int e = tid; e < nz * nz; e += NW_T) { Lz[e] = J.Lz[e]; L2[e] = J.Lz2[e]; }
-    const double zh3 = J.zh * J.zh * J.zh;
-    for (int i = tid; i < nz; i += NW_T) {
-        const double zi = J.zz[i], q = zi / J.zh;
-        s.zz[i] = zi;
-        s.f[i] = 1 - q * q * q;
-        s.df[i] = -3 * zi * zi / zh3;
-        s.Z1[i] = J.Z1[i];
-        s.Z2[i] = J.Z2[i];
-        s.V[i] = J.vy + (J.vy1[i] - J.vy1[nz - 1]);   // :28-29
-    }
-    __syncthreads();
     nw_z2f(nz, Lz, s, s.Z2, s.Z2f);
     __syncthreads();
     nw_residual(nz, Lz, L2, s, s.Z1, s.Z2, s.V, s.Z2f, s.F);
@@ -135,7 +126,7 @@ __global__ __launch_bounds__(NW_T) void k_newton(const NewtonJob* __restrict__ j
 
     int K = 0, converged = 0;
     double err = __builtin_nan("");
-    while (K < J.it_max) {
+    while (K < it_max) {
         // ---- dFun (:37-39) with BCz rows; y <- Fun ----
         for (int r = w; r < n; r += NW_W) {
             const int br = (r >= nz) + (r >= 2 * nz), i = r - br * nz;
@@ -274,8 +265,91 @@ __global__ __launch_bounds__(NW_T) void k_newton(const NewtonJob* __restrict__ j
         for (int i = tid; i < n; i += NW_T) s.F[i] = s.Fn[i];
         __syncthreads();
         if (err != err) break;
-        if (err2 < J.tol) { converged = 1; break; }
+        if (err2 < tol) { converged = 1; break; }
         ++K;
+    }
+    K_out = K;
+    err_out = err;
+    return converged;
+}
+
+// Restart point of a continuation solve: Z1, Z2 rescaled by r with A_t(z=0) set to mu
+// (Statinary_state_with_v.m:50), Vy1 = vy + vy1 - vy1(end) (NewtonRaphson_vy.m:28-29).
+__device__ void nw_restart(int nz, NwVec& s, const double* a, const double* b, const double* v, double vy, double r,
+                           double mu) {
+    for (int i = threadIdx.x; i < nz; i += NW_T) {
+        s.Z1[i] = a[i];
+        s.Z2[i] = (i == nz - 1) ? mu : b[i] * r;
+        s.V[i] = vy + (v[i] - v[nz - 1]);
+    }
+    __syncthreads();
+}
+
+template <bool MLDS>
+__global__ __launch_bounds__(NW_T) void k_newton(const NewtonJob* __restrict__ jobs) {
+    extern __shared__ double dyn[];
+    __shared__ NwVec s;
+    const NewtonJob J = jobs[blockIdx.x];
+    const int nz = J.nz;
+    const int tid = threadIdx.x;
+    double* Lz = dyn;
+    double* L2 = dyn + nz * nz;
+    double* M = MLDS ? dyn + 2 * nz * nz : J.work;   // row-major 3nz x 3nz
+
+    for (int e = tid; e < nz * nz; e += NW_T) { Lz[e] = J.Lz[e]; L2[e] = J.Lz2[e]; }
+    const double zh3 = J.zh * J.zh * J.zh;
+    for (int i = tid; i < nz; i += NW_T) {
+        const double zi = J.zz[i], q = zi / J.zh;
+        s.zz[i] = zi;
+        s.f[i] = 1 - q * q * q;
+        s.df[i] = -3 * zi * zi / zh3;
+    }
+    int K = 0, conv = 0, steps = 0;
+    double err = __builtin_nan("");
+    if (J.mu_seed != J.mu_seed) {
+        // one solve from the caller's guess (NewtonRaphson_vy.m:1-105)
+        for (int i = tid; i < nz; i += NW_T) {
+            s.Z1[i] = J.Z1[i];
+            s.Z2[i] = J.Z2[i];
+            s.V[i] = J.vy + (J.vy1[i] - J.vy1[nz - 1]);   // :28-29
+        }
+        __syncthreads();
+        conv = nw_solve(nz, Lz, L2, M, s, J.it_max, J.tol, K, err);
+    } else {
+        // mu-continuation: the guess rescaled to the seed's A_t(z=0) = mu_seed and solved there, then mu
+        // stepped to the guess's own A_t(z=0) in steps of at most dmu, each from the last solution
+        // rescaled (Statinary_state_with_v.m:11-17,50-52).  A step that does not converge is retried at
+        // half the length, down to dmu/64.
+        const double mt = J.Z2[nz - 1];
+        const int itc = J.it_max < NW_CONT_IT ? J.it_max : NW_CONT_IT;
+        double prev = J.mu_seed;
+        nw_restart(nz, s, J.Z1, J.Z2, J.vy1, J.vy, J.mu_seed / mt, J.mu_seed);
+        conv = nw_solve(nz, Lz, L2, M, s, itc, J.tol, K, err);
+        steps = 1;
+        if (conv) {
+            double h = copysign(J.dmu, mt - prev);
+            const double hmin = J.dmu * (1.0 / 64);
+            while (prev != mt && steps < NW_CONT_MAX_STEPS) {
+                for (int i = tid; i < nz; i += NW_T) { s.C1[i] = s.Z1[i]; s.C2[i] = s.Z2[i]; s.CV[i] = s.V[i]; }
+                __syncthreads();
+                bool ok = false;
+                while (!ok && steps < NW_CONT_MAX_STEPS) {
+                    double m = prev + h;
+                    if (h > 0 ? m >= mt : m <= mt) m = mt;
+                    nw_restart(nz, s, s.C1, s.C2, s.CV, J.vy, m / prev, m);
+                    ok = nw_solve(nz, Lz, L2, M, s, itc, J.tol, K, err) != 0;
+                    ++steps;
+                    if (ok) {
+                        prev = m;
+                    } else {
+                        h *= 0.5;
+                        if (fabs(h) < hmin) break;
+                    }
+                }
+                if (!ok) break;
+            }
+            conv = prev == mt;
+        }
     }
     for (int i = tid; i < nz; i += NW_T) {
         J.z1[i] = s.Z1[i];
@@ -285,16 +359,17 @@ __global__ __launch_bounds__(NW_T) void k_newton(const NewtonJob* __restrict__ j
     if (tid == 0) {
         J.out[0] = (double)K;
         J.out[1] = err;
-        J.out[2] = converged ? 1.0 : 0.0;
+        J.out[2] = conv ? 1.0 : 0.0;
+        J.out[3] = (double)steps;
     }
 }
 
-hipError_t launch_newton(const NewtonJob* jobs, int njobs, int nz, hipStream_t st) {
-    if (nz < 2 || nz > NW_MAXNZ || njobs <= 0) return hipErrorInvalidValue;
-    const bool mlds = newton_lds_resident(nz);
-    const size_t dyn = newton_dyn_bytes(nz, mlds);
+hipError_t launch_newton(const NewtonJob* jobs, int njobs, int maxnz, bool lds, hipStream_t st) {
+    if (maxnz < 2 || maxnz > NW_MAXNZ || njobs <= 0) return hipErrorInvalidValue;
+    if (lds && !newton_lds_resident(maxnz)) return hipErrorInvalidValue;
+    const size_t dyn = newton_dyn_bytes(maxnz, lds);
     hipError_t e;
-    if (mlds) {
+    if (lds) {
         e = hipFuncSetAttribute((const void*)k_newton<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_newton<true>, dim3(njobs), dim3(NW_T), dyn, st, jobs);

@@ -7,8 +7,9 @@
 //     When one finishes (or trips the NaN guard) its state is downloaded, its memory freed and
 //     the ensemble is rebuilt over the remaining members plus the newly admitted ones.
 //   - Stationary Newton solves (NewtonRaphson_vy.m) that are pending at a scheduling point
-//     are solved as batches (one work-group per problem) on a second stream, while the
-//     ensemble's queued steps run on the first, so the two kinds of work overlap on the GPU.
+//     are solved as one batch (one work-group per problem, any mix of radial sizes) on a second
+//     stream, while the ensemble's queued steps run on the first, so the two kinds of work
+//     overlap on the GPU.
 // Each request's result is the one it would get alone: ensemble members and Newton problems
 // are computed independently of their neighbours.
 #include "hqt_internal.h"
@@ -49,6 +50,7 @@ struct Request {
     // Newton
     int32_t nz = 0;
     double zh = 1, small = 0, D = 0, vy = 0;
+    double mu_seed = std::nan(""), dmu = 0;        // mu-continuation seed and step (NaN: one solve)
     std::vector<double> n1, n2, nv;                // initial guesses, then results
     int32_t it = 0;
     double err = 0;
@@ -66,7 +68,9 @@ struct hqt_server {
     std::map<int64_t, std::shared_ptr<Request>> reqs;          // until the result is taken
     std::deque<std::shared_ptr<Request>> pend_evo, pend_newton;
     std::vector<double> lat[2];                                 // completed latencies (s)
-    int64_t counters[8] = {};   // admitted, done evo, done newton, ensemble builds, newton batches, peak bytes, peak members, steps
+    // admitted, done evo, done newton, ensemble builds, newton batches, peak bytes, peak members, steps,
+    // Newton solves (continuation steps included), continued Newton requests
+    int64_t counters[HQT_SERVER_NCOUNTERS] = {};
     std::thread worker;
     // worker-thread state
     std::vector<std::shared_ptr<Request>> active;
@@ -91,42 +95,60 @@ void complete(hqt_server* s, const std::shared_ptr<Request>& r, int status, cons
     s->cv_done.notify_all();
 }
 
-// Newton batches: pending problems grouped by (nz, zh, small, D), one launch per group.
+// Newton batches: every pending problem, whatever its radial size and operator parameters, in one
+// problem list (at most two kernel launches: Jacobians in LDS, then in global memory).
 void run_newton(hqt_server* s, std::vector<std::shared_ptr<Request>>& batch) {
-    std::map<std::tuple<int, double, double, double>, std::vector<std::shared_ptr<Request>>> groups;
-    for (auto& r : batch) groups[std::make_tuple(r->nz, r->zh, r->small, r->D)].push_back(r);
-    for (auto& kv : groups) {
-        auto& g = kv.second;
-        const int nz = std::get<0>(kv.first), nb = (int)g.size();
-        const size_t vec = (size_t)nz * nb;
-        std::vector<double> a(vec), b(vec), v(vec), vy(nb), o1(vec), o2(vec), ov(vec), err(nb);
-        std::vector<int32_t> it(nb);
-        std::vector<uint8_t> conv(nb);
-        for (int k = 0; k < nb; ++k) {
-            std::memcpy(&a[(size_t)k * nz], g[k]->n1.data(), nz * 8);
-            std::memcpy(&b[(size_t)k * nz], g[k]->n2.data(), nz * 8);
-            std::memcpy(&v[(size_t)k * nz], g[k]->nv.data(), nz * 8);
-            vy[k] = g[k]->vy;
-        }
-        const int rc = newton_batch_run(s->nstream, s->nwork, nz, std::get<1>(kv.first), std::get<2>(kv.first),
-                                        std::get<3>(kv.first), nb, a.data(), b.data(), v.data(), vy.data(), 1000, 1e-6,
-                                        o1.data(), o2.data(), ov.data(), it.data(), err.data(), conv.data());
+    const int nb = (int)batch.size();
+    NewtonProblems P;
+    P.resize(nb);
+    std::vector<std::vector<double>> o1(nb), o2(nb), ov(nb);
+    std::vector<double> err(nb);
+    std::vector<int32_t> it(nb), solves(nb);
+    std::vector<uint8_t> conv(nb);
+    for (int k = 0; k < nb; ++k) {
+        const Request& r = *batch[k];
+        P.nz[k] = r.nz;
+        P.zh[k] = r.zh;
+        P.small[k] = r.small;
+        P.D[k] = r.D;
+        P.vy[k] = r.vy;
+        P.mu_seed[k] = r.mu_seed;
+        P.dmu[k] = r.dmu;
+        P.Z1[k] = r.n1.data();
+        P.Z2[k] = r.n2.data();
+        P.vy1[k] = r.nv.data();
+        o1[k].resize(r.nz);
+        o2[k].resize(r.nz);
+        ov[k].resize(r.nz);
+        P.z1[k] = o1[k].data();
+        P.z2[k] = o2[k].data();
+        P.VY1[k] = ov[k].data();
+    }
+    P.it = it.data();
+    P.err = err.data();
+    P.conv = conv.data();
+    P.steps = solves.data();
+    const int rc = newton_problems_run(s->nstream, s->nwork, P, 1000, 1e-6);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->counters[4] += 1;
+    }
+    const std::string emsg = rc ? std::string(hqt_last_error()) : std::string();
+    for (int k = 0; k < nb; ++k) {
+        auto& r = batch[k];
+        if (rc) { complete(s, r, rc, emsg); continue; }
+        r->n1.swap(o1[k]);
+        r->n2.swap(o2[k]);
+        r->nv.swap(ov[k]);
+        r->it = it[k];
+        r->err = err[k];
         {
             std::lock_guard<std::mutex> lk(s->mu);
-            s->counters[4] += 1;
+            s->counters[8] += std::max(1, solves[k]);
+            s->counters[9] += solves[k] > 0;
         }
-        const std::string emsg = rc ? std::string(hqt_last_error()) : std::string();
-        for (int k = 0; k < nb; ++k) {
-            auto& r = g[k];
-            if (rc) { complete(s, r, rc, emsg); continue; }
-            r->n1.assign(o1.begin() + (size_t)k * nz, o1.begin() + (size_t)(k + 1) * nz);
-            r->n2.assign(o2.begin() + (size_t)k * nz, o2.begin() + (size_t)(k + 1) * nz);
-            r->nv.assign(ov.begin() + (size_t)k * nz, ov.begin() + (size_t)(k + 1) * nz);
-            r->it = it[k];
-            r->err = err[k];
-            complete(s, r, conv[k] ? HQT_OK : HQT_ERR_NOCONV,
-                     conv[k] ? "" : "NewtonRaphson_vy: no convergence (it=" + std::to_string(it[k]) + ")");
-        }
+        complete(s, r, conv[k] ? HQT_OK : HQT_ERR_NOCONV,
+                 conv[k] ? "" : "NewtonRaphson_vy: no convergence (it=" + std::to_string(it[k]) + ")");
     }
 }
 
@@ -294,6 +316,37 @@ double percentile(std::vector<double> v, double q) {
 
 }  // namespace
 
+namespace {
+int submit_newton(hqt_server* s, const char* who, int32_t nz, double zh, double small, double D, const double* Z1,
+                  const double* Z2, const double* vy1, double vy, double mu_seed, double dmu, int64_t* id) {
+    if (!s || !Z1 || !Z2 || !vy1 || !id || nz < 3 || nz > 64)
+        return hqt_fail(HQT_ERR_ARG, std::string(who) + ": bad argument (3 <= nz <= 64)");
+    if (int rc = newton_cont_args(who, nz, 1, Z2, &mu_seed, &dmu)) return rc;
+    auto r = std::make_shared<Request>();
+    r->kind = RK_NEWTON;
+    r->nz = nz;
+    r->zh = zh;
+    r->small = small;
+    r->D = D;
+    r->vy = vy;
+    r->mu_seed = mu_seed;
+    r->dmu = dmu;
+    r->n1.assign(Z1, Z1 + nz);
+    r->n2.assign(Z2, Z2 + nz);
+    r->nv.assign(vy1, vy1 + nz);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        r->id = s->next_id++;
+        r->t_submit = Clock::now();
+        s->reqs[r->id] = r;
+        s->pend_newton.push_back(r);
+    }
+    s->cv_work.notify_all();
+    *id = r->id;
+    return HQT_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int hqt_server_create(int64_t hbm_budget, int32_t quantum, hqt_server** out) {
@@ -360,28 +413,14 @@ int hqt_server_submit_evolution(hqt_server* s, const hqt_params* p, const double
 
 int hqt_server_submit_newton(hqt_server* s, int32_t nz, double zh, double small, double D, const double* Z1,
                              const double* Z2, const double* vy1, double vy, int64_t* id) {
-    if (!s || !Z1 || !Z2 || !vy1 || !id || nz < 3 || nz > 64)
-        return hqt_fail(HQT_ERR_ARG, "hqt_server_submit_newton: bad argument (3 <= nz <= 64)");
-    auto r = std::make_shared<Request>();
-    r->kind = RK_NEWTON;
-    r->nz = nz;
-    r->zh = zh;
-    r->small = small;
-    r->D = D;
-    r->vy = vy;
-    r->n1.assign(Z1, Z1 + nz);
-    r->n2.assign(Z2, Z2 + nz);
-    r->nv.assign(vy1, vy1 + nz);
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        r->id = s->next_id++;
-        r->t_submit = Clock::now();
-        s->reqs[r->id] = r;
-        s->pend_newton.push_back(r);
-    }
-    s->cv_work.notify_all();
-    *id = r->id;
-    return HQT_OK;
+    return submit_newton(s, "hqt_server_submit_newton", nz, zh, small, D, Z1, Z2, vy1, vy, std::nan(""), 0.0, id);
+}
+
+int hqt_server_submit_newton_continued(hqt_server* s, int32_t nz, double zh, double small, double D,
+                                       const double* Z1, const double* Z2, const double* vy1, double vy,
+                                       double mu_seed, double dmu, int64_t* id) {
+    return submit_newton(s, "hqt_server_submit_newton_continued", nz, zh, small, D, Z1, Z2, vy1, vy, mu_seed, dmu,
+                         id);
 }
 
 int hqt_server_wait(hqt_server* s, int64_t id, double timeout_s, double* latency_s) {
@@ -457,7 +496,7 @@ int hqt_server_stats(hqt_server* s, int32_t kind, double* p50, double* p99, int6
 int hqt_server_counters(hqt_server* s, int64_t* out) {
     if (!s || !out) return hqt_fail(HQT_ERR_ARG, "bad argument");
     std::lock_guard<std::mutex> lk(s->mu);
-    for (int k = 0; k < 8; ++k) out[k] = s->counters[k];
+    for (int k = 0; k < HQT_SERVER_NCOUNTERS; ++k) out[k] = s->counters[k];
     return HQT_OK;
 }
 

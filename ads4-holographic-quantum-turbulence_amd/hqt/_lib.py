@@ -23,6 +23,7 @@ HQT_DENSE_LINES = 64
 HQT_ORDER_XCD = 128
 HQT_FOLDED_STAGES = 256
 HQT_FACTORED_STAGES = 512
+HQT_SERVER_NCOUNTERS = 10
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "csrc", "libhqt.so")
@@ -30,9 +31,9 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "csrc", "libhqt.so")
 EXPORTS = (
     "hqt_ctx_create", "hqt_ctx_destroy", "hqt_set_state", "hqt_get_state", "hqt_step_n", "hqt_sync",
     "hqt_last_winding", "hqt_rhs", "hqt_order_parameter", "hqt_ff", "hqt_fourdif", "hqt_chebdif",
-    "hqt_newton_vy", "hqt_newton_vy_batch", "hqt_newton_vy_batch_noalloc", "hqt_vortex_census", "hqt_ensemble_create", "hqt_ensemble_step_n", "hqt_ensemble_sync",
+    "hqt_newton_vy", "hqt_newton_vy_batch", "hqt_newton_vy_batch_noalloc", "hqt_newton_vy_continue_batch", "hqt_vortex_census", "hqt_ensemble_create", "hqt_ensemble_step_n", "hqt_ensemble_sync",
     "hqt_ensemble_launches", "hqt_ensemble_destroy", "hqt_server_create", "hqt_server_destroy",
-    "hqt_server_submit_evolution", "hqt_server_submit_newton", "hqt_server_wait", "hqt_server_evolution_result",
+    "hqt_server_submit_evolution", "hqt_server_submit_newton", "hqt_server_submit_newton_continued", "hqt_server_wait", "hqt_server_evolution_result",
     "hqt_server_newton_result", "hqt_server_stats", "hqt_server_counters", "hqt_ctx_footprint", "hqt_ctx_device_bytes",
     "hqt_checkpoint_name", "hqt_checkpoint_write_host", "hqt_checkpoint_save", "hqt_checkpoint_save_async",
     "hqt_checkpoint_wait", "hqt_init_state", "hqt_spline", "hqt_debug_field", "hqt_profile", "hqt_kernel_stats", "hqt_calibration_copy",
@@ -107,6 +108,8 @@ def lib():
     L.hqt_server_submit_evolution.argtypes = [vp, C.POINTER(Params), dp, dp, dp, dp, dp, i32, C.POINTER(C.c_int64)]
     L.hqt_server_submit_newton.argtypes = [vp, i32, C.c_double, C.c_double, C.c_double, dp, dp, dp, C.c_double,
                                            C.POINTER(C.c_int64)]
+    L.hqt_server_submit_newton_continued.argtypes = [vp, i32, C.c_double, C.c_double, C.c_double, dp, dp, dp,
+                                                     C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int64)]
     L.hqt_server_wait.argtypes = [vp, C.c_int64, C.c_double, dp]
     L.hqt_server_evolution_result.argtypes = [vp, C.c_int64, dp, dp, dp, dp, dp, C.POINTER(i32)]
     L.hqt_server_newton_result.argtypes = [vp, C.c_int64, dp, dp, dp, C.POINTER(i32), dp]
@@ -130,6 +133,9 @@ def lib():
     L.hqt_newton_vy_batch.argtypes = [i32, C.c_double, C.c_double, C.c_double, i32, dp, dp, dp, dp, i32,
                                       C.c_double, dp, dp, dp, C.POINTER(i32), dp, C.POINTER(C.c_uint8)]
     L.hqt_newton_vy_batch_noalloc.argtypes = L.hqt_newton_vy_batch.argtypes
+    L.hqt_newton_vy_continue_batch.argtypes = [i32, C.c_double, C.c_double, C.c_double, i32, dp, dp, dp, dp, dp,
+                                               C.c_double, i32, C.c_double, dp, dp, dp, C.POINTER(i32), dp,
+                                               C.POINTER(C.c_uint8), C.POINTER(i32)]
     L.hqt_debug_field.argtypes = [vp, C.c_char_p, dp]
     L.hqt_profile.argtypes = [vp, i32]
     L.hqt_calibration_copy.argtypes = [C.c_int64, dp]

@@ -18,7 +18,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (HQT_ERR_ARG, HQT_ERR_NAN, HQT_ERR_NOCONV, HQT_ERR_TIMEOUT, HQT_FP64, HQT_LINES16, HQT_LINES32, HQT_MIXED, HQT_NO_GRAPH, HQT_RADIAL_W2, HQT_RADIAL_W3, HQT_DENSE_FILTER, HQT_DENSE_LINES, HQT_FACTORED_STAGES, HQT_FOLDED_STAGES, HQT_ORDER_XCD, HQTError, Init, Params, check, dptr, lib)
+from ._lib import (HQT_ERR_ARG, HQT_ERR_NAN, HQT_ERR_NOCONV, HQT_ERR_TIMEOUT, HQT_FP64, HQT_LINES16, HQT_LINES32, HQT_MIXED, HQT_NO_GRAPH, HQT_RADIAL_W2, HQT_RADIAL_W3, HQT_DENSE_FILTER, HQT_DENSE_LINES, HQT_FACTORED_STAGES, HQT_FOLDED_STAGES, HQT_ORDER_XCD, HQT_SERVER_NCOUNTERS, HQTError, Init, Params, check, dptr, lib)
 
 __all__ = ["fourdif", "chebdif", "ff", "NewtonRaphson_vy", "NewtonRaphson_vy_batch", "Stepper", "Ensemble", "Server", "footprint", "make_params", "checkpoint_name", "write_checkpoint", "spline", "HQTError", "mask_factors"]
 
@@ -102,14 +102,18 @@ def NewtonRaphson_vy(nz, zh, small, D, Z1, Z2, vy1, vy, raise_on_failure=False):
 
 
 def NewtonRaphson_vy_batch(nz, zh, small, D, Z1, Z2, vy1, vy, it_max=1000, tol=1e-6, raise_on_failure=False,
-                           allocation_free=False):
+                           allocation_free=False, mu_seed=None, dmu=0.25):
     """NewtonRaphson_vy over a batch on the GPU (one work-group per problem).
 
     ``Z1, Z2, vy1`` are ``[nz, nb]`` (problem ``b`` in column ``b``; an nz-vector is broadcast
     over the batch), ``vy`` is ``[nb]``.  Returns ``z1, z2, VY1`` (``[nz, nb]``), ``it``,
     ``err`` and ``conv`` (``[nb]``), each problem iterated exactly as ``NewtonRaphson_vy``.
     ``allocation_free=True`` runs ``hqt_newton_vy_batch_noalloc`` (no device allocation; the
-    workspace is a fixed per-device array of ``HQT_NEWTON_NOALLOC_BYTES``)."""
+    workspace is a fixed per-device array of ``HQT_NEWTON_NOALLOC_BYTES``).
+    ``mu_seed`` (a number or ``[nb]``; NaN entries solve once) runs
+    ``hqt_newton_vy_continue_batch``: each guess is taken as a solution near ``mu_seed`` and
+    continued in mu to its own A_t(z=0) = ``Z2[-1]`` in steps of at most ``dmu``; the number of
+    solves per problem is then returned as a seventh value."""
     nz = int(nz)
     vy = np.atleast_1d(np.asarray(vy, dtype=np.float64))
     nb = vy.size
@@ -128,13 +132,24 @@ def NewtonRaphson_vy_batch(nz, zh, small, D, Z1, Z2, vy1, vy, it_max=1000, tol=1
     it = np.zeros(nb, dtype=np.int32)
     err = np.zeros(nb)
     conv = np.zeros(nb, dtype=np.uint8)
-    fn = lib().hqt_newton_vy_batch_noalloc if allocation_free else lib().hqt_newton_vy_batch
-    rc = fn(nz, float(zh), float(small), float(D), nb, dptr(a), dptr(b), dptr(v), dptr(vy),
-                                   int(it_max), float(tol), dptr(z1), dptr(z2), dptr(VY1),
-                                   it.ctypes.data_as(C.POINTER(C.c_int32)), dptr(err),
-                                   conv.ctypes.data_as(C.POINTER(C.c_uint8)))
+    tail = (dptr(z1), dptr(z2), dptr(VY1), it.ctypes.data_as(C.POINTER(C.c_int32)), dptr(err),
+            conv.ctypes.data_as(C.POINTER(C.c_uint8)))
+    if mu_seed is not None:
+        if allocation_free:
+            raise HQTError(HQT_ERR_ARG, "NewtonRaphson_vy_batch: continuation has no allocation-free form")
+        ms = np.ascontiguousarray(np.broadcast_to(np.asarray(mu_seed, dtype=np.float64), (nb,)))
+        steps = np.zeros(nb, dtype=np.int32)
+        rc = lib().hqt_newton_vy_continue_batch(nz, float(zh), float(small), float(D), nb, dptr(a), dptr(b), dptr(v),
+                                                dptr(vy), dptr(ms), float(dmu), int(it_max), float(tol), *tail,
+                                                steps.ctypes.data_as(C.POINTER(C.c_int32)))
+    else:
+        fn = lib().hqt_newton_vy_batch_noalloc if allocation_free else lib().hqt_newton_vy_batch
+        rc = fn(nz, float(zh), float(small), float(D), nb, dptr(a), dptr(b), dptr(v), dptr(vy), int(it_max),
+                float(tol), *tail)
     if rc != 0 and (raise_on_failure or rc != HQT_ERR_NOCONV):
         check(rc)
+    if mu_seed is not None:
+        return z1, z2, VY1, it, err, conv.astype(bool), steps
     return z1, z2, VY1, it, err, conv.astype(bool)
 
 
@@ -502,11 +517,18 @@ class Server:
         self._shape[rid.value] = s
         return int(rid.value)
 
-    def submit_newton(self, nz, zh, small, D, Z1, Z2, vy1, vy):
+    def submit_newton(self, nz, zh, small, D, Z1, Z2, vy1, vy, mu_seed=None, dmu=0.25):
+        """A stationary solve; with ``mu_seed`` the guess is continued in mu from that seed to its
+        own A_t(z=0) (``hqt_server_submit_newton_continued``)."""
         a, b, v = (np.ascontiguousarray(np.ravel(x), dtype=np.float64) for x in (Z1, Z2, vy1))
         rid = C.c_int64(0)
-        check(lib().hqt_server_submit_newton(self._h, int(nz), float(zh), float(small), float(D), dptr(a), dptr(b),
-                                             dptr(v), float(vy), C.byref(rid)))
+        if mu_seed is None:
+            check(lib().hqt_server_submit_newton(self._h, int(nz), float(zh), float(small), float(D), dptr(a),
+                                                 dptr(b), dptr(v), float(vy), C.byref(rid)))
+        else:
+            check(lib().hqt_server_submit_newton_continued(self._h, int(nz), float(zh), float(small), float(D),
+                                                           dptr(a), dptr(b), dptr(v), float(vy), float(mu_seed),
+                                                           float(dmu), C.byref(rid)))
         self._kind[rid.value] = "newton"
         self._shape[rid.value] = int(nz)
         return int(rid.value)
@@ -539,10 +561,10 @@ class Server:
         return p50.value, p99.value, int(n.value)
 
     def counters(self):
-        out = (C.c_int64 * 8)()
+        out = (C.c_int64 * HQT_SERVER_NCOUNTERS)()
         check(lib().hqt_server_counters(self._h, out))
         keys = ("admitted", "evolutions_done", "newton_done", "ensemble_builds", "newton_batches", "peak_bytes",
-                "peak_resident", "steps")
+                "peak_resident", "steps", "newton_solves", "newton_continued")
         return dict(zip(keys, [int(x) for x in out]))
 
     def close(self):

@@ -9,8 +9,8 @@ removal, filter, NaN guard) with the state resident in HBM.
 Prints ONE JSON line.  ``value`` is FP64 steps/s; the mixed-precision form is reported beside
 it with its per-field error against the FP64 run after the same steps.  Beside them:
 ``ensemble`` (many independent evolutions of uneven size stepped as one batch) and
-``serving`` (a Poisson stream of Newton solves and evolution segments through the serving
-loop, with p50/p99 latency).  ``roofline`` is the
+``serving`` (the serving loop's saturated capacity for Newton solves and for evolution steps,
+then a Poisson stream of both at a sweep of rates, with p50/p99 latency per rate).  ``roofline`` is the
 dominant kernel's algorithmic FLOP rate from HIP events recorded around its launches on the
 library's stream (a profiled repeat of the timed steps); ``cpu_baseline`` is the NumPy oracle
 (the reference's dense-matrix form) timed on a bounded sample on this host.
@@ -330,8 +330,9 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving (Poisson arrivals) leg")
     ap.add_argument("--ensemble-members", type=int, default=48)
     ap.add_argument("--ensemble-steps", type=int, default=20)
-    ap.add_argument("--serving-rate", type=float, default=2.0, help="Poisson arrival rate, requests/s")
-    ap.add_argument("--serving-seconds", type=float, default=15.0)
+    ap.add_argument("--serving-rates", default="1,2,4,8",
+                    help="Poisson arrival rates of the serving sweep, requests/s (comma-separated)")
+    ap.add_argument("--serving-seconds", type=float, default=8.0, help="arrival span of each rate of the sweep")
     ap.add_argument("--folded", action="store_true",
                     help="RK-stage x/y derivatives as folded matrix products instead of the factored-DFT form")
     ap.add_argument("--dense-lines", action="store_true",
@@ -455,7 +456,8 @@ def main():
 
     if not args.no_serving:
         from hqt import loadgen
-        out["serving"] = loadgen.run(rate=args.serving_rate, duration=args.serving_seconds, seed=0)
+        rates = tuple(float(r) for r in args.serving_rates.split(",") if r.strip())
+        out["serving"] = loadgen.sweep(rates=rates, duration=args.serving_seconds, seed=0)
 
     if not args.no_cpu:
         cps, times = cpu_baseline(nx, ny, nz, L, state, axb, warmup=args.cpu_warmup, timed=args.cpu_steps)

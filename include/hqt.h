@@ -205,6 +205,12 @@ int hqt_server_submit_evolution(hqt_server* srv, const hqt_params* p, const doub
                                 int64_t* id);
 int hqt_server_submit_newton(hqt_server* srv, int32_t nz, double zh, double small, double D, const double* Z1,
                              const double* Z2, const double* vy1, double vy, int64_t* id);
+/* A Newton request solved by mu-continuation from a seed, as hqt_newton_vy_continue_batch with
+ * mu_seed and dmu (a NaN mu_seed: as hqt_server_submit_newton).  The result's it and err are those
+ * of the last solve. */
+int hqt_server_submit_newton_continued(hqt_server* srv, int32_t nz, double zh, double small, double D,
+                                       const double* Z1, const double* Z2, const double* vy1, double vy,
+                                       double mu_seed, double dmu, int64_t* id);
 /* Wait until request id is done (timeout_s < 0: no limit; HQT_ERR_TIMEOUT otherwise) and
  * return its status; *latency_s = completion - submission on the host clock. */
 int hqt_server_wait(hqt_server* srv, int64_t id, double timeout_s, double* latency_s);
@@ -217,8 +223,10 @@ int hqt_server_newton_result(hqt_server* srv, int64_t id, double* z1, double* z2
 /* Latency percentiles (nearest rank, seconds) over completed requests of kind 0 (evolution),
  * 1 (Newton) or 2 (both). */
 int hqt_server_stats(hqt_server* srv, int32_t kind, double* p50, double* p99, int64_t* completed);
-/* out[8]: evolutions admitted, evolutions done, Newton done, ensemble (re)builds, Newton
- * batches, peak resident device bytes, peak resident evolutions, ensemble steps run. */
+/* out[HQT_SERVER_NCOUNTERS]: evolutions admitted, evolutions done, Newton done, ensemble
+ * (re)builds, Newton batches, peak resident device bytes, peak resident evolutions, ensemble steps
+ * run, Newton solves run (each step of a continuation counted), Newton requests continued. */
+#define HQT_SERVER_NCOUNTERS 10
 int hqt_server_counters(hqt_server* srv, int64_t* out);
 /* Device bytes a context with parameters p allocates, and those a context holds. */
 int hqt_ctx_footprint(const hqt_params* p, int64_t* bytes);
@@ -257,13 +265,30 @@ int hqt_newton_vy_batch(int32_t nz, double zh, double small, double D, int32_t n
                         int32_t it_max, double tol, double* z1, double* z2, double* VY1,
                         int32_t* it, double* err, uint8_t* conv);
 
+/* mu-continuation form of hqt_newton_vy_batch: the stationary-state notebook's continuation in mu
+ * (scripts/legacy/Statinary_state_with_v.m:11-17 steps mu by hand, :50 rescales the seed profile's
+ * A_t, :52 solves) run inside the GPU work-group of each problem.  Problem b's target is its guess's
+ * own A_t(z=0) = Z2[nz-1 + nz*b]; the guess, rescaled so that A_t(z=0) = mu_seed[b], is solved at
+ * the seed's mu, then mu is stepped to the target in steps of at most dmu, each from the last
+ * solution rescaled the same way.  A step that does not converge is retried at half the length,
+ * down to dmu/64.  Each solve stops at min(it_max, 100) iterations; at most 1024 solves.  it and
+ * err are those of the last solve, steps[b] the number of solves; conv[b] = 1 when the target was
+ * reached.  A NaN mu_seed[b] solves problem b once from its guess, as hqt_newton_vy_batch does.
+ * mu_seed[b] and the target must be finite, nonzero and of one sign, dmu finite and positive
+ * (HQT_ERR_ARG otherwise).  Returns as hqt_newton_vy_batch. */
+int hqt_newton_vy_continue_batch(int32_t nz, double zh, double small, double D, int32_t nb,
+                                 const double* Z1, const double* Z2, const double* vy1, const double* vy,
+                                 const double* mu_seed, double dmu, int32_t it_max, double tol, double* z1,
+                                 double* z2, double* VY1, int32_t* it, double* err, uint8_t* conv,
+                                 int32_t* steps);
+
 /* hqt_newton_vy_batch without any device allocation, for callers that cannot call hipMalloc
  * (code under HIP graph capture, tight serving loops, harnesses that forbid allocation).  No
  * counterpart in the reference.  Same arguments, results and status as hqt_newton_vy_batch (the same
  * kernel on the same inputs: bitwise the same outputs); the device workspace (operators, inputs,
  * results, job descriptors and, for nz > 41, the 3nz x 3nz Jacobians) lives in a fixed per-device
  * array of HQT_NEWTON_NOALLOC_BYTES bytes that each GPU the library runs on holds until the process
- * ends.  A batch needs 16 nz^2 + 48 nz nb + 144 nb bytes, plus 72 nz^2 nb for nz > 41 (about 8000
+ * ends.  A batch needs 16 nz^2 + 48 nz nb + 168 nb bytes, plus 72 nz^2 nb for nz > 41 (about 7800
  * problems fit at nz <= 41, 100 at nz = 48, 56 at nz = 64); a larger one is rejected with
  * HQT_ERR_ARG before any GPU work.  Calls on one GPU share its array: the library
  * runs them one at a time (a per-device lock), so calls from several host threads wait for one

@@ -17,6 +17,8 @@ Reference files restated here (paths relative to the reference repository):
   constraint, boundary reset, winding removal, filter) -> :func:`step`
 * ``src/numerics/ff.m:7-11``                   -> :func:`ff`
 * ``src/solvers/NewtonRaphson_vy.m:8-105``      -> :func:`NewtonRaphson_vy`
+* ``scripts/legacy/Statinary_state_with_v.m:11-17,50-52`` (continuation in mu
+  by rescaled seeds)                            -> :func:`newton_mu_continuation`
 * ``src/diagnostics/countN.m:7-21``             -> :func:`countN`
 * ``scripts/simulation/tevol_driven_cluster.m:81-102`` and
   ``scripts/simulation/tevol_lattice.m:101-135`` (initial fields) ->
@@ -481,6 +483,57 @@ def NewtonRaphson_vy(nz, zh, small, D, Z1, Z2, vy1, vy, it_max=1000, tol=1e-6):
             break
         K += 1
     return Z1, Z2, Vy1, K, err
+
+
+def newton_mu_continuation(nz, zh, small, D, Z1, Z2, vy1, vy, mu_seed, dmu, it_max=1000, tol=1e-6):
+    """Continuation in mu by rescaled seeds, the procedure of
+    ``scripts/legacy/Statinary_state_with_v.m:11-17,50-52`` (load a solved profile, ``Z2=Z2/6*mu``,
+    solve again, step mu) written as a loop.  The target mu is the guess's own A_t(z=0) = ``Z2[-1]``.
+    The guess rescaled to A_t(z=0) = ``mu_seed`` is solved first; then mu steps to the target by at
+    most ``dmu``, each solve starting from the last solution with Z2 rescaled by m/prev and its
+    boundary entry set to m.  A step that does not converge is retried at half its length until the
+    length drops below ``dmu/64``.  Every solve is :func:`NewtonRaphson_vy` with
+    ``min(it_max, 100)`` iterations; at most 1024 solves.  Returns ``(z1, z2, Vy1, it, err, conv,
+    solves)``: the last solve's iterate, ``it`` and ``err``, whether the target was reached, and
+    the number of solves."""
+    CONT_IT, MAX_SOLVES = 100, 1024
+    itc = min(int(it_max), CONT_IT)
+    Z1 = np.array(Z1, dtype=np.float64).reshape(nz)
+    Z2 = np.array(Z2, dtype=np.float64).reshape(nz)
+    V = np.array(vy1, dtype=np.float64).reshape(nz)
+    mt = float(Z2[nz - 1])
+
+    def solve(a, b, v, r, m):
+        b = b * r
+        b[nz - 1] = m
+        z1, z2, vv, K, err = NewtonRaphson_vy(nz, zh, small, D, a, b, v, vy, it_max=itc, tol=tol)
+        return z1, z2, vv, K, err, (K < itc and not math.isnan(err))
+
+    prev = float(mu_seed)
+    z1, z2, vv, K, err, ok = solve(Z1, Z2, V, prev / mt, prev)
+    solves = 1
+    if not ok:
+        return z1, z2, vv, K, err, False, solves
+    h = math.copysign(dmu, mt - prev)
+    hmin = dmu * (1.0 / 64)
+    while prev != mt and solves < MAX_SOLVES:
+        C1, C2, CV = z1, z2, vv
+        ok = False
+        while not ok and solves < MAX_SOLVES:
+            m = prev + h
+            if (m >= mt) if h > 0 else (m <= mt):
+                m = mt
+            z1, z2, vv, K, err, ok = solve(C1, C2, CV, m / prev, m)
+            solves += 1
+            if ok:
+                prev = m
+            else:
+                h *= 0.5
+                if abs(h) < hmin:
+                    break
+        if not ok:
+            break
+    return z1, z2, vv, K, err, prev == mt, solves
 
 
 # --------------------------------------------------------------------------------------

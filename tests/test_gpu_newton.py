@@ -6,7 +6,10 @@ the converged profiles within 1e-9 (both stop at max|Fun_next| < 1e-6 after the 
 steps; the remaining difference is LU rounding).  Covered: the reference's own solver cases
 (tests/hqt_smoke_test.m:180-183), a vy sweep, the LDS-resident (nz <= 41) and the
 global-workspace (nz > 41) Jacobian, the exact normal-phase solution, a NaN problem inside a
-batch, and batch-order independence."""
+batch, and batch-order independence.  The continuation in mu (hqt_newton_vy_continue_batch)
+against the oracle's newton_mu_continuation on the superfluid branch: the same number of solves and
+retries, the same last iteration count, the profiles within 1e-9; below the branch's end, a
+solution at the target in the normal phase."""
 
 import math
 
@@ -110,3 +113,81 @@ def test_batch_order_independent(golden_profile):
     for k in range(3):
         assert np.array_equal(a[k], b[k][:, ::-1])
     assert np.array_equal(a[3], b[3][::-1])
+
+
+MU_FOLD = 5.8     # the superfluid branch at vy = -2*pi*7/20 ends at a fold near mu = 5.79
+
+
+def _is_solution_at(nz, z1, z2, v, target, vy=VY_REF):
+    """The oracle's NewtonRaphson_vy started from (z1, z2, v) stops at iteration 0 (its residual is
+    already below tolerance) and A_t(z=0) is the target."""
+    from oracle import hqt_oracle as O
+    r = O.NewtonRaphson_vy(nz, 1, 0, 0, z1, z2, v, vy)
+    return r[3] == 0 and abs(z2[-1] - target) <= 1e-12 * abs(target)
+
+
+def _check_continuation(nz, mus, seeds, dmu=0.25, tol=1e-9):
+    """hqt_newton_vy_continue_batch on serving-workload guesses against the oracle's
+    newton_mu_continuation.  On the superfluid branch (target above the fold) each solve converges
+    in a few iterations from the last one: the same convergence, number of solves and last iteration
+    count, and the profiles within tol.  Below the fold the walk leaves the branch through solves
+    that wander for tens of iterations, where rounding decides the path; there the result must be a
+    solution at the target (checked by the oracle's residual) in the normal phase."""
+    import hqt
+    from hqt import loadgen
+    from oracle import hqt_oracle as O
+    G = [loadgen.newton_guess(nz, mu) for mu in mus]
+    Z1, Z2, V = (np.stack([g[k] for g in G], axis=1) for k in range(3))
+    vys = np.full(len(mus), VY_REF)
+    z1, z2, VY1, it, err, conv, steps = hqt.NewtonRaphson_vy_batch(nz, 1, 0, 0, Z1, Z2, V, vys,
+                                                                   mu_seed=np.asarray(seeds, dtype=float), dmu=dmu)
+    for b, (mu, sd) in enumerate(zip(mus, seeds)):
+        if not math.isnan(sd) and mu < MU_FOLD:
+            assert conv[b], (nz, mu)
+            assert _is_solution_at(nz, z1[:, b], z2[:, b], VY1[:, b], Z2[-1, b]), (nz, mu)
+            assert np.max(np.abs(z1[:, b])) < 1e-6, (nz, mu)
+            continue
+        if math.isnan(sd):
+            r1, r2, rv, rit, rerr = O.NewtonRaphson_vy(nz, 1, 0, 0, Z1[:, b], Z2[:, b], V[:, b], VY_REF)
+            rconv, rsteps = (rit < 1000 and np.isfinite(rerr)), 0
+        else:
+            r1, r2, rv, rit, rerr, rconv, rsteps = O.newton_mu_continuation(nz, 1, 0, 0, Z1[:, b], Z2[:, b], V[:, b],
+                                                                            VY_REF, sd, dmu)
+        assert (conv[b], steps[b], it[b]) == (rconv, rsteps, rit), (nz, mu, sd, conv[b], steps[b], it[b], rsteps, rit)
+        scale = max(1.0, float(np.max(np.abs(r2))))
+        for name, u, v in (("z1", z1[:, b], r1), ("z2", z2[:, b], r2), ("VY1", VY1[:, b], rv)):
+            d = float(np.max(np.abs(u - v)))
+            assert d < tol * scale, f"nz={nz} mu={mu} {name}: |diff| {d:.3e}"
+    return conv, steps, z1
+
+
+def test_mu_continuation_matches_oracle():
+    """Continued problems (above and below the end of the superfluid branch near mu = 5.79) and a
+    plain one (NaN seed) in one launch, on the LDS-resident Jacobian (nz = 32)."""
+    mus = [8.25, 9.25, 7.5, 10.0, 5.5, 3.5, 6.5]
+    seeds = [6.0] * 6 + [math.nan]
+    print("continuation nz=32 mus", mus, flush=True)
+    conv, steps, z1 = _check_continuation(32, mus, seeds)
+    assert conv.all()
+    assert (steps[:6] >= 2).all() and steps[6] == 0
+    assert (np.max(np.abs(z1[:, :4]), axis=0) > 1.0).all()       # superfluid above the fold
+    assert (np.max(np.abs(z1[:, 4:6]), axis=0) < 1e-6).all()     # normal phase below it
+
+
+def test_mu_continuation_global_workspace():
+    """nz = 48 (Jacobian in the global workspace), where the plain solve from these guesses wanders
+    for its 1000 iterations at mu = 9.25."""
+    mus = [9.25, 8.5, 5.75, 6.25]
+    print("continuation nz=48 mus", mus, flush=True)
+    conv, steps, _ = _check_continuation(48, mus, [6.0] * 4)
+    assert conv.all()
+
+
+def test_mu_continuation_halves_failed_steps():
+    """Steps of up to 3.0 from mu = 6 to 8.5: the first step fails within its 100 iterations and is
+    retried at half the length, the oracle and the GPU taking the same retries."""
+    for nz in (24, 32):
+        print("continuation halving nz", nz, flush=True)
+        conv, steps, z1 = _check_continuation(nz, [8.5], [6.0], dmu=3.0)
+        assert conv.all() and steps[0] > 2
+        assert np.max(np.abs(z1)) > 1.0

@@ -110,3 +110,30 @@ def test_oversize_and_nan_requests():
     assert st_good == 0
     for x, y in zip(out, _alone(64, 12, 6.0, "fp64", 3, 10)):
         assert np.array_equal(x, y)
+
+
+def test_mixed_size_newton_batch_equals_alone():
+    """Continued Newton requests of different radial sizes, submitted together, run in one problem
+    list (nz <= 41 with the Jacobian in LDS and nz > 41 in global memory): each result is bitwise the
+    one hqt_newton_vy_continue_batch gives for that problem alone."""
+    import hqt
+    from hqt import loadgen
+    cases = [(16, 7.3), (24, 9.1), (33, 6.4), (41, 8.8), (44, 7.7), (48, 9.6)]
+    print("mixed-size Newton batch", cases, flush=True)
+    alone = []
+    for nz, mu in cases:
+        Z1, Z2, V = loadgen.newton_guess(nz, mu)
+        alone.append(hqt.NewtonRaphson_vy_batch(nz, 1, 0, 0, Z1, Z2, V, np.array([VY_REF]), mu_seed=6.0, dmu=0.25))
+    with hqt.Server(1 << 30, quantum=5) as srv:
+        ids = []
+        for nz, mu in cases:
+            Z1, Z2, V = loadgen.newton_guess(nz, mu)
+            ids.append(srv.submit_newton(nz, 1, 0, 0, Z1, Z2, V, VY_REF, mu_seed=6.0, dmu=0.25))
+        outs = [srv.result(i, timeout=60.0) for i in ids]
+        c = srv.counters()
+    for (nz, mu), (st, lat, (z1, z2, v, it, err)), r in zip(cases, outs, alone):
+        assert st == 0 and r[5][0]
+        assert np.array_equal(z1, r[0][:, 0]) and np.array_equal(z2, r[1][:, 0]) and np.array_equal(v, r[2][:, 0])
+        assert it == r[3][0] and err == r[4][0]
+    assert c["newton_continued"] == len(cases)
+    assert c["newton_solves"] == sum(int(r[6][0]) for r in alone)

@@ -9,8 +9,10 @@
   (nx = ny in {64, 96, 128, 192, 256}, nz in {12, 16, 20, 24}, mu ~ U[4, 8], kk in 3..9,
   8-64 windings, FP64 and mixed): bitwise equal to each member stepped alone.
 * A serving stream drawn from the serving workload's ranges (Newton nz in [16, 48],
-  mu ~ U[3.5, 10]; evolutions nx = ny in [64, 301], nz in [12, 24], mu ~ U[4, 8]): every evolution
-  bitwise equal to running alone, every Newton solve equal to the oracle's NewtonRaphson_vy.
+  mu ~ U[3.5, 10], continued in mu from the seed profile's mu = 6; evolutions nx = ny in [64, 301],
+  nz in [12, 24], mu ~ U[4, 8]): every evolution bitwise equal to running alone, every Newton
+  request converged: equal to the oracle's continuation (newton_mu_continuation) on the
+  superfluid branch, a normal-phase solution at the target below the branch's end.
 """
 
 import math
@@ -134,7 +136,7 @@ def test_config5_serving_stream():
     from oracle import hqt_oracle as O
     rng = np.random.default_rng(8)
     newton = []
-    for _ in range(6):
+    for _ in range(8):
         nz = int(rng.integers(16, 49))
         mu = float(rng.uniform(3.5, 10.0))
         newton.append((nz, mu, loadgen.newton_guess(nz, mu)))
@@ -159,8 +161,12 @@ def test_config5_serving_stream():
                                                            ax_b=ic.sinusoidal_drive(n, n, box, kk, 0.1))))
             if k < len(newton):
                 nz, mu, (Z1, Z2, V) = newton[k]
-                ids.append(("new", k, srv.submit_newton(nz, 1, 0, 0, Z1, Z2, V, loadgen.VY_REF)))
+                ids.append(("new", k, srv.submit_newton(nz, 1, 0, 0, Z1, Z2, V, loadgen.VY_REF,
+                                                        mu_seed=loadgen.MU_SEED, dmu=loadgen.DMU)))
         results = {(kind, k): srv.result(rid, timeout=120.0) for kind, k, rid in ids}
+        counters = srv.counters()
+    assert counters["newton_continued"] == len(newton)
+    assert counters["newton_solves"] > len(newton)
     for k, (n, nz, mu, steps, kk, box, state) in enumerate(evo):
         st_, lat, out = results[("evo", k)]
         assert st_ == 0
@@ -171,9 +177,18 @@ def test_config5_serving_stream():
                 assert np.array_equal(a, b), f"evolution {k}"
     for k, (nz, mu, (Z1, Z2, V)) in enumerate(newton):
         st_, lat, (z1, z2, v, it, err) = results[("new", k)]
-        r1, r2, rv, rit, rerr = O.NewtonRaphson_vy(nz, 1, 0, 0, Z1, Z2, V, loadgen.VY_REF)
-        assert (st_ == 0) == (rit < 1000 and np.isfinite(rerr)), (k, nz, mu, st_, rit)
-        if st_ == 0:
-            assert it == rit, (k, it, rit)
-            for a, b in ((z1, r1), (z2, r2), (v, rv)):
-                assert np.max(np.abs(a - b)) <= 1e-9 * max(np.max(np.abs(b)), 1.0), (k, nz, mu)
+        assert st_ == 0, (k, nz, mu, st_)
+        if mu < 5.8:
+            # below the end of the superfluid branch (a fold near mu = 5.79 at this flow) the walk leaves
+            # the branch through wandering solves whose path rounding decides: the result must be a
+            # solution at the target (the oracle's residual) in the normal phase
+            r = O.NewtonRaphson_vy(nz, 1, 0, 0, z1, z2, v, loadgen.VY_REF)
+            assert r[3] == 0 and abs(z2[-1] - Z2[-1]) <= 1e-12 * abs(Z2[-1]), (k, nz, mu)
+            assert np.max(np.abs(z1)) < 1e-6, (k, nz, mu)
+            continue
+        r1, r2, rv, rit, rerr, rconv, rsolves = O.newton_mu_continuation(nz, 1, 0, 0, Z1, Z2, V, loadgen.VY_REF,
+                                                                         loadgen.MU_SEED, loadgen.DMU)
+        assert rconv, (k, nz, mu)
+        assert it == rit, (k, it, rit)
+        for a, b in ((z1, r1), (z2, r2), (v, rv)):
+            assert np.max(np.abs(a - b)) <= 1e-9 * max(np.max(np.abs(b)), 1.0), (k, nz, mu)

@@ -133,6 +133,35 @@ def test_newton_batch_rejects_bad_arguments(nz, nb):
     assert "hqt_newton_vy_batch" in L.hqt_last_error().decode()
 
 
+@pytest.mark.parametrize("seed,target,dmu", [(0.0, 6.0, 0.25), (6.0, -6.0, 0.25), (6.0, 8.0, 0.0),
+                                              (math.inf, 8.0, 0.25), (6.0, 8.0, math.nan), (6.0, 0.0, 0.25)])
+def test_newton_continuation_rejects_bad_arguments(seed, target, dmu):
+    """hqt_newton_vy_continue_batch checks each continued problem (seed and target A_t(z=0) finite,
+    nonzero and of one sign; step finite and positive) before touching the GPU; the server's
+    continued submission makes the same checks."""
+    import ctypes as C
+    import hqt
+    L = hqt.lib()
+    nz, nb = 8, 2
+    Z2 = np.ones((nz, nb))
+    Z2[-1, :] = 6.0
+    Z2[-1, 1] = target
+    other = np.zeros(nz * nb)
+    ms = np.array([math.nan, seed])
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    it = np.zeros(nb, dtype=np.int32)
+    steps = np.zeros(nb, dtype=np.int32)
+    conv = np.zeros(nb, dtype=np.uint8)
+    Z2f = np.asfortranarray(Z2)
+    rc = L.hqt_newton_vy_continue_batch(nz, 1.0, 0.0, 0.0, nb, dp(other), dp(Z2f), dp(other), dp(np.zeros(nb)),
+                                        dp(ms), dmu, 0, 0.0, dp(other), dp(other), dp(other),
+                                        it.ctypes.data_as(C.POINTER(C.c_int32)), dp(np.zeros(nb)),
+                                        conv.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                        steps.ctypes.data_as(C.POINTER(C.c_int32)))
+    assert rc == hqt.HQT_ERR_ARG
+    assert "problem 1" in L.hqt_last_error().decode()
+
+
 @pytest.mark.parametrize("kw", [dict(nx=1), dict(nz=3), dict(nz=33), dict(nx=600), dict(Lx=0.0), dict(Ly=-1.0)])
 def test_ctx_create_rejects_bad_arguments(kw):
     """Argument checks happen before any device call, so they run without a GPU."""

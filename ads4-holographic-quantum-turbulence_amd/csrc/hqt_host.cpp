@@ -344,6 +344,58 @@ int make_spec_pass(hqt_ctx* c, LPass& P, int dir) {
     return HQT_OK;
 }
 
+// The filter pass of one direction (the cross term through the slice matrices G) in the factored-DFT form,
+// when its line length has one and the mask is one contiguous range of DFT indices whose unpaired modes
+// (masked with the conjugate not masked, or the reverse) fall on complex columns of the factorization;
+// otherwise unchanged.
+int make_spec_filter(hqt_ctx* c, LPass& P, int dir, const std::vector<uint8_t>& mask) {
+    const int N = P.N, fid = spec_fact_id(N);
+    if (fid < 0 || !c->stab[dir] || !c->gfilt) return HQT_OK;
+    const int N2 = kSpecFact[fid][1];
+    int lo = -1, hi = -2;
+    for (int k = 0; k < N; ++k)
+        if (mask[k]) {
+            if (lo < 0) lo = k;
+            else if (k != hi + 1) return HQT_OK;          // not one range
+            hi = k;
+        }
+    int nkp = 0, kp[2] = {0, 0};
+    double ke[2] = {0, 0};
+    for (int k = 1; 2 * k < N; ++k)
+        if (mask[k] != mask[N - k]) {
+            if (nkp == 2) return HQT_OK;
+            const int k2 = k % N2;
+            if (k2 == 0 || 2 * k2 == N2) return HQT_OK;    // a real column of the factorization
+            kp[nkp] = k;
+            ke[nkp] = -(double)((mask[k] ? 1 : 0) - (mask[N - k] ? 1 : 0));
+            ++nkp;
+        }
+    for (int g = 0; g < P.ngroups; ++g)
+        for (int o = 0; o < P.g[g].nout; ++o) {
+            LOut& O = P.g[g].out[o];
+            for (int e = 0; e < O.nterms; ++e) {
+                LTerm& tm = O.term[e];
+                const int op = tm.lr ? -1 : P.g[g].slot_op[tm.slot];
+                if (tm.lr && O.gmode == 0)
+                    return fail(HQT_ERR_ARG, "internal: factored-DFT filter pass with a rank-4 term outside the cross term");
+                if (tm.lr) tm.sk = 0;
+                else if (op == OP_PRX || op == OP_QRY) tm.sk = 4;
+                else return fail(HQT_ERR_ARG, "internal: factored-DFT filter pass with an unexpected term");
+            }
+        }
+    P.fold = 3;
+    P.Mp = 64 * (fid + 1);
+    P.S = N | 1;
+    P.sp = 2.0 * M_PI / (dir ? c->p.Ly : c->p.Lx);
+    P.st = c->stab[dir];
+    P.mlo = lo;
+    P.mhi = hi;
+    P.nkp = nkp;
+    for (int q = 0; q < 2; ++q) { P.kp[q] = kp[q]; P.ke[q] = ke[q]; }
+    set_lb(P, 16, dir, 1);
+    return HQT_OK;
+}
+
 template <class F>
 int run(hqt_ctx* c, int name, F&& f) {
     hipEvent_t a = nullptr, b = nullptr;
@@ -712,6 +764,11 @@ int build_context(hqt_ctx* c) {
             LGroup& h = add_group(Q, c->fa[i], T, c->fb[i], T);
             if (c->lr[1]) add_out(h, f[i], fdt[i], 0, OP_QRY, 1, OP_NUVY, nullptr, DT_F64, 1, 0, 1);
             else add_out(h, f[i], fdt[i], 0, OP_QRY, 1, OP_NQIY, nullptr, DT_F64, 1);
+        }
+        // with the stages' x/y derivatives in the factored-DFT form, the filter too (same rule)
+        if (c->fold && factored && !(c->p.flags & HQT_FOLDED_FILTER)) {
+            if ((rc = make_spec_filter(c, P, 0, fx))) return rc;
+            if ((rc = make_spec_filter(c, Q, 1, fy))) return rc;
         }
     }
     // the folded kernel keeps one output's accumulators: a group has one source or one output;

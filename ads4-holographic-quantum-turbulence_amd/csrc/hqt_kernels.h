@@ -77,7 +77,9 @@ constexpr int LP_THREADS = 256;
 // lr = 1: rank-4 operator U V^T with the [V (Np x 4) | U (Mp x 4)] row-major table at P.ops[slot].
 // pad: parity of a dense operator's column (+1 even, -1 odd); rsum: the operator's row sum
 // (C 1 = rsum 1: 1 for the filter's real part, 0 for the derivatives).  Both used by the folded form.
-// sk (factored-DFT passes): the term's spectral multiplier, 1 d/dx, 2 d2/dx2, 3 -d/dx.
+// sk (factored-DFT passes): the term's spectral multiplier, 1 d/dx, 2 d2/dx2, 3 -d/dx, 4 the filter's
+// real part (the even part of the mask's complement, LPass::mlo .. mhi); 0 for a rank-4 term (its
+// output's gmode carries it).
 struct LTerm { int32_t tile; int32_t slot; int32_t lr; int32_t pad; int32_t rsum; int32_t sk; };
 // gmode (folded filter, the rank-4 x rank-4 cross term of Re((P_x (x) P_y) x) as U_x G U_y^T per z-slice):
 //   1 (x-pass): instead of storing the output, write per-16-line-block partials of G = D_x V_y, D_x the
@@ -120,6 +122,11 @@ struct LPass {
     // stride, sp = 2 pi / L (the derivative scale of the direction), st its trig table (pass precision)
     double sp;
     const void* st;
+    // factored-DFT filter: the masked DFT indices are mlo .. mhi; the unpaired ones (masked with the
+    // conjugate index not masked, or the reverse) are kp[0 .. nkp), ke = -1 or +1 the sign
+    // -(mask[kp] - mask[N - kp]) of the rank-4 imaginary part they leave
+    int32_t mlo, mhi, nkp, kp[2];
+    double ke[2];
 };
 
 // Volume fields are [nx, ny, nz] with rows rp >= nx elements apart (rp a multiple of 16, so that a

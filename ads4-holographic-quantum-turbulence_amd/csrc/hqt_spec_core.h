@@ -27,11 +27,23 @@
 // back and written as soon as its last source is in.
 //
 // Each line enters relative to its first value x_0 (the deviation x - x_0 is formed in the source's
-// own precision at staging): the operators here annihilate constants, and in FP32 this keeps the
+// own precision at staging): the derivatives annihilate constants, and in FP32 this keeps the
 // rounding of the transforms from acting on a line's mean.
+//
+// The Fourier filter (ff.m:7-11, w = real(ifft2(b)) with the masked modes of b = fft2(x) zeroed) runs
+// through the same passes.  For a real field and the separable keep-mask a(kx) b(ky) (the complement of
+// fx | fy, tevol_driven_cluster.m:27-31), real(ifft2(a b fft2(x))) keeps the even part of the mask:
+// w = (A_e (x) B_e) x + (A_o (x) B_o) x, a_e(k) = (a(k) + a(-k)) / 2, a_o = (a(k) - a(-k)) / 2.  A_e is a
+// spectral multiplier (term kind 4, the line's x_0 times its row sum a(0) added back at the output).
+// a_o is nonzero only on the unpaired masked modes (two for the reference's masks), so the second
+// product is rank 4 per direction: the x-pass reads the four real DFT coefficients of those modes off
+// each line's spectrum and writes per-block partials of the 4 x 4 slice matrix G (as the folded
+// passes do); the y-pass forms U_x G per line and adds it to the spectrum at the same modes before
+// the inverse transform.
 #pragma once
 #include <utility>
 
+#include "hqt_fold_core.h"
 #include "hqt_line_core.h"
 
 namespace hqt {
@@ -67,6 +79,14 @@ inline int spec_table_len(int fid) {
     const int N1 = kSpecFact[fid][0], N2 = kSpecFact[fid][1], H = N2 / 2;
     return 2 * (H + 1) + 2 * N1 + 2 * (H + 1) * N1;
 }
+
+// Byte offset of the filter cross-term area (80 doubles, then 16 x 4 values of the pass precision) in the
+// dynamic LDS of a factored-DFT pass: after the two line regions and the trig table, 8-byte aligned.
+__host__ __device__ inline int spec_cross_offset(int S, int tab, int es) {
+    return ((2 * 16 * S + tab) * es + 7) & ~7;
+}
+// Bytes of that area.
+__host__ __device__ inline int spec_cross_bytes(int es) { return 80 * 8 + 16 * 4 * es; }
 
 #if defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC__)
 
@@ -161,8 +181,10 @@ __device__ __forceinline__ void spec_mul(int kind, T p, int k, int N, T xr, T xi
     }
 }
 
-// R: type of the prefetch registers (double when a source of the pass is double).
-template <typename T, int DIR, int N1, int N2, typename R>
+// R: type of the prefetch registers (double when a source of the pass is double).  FILT: the filter's
+// passes (term kind 4, the cross term's gmode outputs, the line constant at the output); the RK-stage
+// passes are compiled without that code.
+template <typename T, int DIR, int N1, int N2, typename R, bool FILT>
 __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_spec_pass(
     const LPass* __restrict__ Pp, const Members M, const int xcd_order) {
     typedef SpecGeom<N1, N2> G;
@@ -188,6 +210,10 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
     T* Rg = reinterpret_cast<T*>(smem_raw);           // lines / spectrum, LB x S
     T* Bg = Rg + LB * S;                              // inverse stage-B output / E, O halves, LB x S
     T* TW = Bg + LB * S;                              // trig table (SpecGeom layout)
+    // filter cross term (passes with gmode outputs only): G and its per-wave partials (80 doubles), then the
+    // lines' rank-4 dots or coefficients (LB x 4)
+    double* GL = reinterpret_cast<double*>(smem_raw + spec_cross_offset(S, G::TAB, (int)sizeof(T)));
+    T* DL = reinterpret_cast<T*>(GL + 80);
     const LGroup& Gr = P.g[gy];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -352,9 +378,17 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
         }
     };
 
-    // the multiplier of a generic column: d/dx i kap, d2/dx2 -kap^2, -d/dx -i kap
-    auto gmul = [&](int kind, T kap, T xr, T xi, T& yr, T& yi) {
-        if (kind == 2) {
+    // the filter's real part at DFT index k: 1 - (mask(k) + mask(-k)) / 2
+    auto fmul = [&](int k) -> T {
+        const int kn = k == 0 ? 0 : N - k;
+        return (T)1 - (T)0.5 * (T)((k >= P.mlo && k <= P.mhi) + (kn >= P.mlo && kn <= P.mhi));
+    };
+    // the multiplier of a generic column: d/dx i kap, d2/dx2 -kap^2, -d/dx -i kap, the filter's real part
+    auto gmul = [&](int kind, T kap, int k, T xr, T xi, T& yr, T& yi) {
+        if (FILT && kind == 4) {
+            const T m = fmul(k);
+            yr = m * xr; yi = m * xi;
+        } else if (kind == 2) {
             const T m = -kap * kap;
             yr = m * xr; yi = m * xi;
         } else {
@@ -362,7 +396,9 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
             yr = -kk * xi; yi = kk * xr;
         }
     };
-    auto inv_b = [&](int kind, bool acc) {
+    // gadd: add the filter's rank-4 cross term at the unpaired modes, (ke / 2) (c_1 + i c_0) for mode kp and
+    // its conjugate at N - kp, from the line's coefficients c = DL[l][2p .. 2p + 1]
+    auto inv_b = [&](int kind, bool acc, bool gadd) {
         T cb[N1], sb[N1];
 #pragma unroll
         for (int m = 0; m < N1; ++m) { cb[m] = CB[m]; sb[m] = SB[m]; }
@@ -376,7 +412,21 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
                 const T* tw = TWD + 2 * k2 * N1;
                 const T pk2 = p * (T)k2;
 #pragma unroll
-                for (int k1 = 0; k1 < N1; ++k1) gmul(kind, pk2 + pc[k1], col[k1 * N2], col[k1 * N2 + H], ar[k1], ai[k1]);
+                for (int k1 = 0; k1 < N1; ++k1)
+                    gmul(kind, pk2 + pc[k1], k2 + N2 * k1, col[k1 * N2], col[k1 * N2 + H], ar[k1], ai[k1]);
+                if (FILT && gadd) {
+                    for (int q = 0; q < P.nkp; ++q) {
+                        const int kq = P.kp[q];
+                        const T hc = (T)(0.5 * P.ke[q]);
+                        const T dr = hc * DL[l * 4 + 2 * q + 1], di = hc * DL[l * 4 + 2 * q];
+#pragma unroll
+                        for (int k1 = 0; k1 < N1; ++k1) {
+                            const int k = k2 + N2 * k1;
+                            if (k == kq) { ar[k1] += dr; ai[k1] += di; }
+                            else if (k == N - kq) { ar[k1] += dr; ai[k1] -= di; }
+                        }
+                    }
+                }
                 dft_n1<G, 1>(ar, ai, cb, sb, zr, zi);
 #pragma unroll
                 for (int n1 = 0; n1 < N1; ++n1) {
@@ -400,7 +450,12 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
                     if (k1 == pk) { xr = col[k1 * N2]; xi = (T)0; }
                     else if (k1 < pk) { xr = col[k1 * N2]; xi = col[pk * N2]; }
                     else { xr = col[pk * N2]; xi = -col[k1 * N2]; }
-                    spec_mul(kind, p, k2 + N2 * k1, N, xr, xi, ar[k1], ai[k1]);
+                    if (FILT && kind == 4) {
+                        const T m = fmul(k2 + N2 * k1);
+                        ar[k1] = m * xr; ai[k1] = m * xi;
+                    } else {
+                        spec_mul(kind, p, k2 + N2 * k1, N, xr, xi, ar[k1], ai[k1]);
+                    }
                 }
                 dft_n1<G, 1>(ar, ai, cb, sb, zr, zi);
 #pragma unroll
@@ -458,6 +513,12 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
     // (Loading the addend earlier, before an inverse stage, costs more in register spills than its latency.)
     auto store = [&](const LOut& O, int line0, int z, int x0) -> bool {
         const int xb = DIR == 0 ? xline_base_at(P, line0, z, mrow) : 0;
+        // the line's constant: its first value times the row sum of the output's operator (the filter's
+        // real part; one source per such output)
+        int rs = 0;
+        if (FILT)
+            for (int e = 0; e < O.nterms; ++e) rs += O.term[e].rsum;
+        const T cst = rs ? (T)rs * (T)pref : (T)0;
         const T* row = Bg + mrow * S;
         T add[PF];
         auto load_add = [&](auto tag) {
@@ -490,7 +551,7 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
                 const int m = n2 <= H ? n2 : N2 - n2;      // E[m] -+ O[m]: - for n2 <= HI, + past the half
                 const T ov = (m >= 1 && m <= HI) ? e[H + m] : (T)0;
                 const T y = n2 <= HI ? e[m] - ov : e[m] + ov;
-                const T v = y + add[r];
+                const T v = (y + cst) + add[r];
                 const uint32_t ei = elem(line0, z, x0, xb, j);
                 bstr<S_>(rs, ei, (S_)v);
                 bad |= ei != Z_OOB && v != v;
@@ -517,6 +578,28 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
         __syncthreads();
         for (int o = 0; o < Gr.nout; ++o) {
             const LOut& O = Gr.out[o];
+            if (FILT && DIR == 0 && O.gmode == 1) {
+                // filter x-pass: the unit's rank-4 dots (cos, sin sums of the unpaired modes: Re X[kp], -Im X[kp])
+                // from the spectrum, then this 16-line block's partials of the cross-term matrix
+                for (int idx = tid; idx < LB * 4; idx += LP_THREADS) {
+                    const int l = idx >> 2, sl = idx & 3, q = sl >> 1;
+                    T v = (T)0;
+                    if (q < P.nkp) {
+                        int k = P.kp[q], sg = 1;
+                        int k2 = k % N2;
+                        if (!(k2 >= 1 && k2 <= HI)) { k = N - k; k2 = k % N2; sg = -1; }   // stored conjugate
+                        const T* e = Rg + l * S + (k / N2) * N2 + k2;
+                        v = (sl & 1) ? (T)(-sg) * e[H] : e[0];
+                    }
+                    DL[idx] = v;
+                }
+                __syncthreads();
+                fold_g_partials<T, 1>(P, O, DL, line0, nl);
+                __syncthreads();
+                continue;
+            }
+            const bool gadd = FILT && DIR == 1 && O.gmode == 2;
+            if (gadd) fold_lr_from_g<T, 1>(P, O, DL, GL, x0, z);   // DL[l][s] = sum_r U_x[x0 + l][r] G[r][s]
             // the output is finished when its last source is this unit's
             bool last = false;
             for (int e = 0; e < O.nterms; ++e) last |= O.term[e].tile == t;
@@ -525,8 +608,8 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
             for (int e = 0; e < O.nterms; ++e) {
                 const LTerm& tm = O.term[e];
                 if (tm.tile < t) first = false;
-                if (tm.tile != t) continue;
-                inv_b(tm.sk, !first);
+                if (tm.tile != t || tm.lr) continue;          // a rank-4 term is the output's gmode
+                inv_b(tm.sk, !first, gadd);
                 __syncthreads();
                 first = false;
             }
@@ -542,9 +625,9 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
     }
 }
 
-template <typename T, int DIR, int N1, int N2, typename R>
+template <typename T, int DIR, int N1, int N2, typename R, bool FILT>
 hipError_t spec_launch_one(const LPass* d, Members M, dim3 grid, size_t lds, hipStream_t s, int order) {
-    const void* fn = reinterpret_cast<const void*>(&k_spec_pass<T, DIR, N1, N2, R>);
+    const void* fn = reinterpret_cast<const void*>(&k_spec_pass<T, DIR, N1, N2, R, FILT>);
     if (lds > 64 * 1024) {
         static bool done = false;
         if (!done) {
@@ -553,36 +636,36 @@ hipError_t spec_launch_one(const LPass* d, Members M, dim3 grid, size_t lds, hip
             done = true;
         }
     }
-    hipLaunchKernelGGL((k_spec_pass<T, DIR, N1, N2, R>), grid, dim3(LP_THREADS), lds, s, d, M, order);
+    hipLaunchKernelGGL((k_spec_pass<T, DIR, N1, N2, R, FILT>), grid, dim3(LP_THREADS), lds, s, d, M, order);
     return hipGetLastError();
 }
 
-template <typename T, int DIR, typename R>
+template <typename T, int DIR, typename R, bool FILT>
 hipError_t spec_launch_r(const LPass* d, Members M, dim3 grid, int fid, size_t lds, hipStream_t s, int order) {
     switch (fid) {
-        case 0: return spec_launch_one<T, DIR, 7, 43, R>(d, M, grid, lds, s, order);
-        case 1: return spec_launch_one<T, DIR, 4, 8, R>(d, M, grid, lds, s, order);
-        case 2: return spec_launch_one<T, DIR, 8, 8, R>(d, M, grid, lds, s, order);
-        case 3: return spec_launch_one<T, DIR, 8, 12, R>(d, M, grid, lds, s, order);
-        case 4: return spec_launch_one<T, DIR, 8, 16, R>(d, M, grid, lds, s, order);
-        case 5: return spec_launch_one<T, DIR, 8, 24, R>(d, M, grid, lds, s, order);
-        case 6: return spec_launch_one<T, DIR, 8, 32, R>(d, M, grid, lds, s, order);
+        case 0: return spec_launch_one<T, DIR, 7, 43, R, FILT>(d, M, grid, lds, s, order);
+        case 1: return spec_launch_one<T, DIR, 4, 8, R, FILT>(d, M, grid, lds, s, order);
+        case 2: return spec_launch_one<T, DIR, 8, 8, R, FILT>(d, M, grid, lds, s, order);
+        case 3: return spec_launch_one<T, DIR, 8, 12, R, FILT>(d, M, grid, lds, s, order);
+        case 4: return spec_launch_one<T, DIR, 8, 16, R, FILT>(d, M, grid, lds, s, order);
+        case 5: return spec_launch_one<T, DIR, 8, 24, R, FILT>(d, M, grid, lds, s, order);
+        case 6: return spec_launch_one<T, DIR, 8, 32, R, FILT>(d, M, grid, lds, s, order);
         default: return hipErrorInvalidValue;
     }
 }
 
-// Every factored-DFT instance of one (precision, direction) family; explicitly instantiated by the
-// hqt_spec_*.hip translation units.  fid: index into kSpecFact; wide: a double source in an FP32 pass
-// (prefetch registers in double).
-template <typename T, int DIR>
+// Every factored-DFT instance of one (precision, direction, filter or RK-stage) family; explicitly
+// instantiated by the hqt_spec_*.hip translation units.  fid: index into kSpecFact; wide: a double source
+// in an FP32 pass (prefetch registers in double).
+template <typename T, int DIR, bool FILT>
 hipError_t spec_launch_td(const LPass* d, Members M, dim3 grid, int fid, size_t lds, hipStream_t s, int order,
                           int wide) {
     if constexpr (sizeof(T) == 8) {
         (void)wide;
-        return spec_launch_r<T, DIR, double>(d, M, grid, fid, lds, s, order);
+        return spec_launch_r<T, DIR, double, FILT>(d, M, grid, fid, lds, s, order);
     } else {
-        if (wide) return spec_launch_r<T, DIR, double>(d, M, grid, fid, lds, s, order);
-        return spec_launch_r<T, DIR, float>(d, M, grid, fid, lds, s, order);
+        if (wide) return spec_launch_r<T, DIR, double, FILT>(d, M, grid, fid, lds, s, order);
+        return spec_launch_r<T, DIR, float, FILT>(d, M, grid, fid, lds, s, order);
     }
 }
 

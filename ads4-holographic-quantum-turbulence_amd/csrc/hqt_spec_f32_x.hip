@@ -1,6 +1,6 @@
-// Factored-DFT line passes: explicit instances of the float x-direction family.
+// Factored-DFT line passes: explicit instances of the float x-direction family, RK-stage passes.
 #include "hqt_spec_core.h"
 
 namespace hqt {
-template hipError_t spec_launch_td<float, 0>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
+template hipError_t spec_launch_td<float, 0, false>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
 }  // namespace hqt

@@ -179,14 +179,18 @@ def capacity(kind: str, count: int, seed: int = 0, budget: int = 64 << 30, quant
 
 
 def sweep(rates=(1.0, 2.0, 4.0, 8.0), duration: float = 8.0, seed: int = 0, newton_burst: int = 128,
-          evo_burst: int = 12, budget: int = 64 << 30, quantum: int = 25, precision: str = "fp64"):
+          evo_burst: int = 12, budget: int = 64 << 30, quantum: int = 25, precision: str = "fp64", log=None):
     """Capacity of each kind (bursts), then the mixed stream at each rate: offered vs completed
     requests/s, Newton solves/s, evolution steps/s and p50/p99 latency per rate.  ``load`` is the
     evolution work the stream offers per second (point updates over its arrival span) over the
     burst's capacity: the queue of a stream with load >= 1 grows without bound, and
-    ``saturation_rate_per_s`` is the first rate where it does."""
+    ``saturation_rate_per_s`` is the first rate where it does.  ``log``: called with a line of text
+    after each part."""
+    say = log if log is not None else (lambda msg: None)
     cap_n = capacity("newton", newton_burst, seed, budget, quantum, precision)
+    say(f"serving: Newton burst of {newton_burst}: {cap_n['newton']['solves_per_s']} solves/s")
     cap_e = capacity("evo", evo_burst, seed, budget, quantum, precision)
+    say(f"serving: evolution burst of {evo_burst}: {cap_e['evolution']['steps_per_s']} steps/s")
     per_rate = []
     sat = None
     cap_pu = cap_e["evolution"]["point_updates_per_s"]
@@ -201,6 +205,7 @@ def sweep(rates=(1.0, 2.0, 4.0, 8.0), duration: float = 8.0, seed: int = 0, newt
                                                                "p99_s", "status")}
         short["newton_solves_run"] = rec["counters"]["newton_solves"]
         per_rate.append(short)
+        say(f"serving: rate {r}/s: load {short['load']}, p50 {short['p50_s']} s, p99 {short['p99_s']} s")
         if sat is None and load is not None and load >= 1.0:
             sat = r
     return {

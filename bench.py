@@ -75,6 +75,12 @@ def line_pass_fields(name, stage_mix=(1, 3)):
         return (18 * stage_mix[0] + 16 * stage_mix[1]) / sum(stage_mix)
     if name in ("line_x_filter", "line_y_filter"):
         return 10
+    if name == "zstage":            # 16 field reads + RK bookkeeping (4 evolved fields: 2 reads, 2 writes each)
+        return 32
+    if name == "line_cons":         # A_x, A_y in; d_x A_x, d_y A_y out
+        return 4
+    if name == "zpost":             # Psi (2), d_x A_x, d_y A_y, rho in; A_t out
+        return 6
     return None
 
 
@@ -84,6 +90,8 @@ def factored_transforms(name, stage_mix=(1, 3)):
         return (13 * stage_mix[0] + 11 * stage_mix[1]) / sum(stage_mix)
     if name == "line_y_stage":      # R, I: 1 + 2 each; A_y: 1 + 1; (A_x, dx A_y): 2 + 1; dx A_x: 1 + 1 (+ A_t: 1 + 1)
         return (15 * stage_mix[0] + 13 * stage_mix[1]) / sum(stage_mix)
+    if name in ("line_x_filter", "line_y_filter"):   # 5 fields: 1 + 1 each
+        return 10
     return None
 
 
@@ -141,6 +149,11 @@ def run_gpu(nx, ny, nz, L, precision, steps, warmup, state, axb, profile_steps, 
     return (t1 - t0), final, stats
 
 
+def progress(msg):
+    """One progress line on stderr (stdout carries only the JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_info():
     """CPU model, logical CPUs visible to this process and the BLAS/OpenMP thread settings."""
     model = None
@@ -168,13 +181,16 @@ def cpu_baseline(nx, ny, nz, L, state, axb, warmup=2, timed=5):
     ops = O.Operators(nx, ny, nz, L, L)
     prm = O.Params(mu=6.0, h=0.02, ax_b=axb, ay_b=0.0, winding=True)
     S = list(state)
-    for _ in range(warmup):
+    for i in range(warmup):
+        t0 = time.perf_counter()
         *S, _w = O.step(ops, prm, *S)
+        progress(f"cpu oracle warm-up step {i + 1}/{warmup}: {time.perf_counter() - t0:.1f} s")
     times = []
-    for _ in range(timed):
+    for i in range(timed):
         t0 = time.perf_counter()
         *S, _w = O.step(ops, prm, *S)
         times.append(time.perf_counter() - t0)
+        progress(f"cpu oracle timed step {i + 1}/{timed}: {times[-1]:.1f} s")
     return 1.0 / float(np.median(times)), times
 
 
@@ -218,6 +234,7 @@ def ensemble_leg(members, steps, seed=4):
         t0 = time.perf_counter()
         ens.step(steps)
         tens = (time.perf_counter() - t0) / steps
+        progress(f"ensemble {prec}: {1e3 * tens:.2f} ms/step, one by one {1e3 * tseq:.2f}")
         out[prec] = {"ms_per_step": round(1e3 * tens, 3), "member_steps_per_s": round(members / tens, 1),
                      "point_updates_per_s": round(pts / tens, 1), "launches_per_step": ens.launches_per_step,
                      "members_one_by_one_ms_per_step": round(1e3 * tseq, 3), "speedup": round(tseq / tens, 3)}
@@ -365,6 +382,7 @@ def main():
     t64, fin64, stats = run_gpu(nx, ny, nz, L, "fp64", args.steps, args.warmup, state, axb, prof_steps, args.dense_lines,
                                 args.folded)
     sps = args.steps / t64
+    progress(f"fp64 leg: {sps:.1f} steps/s")
 
     # dominant kernel roofline (HIP events on the library's stream)
     kernels = {}
@@ -447,17 +465,20 @@ def main():
             mixed["O_abs_rel_err"] = float(np.max(np.abs(np.abs(Om) - np.abs(O64))) / np.max(np.abs(O64)))
             mixed["vortices"] = diagnostics.countN(Om)
         out["mixed"] = mixed
+        progress(f"mixed leg: {mixed['value']} steps/s")
 
     if not args.no_accuracy and not args.no_mixed:
         out["mixed_accuracy"] = mixed_accuracy_leg()
+        progress("mixed accuracy leg done")
 
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(args.ensemble_members, args.ensemble_steps)
+        progress("ensemble leg done")
 
     if not args.no_serving:
         from hqt import loadgen
         rates = tuple(float(r) for r in args.serving_rates.split(",") if r.strip())
-        out["serving"] = loadgen.sweep(rates=rates, duration=args.serving_seconds, seed=0)
+        out["serving"] = loadgen.sweep(rates=rates, duration=args.serving_seconds, seed=0, log=progress)
 
     if not args.no_cpu:
         cps, times = cpu_baseline(nx, ny, nz, L, state, axb, warmup=args.cpu_warmup, timed=args.cpu_steps)

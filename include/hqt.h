@@ -55,6 +55,9 @@ extern "C" {
                                eigenvalues, one inverse per output; results agree to rounding) */
 #define HQT_FACTORED_STAGES 512 /* HQT_MIXED: the factored-DFT form of the RK stages' x/y derivatives where it
                                exists (the mixed form's default is the folded FP32 matrix products) */
+#define HQT_FOLDED_FILTER 1024 /* with the factored-DFT stages: keep the filter passes as folded matrix products
+                               (by default the filter takes the factored-DFT form with the stages, where
+                               the mask is one range whose unpaired modes fall on complex columns) */
 
 #define HQT_FP64   0        /* plain form: FP64 end to end */
 #define HQT_MIXED  1        /* x/y operators, filter, stage arithmetic FP32; z operators FP64 */

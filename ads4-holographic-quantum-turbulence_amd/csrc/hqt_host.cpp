@@ -37,6 +37,18 @@ inline int fail(int code, const std::string& msg) { return hqt_fail(code, msg); 
 
 // The folded filter takes the cross term through 4 x 4 slice matrices when both directions have the
 // rank-4 imaginary part and a 16-line block of x-lines touches at most two z-slices (ny >= 16).
+// Whether the x/y passes of line length N take the factored-DFT form.  HQT_FACTORED_STAGES and
+// HQT_FOLDED_STAGES force one form; by default FP64 passes of N >= 160 take it: per RK4 step the factored
+// FP64 passes beat the folded FP64 matrix products at N = 192, 256 and 301 (1.08x, 1.24x, 1.23x) and lose
+// at 64, 96 and 128 (0.78x, 0.84x, 0.96x), where a line's transform leaves the work-group mostly waiting
+// on its barriers.  The mixed form keeps the folded FP32 products, which run at twice the FP64 matrix
+// rate (the folded form was faster at every one of those sizes).
+bool factored_form(const hqt_params& p, int N) {
+    if (spec_fact_id(N) < 0 || (p.flags & HQT_DENSE_LINES) || (p.flags & HQT_FOLDED_STAGES)) return false;
+    if (p.flags & HQT_FACTORED_STAGES) return true;
+    return p.precision == HQT_FP64 && N >= 160;
+}
+
 bool gfilt_on(const hqt_params& p, bool lrx, bool lry) {
     return !(p.flags & HQT_DENSE_LINES) && lrx && lry && p.ny >= 16;
 }
@@ -57,10 +69,8 @@ int64_t ctx_footprint(const hqt_params& p) {
     b += 6 * ((int64_t)line_geom(p.nx).TL + (int64_t)line_geom(p.ny).TL) * es;   // x/y operator tables
     const bool fold = !(p.flags & HQT_DENSE_LINES);
     if (fold) b += 6 * ((int64_t)fold_geom(p.nx).TL + (int64_t)fold_geom(p.ny).TL) * es;   // folded tables
-    const bool factored = p.precision == HQT_MIXED ? (p.flags & HQT_FACTORED_STAGES) != 0 : !(p.flags & HQT_FOLDED_STAGES);
-    if (fold && factored)                                 // trig tables of the factored-DFT stage passes
-        for (int n : {p.nx, p.ny})
-            if (spec_fact_id(n) >= 0) b += (int64_t)spec_table_len(spec_fact_id(n)) * es;
+    for (int n : {p.nx, p.ny})                            // trig tables of the factored-DFT passes
+        if (factored_form(p, n)) b += (int64_t)spec_table_len(spec_fact_id(n)) * es;
     bool lr[2] = {false, false};
     for (int d = 0; d < 2; ++d) {                         // rank-4 filter tables, when used
         if (p.flags & HQT_DENSE_FILTER) break;
@@ -707,16 +717,16 @@ int build_context(hqt_ctx* c) {
     void* SR_[4] = {c->SB[0], c->SB[1], c->SB[2], c->SB[3]};
     stage_x(c->hp[PXR], SR_, c->AtR);
     stage_y(c->hp[PYR], SR_, c->AtR);
-    // the RK stages' (and the RHS entry's) derivative passes in the factored-DFT form where it exists: by
-    // default in FP64, where the VALU transforms beat the FP64 matrix products; FP32 matrix products run at
-    // twice the FP64 rate while the FP32 vector FMA does not, so the mixed form keeps the folded passes
-    // unless HQT_FACTORED_STAGES asks for them
-    const bool factored = c->fp32 ? (c->p.flags & HQT_FACTORED_STAGES) != 0 : !(c->p.flags & HQT_FOLDED_STAGES);
-    if (c->fold && factored) {
+    // the RK stages' (and the RHS entry's) derivative passes of each direction in the factored-DFT form
+    // where factored_form picks it
+    const bool fac[2] = {c->fold && factored_form(c->p, nx), c->fold && factored_form(c->p, ny)};
+    if (fac[0]) {
         if ((rc = build_spec_table(c, nx, 0))) return rc;
-        if ((rc = build_spec_table(c, ny, 1))) return rc;
         for (int p : {PX1, PX2, PX3, PX4, PXR})
             if ((rc = make_spec_pass(c, c->hp[p], 0))) return rc;
+    }
+    if (fac[1]) {
+        if ((rc = build_spec_table(c, ny, 1))) return rc;
         for (int p : {PY1, PY2, PY3, PY4, PYR})
             if ((rc = make_spec_pass(c, c->hp[p], 1))) return rc;
     }
@@ -765,10 +775,10 @@ int build_context(hqt_ctx* c) {
             if (c->lr[1]) add_out(h, f[i], fdt[i], 0, OP_QRY, 1, OP_NUVY, nullptr, DT_F64, 1, 0, 1);
             else add_out(h, f[i], fdt[i], 0, OP_QRY, 1, OP_NQIY, nullptr, DT_F64, 1);
         }
-        // with the stages' x/y derivatives in the factored-DFT form, the filter too (same rule)
-        if (c->fold && factored && !(c->p.flags & HQT_FOLDED_FILTER)) {
-            if ((rc = make_spec_filter(c, P, 0, fx))) return rc;
-            if ((rc = make_spec_filter(c, Q, 1, fy))) return rc;
+        // the filter of a direction whose stage passes are factored, too
+        if (!(c->p.flags & HQT_FOLDED_FILTER)) {
+            if (fac[0] && (rc = make_spec_filter(c, P, 0, fx))) return rc;
+            if (fac[1] && (rc = make_spec_filter(c, Q, 1, fy))) return rc;
         }
     }
     // the folded kernel keeps one output's accumulators: a group has one source or one output;

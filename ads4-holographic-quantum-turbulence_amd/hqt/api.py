@@ -179,8 +179,8 @@ def make_params(nx, ny, nz, Lx, Ly, mu=6.0, dt=0.02, winding=True, filter=True, 
     products (func_Lx's own form) instead of their even/odd folded form.  ``xcd_order`` (with ``lines``)
     runs the folded line passes in the XCD-aware block order.  ``factored`` chooses the form of the RK
     stages' x/y derivatives where the line length has a factored-DFT form (one DFT per source line, the
-    operators' eigenvalues, one inverse per output): None takes it in FP64 and the folded matrix
-    products in the mixed form, True / False force one form.  The filter follows the stages' form
+    operators' eigenvalues, one inverse per output): None takes it for FP64 lines of 160 points or more and
+    the folded matrix products otherwise, True / False force one form.  The filter follows the stages' form
     unless ``folded_filter`` keeps its folded matrix products."""
     p = Params()
     p.nx, p.ny, p.nz = int(nx), int(ny), int(nz)

@@ -391,7 +391,7 @@ def main():
     dom = max(stats.items(), key=lambda kv: kv[1][1])[0] if stats else None
     roof = None
     fact = {301: (7, 43), 32: (4, 8), 64: (8, 8), 96: (8, 12), 128: (8, 16), 192: (8, 24), 256: (8, 32)}
-    factored_form = not args.dense_lines and not args.folded and nx == ny and nx in fact
+    factored_form = not args.dense_lines and not args.folded and nx == ny and nx in fact and nx >= 160
     if dom is not None and line_pass_fields(dom) is not None:
         cnt, ms = stats[dom]
         avg_s = ms / cnt / 1e3

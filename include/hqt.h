@@ -49,15 +49,16 @@ extern "C" {
 #define HQT_DENSE_LINES 64  /* x/y operators and filter as dense Toeplitz products (func_Lx's own form)
                                instead of their even/odd folded form (the default: half the
                                multiply-adds; results agree to rounding) */
-#define HQT_FOLDED_STAGES 256 /* FP64: the RK stages' x/y derivatives as even/odd folded matrix products even
-                               where the line length has a factored-DFT form (the FP64 default there,
-                               N = 32, 64, 96, 128, 192, 256, 301: each line's DFT once, the operators'
+#define HQT_FOLDED_STAGES 256 /* the RK stages' x/y derivatives (and the filter) as even/odd folded matrix
+                               products even where the line length has a factored-DFT form
+                               (N = 32, 64, 96, 128, 192, 256, 301: each line's DFT once, the operators'
                                eigenvalues, one inverse per output; results agree to rounding) */
-#define HQT_FACTORED_STAGES 512 /* HQT_MIXED: the factored-DFT form of the RK stages' x/y derivatives where it
-                               exists (the mixed form's default is the folded FP32 matrix products) */
-#define HQT_FOLDED_FILTER 1024 /* with the factored-DFT stages: keep the filter passes as folded matrix products
-                               (by default the filter takes the factored-DFT form with the stages, where
-                               the mask is one range whose unpaired modes fall on complex columns) */
+#define HQT_FACTORED_STAGES 512 /* the factored-DFT form wherever the line length has one (by default only
+                               FP64 passes of N >= 160 take it; the mixed form keeps the folded FP32
+                               matrix products) */
+#define HQT_FOLDED_FILTER 1024 /* keep the filter passes as folded matrix products where the stages take the
+                               factored-DFT form (by default the filter follows them, where the mask is
+                               one range whose unpaired modes fall on complex columns) */
 
 #define HQT_FP64   0        /* plain form: FP64 end to end */
 #define HQT_MIXED  1        /* x/y operators, filter, stage arithmetic FP32; z operators FP64 */

@@ -1,7 +1,7 @@
-"""The RK stages' x/y derivative passes in their factored-DFT form (the default where the line length
-is 32, 64, 96, 128, 192, 256 or 301: one DFT per source line, the operators' eigenvalues, one inverse
-per output) against the even/odd folded matrix products of the same operators (the fourdif matrices
-of tevol_driven_cluster.m:60-61, which the oracle parity tests pin).
+"""The RK stages' x/y derivative passes in their factored-DFT form (line lengths 32, 64, 96, 128, 192,
+256 and 301, forced here; the FP64 default from 192 up: one DFT per source line, the operators'
+eigenvalues, one inverse per output) against the even/odd folded matrix products of the same operators
+(the fourdif matrices of tevol_driven_cluster.m:60-61, which the oracle parity tests pin).
 
 FP64: F1..F4 on a generic state and the state after 5 steps agree to 1e-12 relative on every supported
 line length, mixed with unsupported ones (non-square grids), so each direction takes its own form.
@@ -87,7 +87,7 @@ def test_factored_filter_matches_folded_filter(grid):
     axb = ic.sinusoidal_drive(nx, ny, 50.0, 3, 0.1)
     res = {1: [], 5: []}
     for folded_filter in (False, True):
-        with hqt.Stepper(nx, ny, nz, 50.0, 40.0, mu=6.0, dt=0.02, ax_b=axb, winding=False,
+        with hqt.Stepper(nx, ny, nz, 50.0, 40.0, mu=6.0, dt=0.02, ax_b=axb, winding=False, factored=True,
                          folded_filter=folded_filter) as st:
             st.set_state(*gen, rho)
             st.step(1)

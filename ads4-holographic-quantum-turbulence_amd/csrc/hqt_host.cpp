@@ -49,6 +49,13 @@ bool factored_form(const hqt_params& p, int N) {
     return p.precision == HQT_FP64 && N >= 160;
 }
 
+// The mixed form's filter passes compute w = x + (P_r - I) x (+ the cross term) instead of P_r x: the
+// source passes through as the output's addend and only the masked modes' part, which is small, goes
+// through the FP32 matrix products or transforms (applying P_r x itself in FP32 rounds a full N-term sum of
+// values of the field's size).  After 200 steps of the driven 301^2 x 20 flow the mixed |O| then differs
+// from the FP64 one by 8e-6 instead of 2.8e-5.
+bool filter_correction(const hqt_ctx* c) { return c->fp32 && c->fold; }
+
 bool gfilt_on(const hqt_params& p, bool lrx, bool lry) {
     return !(p.flags & HQT_DENSE_LINES) && lrx && lry && p.ny >= 16;
 }
@@ -252,7 +259,11 @@ int build_tables(hqt_ctx* c, int N, double L, const uint8_t* mask, int base) {
         // folded tables of the same six operators (columns and parities as above)
         const FoldGeom fg = fold_geom(N);
         std::vector<double> ft;
-        const std::vector<double>* fc[6] = {&col, &c2, &col, &pr, &pi, &pi};
+        // mixed form: the filter's real part enters as its masked part P_r - I, the pass adding the source
+        // itself as the output's addend (filter_correction)
+        std::vector<double> prm = pr;
+        if (filter_correction(c)) prm[0] -= 1.0;
+        const std::vector<double>* fc[6] = {&col, &c2, &col, &prm, &pi, &pi};
         const int fsym[6] = {-1, 1, -1, 1, -1, -1};
         const double fsc[6] = {px, 1.0, -px, 1.0, 1.0, -1.0};
         for (int t = 0; t < 6; ++t) {
@@ -400,6 +411,7 @@ int make_spec_filter(hqt_ctx* c, LPass& P, int dir, const std::vector<uint8_t>& 
     P.st = c->stab[dir];
     P.mlo = lo;
     P.mhi = hi;
+    P.fcorr = filter_correction(c) ? 1 : 0;
     P.nkp = nkp;
     for (int q = 0; q < 2; ++q) { P.kp[q] = kp[q]; P.ke[q] = ke[q]; }
     set_lb(P, 16, dir, 1);
@@ -753,9 +765,10 @@ int build_context(hqt_ctx* c) {
         // Folded with both directions rank 4 (and ny >= 16): the cross term (Im P_x (x) Im P_y) x is
         // U_x G U_y^T per z-slice, so the x-pass writes partials of G (4 x 4 per slice) instead of the
         // intermediate field Im(P_x) x, and the y-pass has the one source Re(P_x) x.
+        const bool corr = filter_correction(c);
         for (int i = 0; i < 5; ++i) {
             LGroup& g = add_group(P, f[i], fdt[i]);
-            add_out(g, c->fa[i], T, 0, OP_PRX);
+            add_out(g, c->fa[i], T, 0, OP_PRX, -1, -1, corr ? f[i] : nullptr, corr ? fdt[i] : DT_F64);
             if (c->gfilt) {
                 double* gp = c->Gp + (size_t)i * 32 * c->gblk;
                 add_out(g, gp, DT_F64, 0, OP_UVX, -1, -1, nullptr, DT_F64, 0, 1);
@@ -763,7 +776,7 @@ int build_context(hqt_ctx* c) {
                 g.out[1].gp = gp;
                 g.out[1].gtab = c->ops[OP_UVY];                                        // V_y: ny x 4
                 LGroup& h = add_group(Q, c->fa[i], T);
-                add_out(h, f[i], fdt[i], 0, OP_QRY, 0, OP_NUVY, nullptr, DT_F64, 1, 0, 1);
+                add_out(h, f[i], fdt[i], 0, OP_QRY, 0, OP_NUVY, corr ? c->fa[i] : nullptr, corr ? T : DT_F64, 1, 0, 1);
                 h.out[0].gmode = 2;
                 h.out[0].gp = gp;
                 h.out[0].gtab = static_cast<const char*>(c->ops[OP_UVX]) + 4 * (size_t)line_geom(nx).Np * c->es;  // U_x
@@ -772,8 +785,8 @@ int build_context(hqt_ctx* c) {
             if (c->lr[0]) add_out(g, c->fb[i], T, 0, OP_UVX, -1, -1, nullptr, DT_F64, 0, 1);
             else add_out(g, c->fb[i], T, 0, OP_PIX);
             LGroup& h = add_group(Q, c->fa[i], T, c->fb[i], T);
-            if (c->lr[1]) add_out(h, f[i], fdt[i], 0, OP_QRY, 1, OP_NUVY, nullptr, DT_F64, 1, 0, 1);
-            else add_out(h, f[i], fdt[i], 0, OP_QRY, 1, OP_NQIY, nullptr, DT_F64, 1);
+            if (c->lr[1]) add_out(h, f[i], fdt[i], 0, OP_QRY, 1, OP_NUVY, corr ? c->fa[i] : nullptr, corr ? T : DT_F64, 1, 0, 1);
+            else add_out(h, f[i], fdt[i], 0, OP_QRY, 1, OP_NQIY, corr ? c->fa[i] : nullptr, corr ? T : DT_F64, 1);
         }
         // the filter of a direction whose stage passes are factored, too
         if (!(c->p.flags & HQT_FOLDED_FILTER)) {
@@ -792,8 +805,10 @@ int build_context(hqt_ctx* c) {
                     LTerm& tm = H.g[g].out[o].term[e];
                     if (tm.lr) continue;
                     const int op = H.g[g].slot_op[tm.slot];
-                    if (op == OP_PRX) tm.rsum = fx[0] ? 0 : 1;
-                    if (op == OP_QRY) tm.rsum = fy[0] ? 0 : 1;
+                    // row sums: of P_r, 1 - mask[0]; of P_r - I (the correction form), -mask[0]
+                    const int corr1 = filter_correction(c) ? 1 : 0;
+                    if (op == OP_PRX) tm.rsum = (fx[0] ? 0 : 1) - corr1;
+                    if (op == OP_QRY) tm.rsum = (fy[0] ? 0 : 1) - corr1;
                 }
             if (H.fold && H.g[g].ntiles > 1 && H.g[g].nout > 1)
                 return fail(HQT_ERR_ARG, "internal: folded line pass group with two sources and two outputs");

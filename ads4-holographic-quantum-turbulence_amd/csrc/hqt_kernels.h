@@ -127,6 +127,8 @@ struct LPass {
     // -(mask[kp] - mask[N - kp]) of the rank-4 imaginary part they leave
     int32_t mlo, mhi, nkp, kp[2];
     double ke[2];
+    int32_t fcorr;             // filter term in the correction form: the multiplier is -(mask(k) + mask(-k)) / 2
+                               // (the masked part, the pass adding the source as the output's addend)
 };
 
 // Volume fields are [nx, ny, nz] with rows rp >= nx elements apart (rp a multiple of 16, so that a

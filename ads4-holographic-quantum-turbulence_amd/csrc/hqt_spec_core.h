@@ -378,10 +378,12 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
         }
     };
 
-    // the filter's real part at DFT index k: 1 - (mask(k) + mask(-k)) / 2
+    // the filter's real part at DFT index k: 1 - (mask(k) + mask(-k)) / 2, or its masked part -(mask(k) +
+    // mask(-k)) / 2 in the correction form (the output then adds the source itself)
     auto fmul = [&](int k) -> T {
         const int kn = k == 0 ? 0 : N - k;
-        return (T)1 - (T)0.5 * (T)((k >= P.mlo && k <= P.mhi) + (kn >= P.mlo && kn <= P.mhi));
+        const T m = (T)0.5 * (T)((k >= P.mlo && k <= P.mhi) + (kn >= P.mlo && kn <= P.mhi));
+        return P.fcorr ? -m : (T)1 - m;
     };
     // the multiplier of a generic column: d/dx i kap, d2/dx2 -kap^2, -d/dx -i kap, the filter's real part
     auto gmul = [&](int kind, T kap, int k, T xr, T xi, T& yr, T& yi) {

@@ -565,6 +565,13 @@ int enqueue_step(hqt_ctx* c) {
                                              st, X.xcd, Y.xcd);
                  })))
                 return rc;
+        } else if (X.fold == 3 && Y.fold == 3 && X.Mp == Y.Mp && X.ngroups == 1 && Y.ngroups == 1) {
+            const size_t lds = std::max(line_pass_lds_bytes(X, f32), line_pass_lds_bytes(Y, f32));
+            if ((rc = run(c, KN_CONS, [&] {
+                     return launch_cons_pair_spec(c->dp + PXC, c->dp + PYC, line_blocks(X), line_blocks(Y), lds,
+                                                  X.Mp / 64 - 1, f32, st, X.xcd, Y.xcd);
+                 })))
+                return rc;
         } else {
             if ((rc = line(c, PXC, 0, KN_LX_CONS))) return rc;
             if ((rc = line(c, PYC, 1, KN_LY_CONS))) return rc;
@@ -753,6 +760,9 @@ int build_context(hqt_ctx* c) {
         // d_y A_y into curl3 (a stage temporary, free after the last stage); the post-step kernel
         // adds d_x A_x, so the two passes are independent
         add_out(h, c->curl3, T, 0, OP_D1Y);
+        // in the factored form where the direction takes it (one launch for both when both do, as folded)
+        if (fac[0] && (rc = make_spec_pass(c, P, 0))) return rc;
+        if (fac[1] && (rc = make_spec_pass(c, Q, 1))) return rc;
     }
     {
         void* f[5] = {c->ZR, c->ZI, c->At, c->ZAx, c->ZAy};

@@ -241,6 +241,9 @@ struct Census {
 // in one launch of nb0 + nb1 work-groups; one context.
 hipError_t launch_cons_pair(const LPass* d0, const LPass* d1, int nb0, int nb1, size_t lds, int Q, int fp32,
                             hipStream_t s, int order0, int order1);
+// The same pair in the factored-DFT form (both passes of factorization fid, descriptors with fold = 3).
+hipError_t launch_cons_pair_spec(const LPass* d0, const LPass* d1, int nb0, int nb1, size_t lds, int fid, int fp32,
+                                 hipStream_t s, int order0, int order1);
 hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, size_t lds, int Q, int nt, int dir,
                             int fp32, int kind, hipStream_t s, int fold = 0, int order = 0);
 size_t line_pass_lds_bytes(const LPass& P, int fp32);

@@ -78,6 +78,18 @@ hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, 
                : line_launch_td<double, 0>(d, M, grid, Q, nt, kind, lds, s);
 }
 
+extern template hipError_t spec_pair_launch<double>(const LPass*, const LPass*, int, int, int, size_t, hipStream_t, int,
+                                                    int);
+extern template hipError_t spec_pair_launch<float>(const LPass*, const LPass*, int, int, int, size_t, hipStream_t, int,
+                                                   int);
+
+hipError_t launch_cons_pair_spec(const LPass* d0, const LPass* d1, int nb0, int nb1, size_t lds, int fid, int fp32,
+                                 hipStream_t s, int order0, int order1) {
+    if (fid < 0 || fid >= kSpecNF || nb0 <= 0 || nb1 <= 0 || lds > 160 * 1024) return hipErrorInvalidValue;
+    return fp32 ? spec_pair_launch<float>(d0, d1, nb0, nb1, fid, lds, s, order0, order1)
+                : spec_pair_launch<double>(d0, d1, nb0, nb1, fid, lds, s, order0, order1);
+}
+
 hipError_t launch_cons_pair(const LPass* d0, const LPass* d1, int nb0, int nb1, size_t lds, int Q, int fp32,
                             hipStream_t s, int order0, int order1) {
     if (Q < 1 || Q > 5 || nb0 <= 0 || nb1 <= 0 || lds > 160 * 1024) return hipErrorInvalidValue;

@@ -184,22 +184,23 @@ __device__ __forceinline__ void spec_mul(int kind, T p, int k, int N, T xr, T xi
 // R: type of the prefetch registers (double when a source of the pass is double).  FILT: the filter's
 // passes (term kind 4, the cross term's gmode outputs, the line constant at the output); the RK-stage
 // passes are compiled without that code.
+// The pass body for work-group (bx0, by0) of an nbx x nby grid (the kernels below pass their own block
+// indices, or a sub-range of them).
 template <typename T, int DIR, int N1, int N2, typename R, bool FILT>
-__global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_spec_pass(
-    const LPass* __restrict__ Pp, const Members M, const int xcd_order) {
+__device__ __forceinline__ void spec_body(const LPass* __restrict__ Pp, const Members M, const int xcd_order,
+                                          const int bx0, const int nbx, const int by0, const int nby) {
     typedef SpecGeom<N1, N2> G;
     constexpr int LB = 16, N = G::N, H = G::H, HI = G::HI;
     constexpr int PF = (N + 15) / 16;                 // samples per thread and line
-    const int nbx = (int)gridDim.x, nby = (int)gridDim.y;
-    const int lin0 = (int)blockIdx.x + nbx * (int)blockIdx.y;
+    const int lin0 = bx0 + nbx * by0;
     int bxl, gy;
     if (xcd_order) {
         const int lin = xcd_block(lin0, nbx * nby);
         bxl = lin / nby;
         gy = lin - bxl * nby;
     } else {
-        bxl = (int)blockIdx.x;
-        gy = (int)blockIdx.y;
+        bxl = bx0;
+        gy = by0;
     }
     const MemberBlock mb = member_block_at(M, bxl, nbx);
     const LPass& P = Pp[mb.m];
@@ -624,6 +625,56 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
             }
             __syncthreads();             // the second region is read before the next output writes it
         }
+    }
+}
+
+template <typename T, int DIR, int N1, int N2, typename R, bool FILT>
+__global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_spec_pass(
+    const LPass* __restrict__ Pp, const Members M, const int xcd_order) {
+    spec_body<T, DIR, N1, N2, R, FILT>(Pp, M, xcd_order, (int)blockIdx.x, (int)gridDim.x, (int)blockIdx.y,
+                                       (int)gridDim.y);
+}
+
+// The constraint's x- and y-pass (one source and one derivative output each, independent of each other) in one
+// launch: the first nb0 work-groups run the x-pass, the rest the y-pass, each exactly as its own launch would.
+template <typename T, int N1, int N2>
+__global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_spec_pair(
+    const LPass* __restrict__ X, const LPass* __restrict__ Y, const int nb0, const int order0, const int order1) {
+    const Members one{nullptr, 1};
+    if ((int)blockIdx.x < nb0) spec_body<T, 0, N1, N2, T, false>(X, one, order0, (int)blockIdx.x, nb0, 0, 1);
+    else spec_body<T, 1, N1, N2, T, false>(Y, one, order1, (int)blockIdx.x - nb0, (int)gridDim.x - nb0, 0, 1);
+}
+
+template <typename T, int N1, int N2>
+hipError_t spec_pair_launch_one(const LPass* x, const LPass* y, int nb0, int nb1, size_t lds, hipStream_t s, int o0,
+                                int o1) {
+    const void* fn = reinterpret_cast<const void*>(&k_spec_pair<T, N1, N2>);
+    if (lds > 64 * 1024) {
+        static bool done = false;
+        if (!done) {
+            const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            if (e != hipSuccess) return e;
+            done = true;
+        }
+    }
+    hipLaunchKernelGGL((k_spec_pair<T, N1, N2>), dim3((unsigned)(nb0 + nb1)), dim3(LP_THREADS), lds, s, x, y, nb0, o0, o1);
+    return hipGetLastError();
+}
+
+// The constraint pair of one precision in the factored-DFT form (both passes of factorization fid); explicitly
+// instantiated by hqt_spec_pair.hip.
+template <typename T>
+hipError_t spec_pair_launch(const LPass* x, const LPass* y, int nb0, int nb1, int fid, size_t lds, hipStream_t s,
+                            int o0, int o1) {
+    switch (fid) {
+        case 0: return spec_pair_launch_one<T, 7, 43>(x, y, nb0, nb1, lds, s, o0, o1);
+        case 1: return spec_pair_launch_one<T, 4, 8>(x, y, nb0, nb1, lds, s, o0, o1);
+        case 2: return spec_pair_launch_one<T, 8, 8>(x, y, nb0, nb1, lds, s, o0, o1);
+        case 3: return spec_pair_launch_one<T, 8, 12>(x, y, nb0, nb1, lds, s, o0, o1);
+        case 4: return spec_pair_launch_one<T, 8, 16>(x, y, nb0, nb1, lds, s, o0, o1);
+        case 5: return spec_pair_launch_one<T, 8, 24>(x, y, nb0, nb1, lds, s, o0, o1);
+        case 6: return spec_pair_launch_one<T, 8, 32>(x, y, nb0, nb1, lds, s, o0, o1);
+        default: return hipErrorInvalidValue;
     }
 }
 

@@ -94,7 +94,7 @@ void add_line(std::vector<Launch>& plan, PlanImage& img, const std::vector<hqt_c
               bool (*use)(const hqt_ctx*)) {
     auto cls = classes<std::tuple<int, int, int, int>>(
         mem, [&](hqt_ctx* c) {
-            return std::make_tuple(c->fp32, c->hp[pass].Mp / 64, c->hp[pass].lb / 16, c->hp[pass].fold);
+            return std::make_tuple(pass_fp32(c, pass), c->hp[pass].Mp / 64, c->hp[pass].lb / 16, c->hp[pass].fold);
         },
         [&](hqt_ctx* c) { return use(c); });
     for (auto& kv : cls) {
@@ -112,7 +112,7 @@ void add_line(std::vector<Launch>& plan, PlanImage& img, const std::vector<hqt_c
         L.lkind = (name == KN_LX_FILT || name == KN_LY_FILT) ? 2 : (name == KN_LX_CONS || name == KN_LY_CONS) ? 1 : 0;
         for (hqt_ctx* c : kv.second) {
             L.ngroups = std::max(L.ngroups, c->hp[pass].ngroups);
-            L.lds = std::max(L.lds, line_pass_lds_bytes(c->hp[pass], c->fp32));
+            L.lds = std::max(L.lds, line_pass_lds_bytes(c->hp[pass], pass_fp32(c, pass)));
         }
         plan.push_back(L);
     }

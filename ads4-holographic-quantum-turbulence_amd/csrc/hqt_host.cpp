@@ -38,15 +38,17 @@ inline int fail(int code, const std::string& msg) { return hqt_fail(code, msg); 
 // The folded filter takes the cross term through 4 x 4 slice matrices when both directions have the
 // rank-4 imaginary part and a 16-line block of x-lines touches at most two z-slices (ny >= 16).
 // Whether the x/y passes of line length N take the factored-DFT form.  HQT_FACTORED_STAGES and
-// HQT_FOLDED_STAGES force one form; by default FP64 passes of N >= 160 take it: per RK4 step the factored
-// FP64 passes beat the folded FP64 matrix products at N = 192, 256 and 301 (1.08x, 1.24x, 1.23x) and lose
-// at 64, 96 and 128 (0.78x, 0.84x, 0.96x), where a line's transform leaves the work-group mostly waiting
-// on its barriers.  The mixed form keeps the folded FP32 products, which run at twice the FP64 matrix
-// rate (the folded form was faster at every one of those sizes).
+// HQT_FOLDED_STAGES force one form.  By default:
+//   * the mixed form takes it at every supported N: an FP32 transform rounds like an FFT (an error of a few
+//     ulps of the line's size), while an FP32 matrix product rounds an N-term sum of operator entries times
+//     values; on the 64^2 vortex lattice the folded FP32 products left a condensate error of 6e-3 after 100
+//     time units, and FP32 transforms in a CPU model of the same step 2e-5;
+//   * FP64 passes take it from N = 128: per RK4 step the factored FP64 passes beat the folded FP64 matrix
+//     products at 128, 192, 256 and 301 (1.09x, 1.21x, 1.43x, 1.26x) and lose at 64 (0.87x).
 bool factored_form(const hqt_params& p, int N) {
     if (spec_fact_id(N) < 0 || (p.flags & HQT_DENSE_LINES) || (p.flags & HQT_FOLDED_STAGES)) return false;
     if (p.flags & HQT_FACTORED_STAGES) return true;
-    return p.precision == HQT_FP64 && N >= 160;
+    return p.precision == HQT_MIXED || N >= 128;
 }
 
 // The mixed form's filter passes compute w = x + (P_r - I) x (+ the cross term) instead of P_r x: the
@@ -55,6 +57,11 @@ bool factored_form(const hqt_params& p, int N) {
 // values of the field's size).  After 200 steps of the driven 301^2 x 20 flow the mixed |O| then differs
 // from the FP64 one by 8e-6 instead of 2.8e-5.
 bool filter_correction(const hqt_ctx* c) { return c->fp32 && c->fold; }
+
+// Data type of the filter's operator tables and intermediate fields (FP64 when the filter computes in FP64),
+// and the element size of a data type.
+int filt_dt(const hqt_ctx* c) { return c->filt64 ? DT_F64 : c->tdt; }
+size_t dt_bytes(int dt) { return dt == DT_F64 ? 8 : 4; }
 
 bool gfilt_on(const hqt_params& p, bool lrx, bool lry) {
     return !(p.flags & HQT_DENSE_LINES) && lrx && lry && p.ny >= 16;
@@ -67,17 +74,21 @@ int64_t ctx_footprint(const hqt_params& p) {
     auto r8 = [](int64_t b) { return b == 0 ? 8 : b; };
     int64_t b = 0;
     b += 42 * np * es;                                   // state, stage, derivative and filter buffers (TS)
+    if (p.precision == HQT_MIXED && !(p.flags & HQT_FP32_FILTER)) b += 10 * np * 4;   // the filter's FP64 intermediates
     b += 3 * np * 8;                                     // At, rho, AtR
     b += ncol * 8 + 2 * ncol * 8 + r8(p.ny * 8) + 8;     // arho, O, part, W
     b += 2 * ncol * 8 + r8(p.ny * 8) + 16;               // axb, ayb, yph, flags
     b += (int64_t)NPASS * (int64_t)sizeof(LPass) + (int64_t)sizeof(hqt_ctx::DevDesc);
     b += 2 * nxy * 8 + nxy + 8 + 2 * (int64_t)census_blocks(nxy) * 8;   // diagnostics
     b += (nz * nz + (nz - 1) * (nz - 1) + 5 * nz) * 8;   // radial operators
-    b += 6 * ((int64_t)line_geom(p.nx).TL + (int64_t)line_geom(p.ny).TL) * es;   // x/y operator tables
+    // the filter's tables (3 of the 6 per direction, and the rank-4 ones) are FP64 when the mixed form's filter
+    // computes in FP64
+    const int64_t fes = (p.precision == HQT_MIXED && !(p.flags & HQT_FP32_FILTER)) ? 8 : es;
+    b += 3 * ((int64_t)line_geom(p.nx).TL + (int64_t)line_geom(p.ny).TL) * (es + fes);   // x/y operator tables
     const bool fold = !(p.flags & HQT_DENSE_LINES);
-    if (fold) b += 6 * ((int64_t)fold_geom(p.nx).TL + (int64_t)fold_geom(p.ny).TL) * es;   // folded tables
-    for (int n : {p.nx, p.ny})                            // trig tables of the factored-DFT passes
-        if (factored_form(p, n)) b += (int64_t)spec_table_len(spec_fact_id(n)) * es;
+    if (fold) b += 3 * ((int64_t)fold_geom(p.nx).TL + (int64_t)fold_geom(p.ny).TL) * (es + fes);   // folded tables
+    for (int n : {p.nx, p.ny})                            // trig tables of the factored-DFT passes (+ FP64 copy)
+        if (factored_form(p, n)) b += (int64_t)spec_table_len(spec_fact_id(n)) * (es + (fes != es ? 8 : 0));
     bool lr[2] = {false, false};
     for (int d = 0; d < 2; ++d) {                         // rank-4 filter tables, when used
         if (p.flags & HQT_DENSE_FILTER) break;
@@ -87,8 +98,8 @@ int64_t ctx_footprint(const hqt_params& p) {
         std::vector<double> t;
         if (filter_lowrank(n, mk.data(), line_geom(n), 1.0, t)) {
             lr[d] = true;
-            b += 2 * (int64_t)t.size() * es;
-            if (fold) b += 2 * (8 * (int64_t)fold_geom(n).Kp + 8 * (int64_t)fold_geom(n).Mp) * es;
+            b += 2 * (int64_t)t.size() * fes;
+            if (fold) b += 2 * (8 * (int64_t)fold_geom(n).Kp + 8 * (int64_t)fold_geom(n).Mp) * fes;
         }
     }
     if (gfilt_on(p, lr[0], lr[1])) b += 5 * (((int64_t)p.ny * p.nz + 15) / 16) * 32 * 8;   // cross-term partials
@@ -100,6 +111,10 @@ int hqt_sync_streams(hqt_ctx* c) {
     HIPC(hipStreamSynchronize(c->stream));
     return HQT_OK;
 }
+
+// Whether a pass of the context runs the FP32 kernels: the mixed form's passes, except its filter when that
+// computes in FP64 (filt64).
+int pass_fp32(const hqt_ctx* c, int p) { return c->fp32 && !(c->filt64 && (p == PXF || p == PYF)); }
 
 namespace {
 
@@ -268,9 +283,10 @@ int build_tables(hqt_ctx* c, int N, double L, const uint8_t* mask, int base) {
         const double fsc[6] = {px, 1.0, -px, 1.0, 1.0, -1.0};
         for (int t = 0; t < 6; ++t) {
             fold_table(*fc[t], fsym[t], fg, fsc[t], ft);
-            int rc = dalloc(c, ft.size() * c->es, &c->fops[base + t]);
+            const int dt = t >= 3 ? filt_dt(c) : c->tdt;     // tables 3-5: the filter's
+            int rc = dalloc(c, ft.size() * dt_bytes(dt), &c->fops[base + t]);
             if (rc) return rc;
-            rc = upload_as(c, c->fops[base + t], ft.data(), ft.size(), c->tdt);
+            rc = upload_as(c, c->fops[base + t], ft.data(), ft.size(), dt);
             if (rc) return rc;
         }
     }
@@ -279,9 +295,10 @@ int build_tables(hqt_ctx* c, int N, double L, const uint8_t* mask, int base) {
     toeplitz_table(pi, pir, Np, 1.0, tabs[4]);
     toeplitz_table(pi, pir, Np, -1.0, tabs[5]);
     for (int t = 0; t < 6; ++t) {
-        int rc = dalloc(c, tabs[t].size() * c->es, &c->ops[base + t]);
+        const int dt = t >= 3 ? filt_dt(c) : c->tdt;
+        int rc = dalloc(c, tabs[t].size() * dt_bytes(dt), &c->ops[base + t]);
         if (rc) return rc;
-        rc = upload_as(c, c->ops[base + t], tabs[t].data(), tabs[t].size(), c->tdt);
+        rc = upload_as(c, c->ops[base + t], tabs[t].data(), tabs[t].size(), dt);
         if (rc) return rc;
     }
     const int dir = base == OP_D1X ? 0 : 1, lrbase = dir ? OP_UVY : OP_UVX;
@@ -291,16 +308,17 @@ int build_tables(hqt_ctx* c, int N, double L, const uint8_t* mask, int base) {
     if (c->lr[dir]) {
         const std::vector<double>* tt[2] = {&uv, &nuv};
         for (int t = 0; t < 2; ++t) {
-            int rc = dalloc(c, tt[t]->size() * c->es, &c->ops[lrbase + t]);
+            const int dt = filt_dt(c);
+            int rc = dalloc(c, tt[t]->size() * dt_bytes(dt), &c->ops[lrbase + t]);
             if (rc) return rc;
-            rc = upload_as(c, c->ops[lrbase + t], tt[t]->data(), tt[t]->size(), c->tdt);
+            rc = upload_as(c, c->ops[lrbase + t], tt[t]->data(), tt[t]->size(), dt);
             if (rc) return rc;
             if (c->fold) {
                 std::vector<double> ft;
                 fold_lowrank(*tt[t], Np, fold_geom(N), ft);
-                rc = dalloc(c, ft.size() * c->es, &c->fops[lrbase + t]);
+                rc = dalloc(c, ft.size() * dt_bytes(dt), &c->fops[lrbase + t]);
                 if (rc) return rc;
-                rc = upload_as(c, c->fops[lrbase + t], ft.data(), ft.size(), c->tdt);
+                rc = upload_as(c, c->fops[lrbase + t], ft.data(), ft.size(), dt);
                 if (rc) return rc;
             }
         }
@@ -338,7 +356,10 @@ int build_spec_table(hqt_ctx* c, int N, int dir) {
             cos_sin_2pi((long long)n1 * k2, N, t[tw + 2 * ((size_t)k2 * N1 + n1)], t[tw + 2 * ((size_t)k2 * N1 + n1) + 1]);
     int rc = dalloc(c, t.size() * c->es, &c->stab[dir]);
     if (rc) return rc;
-    return upload_as(c, c->stab[dir], t.data(), t.size(), c->tdt);
+    if ((rc = upload_as(c, c->stab[dir], t.data(), t.size(), c->tdt))) return rc;
+    if (!c->filt64) return HQT_OK;
+    if ((rc = dalloc(c, t.size() * 8, &c->stab64[dir]))) return rc;
+    return upload_as(c, c->stab64[dir], t.data(), t.size(), DT_F64);
 }
 
 // An RK-stage pass (groups of d/dx, d2/dx2, -d/dx terms only) in the factored-DFT form, when its line
@@ -356,6 +377,26 @@ int make_spec_pass(hqt_ctx* c, LPass& P, int dir) {
                 else if (op == OP_ND1X || op == OP_ND1Y) tm.sk = 3;
                 else return fail(HQT_ERR_ARG, "internal: factored-DFT pass with a term that is not a derivative");
             }
+    // one output per group (the single-region kernel transforms a group's sources once per output): a group's
+    // second output becomes a group of its own on the same sources
+    const int ng0 = P.ngroups;
+    for (int g = 0; g < ng0; ++g)
+        while (P.g[g].nout > 1) {
+            if (P.ngroups == LP_MAXG) return fail(HQT_ERR_ARG, "internal: factored-DFT pass with too many outputs");
+            LGroup& h = P.g[P.ngroups++];
+            h = P.g[g];
+            h.nout = 1;
+            h.out[0] = P.g[g].out[P.g[g].nout - 1];
+            --P.g[g].nout;
+        }
+    for (int g = 0; g < P.ngroups; ++g) {
+        const LOut& O = P.g[g].out[0];
+        if (O.nterms != P.g[g].ntiles || O.gmode)
+            return fail(HQT_ERR_ARG, "internal: factored-DFT output without one term per source");
+        for (int e = 0; e < O.nterms; ++e)
+            if (O.term[e].tile != e || O.term[e].lr)
+                return fail(HQT_ERR_ARG, "internal: factored-DFT output without one term per source");
+    }
     P.fold = 3;
     P.Mp = 64 * (fid + 1);
     P.S = N | 1;
@@ -409,6 +450,7 @@ int make_spec_filter(hqt_ctx* c, LPass& P, int dir, const std::vector<uint8_t>& 
     P.S = N | 1;
     P.sp = 2.0 * M_PI / (dir ? c->p.Ly : c->p.Lx);
     P.st = c->stab[dir];
+    if (c->filt64) P.st = c->stab64[dir];
     P.mlo = lo;
     P.mhi = hi;
     P.fcorr = filter_correction(c) ? 1 : 0;
@@ -444,8 +486,9 @@ int line_kind(int name) {
 int line(hqt_ctx* c, int pass, int dir, int name) {
     const LPass& H = c->hp[pass];
     return run(c, name, [&] {
-        return launch_line_pass(c->dp + pass, kOne, line_blocks(H), H.ngroups, line_pass_lds_bytes(H, c->fp32), H.Mp / 64,
-                                H.lb / 16, dir, c->fp32, line_kind(name), c->stream, H.fold, H.xcd);
+        const int f32 = pass_fp32(c, pass);
+        return launch_line_pass(c->dp + pass, kOne, line_blocks(H), H.ngroups, line_pass_lds_bytes(H, f32), H.Mp / 64,
+                                H.lb / 16, dir, f32, line_kind(name), c->stream, H.fold, H.xcd);
     });
 }
 
@@ -466,6 +509,7 @@ int pass_kind(int p) { return (p == PXF || p == PYF) ? 2 : (p == PXC || p == PYC
 
 int pass_dir(int p) { return ((p >= PY1 && p <= PY4) || p == PYC || p == PYF || p == PYR) ? 1 : 0; }
 
+
 constexpr size_t kLdsCap = 160 * 1024;
 
 // Line tiles of every line pass: lines per tile (16 or 32), tiles per work-group (1, 2 or 4) and, for
@@ -484,7 +528,8 @@ int tune_lines(hqt_ctx* c) {
         LPass& H = c->hp[p];
         const int dir = pass_dir(p);
         int choice = forced ? 100 * forced + 1 + ((c->p.flags & HQT_ORDER_XCD) ? 10000 : 0) : 0;
-        const auto key = std::make_tuple(p, c->fp32, c->nx, c->ny, c->nz, (int)H.fold);
+        const int f32 = pass_fp32(c, p);
+        const auto key = std::make_tuple(p, f32, c->nx, c->ny, c->nz, (int)H.fold);
         if (!choice) {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             auto it = g_tune.find(key);
@@ -503,7 +548,7 @@ int tune_lines(hqt_ctx* c) {
                 for (int tpb : {1, 2, 4}) {
                     if (dir == 1 && tpb != 1 && !H.fold) continue;   // dense y-lines: one tile per work-group
                     set_lb(H, lb, dir, tpb);
-                    const size_t lds = line_pass_lds_bytes(H, c->fp32);
+                    const size_t lds = line_pass_lds_bytes(H, f32);
                     if (lds > kLdsCap) continue;
                     float ms = 0;
                     hipError_t e = hipMemcpy(c->dp + p, &H, sizeof(H), hipMemcpyHostToDevice);
@@ -511,7 +556,7 @@ int tune_lines(hqt_ctx* c) {
                         if (rep == 1) e = hipEventRecord(ea, c->stream);
                         if (e == hipSuccess)
                             e = launch_line_pass(c->dp + p, kOne, line_blocks(H), H.ngroups, lds, H.Mp / 64, lb / 16, dir,
-                                                 c->fp32, pass_kind(p), c->stream, H.fold, order);
+                                                 f32, pass_kind(p), c->stream, H.fold, order);
                     }
                     if (e == hipSuccess) e = hipEventRecord(eb, c->stream);
                     if (e == hipSuccess) e = hipEventSynchronize(eb);
@@ -531,7 +576,7 @@ int tune_lines(hqt_ctx* c) {
         // tile per work-group and keeps the launch order
         set_lb(H, H.fold >= 3 ? 16 : (choice % 10000) / 100, dir, (dir == 1 && !H.fold) ? 1 : choice % 100);
         H.xcd = H.fold ? choice / 10000 : 0;
-        if (line_pass_lds_bytes(H, c->fp32) > kLdsCap) set_lb(H, 16, dir, 1);
+        if (line_pass_lds_bytes(H, f32) > kLdsCap) set_lb(H, 16, dir, 1);
     }
     // occupancy target of the radial stage kernel: fixed, so that results never depend on a timing.
     // Measured on the MI355X (301^2 for nz = 12..20, 256^2 for nz = 21..32, both precisions): 2 waves
@@ -619,7 +664,8 @@ int build_context(hqt_ctx* c) {
     ALLOC(c->dyR, np * es); ALLOC(c->dyI, np * es); ALLOC(c->lapR, np * es); ALLOC(c->lapI, np * es);
     ALLOC(c->divA, np * es); ALLOC(c->curl3, np * es); ALLOC(c->curl4, np * es);
     ALLOC(c->dxAt, np * es); ALLOC(c->dyAt, np * es);
-    for (int i = 0; i < 5; ++i) { ALLOC(c->fa[i], np * es); ALLOC(c->fb[i], np * es); }
+    const size_t fes = dt_bytes(filt_dt(c));
+    for (int i = 0; i < 5; ++i) { ALLOC(c->fa[i], np * fes); ALLOC(c->fb[i], np * fes); }
     ALLOC(c->At, np * 8); ALLOC(c->rho, np * 8); ALLOC(c->AtR, np * 8);
     // per-column arrays of the radial kernels: one entry per column of the padded plane
     ALLOC(c->arho, ncol * 8); ALLOC(c->O, 2 * ncol * 8); ALLOC(c->part, (size_t)ny * 8); ALLOC(c->W, 8);
@@ -767,10 +813,12 @@ int build_context(hqt_ctx* c) {
     {
         void* f[5] = {c->ZR, c->ZI, c->At, c->ZAx, c->ZAy};
         int fdt[5] = {T, T, DT_F64, T, T};
+        // FT: the filter's intermediates (FP64 when the filter computes in FP64)
+        const int FT = filt_dt(c), fes = (int)dt_bytes(FT);
         LPass& P = c->hp[PXF];
-        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo, (int)c->es, c->ps, c->rp);
+        P = base_pass(nx, nx, ny, nz, 0, ops, c->flags, fo, fes, c->ps, c->rp);
         LPass& Q = c->hp[PYF];
-        Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo, (int)c->es, c->ps, c->rp);
+        Q = base_pass(ny, nx, ny, nz, 1, ops, c->flags, fo, fes, c->ps, c->rp);
         // Im(P) has rank 4 for the reference's masks: applied as U (V^T x) instead of a dense product.
         // Folded with both directions rank 4 (and ny >= 16): the cross term (Im P_x (x) Im P_y) x is
         // U_x G U_y^T per z-slice, so the x-pass writes partials of G (4 x 4 per slice) instead of the
@@ -778,25 +826,25 @@ int build_context(hqt_ctx* c) {
         const bool corr = filter_correction(c);
         for (int i = 0; i < 5; ++i) {
             LGroup& g = add_group(P, f[i], fdt[i]);
-            add_out(g, c->fa[i], T, 0, OP_PRX, -1, -1, corr ? f[i] : nullptr, corr ? fdt[i] : DT_F64);
+            add_out(g, c->fa[i], FT, 0, OP_PRX, -1, -1, corr ? f[i] : nullptr, corr ? fdt[i] : DT_F64);
             if (c->gfilt) {
                 double* gp = c->Gp + (size_t)i * 32 * c->gblk;
                 add_out(g, gp, DT_F64, 0, OP_UVX, -1, -1, nullptr, DT_F64, 0, 1);
                 g.out[1].gmode = 1;
                 g.out[1].gp = gp;
                 g.out[1].gtab = c->ops[OP_UVY];                                        // V_y: ny x 4
-                LGroup& h = add_group(Q, c->fa[i], T);
-                add_out(h, f[i], fdt[i], 0, OP_QRY, 0, OP_NUVY, corr ? c->fa[i] : nullptr, corr ? T : DT_F64, 1, 0, 1);
+                LGroup& h = add_group(Q, c->fa[i], FT);
+                add_out(h, f[i], fdt[i], 0, OP_QRY, 0, OP_NUVY, corr ? c->fa[i] : nullptr, corr ? FT : DT_F64, 1, 0, 1);
                 h.out[0].gmode = 2;
                 h.out[0].gp = gp;
-                h.out[0].gtab = static_cast<const char*>(c->ops[OP_UVX]) + 4 * (size_t)line_geom(nx).Np * c->es;  // U_x
+                h.out[0].gtab = static_cast<const char*>(c->ops[OP_UVX]) + 4 * (size_t)line_geom(nx).Np * fes;  // U_x
                 continue;
             }
-            if (c->lr[0]) add_out(g, c->fb[i], T, 0, OP_UVX, -1, -1, nullptr, DT_F64, 0, 1);
-            else add_out(g, c->fb[i], T, 0, OP_PIX);
-            LGroup& h = add_group(Q, c->fa[i], T, c->fb[i], T);
-            if (c->lr[1]) add_out(h, f[i], fdt[i], 0, OP_QRY, 1, OP_NUVY, corr ? c->fa[i] : nullptr, corr ? T : DT_F64, 1, 0, 1);
-            else add_out(h, f[i], fdt[i], 0, OP_QRY, 1, OP_NQIY, corr ? c->fa[i] : nullptr, corr ? T : DT_F64, 1);
+            if (c->lr[0]) add_out(g, c->fb[i], FT, 0, OP_UVX, -1, -1, nullptr, DT_F64, 0, 1);
+            else add_out(g, c->fb[i], FT, 0, OP_PIX);
+            LGroup& h = add_group(Q, c->fa[i], FT, c->fb[i], FT);
+            if (c->lr[1]) add_out(h, f[i], fdt[i], 0, OP_QRY, 1, OP_NUVY, corr ? c->fa[i] : nullptr, corr ? FT : DT_F64, 1, 0, 1);
+            else add_out(h, f[i], fdt[i], 0, OP_QRY, 1, OP_NQIY, corr ? c->fa[i] : nullptr, corr ? FT : DT_F64, 1);
         }
         // the filter of a direction whose stage passes are factored, too
         if (!(c->p.flags & HQT_FOLDED_FILTER)) {
@@ -823,7 +871,7 @@ int build_context(hqt_ctx* c) {
             if (H.fold && H.g[g].ntiles > 1 && H.g[g].nout > 1)
                 return fail(HQT_ERR_ARG, "internal: folded line pass group with two sources and two outputs");
             for (int t = 0; t < H.g[g].ntiles; ++t)
-                if (c->fp32 && H.g[g].src_dt[t] == DT_F64) {
+                if (pass_fp32(c, p) && H.g[g].src_dt[t] == DT_F64) {
                     if (H.fold == 1) H.fold = 2;          // folded, double prefetch registers
                     else if (H.fold == 3) H.fold = 4;     // factored-DFT, double prefetch registers
                 }
@@ -917,6 +965,7 @@ int hqt_ctx_create(const hqt_params* p, hqt_ctx** out) {
     c->tdt = c->fp32 ? DT_F32 : DT_F64;
     c->fold = (p->flags & HQT_DENSE_LINES) ? 0 : 1;
     c->es = c->fp32 ? 4 : 8;
+    c->filt64 = c->fp32 && !(p->flags & HQT_FP32_FILTER);
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return fail(HQT_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
     hqt_params tmp = *p;

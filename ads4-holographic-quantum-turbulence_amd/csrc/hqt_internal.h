@@ -49,6 +49,10 @@ struct hqt_ctx {
     int fp32 = 0;
     int tdt = DT_F64;
     size_t es = 8;
+    // mixed form: the filter's passes compute in FP64 (FP32 fields in and out, FP64 intermediates fa / fb and
+    // FP64 operator and trig tables): an FP32 filter leaves its masked modes at the rounding level of the whole
+    // field every step, and the dynamics grow them
+    int filt64 = 0;
     hipStream_t stream = nullptr;
     std::vector<void*> allocs;
     int64_t bytes = 0;               // device bytes allocated by this context
@@ -68,6 +72,7 @@ struct hqt_ctx {
     void* fops[OP_COUNT];            // folded tables of the same operators (NULL where unused)
     int fold = 1;                    // line passes in the even/odd folded form
     void* stab[2] = {nullptr, nullptr};   // trig tables of the factored-DFT passes (x, y), or NULL
+    void* stab64[2] = {nullptr, nullptr}; // the same in FP64 for the filter's passes (filt64), or NULL
     int gfilt = 0;                   // folded filter with the cross term through the 4 x 4 slice matrices
     int64_t gblk = 0;                // 16-line blocks of the x-lines (partials of the cross term)
     double* Gp = nullptr;            // [5 fields][gblk][2][16] cross-term partials
@@ -170,4 +175,6 @@ int64_t ctx_footprint(const hqt_params& p);
 void checkpoint_writer_release(hqt_ctx* c);
 
 // Work-groups of a line pass along grid.x (line tiles / tiles per work-group, rounded up).
+// Whether pass p of the context runs the FP32 kernels (hqt_host.cpp).
+int pass_fp32(const hqt_ctx* c, int p);
 inline int line_blocks(const LPass& P) { return (P.ntiles + P.tpb - 1) / P.tpb; }

@@ -1,7 +1,7 @@
 // Line-operator passes: dispatch over the (precision, direction) families whose kernels are
 // compiled in hqt_line_*.hip (the kernel itself is in hqt_line_core.h).
 #include "hqt_fold_core.h"
-#include "hqt_spec_core.h"
+#include "hqt_spec1_core.h"
 
 namespace hqt {
 
@@ -13,22 +13,36 @@ extern template hipError_t fold_launch_td<double, 0>(const LPass*, Members, dim3
 extern template hipError_t fold_launch_td<double, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int, int);
 extern template hipError_t fold_launch_td<float, 0>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int, int);
 extern template hipError_t fold_launch_td<float, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int, int);
-extern template hipError_t spec_launch_td<double, 0, false>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
-extern template hipError_t spec_launch_td<double, 1, false>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
-extern template hipError_t spec_launch_td<float, 0, false>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
-extern template hipError_t spec_launch_td<float, 1, false>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
 extern template hipError_t spec_launch_td<double, 0, true>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
 extern template hipError_t spec_launch_td<double, 1, true>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
 extern template hipError_t spec_launch_td<float, 0, true>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
 extern template hipError_t spec_launch_td<float, 1, true>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
+extern template hipError_t spec1_launch_td<double, 0>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
+extern template hipError_t spec1_launch_td<double, 1>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
+extern template hipError_t spec1_launch_td<float, 0>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
+extern template hipError_t spec1_launch_td<float, 1>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
 extern template hipError_t fold_pair_launch_cons<double>(const LPass*, const LPass*, int, int, int, size_t, hipStream_t, int, int);
 extern template hipError_t fold_pair_launch_cons<float>(const LPass*, const LPass*, int, int, int, size_t, hipStream_t, int, int);
 
+// A factored-DFT pass is the filter's (hqt_spec_core.h: the filter's real part, or the cross term) or an RK-stage /
+// constraint pass of derivative terms with one output per group (hqt_spec1_core.h).
+bool spec_filter_pass(const LPass& P) {
+    for (int g = 0; g < P.ngroups; ++g)
+        for (int o = 0; o < P.g[g].nout; ++o) {
+            if (P.g[g].out[o].gmode != 0) return true;
+            for (int e = 0; e < P.g[g].out[o].nterms; ++e)
+                if (P.g[g].out[o].term[e].sk == 4 || P.g[g].out[o].term[e].lr) return true;
+        }
+    return false;
+}
+
 size_t line_pass_lds_bytes(const LPass& P, int fp32) {
     if (P.fold == 3 || P.fold == 4) {
-        // factored-DFT form: the line region and the inverse region (lb x S each) and the trig table
         const int fid = P.Mp / 64 - 1;
         const int es = fp32 ? (int)sizeof(float) : (int)sizeof(double);
+        // the RK-stage and constraint passes (hqt_spec1_core.h): one line region (lb x S)
+        if (!spec_filter_pass(P)) return (size_t)es * (size_t)P.lb * P.S;
+        // the filter (hqt_spec_core.h): the line region and the inverse region (lb x S each) and the trig table
         bool cross = false;
         for (int g = 0; g < P.ngroups; ++g)
             for (int o = 0; o < P.g[g].nout; ++o) cross |= P.g[g].out[o].gmode != 0;
@@ -59,10 +73,11 @@ hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, 
             return dir ? spec_launch_td<double, 1, true>(d, M, grid, Q - 1, lds, s, order, wide)
                        : spec_launch_td<double, 0, true>(d, M, grid, Q - 1, lds, s, order, wide);
         }
-        if (fp32) return dir ? spec_launch_td<float, 1, false>(d, M, grid, Q - 1, lds, s, order, wide)
-                             : spec_launch_td<float, 0, false>(d, M, grid, Q - 1, lds, s, order, wide);
-        return dir ? spec_launch_td<double, 1, false>(d, M, grid, Q - 1, lds, s, order, wide)
-                   : spec_launch_td<double, 0, false>(d, M, grid, Q - 1, lds, s, order, wide);
+        // the RK-stage and constraint passes: single-region form
+        if (fp32) return dir ? spec1_launch_td<float, 1>(d, M, grid, Q - 1, lds, s, order, wide)
+                             : spec1_launch_td<float, 0>(d, M, grid, Q - 1, lds, s, order, wide);
+        return dir ? spec1_launch_td<double, 1>(d, M, grid, Q - 1, lds, s, order, wide)
+                   : spec1_launch_td<double, 0>(d, M, grid, Q - 1, lds, s, order, wide);
     }
     if (fold) {
         if (Q > 5) return hipErrorInvalidValue;

@@ -1,0 +1,6 @@
+// Factored-DFT line passes, single-region form: explicit instances of the float x-direction family.
+#include "hqt_spec1_core.h"
+
+namespace hqt {
+template hipError_t spec1_launch_td<float, 0>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
+}  // namespace hqt

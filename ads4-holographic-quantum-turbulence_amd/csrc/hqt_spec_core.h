@@ -90,6 +90,12 @@ __host__ __device__ inline int spec_cross_bytes(int es) { return 80 * 8 + 16 * 4
 
 #if defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC__)
 
+// Fused multiply-add in the pass precision (__builtin_fma alone would widen float operands to double).
+template <typename T> __device__ __forceinline__ T fmat(T a, T b, T c) {
+    if constexpr (sizeof(T) == 4) return __builtin_fmaf(a, b, c);
+    else return __builtin_fma(a, b, c);
+}
+
 // cos / sin (2 pi m / N2) for a compile-time m in [0, N2) from the H + 1 stored values.
 template <typename G, typename T> __device__ __forceinline__ T cos_m(const T (&ca)[G::H + 1], int m) {
     return m <= G::H ? ca[m] : ca[G::N2 - m];
@@ -105,7 +111,7 @@ template <typename G, typename T> __device__ __forceinline__ T sin_m(const T (&s
 template <typename G, int K, typename T, int... J>
 __device__ __forceinline__ T cos_dot(const T (&in)[G::H + 1], const T (&ca)[G::H + 1], std::integer_sequence<int, J...>) {
     T acc = in[0];
-    ((acc = __builtin_fma(in[J + 1], cos_m<G>(ca, ((J + 1) * K) % G::N2), acc)), ...);
+    ((acc = fmat(in[J + 1], cos_m<G>(ca, ((J + 1) * K) % G::N2), acc)), ...);
     return acc;
 }
 template <typename G, typename T, int... K>
@@ -120,7 +126,7 @@ __device__ __forceinline__ void cos_tf(const T (&in)[G::H + 1], const T (&ca)[G:
 template <typename G, int K, typename T, int... J>
 __device__ __forceinline__ T sin_dot(const T (&in)[G::HI + 1], const T (&sa)[G::H + 1], std::integer_sequence<int, J...>) {
     T acc = (T)0;
-    ((acc = __builtin_fma(in[J + 1], sin_m<G>(sa, ((J + 1) * K) % G::N2), acc)), ...);
+    ((acc = fmat(in[J + 1], sin_m<G>(sa, ((J + 1) * K) % G::N2), acc)), ...);
     return acc;
 }
 template <typename G, typename T, int... K>
@@ -150,14 +156,14 @@ __device__ __forceinline__ void dft_n1(const T (&ar)[G::N1], const T (&ai)[G::N1
         T pr = ar[0], pi = ai[0], qr = (T)0, qi = (T)0;
         if constexpr (G::EV1) {
             const T s = (k & 1) ? (T)-1 : (T)1;
-            pr = __builtin_fma(s, ar[G::H1], pr);
-            pi = __builtin_fma(s, ai[G::H1], pi);
+            pr = fmat(s, ar[G::H1], pr);
+            pi = fmat(s, ai[G::H1], pi);
         }
 #pragma unroll
         for (int j = 1; j <= HI1; ++j) {
             const T c = cb[(j * k) % N1], s = sb[(j * k) % N1];
-            pr = __builtin_fma(ur[j], c, pr); pi = __builtin_fma(ui[j], c, pi);
-            qr = __builtin_fma(vr[j], s, qr); qi = __builtin_fma(vi[j], s, qi);
+            pr = fmat(ur[j], c, pr); pi = fmat(ui[j], c, pi);
+            qr = fmat(vr[j], s, qr); qi = fmat(vi[j], s, qi);
         }
         // z[k] = P + sgn i Q, z[N1 - k] = P - sgn i Q
         zr[k] = pr - SGN * qi; zi[k] = pi + SGN * qr;
@@ -350,8 +356,8 @@ __device__ __forceinline__ void spec_body(const LPass* __restrict__ Pp, const Me
 #pragma unroll
                 for (int n1 = 0; n1 < N1; ++n1) {
                     const T xr = col[n1 * N2], xi = col[n1 * N2 + H], cs = tw[2 * n1], sn = tw[2 * n1 + 1];
-                    ar[n1] = __builtin_fma(xr, cs, xi * sn);          // (xr + i xi)(c - i s)
-                    ai[n1] = __builtin_fma(xi, cs, -(xr * sn));
+                    ar[n1] = fmat(xr, cs, xi * sn);          // (xr + i xi)(c - i s)
+                    ai[n1] = fmat(xi, cs, -(xr * sn));
                 }
                 dft_n1<G, -1>(ar, ai, cb, sb, zr, zi);
 #pragma unroll
@@ -434,8 +440,8 @@ __device__ __forceinline__ void spec_body(const LPass* __restrict__ Pp, const Me
 #pragma unroll
                 for (int n1 = 0; n1 < N1; ++n1) {
                     const T cs = tw[2 * n1], sn = tw[2 * n1 + 1];  // times exp(+2 pi i n1 k2 / N)
-                    T br = __builtin_fma(zr[n1], cs, -(zi[n1] * sn));
-                    T bi = __builtin_fma(zi[n1], cs, zr[n1] * sn);
+                    T br = fmat(zr[n1], cs, -(zi[n1] * sn));
+                    T bi = fmat(zi[n1], cs, zr[n1] * sn);
                     if (acc) { br += dst[n1 * N2]; bi += dst[n1 * N2 + H]; }
                     dst[n1 * N2] = br;
                     dst[n1 * N2 + H] = bi;
@@ -463,7 +469,7 @@ __device__ __forceinline__ void spec_body(const LPass* __restrict__ Pp, const Me
                 dft_n1<G, 1>(ar, ai, cb, sb, zr, zi);
 #pragma unroll
                 for (int n1 = 0; n1 < N1; ++n1) {
-                    T br = __builtin_fma(zr[n1], tw[2 * n1], -(zi[n1] * tw[2 * n1 + 1]));   // real part
+                    T br = fmat(zr[n1], tw[2 * n1], -(zi[n1] * tw[2 * n1 + 1]));   // real part
                     if (acc) br += dst[n1 * N2];
                     dst[n1 * N2] = br;
                 }
@@ -633,49 +639,6 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
     const LPass* __restrict__ Pp, const Members M, const int xcd_order) {
     spec_body<T, DIR, N1, N2, R, FILT>(Pp, M, xcd_order, (int)blockIdx.x, (int)gridDim.x, (int)blockIdx.y,
                                        (int)gridDim.y);
-}
-
-// The constraint's x- and y-pass (one source and one derivative output each, independent of each other) in one
-// launch: the first nb0 work-groups run the x-pass, the rest the y-pass, each exactly as its own launch would.
-template <typename T, int N1, int N2>
-__global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_spec_pair(
-    const LPass* __restrict__ X, const LPass* __restrict__ Y, const int nb0, const int order0, const int order1) {
-    const Members one{nullptr, 1};
-    if ((int)blockIdx.x < nb0) spec_body<T, 0, N1, N2, T, false>(X, one, order0, (int)blockIdx.x, nb0, 0, 1);
-    else spec_body<T, 1, N1, N2, T, false>(Y, one, order1, (int)blockIdx.x - nb0, (int)gridDim.x - nb0, 0, 1);
-}
-
-template <typename T, int N1, int N2>
-hipError_t spec_pair_launch_one(const LPass* x, const LPass* y, int nb0, int nb1, size_t lds, hipStream_t s, int o0,
-                                int o1) {
-    const void* fn = reinterpret_cast<const void*>(&k_spec_pair<T, N1, N2>);
-    if (lds > 64 * 1024) {
-        static bool done = false;
-        if (!done) {
-            const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-            if (e != hipSuccess) return e;
-            done = true;
-        }
-    }
-    hipLaunchKernelGGL((k_spec_pair<T, N1, N2>), dim3((unsigned)(nb0 + nb1)), dim3(LP_THREADS), lds, s, x, y, nb0, o0, o1);
-    return hipGetLastError();
-}
-
-// The constraint pair of one precision in the factored-DFT form (both passes of factorization fid); explicitly
-// instantiated by hqt_spec_pair.hip.
-template <typename T>
-hipError_t spec_pair_launch(const LPass* x, const LPass* y, int nb0, int nb1, int fid, size_t lds, hipStream_t s,
-                            int o0, int o1) {
-    switch (fid) {
-        case 0: return spec_pair_launch_one<T, 7, 43>(x, y, nb0, nb1, lds, s, o0, o1);
-        case 1: return spec_pair_launch_one<T, 4, 8>(x, y, nb0, nb1, lds, s, o0, o1);
-        case 2: return spec_pair_launch_one<T, 8, 8>(x, y, nb0, nb1, lds, s, o0, o1);
-        case 3: return spec_pair_launch_one<T, 8, 12>(x, y, nb0, nb1, lds, s, o0, o1);
-        case 4: return spec_pair_launch_one<T, 8, 16>(x, y, nb0, nb1, lds, s, o0, o1);
-        case 5: return spec_pair_launch_one<T, 8, 24>(x, y, nb0, nb1, lds, s, o0, o1);
-        case 6: return spec_pair_launch_one<T, 8, 32>(x, y, nb0, nb1, lds, s, o0, o1);
-        default: return hipErrorInvalidValue;
-    }
 }
 
 template <typename T, int DIR, int N1, int N2, typename R, bool FILT>

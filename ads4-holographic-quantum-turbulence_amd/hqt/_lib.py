@@ -24,6 +24,7 @@ HQT_ORDER_XCD = 128
 HQT_FOLDED_STAGES = 256
 HQT_FACTORED_STAGES = 512
 HQT_FOLDED_FILTER = 1024
+HQT_FP32_FILTER = 2048
 HQT_SERVER_NCOUNTERS = 10
 
 _HERE = os.path.dirname(os.path.abspath(__file__))

@@ -18,7 +18,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (HQT_ERR_ARG, HQT_ERR_NAN, HQT_ERR_NOCONV, HQT_ERR_TIMEOUT, HQT_FP64, HQT_LINES16, HQT_LINES32, HQT_MIXED, HQT_NO_GRAPH, HQT_RADIAL_W2, HQT_RADIAL_W3, HQT_DENSE_FILTER, HQT_DENSE_LINES, HQT_FACTORED_STAGES, HQT_FOLDED_FILTER, HQT_FOLDED_STAGES, HQT_ORDER_XCD, HQT_SERVER_NCOUNTERS, HQTError, Init, Params, check, dptr, lib)
+from ._lib import (HQT_ERR_ARG, HQT_ERR_NAN, HQT_ERR_NOCONV, HQT_ERR_TIMEOUT, HQT_FP64, HQT_LINES16, HQT_LINES32, HQT_MIXED, HQT_NO_GRAPH, HQT_RADIAL_W2, HQT_RADIAL_W3, HQT_DENSE_FILTER, HQT_DENSE_LINES, HQT_FACTORED_STAGES, HQT_FOLDED_FILTER, HQT_FOLDED_STAGES, HQT_FP32_FILTER, HQT_ORDER_XCD, HQT_SERVER_NCOUNTERS, HQTError, Init, Params, check, dptr, lib)
 
 __all__ = ["fourdif", "chebdif", "ff", "NewtonRaphson_vy", "NewtonRaphson_vy_batch", "Stepper", "Ensemble", "Server", "footprint", "make_params", "checkpoint_name", "write_checkpoint", "spline", "HQTError", "mask_factors"]
 
@@ -169,7 +169,7 @@ def _c128F(a, shape):
 
 def make_params(nx, ny, nz, Lx, Ly, mu=6.0, dt=0.02, winding=True, filter=True, precision="fp64", ax_b=0.0,
                 ay_b=0.0, graph=True, lines=None, radial_waves=None, dense_filter=False, dense_lines=False,
-                xcd_order=False, factored=None, folded_filter=False):
+                xcd_order=False, factored=None, folded_filter=False, fp32_filter=False):
     """``hqt_params`` for a run, plus the arrays its boundary pointers borrow (keep them alive
     while the params are in use).  ``lines`` = 16 or 32 fixes the lines per work-group of the
     x/y passes; None lets the library measure and pick per pass.  ``radial_waves`` = 2 or 3 fixes the
@@ -179,9 +179,11 @@ def make_params(nx, ny, nz, Lx, Ly, mu=6.0, dt=0.02, winding=True, filter=True, 
     products (func_Lx's own form) instead of their even/odd folded form.  ``xcd_order`` (with ``lines``)
     runs the folded line passes in the XCD-aware block order.  ``factored`` chooses the form of the RK
     stages' x/y derivatives where the line length has a factored-DFT form (one DFT per source line, the
-    operators' eigenvalues, one inverse per output): None takes it for FP64 lines of 160 points or more and
-    the folded matrix products otherwise, True / False force one form.  The filter follows the stages' form
-    unless ``folded_filter`` keeps its folded matrix products."""
+    operators' eigenvalues, one inverse per output): None takes it in the mixed form at every supported line
+    length (FP32 transforms round far less than FP32 matrix products) and for FP64 lines of 128 points or
+    more, the folded matrix products otherwise; True / False force one form.  The filter follows the stages' form
+    unless ``folded_filter`` keeps its folded matrix products.  In the mixed form the filter computes in FP64
+    (FP32 state in and out) unless ``fp32_filter`` runs it in FP32."""
     p = Params()
     p.nx, p.ny, p.nz = int(nx), int(ny), int(nz)
     p.Lx, p.Ly, p.mu, p.dt = float(Lx), float(Ly), float(mu), float(dt)
@@ -194,7 +196,8 @@ def make_params(nx, ny, nz, Lx, Ly, mu=6.0, dt=0.02, winding=True, filter=True, 
                | {None: 0, 2: HQT_RADIAL_W2, 3: HQT_RADIAL_W3}[radial_waves]
                | (HQT_DENSE_FILTER if dense_filter else 0) | (HQT_DENSE_LINES if dense_lines else 0)
                | (HQT_ORDER_XCD if xcd_order else 0) | (HQT_FOLDED_STAGES if factored is False else 0)
-               | (HQT_FACTORED_STAGES if factored is True else 0) | (HQT_FOLDED_FILTER if folded_filter else 0))
+               | (HQT_FACTORED_STAGES if factored is True else 0) | (HQT_FOLDED_FILTER if folded_filter else 0)
+               | (HQT_FP32_FILTER if fp32_filter else 0))
     keep = []
     for name, val in (("ax_b", ax_b), ("ay_b", ay_b)):
         if np.ndim(val) == 0:
@@ -252,12 +255,14 @@ class Stepper:
 
     def __init__(self, nx, ny, nz, Lx, Ly, mu=6.0, dt=0.02, winding=True, filter=True,
                  precision="fp64", ax_b=0.0, ay_b=0.0, graph=True, lines=None, radial_waves=None,
-                 dense_filter=False, dense_lines=False, xcd_order=False, factored=None, folded_filter=False):
+                 dense_filter=False, dense_lines=False, xcd_order=False, factored=None, folded_filter=False,
+                 fp32_filter=False):
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.shape = (self.nx, self.ny, self.nz)
         self.precision = precision
         p, keep = make_params(nx, ny, nz, Lx, Ly, mu, dt, winding, filter, precision, ax_b, ay_b, graph, lines,
-                              radial_waves, dense_filter, dense_lines, xcd_order, factored, folded_filter)
+                              radial_waves, dense_filter, dense_lines, xcd_order, factored, folded_filter,
+                              fp32_filter)
         self._h = C.c_void_p()
         check(lib().hqt_ctx_create(C.byref(p), C.byref(self._h)))
 

@@ -53,15 +53,17 @@ extern "C" {
                                products even where the line length has a factored-DFT form
                                (N = 32, 64, 96, 128, 192, 256, 301: each line's DFT once, the operators'
                                eigenvalues, one inverse per output; results agree to rounding) */
-#define HQT_FACTORED_STAGES 512 /* the factored-DFT form wherever the line length has one (by default only
-                               FP64 passes of N >= 160 take it; the mixed form keeps the folded FP32
-                               matrix products) */
+#define HQT_FACTORED_STAGES 512 /* the factored-DFT form wherever the line length has one (by default the
+                               mixed form takes it at every such N, FP64 passes from N = 128) */
 #define HQT_FOLDED_FILTER 1024 /* keep the filter passes as folded matrix products where the stages take the
                                factored-DFT form (by default the filter follows them, where the mask is
                                one range whose unpaired modes fall on complex columns) */
+#define HQT_FP32_FILTER 2048 /* mixed form: the filter's passes in FP32 arithmetic with FP32 intermediates
+                               (by default they compute in FP64, reading and writing the FP32 state) */
 
 #define HQT_FP64   0        /* plain form: FP64 end to end */
-#define HQT_MIXED  1        /* x/y operators, filter, stage arithmetic FP32; z operators FP64 */
+#define HQT_MIXED  1        /* x/y operators and stage arithmetic FP32 (state stored FP32); z operators,
+                               A_t, the constraint solve, boundary rows and the filter's arithmetic FP64 */
 
 typedef struct hqt_ctx hqt_ctx;
 

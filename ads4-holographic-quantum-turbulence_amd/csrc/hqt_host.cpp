@@ -377,11 +377,16 @@ int make_spec_pass(hqt_ctx* c, LPass& P, int dir) {
                 else if (op == OP_ND1X || op == OP_ND1Y) tm.sk = 3;
                 else return fail(HQT_ERR_ARG, "internal: factored-DFT pass with a term that is not a derivative");
             }
-    // one output per group (the single-region kernel transforms a group's sources once per output): a group's
-    // second output becomes a group of its own on the same sources
+    // A group keeps two outputs only where they share one source and the pass can afford a second LDS line
+    // region: the mixed form's y-passes (two FP32 regions are 38.5 KB at N = 301, and the two outputs share one
+    // forward transform: line_y_stage 71 -> 67 us at 301^2 x 20, 19.0 -> 17.5 us at 128^2 x 12).  Elsewhere a
+    // group's second output becomes a group of its own on the same sources (the kernel transforms them again):
+    // the mixed x-passes lose with fewer, longer work-groups (49.5 -> 49.1 us at 301^2, 12.8 -> 14.6 us at
+    // 128^2), and two FP64 regions would leave 2 work-groups per CU instead of 3.
+    const bool share = c->fp32 != 0 && dir == 1;
     const int ng0 = P.ngroups;
     for (int g = 0; g < ng0; ++g)
-        while (P.g[g].nout > 1) {
+        while (P.g[g].nout > (share && P.g[g].ntiles == 1 ? 2 : 1)) {
             if (P.ngroups == LP_MAXG) return fail(HQT_ERR_ARG, "internal: factored-DFT pass with too many outputs");
             LGroup& h = P.g[P.ngroups++];
             h = P.g[g];
@@ -389,14 +394,18 @@ int make_spec_pass(hqt_ctx* c, LPass& P, int dir) {
             h.out[0] = P.g[g].out[P.g[g].nout - 1];
             --P.g[g].nout;
         }
-    for (int g = 0; g < P.ngroups; ++g) {
-        const LOut& O = P.g[g].out[0];
-        if (O.nterms != P.g[g].ntiles || O.gmode)
-            return fail(HQT_ERR_ARG, "internal: factored-DFT output without one term per source");
-        for (int e = 0; e < O.nterms; ++e)
-            if (O.term[e].tile != e || O.term[e].lr)
+    for (int g = 0; g < P.ngroups; ++g)
+        for (int o = 0; o < P.g[g].nout; ++o) {
+            LOut& O = P.g[g].out[o];
+            if (O.nterms != P.g[g].ntiles || O.gmode)
                 return fail(HQT_ERR_ARG, "internal: factored-DFT output without one term per source");
-    }
+            for (int e = 0; e < O.nterms; ++e)
+                if (O.term[e].tile != e || O.term[e].lr)
+                    return fail(HQT_ERR_ARG, "internal: factored-DFT output without one term per source");
+        }
+    // of two outputs, the one with an addend first (its addend is loaded with the source)
+    for (int g = 0; g < P.ngroups; ++g)
+        if (P.g[g].nout == 2 && !P.g[g].out[0].addend && P.g[g].out[1].addend) std::swap(P.g[g].out[0], P.g[g].out[1]);
     P.fold = 3;
     P.Mp = 64 * (fid + 1);
     P.S = N | 1;

@@ -5,18 +5,21 @@
 // hqt_spec_core.h (N = N1 N2, stage A a real DFT of length N2 of each of the N1 sub-sequences,
 // stage B a complex DFT of length N1 per kept column); what differs is how a work-group uses the CU:
 //
-//   * one output per group.  A pass's groups are split on the host so that every group has exactly one
-//     output (a source with two outputs becomes two groups, each transforming the source again), so the
-//     spectrum never has to outlive its output;
+//   * one output per group, as a rule.  A pass's groups are split on the host so that a group has one output
+//     (a source with two outputs becomes two groups, each transforming the source again), so the spectrum
+//     never has to outlive its output.  Where a second LDS region is affordable (the mixed form's y-passes),
+//     a source keeps its two outputs and stage B writes the second one's columns there;
 //   * stage B fused with the spectral multiply and its own inverse: a column task transforms its column
 //     forward, multiplies it by the output term's eigenvalues and transforms it back in registers, in
-//     place in the one LDS line region.  No second region, no trig table in LDS: a work-group holds
-//     16 lines (38.5 KB in FP64, half that in FP32), so four work-groups (16 waves) share a CU;
+//     place in the LDS line region.  No trig table in LDS: a work-group holds 16 lines (38.5 KB in FP64,
+//     half that in FP32), so three FP64 (four FP32) work-groups share a CU;
 //   * stage A accumulates in streaming form (acc[k] += v_j * cos(2 pi j k / N2) over the folded inputs
 //     v_j), with the cos / sin constants of the compile-time sizes read by scalar loads into SGPRs, so the
-//     phase holds one accumulator per output and no constant in vector registers;
+//     phase holds one accumulator per output and no constant in vector registers; the inverse writes the
+//     line back in natural order (even half to both mirror positions, then the odd half subtracted and
+//     added), so the output phase reads it like the input phase wrote it;
 //   * an output with two sources (d_y(d_y A_x - d_x A_y)) keeps the first source's result in registers and
-//     adds it when the second source's result is stored.
+//     adds it when the second source's result is stored; the first output's addend is loaded with the source.
 //
 // The stage passes apply d/dx (i p s), d2/dx2 (-(p s)^2) and -d/dx (-i p s), s the signed wavenumber (0 at
 // the Nyquist mode of even N): exactly the eigenvalues of the fourdif matrix Lx = p Dx and of Lx Lx.
@@ -88,8 +91,9 @@ __device__ __forceinline__ void inv_sin_stream(const T* row, T rn2, T (&acc)[G::
     ((sin_axpy<G, K + 1, false>(acc, row[G::H + K + 1] * rn2, sa, std::make_integer_sequence<int, G::HI>{})), ...);
 }
 
-// The pass body for work-group (bx0, by0) of an nbx x nby grid.  Every group of the pass has one output,
-// whose terms are one per source (term e on source e), each a d/dx, d2/dx2 or -d/dx multiplier.
+// The pass body for work-group (bx0, by0) of an nbx x nby grid.  A group has one output whose terms are one per
+// source (term e on source e), or one source and two outputs of one term each (the second LDS region then holds
+// the second output); every term a d/dx, d2/dx2 or -d/dx multiplier.
 template <typename T, int DIR, int N1, int N2, typename R>
 __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const Members M, const int xcd_order,
                                            const int bx0, const int nbx, const int by0, const int nby) {
@@ -182,13 +186,13 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
         __syncthreads();
     };
 
-    auto inv_a = [&]() {
+    auto inv_a = [&](T* Rb) {
         const int ut = fresh((int)threadIdx.x);
         const int ta = 64 * (wave & 1) + (ut & 63);     // (the lane is unchanged by the rotation)
         const bool act = ta < LB * N1;
         const int la = act ? ta % LB : 0, n1a = act ? ta / LB : 0;
         const T rn = (T)(1.0 / N), rn2 = (T)(2.0 / N);
-        T* row = Rg + la * S + n1a * N2;
+        T* row = Rb + la * S + n1a * N2;
         T acc[H + 1];
         const cptr<T> tb = phase_table<T>(P.st);
         if (!odd) {
@@ -202,14 +206,24 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
             for (int m = 0; m <= H; ++m) sa[m] = tb[G::O_SA + m];
             if (wact) inv_sin_stream<G>(row, rn2, acc, sa, std::make_integer_sequence<int, HI>{});
         }
+        // the line back in natural order, y[n1 + N1 n2] = E[n2] - O[n2] (n2 <= HI) and E[m] + O[m] at
+        // n2 = N2 - m: the even half is written to both positions, then the odd half is subtracted and
+        // added (every address a compile-time offset from the task's first sample)
         __syncthreads();
-        if (act) {
-            if (!odd) {
+        T* y = Rb + la * S + n1a;
+        if (act && !odd) {
 #pragma unroll
-                for (int k = 0; k <= H; ++k) row[k] = acc[k];
-            } else {
+            for (int m = 0; m <= H; ++m) {
+                y[N1 * m] = acc[m];
+                if (m >= 1 && N2 - m != m) y[N1 * (N2 - m)] = acc[m];
+            }
+        }
+        __syncthreads();
+        if (act && odd) {
 #pragma unroll
-                for (int k = 1; k <= HI; ++k) row[H + k] = acc[k];
+            for (int m = 1; m <= HI; ++m) {
+                y[N1 * m] -= acc[m];
+                y[N1 * (N2 - m)] += acc[m];
             }
         }
         __syncthreads();
@@ -227,17 +241,21 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
             yr = -kk * xi; yi = kk * xr;
         }
     };
-    auto fused_b = [&](int kind) {
+    // kinds: the multiplier of the region's output; TWO: a second output from the same forward transform,
+    // its column written to the second region R2 (same offsets)
+    auto fused_b = [&](auto two, int kind0, int kind1) {
+        constexpr bool TWO = decltype(two)::value;
         const cptr<T> tb = phase_table<T>(P.st);
         T cb[N1], sb[N1];
 #pragma unroll
         for (int m = 0; m < N1; ++m) { cb[m] = tb[G::O_CB + m]; sb[m] = tb[G::O_SB + m]; }
         for (int t = (fresh((int)threadIdx.x) + rot) & (LP_THREADS - 1); t < NTB; t += LP_THREADS) {
             const int l = t % LB, c = t / LB;
-            T ar[N1], ai[N1], zr[N1], zi[N1];
+            T ar[N1], ai[N1], fr[N1], fi[N1], zr[N1], zi[N1];
             if (c < HI) {
                 const int k2 = 1 + c;
-                T* col = Rg + l * S + k2;
+                const int off = l * S + k2;
+                const T* col = Rg + off;
                 const T* tw = twg + 2 * k2 * N1;
 #pragma unroll
                 for (int n1 = 0; n1 < N1; ++n1) {
@@ -245,21 +263,27 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
                     ar[n1] = fmat(xr, cs, xi * sn);          // (xr + i xi)(c - i s)
                     ai[n1] = fmat(xi, cs, -(xr * sn));
                 }
-                dft_n1<G, -1>(ar, ai, cb, sb, zr, zi);
-                // the generic columns' wavenumbers: k = k2 + N2 k1 wraps to k - N exactly when 2 k1 >= N1
+                dft_n1<G, -1>(ar, ai, cb, sb, fr, fi);
+                // multiplier, inverse DFT, twiddle back, into region dst; the generic columns' wavenumbers:
+                // k = k2 + N2 k1 wraps to k - N exactly when 2 k1 >= N1
+                auto back = [&](int kind, T* dst) {
 #pragma unroll
-                for (int k1 = 0; k1 < N1; ++k1)
-                    mul_b(kind, p * (T)(k2 + N2 * k1 - (2 * k1 >= N1 ? N : 0)), zr[k1], zi[k1], ar[k1], ai[k1]);
-                dft_n1<G, 1>(ar, ai, cb, sb, zr, zi);
+                    for (int k1 = 0; k1 < N1; ++k1)
+                        mul_b(kind, p * (T)(k2 + N2 * k1 - (2 * k1 >= N1 ? N : 0)), fr[k1], fi[k1], ar[k1], ai[k1]);
+                    dft_n1<G, 1>(ar, ai, cb, sb, zr, zi);
 #pragma unroll
-                for (int n1 = 0; n1 < N1; ++n1) {
-                    const T cs = tw[2 * n1], sn = tw[2 * n1 + 1];  // times exp(+2 pi i n1 k2 / N)
-                    col[n1 * N2] = fmat(zr[n1], cs, -(zi[n1] * sn));
-                    col[n1 * N2 + H] = fmat(zi[n1], cs, zr[n1] * sn);
-                }
+                    for (int n1 = 0; n1 < N1; ++n1) {
+                        const T cs = tw[2 * n1], sn = tw[2 * n1 + 1];  // times exp(+2 pi i n1 k2 / N)
+                        dst[n1 * N2] = fmat(zr[n1], cs, -(zi[n1] * sn));
+                        dst[n1 * N2 + H] = fmat(zi[n1], cs, zr[n1] * sn);
+                    }
+                };
+                back(kind0, Rg + off);
+                if constexpr (TWO) back(kind1, Rg + LB * S + off);
             } else {
                 const int k2 = c > HI ? H : 0;
-                T* col = Rg + l * S + k2;
+                const int off = l * S + k2;
+                const T* col = Rg + off;
                 const T* tw = twg + 2 * k2 * N1;
 #pragma unroll
                 for (int n1 = 0; n1 < N1; ++n1) {
@@ -267,34 +291,67 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
                     ar[n1] = xr * tw[2 * n1];
                     ai[n1] = -(xr * tw[2 * n1 + 1]);
                 }
-                dft_n1<G, -1>(ar, ai, cb, sb, zr, zi);
+                dft_n1<G, -1>(ar, ai, cb, sb, fr, fi);
+                auto back = [&](int kind, T* dst) {
 #pragma unroll
-                for (int k1 = 0; k1 < N1; ++k1) spec_mul(kind, p, k2 + N2 * k1, N, zr[k1], zi[k1], ar[k1], ai[k1]);
-                dft_n1<G, 1>(ar, ai, cb, sb, zr, zi);
+                    for (int k1 = 0; k1 < N1; ++k1) spec_mul(kind, p, k2 + N2 * k1, N, fr[k1], fi[k1], ar[k1], ai[k1]);
+                    dft_n1<G, 1>(ar, ai, cb, sb, zr, zi);
 #pragma unroll
-                for (int n1 = 0; n1 < N1; ++n1) col[n1 * N2] = fmat(zr[n1], tw[2 * n1], -(zi[n1] * tw[2 * n1 + 1]));
+                    for (int n1 = 0; n1 < N1; ++n1) dst[n1 * N2] = fmat(zr[n1], tw[2 * n1], -(zi[n1] * tw[2 * n1 + 1]));
+                };
+                back(kind0, Rg + off);
+                if constexpr (TWO) back(kind1, Rg + LB * S + off);
             }
         }
         __syncthreads();
     };
 
-    // y[n1 + N1 n2] = E[n2] -+ O[n2] of sample jt + 16 r of line mrow, from the inverse stage A's halves
-    auto outval = [&](int mrow, int jt, int r) -> T {
-        const int n1j = jt % N1, n2j = jt / N1;
-        const int j = jt + 16 * r;
-        int n1 = n1j + (16 * r) % N1, n2 = n2j + (16 * r) / N1;
-        if (n1 >= N1) { n1 -= N1; ++n2; }
-        if (j >= N) { n1 = 0; n2 = 0; }            // samples past the line: no LDS read out of the row
-        const T* e = Rg + mrow * S + n1 * N2;
-        const int m = n2 <= H ? n2 : N2 - n2;
-        const T ov = (m >= 1 && m <= HI) ? e[H + m] : (T)0;
-        return n2 <= HI ? e[m] - ov : e[m] + ov;
+    // store of output Ou from region Rr: the transform (+ hold: the addend and, for a two-source output, the first
+    // source's result); the NaN flag from the stored values when the output checks them
+    struct Line { bool ok; uint32_t e0, b; int mrow, jt; };
+    auto store = [&](const LOut& Ou, const T* Rr, const Line& L, const T (&hold)[PF]) {
+        bool bad = false;
+        auto st = [&](auto tag) {
+            using S_ = decltype(tag);
+            const Rs rs = rs_make(Ou.out, fbytes<S_>(P));
+#pragma unroll
+            for (int r = 0; r < PF; ++r) {
+                const T v = Rr[L.mrow * S + min(L.jt + 16 * r, N - 1)] + hold[r];
+                const uint32_t ei = eidx(L.ok, L.jt, L.b, r);
+                bstr<S_>(rs, ei, (S_)v);
+                bad |= ei != Z_OOB && v != v;
+            }
+        };
+        if (Ou.out_dt == DT_F64) st(double{});
+        else st(float{});
+        if (Ou.nan_check && __any(bad) && (threadIdx.x & 63) == 0) {
+            const int step = __atomic_load_n(&P.flags[1], __ATOMIC_RELAXED);
+            atomicCAS(&P.flags[0], 0, step > 0 ? step : 1);
+        }
+    };
+    // an output's addend into hold (zeros without one)
+    auto load_addend = [&](const LOut& Ou, const Line& L, T (&hold)[PF]) {
+        auto ld = [&](auto tag) {
+            using S_ = decltype(tag);
+            const Rs rs = rs_make(Ou.addend, fbytes<S_>(P));
+#pragma unroll
+            for (int r = 0; r < PF; ++r) hold[r] = (T)bldr<S_>(rs, eidx(L.ok, L.jt, L.b, r));
+        };
+        if (!Ou.addend) {
+#pragma unroll
+            for (int r = 0; r < PF; ++r) hold[r] = (T)0;
+        } else if (Ou.add_dt == DT_F64) {
+            ld(double{});
+        } else {
+            ld(float{});
+        }
     };
 
-    // the tile loop, compiled once for one source and once for two (only the second holds the first source's
-    // result in registers)
-    auto tiles = [&](auto two) {
-    constexpr int NS = decltype(two)::value ? 2 : 1;
+    // the tile loop, compiled for the three group shapes: one source and one output, two sources summed into
+    // one output (the first source's result held in registers), one source and two outputs (one forward
+    // transform, the second output's columns in the second LDS region)
+    auto tiles = [&](auto ns_, auto no_) {
+    constexpr int NS = decltype(ns_)::value, NO = decltype(no_)::value;
     for (int it = 0; it < ntile; ++it) {
         const int lt = tile0 + it;
         int line0, z, x0;
@@ -308,7 +365,6 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
             z = lt / P.nxc;
         }
         // the thread's line: valid, its first element, and the offset of its sample jt
-        struct Line { bool ok; uint32_t e0, b; int mrow, jt; };
         auto line_at = [&](int ut) {
             Line L;
             L.mrow = row_of(ut);
@@ -318,8 +374,8 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
             L.b = L.e0 + (DIR == 0 ? (uint32_t)L.jt : (uint32_t)L.jt * (uint32_t)P.rp);
             return L;
         };
-        // hold: the output's addend (loaded with the first source, so its latency overlaps the transforms) plus,
-        // for a two-source output, the first source's result
+        // hold: the first output's addend (loaded with the first source, so its latency overlaps the transforms)
+        // plus, for a two-source output, the first source's result
         T hold[PF];
 #pragma unroll
         for (int t = 0; t < NS; ++t) {
@@ -336,22 +392,7 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
                 };
                 if (Gr.src_dt[t] == DT_F64) load(double{});
                 else load(float{});
-                if (t == 0) {
-                    auto load_add = [&](auto tag) {
-                        using S_ = decltype(tag);
-                        const Rs rs = rs_make(O.addend, fbytes<S_>(P));
-#pragma unroll
-                        for (int r = 0; r < PF; ++r) hold[r] = (T)bldr<S_>(rs, eidx(L.ok, L.jt, L.b, r));
-                    };
-                    if (!O.addend) {
-#pragma unroll
-                        for (int r = 0; r < PF; ++r) hold[r] = (T)0;
-                    } else if (O.add_dt == DT_F64) {
-                        load_add(double{});
-                    } else {
-                        load_add(float{});
-                    }
-                }
+                if (t == 0) load_addend(O, L, hold);
                 __syncthreads();                 // the previous unit is done with the line region
 #pragma unroll
                 for (int r = 0; r < PF; ++r) {
@@ -364,38 +405,32 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
             int kind = 1;
             for (int e = 0; e < O.nterms; ++e)
                 if (O.term[e].tile == t) kind = O.term[e].sk;
-            fused_b(kind);
-            inv_a();
+            fused_b(std::integral_constant<bool, NO == 2>{}, kind, NO == 2 ? Gr.out[1].term[0].sk : 1);
+            inv_a(Rg);
             const Line L = line_at(fresh((int)threadIdx.x));
             if (t + 1 < NS) {
 #pragma unroll
-                for (int r = 0; r < PF; ++r) hold[r] += outval(L.mrow, L.jt, r);
+                for (int r = 0; r < PF; ++r) hold[r] += Rg[L.mrow * S + min(L.jt + 16 * r, N - 1)];
                 continue;
             }
-            // store: the output (+ the addend, + the first source's result)
-            bool bad = false;
-            auto st = [&](auto tag) {
-                using S_ = decltype(tag);
-                const Rs rs = rs_make(O.out, fbytes<S_>(P));
-#pragma unroll
-                for (int r = 0; r < PF; ++r) {
-                    const T v = outval(L.mrow, L.jt, r) + hold[r];
-                    const uint32_t ei = eidx(L.ok, L.jt, L.b, r);
-                    bstr<S_>(rs, ei, (S_)v);
-                    bad |= ei != Z_OOB && v != v;
-                }
-            };
-            if (O.out_dt == DT_F64) st(double{});
-            else st(float{});
-            if (O.nan_check && __any(bad) && (threadIdx.x & 63) == 0) {
-                const int step = __atomic_load_n(&P.flags[1], __ATOMIC_RELAXED);
-                atomicCAS(&P.flags[0], 0, step > 0 ? step : 1);
+            store(O, Rg, L, hold);
+            if constexpr (NO == 2) {
+                inv_a(Rg + LB * S);
+                const Line L2 = line_at(fresh((int)threadIdx.x));
+                load_addend(Gr.out[1], L2, hold);
+                store(Gr.out[1], Rg + LB * S, L2, hold);
             }
         }
     }
     };
-    if (nsrc == 2) tiles(std::true_type{});
-    else tiles(std::false_type{});
+    // (the host gives FP64 passes one output per group: their kernels leave out the two-output shape)
+    if (nsrc == 2) tiles(std::integral_constant<int, 2>{}, std::integral_constant<int, 1>{});
+    else if constexpr (sizeof(T) == 4) {
+        if (Gr.nout == 2) tiles(std::integral_constant<int, 1>{}, std::integral_constant<int, 2>{});
+        else tiles(std::integral_constant<int, 1>{}, std::integral_constant<int, 1>{});
+    } else {
+        tiles(std::integral_constant<int, 1>{}, std::integral_constant<int, 1>{});
+    }
 }
 
 template <typename T, int DIR, int N1, int N2, typename R>

@@ -1,0 +1,160 @@
+"""Error-budget model of the mixed-precision RK4 step (CPU, NumPy).
+
+TEST INFRASTRUCTURE ONLY, like the rest of ``oracle/``: it is run by hand to attribute the mixed form's
+error to its sources, and pinned by ``tests/test_oracle_mixed_model.py``; nothing in the product imports it.
+
+It restates one RK4 step of ``scripts/simulation/tevol_driven_cluster.m:120-142`` (winding removal off, as
+``scripts/simulation/tevol_lattice.m:168-170``) with the x/y derivatives as FFT multipliers (the oracle's
+``form="fft"`` twin) and inserts FP32 rounding where a mixed-precision step would round:
+
+* ``deriv32``: every x/y derivative in FP32 (the source rounded, a single-precision FFT, the multiplier in
+  FP32, the result rounded), as FP32 transforms on FP32 fields do;
+* ``state32``: the stored state, the stage inputs and the RK accumulators rounded to FP32 after every update
+  (A_t and rho stay FP64, as in the mixed form);
+* ``filter32``: the filter's masked part (P - I) x computed by single-precision FFTs and added to x.
+
+With every knob off it is the FP64 step (the test checks it against :func:`oracle.hqt_oracle.step`).  Run as
+``python oracle/mixed_error_model.py <variant> <steps>`` it prints the relative max-norm |O| error against the
+FP64 model at a few checkpoints, on the 64 x 64 x 12 vortex-lattice workload.
+"""
+
+from __future__ import annotations
+
+import math
+import os
+import sys
+
+import numpy as np
+import scipy.fft as sfft
+
+from oracle import hqt_oracle as O
+
+
+def f32(a):
+    return a.astype(np.float32).astype(np.float64)
+
+
+class MixedModel:
+    def __init__(self, nx, ny, nz, L, mu, h, deriv32=False, state32=False, filter32=False):
+        self.ops = O.Operators(nx, ny, nz, L, L)
+        self.nz, self.mu, self.h = nz, mu, h
+        self.kx = 2 * math.pi / L * O.Operators._wavenumbers(nx)
+        self.ky = 2 * math.pi / L * O.Operators._wavenumbers(ny)
+        self.deriv32, self.filter32 = deriv32, filter32
+        self.st = f32 if state32 else (lambda a: a)
+
+    def d(self, X, axis, m):
+        """m-th x (axis 0) or y (axis 1) derivative of a real field: func_Lx / func_Ly applied m times."""
+        k = self.kx if axis == 0 else self.ky
+        shape = [1, 1, 1]
+        shape[axis] = -1
+        mult = ((1j * k) ** m).reshape(shape)
+        if self.deriv32:
+            out = sfft.ifft(mult.astype(np.complex64) * sfft.fft(X.astype(np.float32), axis=axis), axis=axis).real
+            return out.astype(np.float32).astype(np.float64)
+        return np.fft.ifft(mult * np.fft.fft(X, axis=axis), axis=axis).real
+
+    def lz(self, X):
+        return X @ self.ops.Lz.T
+
+    def lzr(self, X):
+        W = np.zeros_like(X)
+        W[:, :, :-1] = X[:, :, :-1] @ self.ops.Lz1.T
+        return W
+
+    def rhs(self, R, I, Ax, Ay, At, dAt):
+        """F1 (real, imaginary), F3, F4 and the z = 0 row of F2 (tevol_driven_cluster.m:72-76)."""
+        d = self.d
+        dxR, dxI, dyR, dyI = d(R, 0, 1), d(I, 0, 1), d(R, 1, 1), d(I, 1, 1)
+        lapR = d(R, 0, 2) + d(R, 1, 2)
+        lapI = d(I, 0, 2) + d(I, 1, 2)
+        dxAx, dyAy, dxAy = d(Ax, 0, 1), d(Ay, 1, 1), d(Ay, 0, 1)
+        curl3 = d(Ax, 1, 2) - d(dxAy, 1, 1)
+        curl4 = d(Ay, 0, 2) - d(dxAx, 1, 1)
+        div = dxAx + dyAy
+        dxAt, dyAt, dzAt = dAt
+        f = self.ops.f1[None, None, :]
+        Psi = R + 1j * I
+        dpsi = self.lz(Psi)
+        E1 = (1j * At * dpsi + 0.5j * Psi * dzAt + 0.5 * f * self.lz(dpsi) + self.ops.c_dpsi * dpsi
+              + self.ops.c_psi * Psi - 0.5 * (Ax * Ax + Ay * Ay) * Psi + 0.5 * (lapR + 1j * lapI)
+              - 0.5j * Psi * div - 1j * (Ax * (dxR + 1j * dxI) + Ay * (dyR + 1j * dyI)))
+        F1 = self.lzr(E1)
+        n2 = R * R + I * I
+        F3 = self.lzr(0.5 * self.lz(dxAt + f * self.lz(Ax)) + 0.5 * curl3 + (R * dxI - I * dxR) - Ax * n2)
+        F4 = self.lzr(0.5 * self.lz(dyAt + f * self.lz(Ay)) + 0.5 * curl4 + (R * dyI - I * dyR) - Ay * n2)
+        return F1.real, F1.imag, F3, F4, self.lz(div)[:, :, -1]
+
+    def ff(self, X):
+        """ff.m:7-11 with the reference's mask; filter32: x + (P - I) x with the masked part in FP32."""
+        mask = self.ops.mask2d[:, :, None]
+        if not self.filter32:
+            return np.real(np.fft.ifft2(np.where(mask, 0, np.fft.fft2(X, axes=(0, 1))), axes=(0, 1)))
+        b = sfft.fft2(X.astype(np.float32), axes=(0, 1))
+        corr = np.real(sfft.ifft2(np.where(mask, -b, 0), axes=(0, 1))).astype(np.float32)
+        return X + corr.astype(np.float64)
+
+    def step(self, s):
+        """One loop body; s = [Re Psi, Im Psi, A_t, A_x, A_y, rho]."""
+        R, I, At, Ax, Ay, rho = s
+        h, nz, st = self.h, self.nz, self.st
+        dAt = (self.d(At, 0, 1), self.d(At, 1, 1), self.lz(At))
+        Z = [R, I, Ax, Ay]
+        Y, K, k2 = Z, [0.0] * 4, 0.0
+        for stg, (c, w) in enumerate(((0.5, 1), (0.5, 2), (1.0, 2), (None, 1))):
+            F = self.rhs(Y[0], Y[1], Y[2], Y[3], At, dAt)
+            ks = [h * F[i] for i in range(4)]
+            for k in ks:
+                k[:, :, nz - 1] = 0
+            k2 = k2 + w * h * F[4]
+            if c is None:
+                Z = [st(Z[i] + (K[i] + ks[i]) / 6) for i in range(4)]
+            else:
+                K = [st(K[i] + w * ks[i]) for i in range(4)]
+                Y = [st(Z[i] + c * ks[i]) for i in range(4)]
+        R, I, Ax, Ay = Z
+        rho = rho - (k2 / 6)[:, :, None]
+        Psi = R + 1j * I
+        cons = self.lz(self.d(Ax, 0, 1) + self.d(Ay, 1, 1)) - 2 * np.imag(np.conj(Psi) * self.lz(Psi))
+        At = self.lzr(self.lzr(cons) - rho) + self.mu
+        R, I, Ax, Ay = R.copy(), I.copy(), Ax.copy(), Ay.copy()
+        R[:, :, -1] = 0
+        I[:, :, -1] = 0
+        Ax[:, :, -1] = 0
+        Ay[:, :, -1] = 0
+        At[:, :, -1] = self.mu
+        return [st(self.ff(R)), st(self.ff(I)), self.ff(At), st(self.ff(Ax)), st(self.ff(Ay)), rho]
+
+
+VARIANTS = {
+    "fp64": {},
+    "mixed": dict(deriv32=True, state32=True, filter32=True),
+    "deriv32": dict(deriv32=True),
+    "state32": dict(state32=True),
+    "mixed_filter64": dict(deriv32=True, state32=True),
+}
+
+
+def main():
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                    "ads4-holographic-quantum-turbulence_amd"))
+    from hqt import ic
+    variant = sys.argv[1]
+    nsteps = int(sys.argv[2]) if len(sys.argv) > 2 else 5000
+    nx = ny = 64
+    nz, L = 12, 50.0
+    z1, z2, z3, z4, rh = ic.initial_state(nx, ny, nz, L, L, mu=6.0, seed=3, lattice=(8, 0.5))
+    runs = {"fp64": MixedModel(nx, ny, nz, L, 6.0, 0.02), variant: MixedModel(nx, ny, nz, L, 6.0, 0.02, **VARIANTS[variant])}
+    state = {k: [m.st(z1.real), m.st(z1.imag), z2.copy(), m.st(z3), m.st(z4), rh.copy()] for k, m in runs.items()}
+    Lz = runs["fp64"].ops.Lz
+    for n in range(1, nsteps + 1):
+        for k, m in runs.items():
+            state[k] = m.step(state[k])
+        if n in (1, 10, 100, 500, 1000, 2000, 5000):
+            a = {k: np.abs((s[0] + 1j * s[1]) @ Lz[-1, :]) for k, s in state.items()}
+            print(f"{variant} step {n}: |O| rel err {np.max(np.abs(a[variant] - a['fp64'])) / np.max(a['fp64']):.2e}",
+                  flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -222,7 +222,7 @@ def test_mixed_rhs_error_bound():
 
 def test_mixed_vs_fp64_trajectory():
     """500 steps (t = 10) on a 64x64x12 lattice of 16 windings: the mixed form keeps the
-    vortex count and the condensate |O| within 1e-3 relative of the FP64 run."""
+    vortex count and the condensate |O| within 1e-5 relative of the FP64 run."""
     import hqt
     from hqt import diagnostics, ic
     nx = ny = 64
@@ -240,7 +240,7 @@ def test_mixed_vs_fp64_trajectory():
     e = float(np.max(np.abs(np.abs(Om) - np.abs(O64))) / np.max(np.abs(O64)))
     print(f"  |O| rel err {e:.2e}, vortices {diagnostics.countN(O64)} {diagnostics.countN(Om)}", flush=True)
     assert diagnostics.countN(Om) == diagnostics.countN(O64)
-    assert e < 1e-3
+    assert e < 1e-5
 
 
 def test_graph_replays_match_eager():

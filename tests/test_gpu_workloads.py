@@ -3,6 +3,12 @@
 * The reference's own driven run at full length: 301 x 301 x 20 on L = (nx-1)/2 = 150, the kk = 7
   profile with uniform A_y(z=0) = -vy, 5000 steps (``scripts/simulation/tevol_driven_cluster.m``).
   Checked by finiteness, charge conservation and the vortex census against the oracle's countN.
+* The mixed-precision workload at full length: 64 x 64 x 12 on L = 50, an 8 x 8 jittered lattice (jitter 0.5)
+  of +-1 windings, winding removal off (``scripts/simulation/tevol_lattice.m:101-135,168-170``), 5000 steps
+  (t = 100) of the mixed form beside the FP64 form: vortex count and plaquette positions equal at t = 10 and
+  t = 100; the condensate |O| within 1e-5 relative (max norm) at t = 10 and within 5e-5 at t = 100 (measured
+  2.0e-5, the limit of FP32 state storage and FP32 derivative transforms carried by the dynamics), its spatial
+  mean within 1e-6; FP32 arithmetic in the filter, by contrast, leaves 6e-3 at t = 100.
 * The fine-grid time-step limit: the oracle runs of ``tests/golden/make_instability.py`` (32^2 x 12
   at spacing 0.17 turns non-finite, at spacing 0.48 stays finite), repeated on the GPU.
 * The batched ensemble of 48 independent evolutions drawn from the ensemble workload's ranges
@@ -24,6 +30,38 @@ import pytest
 from conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
+
+
+def test_mixed_lattice_5000_steps():
+    import hqt
+    from hqt import diagnostics, ic
+    nx = ny = 64
+    nz, Lb = 12, 50.0
+    print("mixed vs fp64: 64x64x12 lattice, 5000 steps", flush=True)
+    S = ic.initial_state(nx, ny, nz, Lb, Lb, mu=6.0, seed=3, lattice=(8, 0.5))
+    g = ic.Grid(nx, ny, nz, Lb, Lb)
+    sts = {p: hqt.Stepper(nx, ny, nz, Lb, Lb, mu=6.0, dt=0.02, precision=p, winding=False) for p in ("fp64", "mixed")}
+    try:
+        for st in sts.values():
+            st.set_state(*S)
+        done = 0
+        for n, tol in ((500, 1e-5), (5000, 5e-5)):
+            O = {}
+            for p, st in sts.items():
+                st.step(n - done)
+                O[p] = diagnostics.order_parameter(g, st.get_state()[0])
+            done = n
+            a64, am = np.abs(O["fp64"]), np.abs(O["mixed"])
+            err = float(np.max(np.abs(am - a64)) / np.max(a64))
+            emean = float(abs(am.mean() - a64.mean()) / a64.mean())
+            print(f"  t = {n * 0.02:g}: |O| max-norm rel err {err:.2e}, mean {emean:.2e}, vortices "
+                  f"{diagnostics.countN(O['fp64'])} {diagnostics.countN(O['mixed'])}", flush=True)
+            assert diagnostics.countN(O["mixed"]) == diagnostics.countN(O["fp64"])
+            assert diagnostics.vortex_positions(O["mixed"]) == diagnostics.vortex_positions(O["fp64"])
+            assert err < tol and emean < 1e-6
+    finally:
+        for st in sts.values():
+            st.close()
 
 
 def test_reference_box_5000_steps():

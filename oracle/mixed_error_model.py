@@ -13,9 +13,9 @@ It restates one RK4 step of ``scripts/simulation/tevol_driven_cluster.m:120-142`
   (A_t and rho stay FP64, as in the mixed form);
 * ``filter32``: the filter's masked part (P - I) x computed by single-precision FFTs and added to x.
 
-With every knob off it is the FP64 step (the test checks it against :func:`oracle.hqt_oracle.step`).  Run as
-``python oracle/mixed_error_model.py <variant> <steps>`` it prints the relative max-norm |O| error against the
-FP64 model at a few checkpoints, on the 64 x 64 x 12 vortex-lattice workload.
+With every knob off it is the FP64 step (the test checks it against :func:`oracle.hqt_oracle.step`).  Run from
+the repository root as ``python -m oracle.mixed_error_model <variant> <steps>`` it prints the relative max-norm
+|O| error against the FP64 model at a few checkpoints, on the 64 x 64 x 12 vortex-lattice workload.
 """
 
 from __future__ import annotations

@@ -11,7 +11,11 @@ It restates one RK4 step of ``scripts/simulation/tevol_driven_cluster.m:120-142`
   FP32, the result rounded), as FP32 transforms on FP32 fields do;
 * ``state32``: the stored state, the stage inputs and the RK accumulators rounded to FP32 after every update
   (A_t and rho stay FP64, as in the mixed form);
-* ``filter32``: the filter's masked part (P - I) x computed by single-precision FFTs and added to x.
+* ``filter32``: the filter's masked part (P - I) x computed by single-precision FFTs and added to x;
+* ``start64`` (with ``state32``): the step-start state kept in FP64, only the stage inputs and the accumulators
+  rounded;
+* ``deriv_io32``: the x/y derivatives' sources and results rounded to FP32, their arithmetic FP64 (what an exact
+  transform of FP32 fields would leave).
 
 With every knob off it is the FP64 step (the test checks it against :func:`oracle.hqt_oracle.step`).  Run from
 the repository root as ``python -m oracle.mixed_error_model <variant> <steps>`` it prints the relative max-norm
@@ -35,13 +39,16 @@ def f32(a):
 
 
 class MixedModel:
-    def __init__(self, nx, ny, nz, L, mu, h, deriv32=False, state32=False, filter32=False):
+    def __init__(self, nx, ny, nz, L, mu, h, deriv32=False, state32=False, filter32=False, start64=False,
+                 deriv_io32=False):
         self.ops = O.Operators(nx, ny, nz, L, L)
         self.nz, self.mu, self.h = nz, mu, h
         self.kx = 2 * math.pi / L * O.Operators._wavenumbers(nx)
         self.ky = 2 * math.pi / L * O.Operators._wavenumbers(ny)
-        self.deriv32, self.filter32 = deriv32, filter32
+        self.deriv32, self.filter32, self.deriv_io32 = deriv32, filter32, deriv_io32
         self.st = f32 if state32 else (lambda a: a)
+        # start64: the step-start state stays FP64 (only the stage inputs and accumulators are rounded)
+        self.sz = (lambda a: a) if start64 else self.st
 
     def d(self, X, axis, m):
         """m-th x (axis 0) or y (axis 1) derivative of a real field: func_Lx / func_Ly applied m times."""
@@ -52,6 +59,8 @@ class MixedModel:
         if self.deriv32:
             out = sfft.ifft(mult.astype(np.complex64) * sfft.fft(X.astype(np.float32), axis=axis), axis=axis).real
             return out.astype(np.float32).astype(np.float64)
+        if self.deriv_io32:     # FP32 source and result, FP64 arithmetic
+            return f32(np.fft.ifft(mult * np.fft.fft(f32(X), axis=axis), axis=axis).real)
         return np.fft.ifft(mult * np.fft.fft(X, axis=axis), axis=axis).real
 
     def lz(self, X):
@@ -108,7 +117,7 @@ class MixedModel:
                 k[:, :, nz - 1] = 0
             k2 = k2 + w * h * F[4]
             if c is None:
-                Z = [st(Z[i] + (K[i] + ks[i]) / 6) for i in range(4)]
+                Z = [self.sz(Z[i] + (K[i] + ks[i]) / 6) for i in range(4)]
             else:
                 K = [st(K[i] + w * ks[i]) for i in range(4)]
                 Y = [st(Z[i] + c * ks[i]) for i in range(4)]
@@ -123,7 +132,8 @@ class MixedModel:
         Ax[:, :, -1] = 0
         Ay[:, :, -1] = 0
         At[:, :, -1] = self.mu
-        return [st(self.ff(R)), st(self.ff(I)), self.ff(At), st(self.ff(Ax)), st(self.ff(Ay)), rho]
+        sz = self.sz
+        return [sz(self.ff(R)), sz(self.ff(I)), self.ff(At), sz(self.ff(Ax)), sz(self.ff(Ay)), rho]
 
 
 VARIANTS = {
@@ -132,6 +142,8 @@ VARIANTS = {
     "deriv32": dict(deriv32=True),
     "state32": dict(state32=True),
     "mixed_filter64": dict(deriv32=True, state32=True),
+    "mixed_filter64_start64": dict(deriv32=True, state32=True, start64=True),
+    "deriv_io32": dict(deriv_io32=True),
 }
 
 
@@ -145,7 +157,8 @@ def main():
     nz, L = 12, 50.0
     z1, z2, z3, z4, rh = ic.initial_state(nx, ny, nz, L, L, mu=6.0, seed=3, lattice=(8, 0.5))
     runs = {"fp64": MixedModel(nx, ny, nz, L, 6.0, 0.02), variant: MixedModel(nx, ny, nz, L, 6.0, 0.02, **VARIANTS[variant])}
-    state = {k: [m.st(z1.real), m.st(z1.imag), z2.copy(), m.st(z3), m.st(z4), rh.copy()] for k, m in runs.items()}
+    c = np.ascontiguousarray
+    state = {k: [m.sz(c(z1.real)), m.sz(c(z1.imag)), c(z2), m.sz(c(z3)), m.sz(c(z4)), c(rh)] for k, m in runs.items()}
     Lz = runs["fp64"].ops.Lz
     for n in range(1, nsteps + 1):
         for k, m in runs.items():

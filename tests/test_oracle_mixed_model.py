@@ -33,7 +33,7 @@ def test_fp64_model_is_the_oracle_step():
         assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
 
 
-@pytest.mark.parametrize("variant", ["mixed", "deriv32", "state32", "mixed_filter64"])
+@pytest.mark.parametrize("variant", ["mixed", "deriv32", "state32", "mixed_filter64", "mixed_filter64_start64", "deriv_io32"])
 def test_knobs_round(variant):
     z1, z2, z3, z4, rh = _state()
     s0 = [z1.real.copy(), z1.imag.copy(), z2.copy(), z3.copy(), z4.copy(), rh.copy()]

@@ -38,7 +38,7 @@ EXPORTS = (
     "hqt_server_submit_evolution", "hqt_server_submit_newton", "hqt_server_submit_newton_continued", "hqt_server_wait", "hqt_server_evolution_result",
     "hqt_server_newton_result", "hqt_server_stats", "hqt_server_counters", "hqt_ctx_footprint", "hqt_ctx_device_bytes",
     "hqt_checkpoint_name", "hqt_checkpoint_write_host", "hqt_checkpoint_save", "hqt_checkpoint_save_async",
-    "hqt_checkpoint_wait", "hqt_init_state", "hqt_spline", "hqt_debug_field", "hqt_profile", "hqt_kernel_stats", "hqt_calibration_copy",
+    "hqt_checkpoint_wait", "hqt_init_state", "hqt_spline", "hqt_debug_field", "hqt_profile", "hqt_kernel_stats", "hqt_calibration_copy", "hqt_calibration_radial",
     "hqt_last_error", "hqt_version",
 )
 
@@ -141,6 +141,7 @@ def lib():
     L.hqt_debug_field.argtypes = [vp, C.c_char_p, dp]
     L.hqt_profile.argtypes = [vp, i32]
     L.hqt_calibration_copy.argtypes = [C.c_int64, dp]
+    L.hqt_calibration_radial.argtypes = [C.c_int64, C.c_int32, C.c_int32, C.c_int32, dp, dp]
     L.hqt_kernel_stats.argtypes = [vp, i32, C.c_char_p, i32, C.POINTER(C.c_int64), dp]
     for name in EXPORTS:
         if name not in ("hqt_last_error", "hqt_version"):

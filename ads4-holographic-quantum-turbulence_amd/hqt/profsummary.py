@@ -17,6 +17,7 @@ from collections import defaultdict
 
 
 def _short(name: str) -> str:
+    name = name.replace("(anonymous namespace)::", "")
     return name.replace("hqt::", "").replace("void ", "").split("(")[0]
 
 

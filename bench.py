@@ -326,6 +326,26 @@ def mixed_accuracy_leg(checkpoints=(500, 5000), seed=3):
     return out
 
 
+def radial_ab(reps=200):
+    """The radial contraction (u = Lz x, v = Lz u per column) in the stage kernel's FP64 MFMA form and as vector
+    FMAs fed through LDS, on the padded boundary planes of 301^2 (nz = 20) and 64^2 (nz = 12): average launch time,
+    algorithmic FP64 FLOPs (2 nz^2 per column per matvec) and HBM bytes (x in, u and v out) per second."""
+    import ctypes
+    import hqt
+    out = []
+    for ncol, nz in ((304 * 301, 20), (64 * 64, 12)):
+        row = {"columns": ncol, "nz": nz}
+        for form, name in ((0, "mfma"), (1, "valu_lds")):
+            ms, ss = ctypes.c_double(0.0), ctypes.c_double(0.0)
+            hqt._lib.check(hqt.lib().hqt_calibration_radial(ncol, nz, form, reps, ctypes.byref(ms), ctypes.byref(ss)))
+            us = ms.value * 1e3
+            row[name] = {"us": round(us, 2), "tflops": round(2 * 2 * nz * nz * ncol / (us * 1e-6) / 1e12, 3),
+                         "gbps": round(3 * 8 * nz * ncol / (us * 1e-6) / 1e9, 1), "sumsq": ss.value}
+        row["mfma_speedup"] = round(row["valu_lds"]["us"] / row["mfma"]["us"], 3)
+        out.append(row)
+    return out
+
+
 def field_errors(a, b):
     names = ("Psi", "A_t", "A_x", "A_y", "rho")
     out = {}
@@ -358,6 +378,9 @@ def main():
                     help="RK-stage x/y derivatives as folded matrix products instead of the factored-DFT form")
     ap.add_argument("--dense-lines", action="store_true",
                     help="x/y operators as dense Toeplitz products (the reference's form) instead of the folded form")
+    ap.add_argument("--radial-ab", action="store_true",
+                    help="only the radial contraction A/B (FP64 MFMA slot matvec vs vector FMAs through LDS), one "
+                         "JSON line")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="also run the plain copy kernel on MB megabytes (traffic-counter calibration)")
     args = ap.parse_args()
@@ -374,6 +397,10 @@ def main():
     if args.calibrate_copy:
         ms = ctypes.c_double(0.0)
         hqt._lib.check(hqt.lib().hqt_calibration_copy(int(args.calibrate_copy) << 20, ctypes.byref(ms)))
+
+    if args.radial_ab:
+        print(json.dumps({"radial_ab": radial_ab()}), flush=True)
+        return
 
     nx = ny = args.n
     nz = args.nz
@@ -395,7 +422,7 @@ def main():
     dom = max(stats.items(), key=lambda kv: kv[1][1])[0] if stats else None
     roof = None
     fact = {301: (7, 43), 32: (4, 8), 64: (8, 8), 96: (8, 12), 128: (8, 16), 192: (8, 24), 256: (8, 32)}
-    factored_form = not args.dense_lines and not args.folded and nx == ny and nx in fact and nx >= 160
+    factored_form = not args.dense_lines and not args.folded and nx == ny and nx in fact and nx >= 128
     if dom is not None and line_pass_fields(dom) is not None:
         cnt, ms = stats[dom]
         avg_s = ms / cnt / 1e3

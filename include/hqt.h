@@ -324,6 +324,13 @@ int hqt_kernel_stats(hqt_ctx* ctx, int32_t idx, char* name, int32_t name_len, in
  * FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic figures of the other kernels. */
 int hqt_calibration_copy(int64_t bytes, double* ms);
 
+/* Apply the Chebyshev derivative twice (u = Lz x, v = Lz u) to ncol columns of nz points (nz = 12 or 20) with
+ * the radial matvec of the stage kernel in `form` 0 (FP64 MFMA on the register slot layout) or 1 (vector FMAs
+ * fed through LDS, the LDS/wavefront alternative), `reps` timed launches after one warm-up; returns the average
+ * duration per launch and the sum of squares of v (the two forms agree to rounding).  Measurement helper with
+ * no counterpart in the reference: it compares the two ways of doing the radial contraction. */
+int hqt_calibration_radial(int64_t ncol, int32_t nz, int32_t form, int32_t reps, double* ms, double* sumsq);
+
 const char* hqt_last_error(void);
 const char* hqt_version(void);
 

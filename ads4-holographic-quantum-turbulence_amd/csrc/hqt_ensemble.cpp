@@ -3,10 +3,12 @@
 // class (same template instance: precision and Q = Mp/64 for the line passes, nz for the
 // radial kernels) over all members of that class; a member owns a contiguous block range of
 // the launch and reads its own descriptor (buffers, sizes, flags) from a device array.  The
-// whole step is captured once into a hipGraph and replayed per step.  The members are split into
-// up to kLanes lanes of about equal point counts; each lane runs its own class launches on its own
-// stream (the lanes share no member, so they meet only at the fork and the join around the step),
-// which overlaps one lane's small or tail launches with another lane's work.
+// members are split into up to kLanes lanes of about equal point counts, members of one shape
+// together where the balance allows (their launches merge); each lane runs its own class launches
+// on its own stream.  The lanes share no member, so they meet only at the fork and the join
+// around a replay, and a lane's steps need not wait for the other lanes: the steps are captured
+// into two hipGraphs, one step and kGraphSteps steps, and step_n replays the long one while it
+// can, which overlaps one lane's small or tail launches with another lane's work across steps.
 //
 // The per-member arithmetic is that of hqt_step_n on the same context (same kernels, same
 // operands), so results are identical to stepping each member alone.
@@ -17,6 +19,8 @@
 #include <map>
 #include <new>
 #include <tuple>
+#include <utility>
+#include <vector>
 
 namespace {
 
@@ -42,6 +46,7 @@ struct PlanImage {
 };
 
 constexpr int kLanes = 4;            // the ensemble's own stream + 3 side streams
+constexpr int kGraphSteps = 8;       // steps per replay of the long graph
 
 }  // namespace
 
@@ -53,7 +58,9 @@ struct hqt_ensemble {
     int lanes = 1;                   // lanes in use
     std::vector<Launch> plan;
     unsigned char* dplan = nullptr;
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph = nullptr;  // one step
+    hipGraphExec_t graphk = nullptr; // kGraphSteps steps, captured on the first step_n that can use it
+    bool graphs = false;             // replay graphs (else launch directly)
 };
 
 namespace {
@@ -162,17 +169,40 @@ void build_plan(hqt_ensemble* e, PlanImage& img, const std::vector<hqt_ctx*>& M,
     for (size_t i = first; i < P.size(); ++i) P[i].lane = lane;
 }
 
-// Lanes: the members by decreasing point count, each to the lane with the fewest points so far.
+// Lanes: members of one shape (precision, nx, ny, nz) form a group, split into pieces of at most a
+// lane's share of the points; the pieces by decreasing point count, each to the lane with the
+// fewest points so far.
 void build_lanes(hqt_ensemble* e, PlanImage& img) {
-    std::vector<hqt_ctx*> order(e->mem);
-    std::stable_sort(order.begin(), order.end(), [](const hqt_ctx* a, const hqt_ctx* b) { return a->npts > b->npts; });
-    const int nl = std::min<int>(kLanes, (int)order.size());
+    std::map<std::tuple<int, int, int, int>, std::vector<hqt_ctx*>> shape;
+    int64_t total = 0;
+    for (hqt_ctx* c : e->mem) {
+        shape[std::make_tuple(c->fp32, c->nx, c->ny, c->nz)].push_back(c);
+        total += c->npts;
+    }
+    const int nl = std::min<int>(kLanes, (int)e->mem.size());
+    const int64_t share = (total + nl - 1) / nl;
+    std::vector<std::pair<int64_t, std::vector<hqt_ctx*>>> pieces;
+    for (auto& kv : shape) {
+        std::vector<hqt_ctx*> cur;
+        int64_t pts = 0;
+        for (hqt_ctx* c : kv.second) {
+            if (!cur.empty() && pts + c->npts > share) {
+                pieces.emplace_back(pts, cur);
+                cur.clear();
+                pts = 0;
+            }
+            cur.push_back(c);
+            pts += c->npts;
+        }
+        pieces.emplace_back(pts, cur);
+    }
+    std::stable_sort(pieces.begin(), pieces.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
     std::vector<std::vector<hqt_ctx*>> lane(nl);
     std::vector<int64_t> load(nl, 0);
-    for (hqt_ctx* c : order) {
+    for (auto& pc : pieces) {
         const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        lane[l].push_back(c);
-        load[l] += c->npts;
+        lane[l].insert(lane[l].end(), pc.second.begin(), pc.second.end());
+        load[l] += pc.first;
     }
     e->lanes = nl;
     for (int l = 0; l < nl; ++l) build_plan(e, img, lane[l], l);
@@ -194,14 +224,16 @@ hipError_t issue(const hqt_ensemble* e, const Launch& L, hipStream_t s) {
     return hipErrorInvalidValue;
 }
 
-int enqueue(const hqt_ensemble* e) {
+// nsteps steps of every lane between one fork and one join.
+int enqueue(const hqt_ensemble* e, int nsteps) {
     auto lane = [&](int l) { return l == 0 ? e->stream : e->side[l - 1]; };
     hipError_t r = hipSuccess;
     if (e->lanes > 1) {
         r = hipEventRecord(e->fork, e->stream);
         for (int l = 1; l < e->lanes && r == hipSuccess; ++l) r = hipStreamWaitEvent(e->side[l - 1], e->fork, 0);
     }
-    for (size_t k = 0; k < e->plan.size() && r == hipSuccess; ++k) r = issue(e, e->plan[k], lane(e->plan[k].lane));
+    for (int i = 0; i < nsteps; ++i)
+        for (size_t k = 0; k < e->plan.size() && r == hipSuccess; ++k) r = issue(e, e->plan[k], lane(e->plan[k].lane));
     for (int l = 1; l < e->lanes && r == hipSuccess; ++l) {
         r = hipEventRecord(e->join[l - 1], e->side[l - 1]);
         if (r == hipSuccess) r = hipStreamWaitEvent(e->stream, e->join[l - 1], 0);
@@ -210,22 +242,23 @@ int enqueue(const hqt_ensemble* e) {
     return HQT_OK;
 }
 
-int build_graph(hqt_ensemble* e) {
+int build_graph(hqt_ensemble* e, int nsteps, hipGraphExec_t* out) {
     hipGraph_t g = nullptr;
     HIPC(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueue(e);
+    const int rc = enqueue(e, nsteps);
     const hipError_t r = hipStreamEndCapture(e->stream, &g);
     if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
     if (r != hipSuccess) return efail(HQT_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(r));
-    const hipError_t ri = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+    const hipError_t ri = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
-    if (ri != hipSuccess) { e->graph = nullptr; return efail(HQT_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ri)); }
+    if (ri != hipSuccess) { *out = nullptr; return efail(HQT_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ri)); }
     return HQT_OK;
 }
 
 void release(hqt_ensemble* e) {
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->graph) (void)hipGraphExecDestroy(e->graph);
+    if (e->graphk) (void)hipGraphExecDestroy(e->graphk);
     if (e->dplan) (void)hipFree(e->dplan);
     for (int l = 0; l < kLanes - 1; ++l) {
         if (e->side[l]) { (void)hipStreamSynchronize(e->side[l]); (void)hipStreamDestroy(e->side[l]); }
@@ -277,8 +310,9 @@ int hqt_ensemble_create(hqt_ctx* const* members, int32_t n, int32_t flags, hqt_e
         if (int rc = hqt_sync_streams(c)) { release(e); return rc; }
         c->owner = e->stream;
     }
-    if (!(flags & HQT_NO_GRAPH)) {
-        if (int rc = build_graph(e)) { release(e); return rc; }
+    e->graphs = !(flags & HQT_NO_GRAPH);
+    if (e->graphs) {
+        if (int rc = build_graph(e, 1, &e->graph)) { release(e); return rc; }
     }
     *out = e;
     return HQT_OK;
@@ -299,13 +333,12 @@ int hqt_ensemble_step_n(hqt_ensemble* e, int32_t nsteps) {
     if (!e) return efail(HQT_ERR_ARG, "NULL ensemble");
     if (nsteps < 0) return efail(HQT_ERR_ARG, "nsteps < 0");
     for (hqt_ctx* c : e->mem) HIPC(hipStreamSynchronize(c->stream));   // members' own queued work first
-    for (int i = 0; i < nsteps; ++i) {
-        if (e->graph) {
-            HIPC(hipGraphLaunch(e->graph, e->stream));
-        } else if (int rc = enqueue(e)) {
-            return rc;
-        }
+    if (!e->graphs) return nsteps > 0 ? enqueue(e, nsteps) : HQT_OK;
+    if (nsteps >= kGraphSteps && !e->graphk) {
+        if (int rc = build_graph(e, kGraphSteps, &e->graphk)) return rc;
     }
+    for (; nsteps >= kGraphSteps; nsteps -= kGraphSteps) HIPC(hipGraphLaunch(e->graphk, e->stream));
+    for (; nsteps > 0; --nsteps) HIPC(hipGraphLaunch(e->graph, e->stream));
     return HQT_OK;
 }
 

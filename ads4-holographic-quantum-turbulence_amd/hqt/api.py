@@ -436,8 +436,8 @@ class Stepper:
 
 class Ensemble:
     """Independent evolutions stepped together (``hqt_ensemble``): the members split into four
-    lanes, each lane one launch per kernel class of the step on its own stream, the whole step
-    replayed from one captured hipGraph.
+    lanes (members of one shape together), each lane one launch per kernel class of the step on
+    its own stream, the steps replayed from captured hipGraphs of eight steps and of one step.
 
     ``steppers`` are :class:`Stepper` objects with their state set; they may differ in grid,
     nz, mu, drive, switches and precision.  Each member's result equals stepping it alone.

@@ -200,8 +200,9 @@ def ensemble_leg(members, steps, seed=4):
     """Batched ensemble of independent evolutions of uneven size: nx = ny in {64, 96, 128, 192,
     256}, nz in {12, 16, 20, 24}, mu ~ U[4.1, 8] (below mu ~ 4.07 the zero-flow state is the
     normal phase), kk in 3..9 drive, 8-64 windings, each built on the device.  Timed: `steps`
-    steps of the whole ensemble (one graph per step) vs the same members stepped one by one
-    (each its own graph and stream)."""
+    steps of the whole ensemble (replays of its 8-step graph) vs the same members stepped one by
+    one (each its own graph and stream, the streams running concurrently).  The warm-up builds
+    both of the ensemble's graphs."""
     import hqt
     from hqt import ic
     rng = np.random.default_rng(seed)
@@ -232,7 +233,7 @@ def ensemble_leg(members, steps, seed=4):
             st.sync()
         tseq = (time.perf_counter() - t0) / steps
         ens = hqt.Ensemble(sts)
-        ens.step(2)
+        ens.step(8)
         t0 = time.perf_counter()
         ens.step(steps)
         tens = (time.perf_counter() - t0) / steps
@@ -261,7 +262,7 @@ def ensemble_leg(members, steps, seed=4):
         st.sync()
     tseq = (time.perf_counter() - t0) / steps
     with hqt.Ensemble(sts) as ens:
-        ens.step(2)
+        ens.step(8)
         t0 = time.perf_counter()
         ens.step(steps)
         tens = (time.perf_counter() - t0) / steps
@@ -370,7 +371,7 @@ def main():
                     help="skip the mixed-precision accuracy leg (64^2 vortex lattice to t = 100)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving (Poisson arrivals) leg")
     ap.add_argument("--ensemble-members", type=int, default=48)
-    ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--serving-rates", default="1,2,4,8",
                     help="Poisson arrival rates of the serving sweep, requests/s (comma-separated)")
     ap.add_argument("--serving-seconds", type=float, default=8.0, help="arrival span of each rate of the sweep")

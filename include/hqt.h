@@ -171,8 +171,11 @@ int hqt_checkpoint_wait(hqt_ctx* ctx);
 /* Many independent evolutions stepped together: the contexts may differ in nx, ny, nz, mu,
  * boundary drive, winding/filter switches and precision.  Every kernel of the step runs once
  * per kernel class (precision and padded line length for the x/y passes, nz for the radial
- * kernels) over all members of that class, and the whole step is one captured hipGraph
- * (unless flags has HQT_NO_GRAPH).  Each member's result equals hqt_step_n on that context
+ * kernels) over all members of that class, on four lanes (streams) of about equal point
+ * counts that meet only at the start and end of a replay; the steps are captured hipGraphs of
+ * one step and of eight steps (unless flags has HQT_NO_GRAPH; the eight-step graph is captured
+ * by the first hqt_ensemble_step_n of eight or more steps).  Each member's result equals
+ * hqt_step_n on that context
  * alone.  A member that hits NaN stops (its later kernels exit at once) while the others go
  * on.  Members keep their state and accessors; while in an ensemble, their calls first wait
  * for the ensemble's queued steps.  Serves the batched evolutions of the reference's

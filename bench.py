@@ -201,8 +201,9 @@ def ensemble_leg(members, steps, seed=4):
     256}, nz in {12, 16, 20, 24}, mu ~ U[4.1, 8] (below mu ~ 4.07 the zero-flow state is the
     normal phase), kk in 3..9 drive, 8-64 windings, each built on the device.  Timed: `steps`
     steps of the whole ensemble (replays of its 8-step graph) vs the same members stepped one by
-    one (each its own graph and stream, the streams running concurrently).  The warm-up builds
-    both of the ensemble's graphs."""
+    one (each its own graph and stream, the streams running concurrently) and one after another
+    (each member's steps to completion before the next member starts, as separate runs go).
+    The warm-up builds both of the ensemble's graphs."""
     import hqt
     from hqt import ic
     rng = np.random.default_rng(seed)
@@ -216,31 +217,46 @@ def ensemble_leg(members, steps, seed=4):
     pts = sum(n * n * nz for n, nz, *_ in specs)
     out = {"members": members, "steps": steps, "point_count": pts,
            "sizes": "nx=ny in {64,96,128,192,256}, nz in {12,16,20,24}, mu~U[4.1,8], kk 3..9, 8-64 windings"}
-    for prec in ("fp64", "mixed"):
-        sts = []
-        for n, nz, mu, kk, nv, s in specs:
-            st = hqt.Stepper(n, n, nz, L, L, mu=mu, dt=0.02, precision=prec, ax_b=ic.sinusoidal_drive(n, n, L, kk, 0.1))
+    profiles = [ic.zero_flow_profile(mu) for _, _, mu, *_ in specs]
+
+    def fresh(sts, warm):
+        # every timed run starts from the members' initial states: the finest grids (dx ~ 0.2 at L = 50) turn
+        # unstable after ~900 steps, as the reference's loop does at that resolution
+        for st, prof, (n, nz, mu, kk, nv, s) in zip(sts, profiles, specs):
             r = np.random.default_rng(s)
             vort = (r.uniform(0, L, nv), r.uniform(0, L, nv), np.where(np.arange(nv) % 2 == 0, 1.0, -1.0))
-            st.init_state(ic.zero_flow_profile(mu), phase_noise=1e-3, seed=s, vortices=vort)
-            sts.append(st)
+            st.init_state(prof, phase_noise=1e-3, seed=s, vortices=vort)
         for st in sts:
-            st.step(2)
+            if warm:
+                st.step(warm)
+
+    for prec in ("fp64", "mixed"):
+        sts = [hqt.Stepper(n, n, nz, L, L, mu=mu, dt=0.02, precision=prec, ax_b=ic.sinusoidal_drive(n, n, L, kk, 0.1))
+               for n, nz, mu, kk, nv, s in specs]
+        fresh(sts, 2)
         t0 = time.perf_counter()
         for st in sts:
             st.step_async(steps)
         for st in sts:
             st.sync()
         tseq = (time.perf_counter() - t0) / steps
+        fresh(sts, 2)
+        t0 = time.perf_counter()
+        for st in sts:          # one after another, as separate runs would go
+            st.step(steps)
+        tser = (time.perf_counter() - t0) / steps
+        fresh(sts, 0)
         ens = hqt.Ensemble(sts)
         ens.step(8)
         t0 = time.perf_counter()
         ens.step(steps)
         tens = (time.perf_counter() - t0) / steps
-        progress(f"ensemble {prec}: {1e3 * tens:.2f} ms/step, one by one {1e3 * tseq:.2f}")
+        progress(f"ensemble {prec}: {1e3 * tens:.2f} ms/step, one by one {1e3 * tseq:.2f}, sequential {1e3 * tser:.2f}")
         out[prec] = {"ms_per_step": round(1e3 * tens, 3), "member_steps_per_s": round(members / tens, 1),
                      "point_updates_per_s": round(pts / tens, 1), "launches_per_step": ens.launches_per_step,
-                     "members_one_by_one_ms_per_step": round(1e3 * tseq, 3), "speedup": round(tseq / tens, 3)}
+                     "members_one_by_one_ms_per_step": round(1e3 * tseq, 3), "speedup": round(tseq / tens, 3),
+                     "members_sequential_ms_per_step": round(1e3 * tser, 3),
+                     "speedup_vs_sequential": round(tser / tens, 3)}
         ens.close()
         for st in sts:
             st.close()

@@ -144,6 +144,7 @@ VARIANTS = {
     "mixed_filter64": dict(deriv32=True, state32=True),
     "mixed_filter64_start64": dict(deriv32=True, state32=True, start64=True),
     "deriv_io32": dict(deriv_io32=True),
+    "mixed_io32": dict(deriv_io32=True, state32=True),
 }
 
 

@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void k_copy_calib(const double4* __restrict__ 
 }
 hipError_t launch_copy_calib(const void* src, void* dst, int64_t bytes, hipStream_t s) {
     const int64_t n = bytes / (int64_t)sizeof(double4);
-    hipLaunchKernelGGL(k_copy_calib, dim3(4096), dim3(256), 0, s, (const double4*)src, (double4*)dst, n);
+    hipLaunchKernelGGL(k_copy_calib, dim3(16384), dim3(256), 0, s, (const double4*)src, (double4*)dst, n);
     return hipGetLastError();
 }
 }  // namespace hqt

@@ -451,6 +451,15 @@ def main():
                 "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": tr, "avg_launch_us": round(avg_s * 1e6, 2),
                 "algorithmic_bytes_per_launch": alg_bytes,
                 "bytes_form": "fields read (sources, addends) + fields written, 8 B per point each"}
+        # the streaming rate a plain copy kernel reaches on this GPU (1 GiB in, 1 GiB out; best of 3)
+        cms = []
+        for _ in range(3):
+            m = ctypes.c_double(0.0)
+            hqt._lib.check(hqt.lib().hqt_calibration_copy(1 << 30, ctypes.byref(m)))
+            cms.append(m.value)
+        copy_gbs = 2.0 * (1 << 30) / (min(cms) * 1e-3) / 1e9
+        roof["copy_GBps"] = round(copy_gbs, 1)
+        roof["frac_of_copy"] = round(ach / copy_gbs, 4)
         if factored_form and factored_transforms(dom) is not None:
             fl = factored_transforms(dom) * factored_flops_per_point(*fact[nx]) * S
             form = "factored DFT (%d x %d): each transform %.1f flops per point" % (fact[nx] + (factored_flops_per_point(*fact[nx]),))

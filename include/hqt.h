@@ -322,9 +322,10 @@ int hqt_profile(hqt_ctx* ctx, int32_t enable);
 int hqt_kernel_stats(hqt_ctx* ctx, int32_t idx, char* name, int32_t name_len, int64_t* count,
                      double* total_ms);
 
-/* Copy `bytes` between two fresh device buffers with a plain streaming kernel (k_copy_calib)
- * and return its duration.  Measurement helper with no counterpart in the reference: its
- * FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic figures of the other kernels. */
+/* Copy `bytes` between two fresh device buffers with a plain streaming kernel (k_copy_calib),
+ * twice, and return the duration of the second copy.  Measurement helper with no counterpart in
+ * the reference: its FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic figures of the
+ * other kernels, and its rate is the streaming bandwidth the HBM-bound kernels are compared with. */
 int hqt_calibration_copy(int64_t bytes, double* ms);
 
 /* Apply the Chebyshev derivative twice (u = Lz x, v = Lz u) to ncol columns of nz points (nz = 12 or 20) with

@@ -382,7 +382,8 @@ int make_spec_pass(hqt_ctx* c, LPass& P, int dir) {
     // forward transform: line_y_stage 71 -> 67 us at 301^2 x 20, 19.0 -> 17.5 us at 128^2 x 12).  Elsewhere a
     // group's second output becomes a group of its own on the same sources (the kernel transforms them again):
     // the mixed x-passes lose with fewer, longer work-groups (49.5 -> 49.1 us at 301^2, 12.8 -> 14.6 us at
-    // 128^2), and two FP64 regions would leave 2 work-groups per CU instead of 3.
+    // 128^2), and two FP64 regions leave 2 work-groups per CU instead of 3 and need 221 VGPRs (FP64 with
+    // shared sources: line_x_stage 69.4 -> 81.0 us, line_y_stage 94.1 -> 117.7 us).
     const bool share = c->fp32 != 0 && dir == 1;
     const int ng0 = P.ngroups;
     for (int g = 0; g < ng0; ++g)
@@ -403,7 +404,7 @@ int make_spec_pass(hqt_ctx* c, LPass& P, int dir) {
                 if (O.term[e].tile != e || O.term[e].lr)
                     return fail(HQT_ERR_ARG, "internal: factored-DFT output without one term per source");
         }
-    // of two outputs, the one with an addend first (its addend is loaded with the source)
+    // of two outputs, the one with an addend first
     for (int g = 0; g < P.ngroups; ++g)
         if (P.g[g].nout == 2 && !P.g[g].out[0].addend && P.g[g].out[1].addend) std::swap(P.g[g].out[0], P.g[g].out[1]);
     P.fold = 3;

@@ -18,8 +18,11 @@
 //     phase holds one accumulator per output and no constant in vector registers; the inverse writes the
 //     line back in natural order (even half to both mirror positions, then the odd half subtracted and
 //     added), so the output phase reads it like the input phase wrote it;
-//   * an output with two sources (d_y(d_y A_x - d_x A_y)) keeps the first source's result in registers and
-//     adds it when the second source's result is stored; the first output's addend is loaded with the source.
+//   * an output with two sources (d_y(d_y A_x - d_x A_y)) stores the first source's result to the output field
+//     and reads it back as the second source's addend; an output's addend is loaded just before the store.
+//     Nothing but the transform's own values is held in registers across the transforms: the FP64 passes fit
+//     168 VGPRs (three work-groups per CU) without spills (holding the addend from the source load on spilled
+//     60 VGPRs of the y-pass to scratch: line_x_stage 74.1 -> 68.3 us, line_y_stage 96.8 -> 94.0 us at 301^2 x 20).
 //
 // The stage passes apply d/dx (i p s), d2/dx2 (-(p s)^2) and -d/dx (-i p s), s the signed wavenumber (0 at
 // the Nyquist mode of even N): exactly the eigenvalues of the fourdif matrix Lx = p Dx and of Lx Lx.
@@ -329,18 +332,21 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
             atomicCAS(&P.flags[0], 0, step > 0 ? step : 1);
         }
     };
-    // an output's addend into hold (zeros without one)
-    auto load_addend = [&](const LOut& Ou, const Line& L, T (&hold)[PF]) {
+    // an output's addend into hold (zeros without one); partial: the output field itself, which holds the
+    // first source's result of a two-source output (stored by the same threads at the same elements)
+    auto load_addend = [&](const LOut& Ou, const Line& L, T (&hold)[PF], bool partial) {
         auto ld = [&](auto tag) {
             using S_ = decltype(tag);
-            const Rs rs = rs_make(Ou.addend, fbytes<S_>(P));
+            const Rs rs = rs_make(partial ? Ou.out : Ou.addend, fbytes<S_>(P));
 #pragma unroll
             for (int r = 0; r < PF; ++r) hold[r] = (T)bldr<S_>(rs, eidx(L.ok, L.jt, L.b, r));
         };
-        if (!Ou.addend) {
+        const void* a = partial ? Ou.out : Ou.addend;
+        const int dt = partial ? Ou.out_dt : Ou.add_dt;
+        if (!a) {
 #pragma unroll
             for (int r = 0; r < PF; ++r) hold[r] = (T)0;
-        } else if (Ou.add_dt == DT_F64) {
+        } else if (dt == DT_F64) {
             ld(double{});
         } else {
             ld(float{});
@@ -348,8 +354,10 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
     };
 
     // the tile loop, compiled for the three group shapes: one source and one output, two sources summed into
-    // one output (the first source's result held in registers), one source and two outputs (one forward
-    // transform, the second output's columns in the second LDS region)
+    // one output (the first source's result stored to the output and read back as the second one's addend),
+    // one source and two outputs (one forward transform, the second output's columns in the second LDS
+    // region).  An output's addend is loaded after stage B, so its latency overlaps the inverse transform and
+    // it occupies no registers during the forward transform and stage B.
     auto tiles = [&](auto ns_, auto no_) {
     constexpr int NS = decltype(ns_)::value, NO = decltype(no_)::value;
     for (int it = 0; it < ntile; ++it) {
@@ -374,8 +382,7 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
             L.b = L.e0 + (DIR == 0 ? (uint32_t)L.jt : (uint32_t)L.jt * (uint32_t)P.rp);
             return L;
         };
-        // hold: the first output's addend (loaded with the first source, so its latency overlaps the transforms)
-        // plus, for a two-source output, the first source's result
+        // hold: the output's addend (for a two-source output's second source, the first source's stored result)
         T hold[PF];
 #pragma unroll
         for (int t = 0; t < NS; ++t) {
@@ -392,7 +399,6 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
                 };
                 if (Gr.src_dt[t] == DT_F64) load(double{});
                 else load(float{});
-                if (t == 0) load_addend(O, L, hold);
                 __syncthreads();                 // the previous unit is done with the line region
 #pragma unroll
                 for (int r = 0; r < PF; ++r) {
@@ -408,16 +414,13 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
             fused_b(std::integral_constant<bool, NO == 2>{}, kind, NO == 2 ? Gr.out[1].term[0].sk : 1);
             inv_a(Rg);
             const Line L = line_at(fresh((int)threadIdx.x));
-            if (t + 1 < NS) {
-#pragma unroll
-                for (int r = 0; r < PF; ++r) hold[r] += Rg[L.mrow * S + min(L.jt + 16 * r, N - 1)];
-                continue;
-            }
+            // (the barriers since the first source's store order it before this read of the same elements)
+            load_addend(O, L, hold, t > 0);
             store(O, Rg, L, hold);
             if constexpr (NO == 2) {
-                inv_a(Rg + LB * S);
                 const Line L2 = line_at(fresh((int)threadIdx.x));
-                load_addend(Gr.out[1], L2, hold);
+                load_addend(Gr.out[1], L2, hold, false);
+                inv_a(Rg + LB * S);
                 store(Gr.out[1], Rg + LB * S, L2, hold);
             }
         }

@@ -136,7 +136,10 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
     const int ntile = min(P.tpb, P.ntiles - tile0);
     const int nsrc = Gr.ntiles;
     const T p = (T)P.sp;
-    const T* twg = static_cast<const T*>(P.st) + G::O_TW;   // twiddles (per-lane reads)
+    // twiddles: per-lane reads, through a global-address-space pointer (a generic one is read with flat loads,
+    // which also count against the LDS wait counter, and is re-read after every LDS store it might alias)
+    typedef const __attribute__((address_space(1))) T gT;
+    gT* __restrict__ const twg = (gT*)(static_cast<const T*>(P.st) + G::O_TW);
 
     // Every phase forms its per-thread indices from a fresh copy of the thread index (an empty asm statement
     // the compiler cannot see through), so they are computed where used instead of being hoisted out of the
@@ -259,10 +262,13 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
                 const int k2 = 1 + c;
                 const int off = l * S + k2;
                 const T* col = Rg + off;
-                const T* tw = twg + 2 * k2 * N1;
+                gT* tw = twg + 2 * k2 * N1;
+                T tc[N1], ts[N1];                         // exp(-2 pi i n1 k2 / N) = tc - i ts, read once per task
+#pragma unroll
+                for (int n1 = 0; n1 < N1; ++n1) { tc[n1] = tw[2 * n1]; ts[n1] = tw[2 * n1 + 1]; }
 #pragma unroll
                 for (int n1 = 0; n1 < N1; ++n1) {
-                    const T xr = col[n1 * N2], xi = col[n1 * N2 + H], cs = tw[2 * n1], sn = tw[2 * n1 + 1];
+                    const T xr = col[n1 * N2], xi = col[n1 * N2 + H], cs = tc[n1], sn = ts[n1];
                     ar[n1] = fmat(xr, cs, xi * sn);          // (xr + i xi)(c - i s)
                     ai[n1] = fmat(xi, cs, -(xr * sn));
                 }
@@ -276,7 +282,7 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
                     dft_n1<G, 1>(ar, ai, cb, sb, zr, zi);
 #pragma unroll
                     for (int n1 = 0; n1 < N1; ++n1) {
-                        const T cs = tw[2 * n1], sn = tw[2 * n1 + 1];  // times exp(+2 pi i n1 k2 / N)
+                        const T cs = tc[n1], sn = ts[n1];  // times exp(+2 pi i n1 k2 / N)
                         dst[n1 * N2] = fmat(zr[n1], cs, -(zi[n1] * sn));
                         dst[n1 * N2 + H] = fmat(zi[n1], cs, zr[n1] * sn);
                     }
@@ -287,12 +293,15 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
                 const int k2 = c > HI ? H : 0;
                 const int off = l * S + k2;
                 const T* col = Rg + off;
-                const T* tw = twg + 2 * k2 * N1;
+                gT* tw = twg + 2 * k2 * N1;
+                T tc[N1], ts[N1];
+#pragma unroll
+                for (int n1 = 0; n1 < N1; ++n1) { tc[n1] = tw[2 * n1]; ts[n1] = tw[2 * n1 + 1]; }
 #pragma unroll
                 for (int n1 = 0; n1 < N1; ++n1) {
                     const T xr = col[n1 * N2];
-                    ar[n1] = xr * tw[2 * n1];
-                    ai[n1] = -(xr * tw[2 * n1 + 1]);
+                    ar[n1] = xr * tc[n1];
+                    ai[n1] = -(xr * ts[n1]);
                 }
                 dft_n1<G, -1>(ar, ai, cb, sb, fr, fi);
                 auto back = [&](int kind, T* dst) {
@@ -300,7 +309,7 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
                     for (int k1 = 0; k1 < N1; ++k1) spec_mul(kind, p, k2 + N2 * k1, N, fr[k1], fi[k1], ar[k1], ai[k1]);
                     dft_n1<G, 1>(ar, ai, cb, sb, zr, zi);
 #pragma unroll
-                    for (int n1 = 0; n1 < N1; ++n1) dst[n1 * N2] = fmat(zr[n1], tw[2 * n1], -(zi[n1] * tw[2 * n1 + 1]));
+                    for (int n1 = 0; n1 < N1; ++n1) dst[n1 * N2] = fmat(zr[n1], tc[n1], -(zi[n1] * ts[n1]));
                 };
                 back(kind0, Rg + off);
                 if constexpr (TWO) back(kind1, Rg + LB * S + off);

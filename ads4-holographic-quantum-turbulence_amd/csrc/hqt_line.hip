@@ -13,18 +13,18 @@ extern template hipError_t fold_launch_td<double, 0>(const LPass*, Members, dim3
 extern template hipError_t fold_launch_td<double, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int, int);
 extern template hipError_t fold_launch_td<float, 0>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int, int);
 extern template hipError_t fold_launch_td<float, 1>(const LPass*, Members, dim3, int, int, int, size_t, hipStream_t, int, int);
-extern template hipError_t spec_launch_td<double, 0, true>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
-extern template hipError_t spec_launch_td<double, 1, true>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
-extern template hipError_t spec_launch_td<float, 0, true>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
-extern template hipError_t spec_launch_td<float, 1, true>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
-extern template hipError_t spec1_launch_td<double, 0>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
-extern template hipError_t spec1_launch_td<double, 1>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
-extern template hipError_t spec1_launch_td<float, 0>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
-extern template hipError_t spec1_launch_td<float, 1>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
+extern template hipError_t spec1_launch_td<double, 0, true>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
+extern template hipError_t spec1_launch_td<double, 1, true>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
+extern template hipError_t spec1_launch_td<float, 0, true>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
+extern template hipError_t spec1_launch_td<float, 1, true>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
+extern template hipError_t spec1_launch_td<double, 0, false>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
+extern template hipError_t spec1_launch_td<double, 1, false>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
+extern template hipError_t spec1_launch_td<float, 0, false>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
+extern template hipError_t spec1_launch_td<float, 1, false>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
 extern template hipError_t fold_pair_launch_cons<double>(const LPass*, const LPass*, int, int, int, size_t, hipStream_t, int, int);
 extern template hipError_t fold_pair_launch_cons<float>(const LPass*, const LPass*, int, int, int, size_t, hipStream_t, int, int);
 
-// A factored-DFT pass is the filter's (hqt_spec_core.h: the filter's real part, or the cross term) or an RK-stage /
+// A factored-DFT pass is the filter's (the filter's real part, or the cross term) or an RK-stage /
 // constraint pass of derivative terms with one output per group (hqt_spec1_core.h).
 bool spec_filter_pass(const LPass& P) {
     for (int g = 0; g < P.ngroups; ++g)
@@ -47,13 +47,9 @@ size_t line_pass_lds_bytes(const LPass& P, int fp32) {
             for (int g = 0; g < P.ngroups; ++g) regions = P.g[g].nout > regions ? P.g[g].nout : regions;
             return (size_t)es * (size_t)regions * (size_t)P.lb * P.S;
         }
-        // the filter (hqt_spec_core.h): the line region and the inverse region (lb x S each) and the trig table
-        bool cross = false;
-        for (int g = 0; g < P.ngroups; ++g)
-            for (int o = 0; o < P.g[g].nout; ++o) cross |= P.g[g].out[o].gmode != 0;
-        // + the filter's cross-term area
-        if (cross) return (size_t)spec_cross_offset(P.S, spec_table_len(fid), es) + (size_t)spec_cross_bytes(es);
-        return (size_t)es * (2 * (size_t)P.lb * P.S + (size_t)spec_table_len(fid));
+        // the filter: the line region and the cross-term area (G, its partials and the lines' coefficients)
+        (void)fid;
+        return (size_t)spec1_cross_offset(P.S, es) + (size_t)spec_cross_bytes(es);
     }
     int maxslots = 0;
     for (int g = 0; g < P.ngroups; ++g) maxslots = P.g[g].nslots > maxslots ? P.g[g].nslots : maxslots;
@@ -73,16 +69,16 @@ hipError_t launch_line_pass(const LPass* d, Members M, int blocks, int ngroups, 
         if (nt != 1 || Q > kSpecNF) return hipErrorInvalidValue;
         const int wide = fold == 4;
         if (kind == 2) {           // the filter
-            if (fp32) return dir ? spec_launch_td<float, 1, true>(d, M, grid, Q - 1, lds, s, order, wide)
-                                 : spec_launch_td<float, 0, true>(d, M, grid, Q - 1, lds, s, order, wide);
-            return dir ? spec_launch_td<double, 1, true>(d, M, grid, Q - 1, lds, s, order, wide)
-                       : spec_launch_td<double, 0, true>(d, M, grid, Q - 1, lds, s, order, wide);
+            if (fp32) return dir ? spec1_launch_td<float, 1, true>(d, M, grid, Q - 1, lds, s, order, wide)
+                                 : spec1_launch_td<float, 0, true>(d, M, grid, Q - 1, lds, s, order, wide);
+            return dir ? spec1_launch_td<double, 1, true>(d, M, grid, Q - 1, lds, s, order, wide)
+                       : spec1_launch_td<double, 0, true>(d, M, grid, Q - 1, lds, s, order, wide);
         }
-        // the RK-stage and constraint passes: single-region form
-        if (fp32) return dir ? spec1_launch_td<float, 1>(d, M, grid, Q - 1, lds, s, order, wide)
-                             : spec1_launch_td<float, 0>(d, M, grid, Q - 1, lds, s, order, wide);
-        return dir ? spec1_launch_td<double, 1>(d, M, grid, Q - 1, lds, s, order, wide)
-                   : spec1_launch_td<double, 0>(d, M, grid, Q - 1, lds, s, order, wide);
+        // the RK-stage and constraint passes
+        if (fp32) return dir ? spec1_launch_td<float, 1, false>(d, M, grid, Q - 1, lds, s, order, wide)
+                             : spec1_launch_td<float, 0, false>(d, M, grid, Q - 1, lds, s, order, wide);
+        return dir ? spec1_launch_td<double, 1, false>(d, M, grid, Q - 1, lds, s, order, wide)
+                   : spec1_launch_td<double, 0, false>(d, M, grid, Q - 1, lds, s, order, wide);
     }
     if (fold) {
         if (Q > 5) return hipErrorInvalidValue;

@@ -31,6 +31,10 @@
 
 namespace hqt {
 
+// Byte offset of the filter cross-term area (80 doubles, then 16 x 4 values of the pass precision, spec_cross_bytes)
+// in the dynamic LDS of a single-region pass: after the line region, 8-byte aligned.
+__host__ __device__ inline int spec1_cross_offset(int S, int es) { return (16 * S * es + 7) & ~7; }
+
 #if defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC__)
 
 // A uniform table pointer viewed in the constant address space: loads with compile-time offsets from it are scalar
@@ -97,7 +101,12 @@ __device__ __forceinline__ void inv_sin_stream(const T* row, T rn2, T (&acc)[G::
 // The pass body for work-group (bx0, by0) of an nbx x nby grid.  A group has one output whose terms are one per
 // source (term e on source e), or one source and two outputs of one term each (the second LDS region then holds
 // the second output); every term a d/dx, d2/dx2 or -d/dx multiplier.
-template <typename T, int DIR, int N1, int N2, typename R>
+// FILT: the Fourier filter's passes (ff.m:7-11).  A group has one source and one output whose term is the filter's
+// real part (kind 4: the even part of the keep-mask, a spectral multiplier; plus the line's first value times the
+// operator's row sum at the output); the x-pass group's second output (gmode 1) takes the spectrum's coefficients at
+// the unpaired masked modes into per-block partials of the 4 x 4 slice matrix G, the y-pass output (gmode 2) adds
+// U_x G of its line at those modes before the inverse transform (the rank-4 cross term of the mask's odd part).
+template <typename T, int DIR, int N1, int N2, typename R, bool FILT = false>
 __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const Members M, const int xcd_order,
                                            const int bx0, const int nbx, const int by0, const int nby) {
     typedef SpecGeom<N1, N2> G;
@@ -124,6 +133,12 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
     T* Rg = reinterpret_cast<T*>(smem_raw);           // LB lines (row stride S), transformed in place
     const LGroup& Gr = P.g[gy];
     const LOut& O = Gr.out[0];
+    // the filter's cross term: G and its per-wave partials (80 doubles) after the line region, then the lines'
+    // coefficients at the unpaired modes (LB x 4)
+    double* const GL = reinterpret_cast<double*>(smem_raw + spec1_cross_offset(S, (int)sizeof(T)));
+    T* const DL = reinterpret_cast<T*>(GL + 80);
+    const LOut* const Og = (FILT && DIR == 0 && Gr.nout == 2 && Gr.out[1].gmode == 1) ? &Gr.out[1] : nullptr;
+    const bool gadd = FILT && DIR == 1 && O.gmode == 2;
     // compute roles by a rotated thread index vt: a work-group of odd index hands waves 2-3 the roles of waves
     // 0-1 and the reverse, so the heavier roles (the even half of stage A, stage B's second round) fall on
     // every SIMD of a CU in turn instead of always on the SIMDs of waves 0-1
@@ -238,8 +253,18 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
     // ---- stage B fused with the multiplier and its inverse: column task (line l, k2), generic columns
     // k2 = 1 .. HI then the real columns k2 = 0 (and H for even N2), in place
     constexpr int NTB = LB * (HI + G::NSP);
-    auto mul_b = [&](int kind, T kap, T xr, T xi, T& yr, T& yi) {
-        if (kind == 2) {
+    // the filter's real part at DFT index k: 1 - (mask(k) + mask(-k)) / 2, or its masked part -(mask(k) +
+    // mask(-k)) / 2 in the correction form (the output then adds the source itself)
+    auto fmul = [&](int k) -> T {
+        const int kn = k == 0 ? 0 : N - k;
+        const T m = (T)0.5 * (T)((k >= P.mlo && k <= P.mhi) + (kn >= P.mlo && kn <= P.mhi));
+        return P.fcorr ? -m : (T)1 - m;
+    };
+    auto mul_b = [&](int kind, T kap, int k, T xr, T xi, T& yr, T& yi) {
+        if (FILT && kind == 4) {
+            const T m = fmul(k);
+            yr = m * xr; yi = m * xi;
+        } else if (kind == 2) {
             const T m = -kap * kap;
             yr = m * xr; yi = m * xi;
         } else {
@@ -273,12 +298,41 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
                     ai[n1] = fmat(xi, cs, -(xr * sn));
                 }
                 dft_n1<G, -1>(ar, ai, cb, sb, fr, fi);
+                if (FILT && Og) {
+                    // the coefficients (Re X[kq], -Im X[kq]) of the unpaired modes, from kq or its conjugate N - kq
+                    // (exactly one of the two lies on a complex column)
+                    for (int q = 0; q < P.nkp; ++q) {
+                        const int kq = P.kp[q];
+#pragma unroll
+                        for (int k1 = 0; k1 < N1; ++k1) {
+                            const int k = k2 + N2 * k1;
+                            if (k == kq) { DL[l * 4 + 2 * q] = fr[k1]; DL[l * 4 + 2 * q + 1] = -fi[k1]; }
+                            else if (k == N - kq) { DL[l * 4 + 2 * q] = fr[k1]; DL[l * 4 + 2 * q + 1] = fi[k1]; }
+                        }
+                    }
+                }
                 // multiplier, inverse DFT, twiddle back, into region dst; the generic columns' wavenumbers:
                 // k = k2 + N2 k1 wraps to k - N exactly when 2 k1 >= N1
                 auto back = [&](int kind, T* dst) {
 #pragma unroll
                     for (int k1 = 0; k1 < N1; ++k1)
-                        mul_b(kind, p * (T)(k2 + N2 * k1 - (2 * k1 >= N1 ? N : 0)), fr[k1], fi[k1], ar[k1], ai[k1]);
+                        mul_b(kind, p * (T)(k2 + N2 * k1 - (2 * k1 >= N1 ? N : 0)), k2 + N2 * k1, fr[k1], fi[k1], ar[k1],
+                              ai[k1]);
+                    if (FILT && gadd) {
+                        // the cross term (ke / 2) (c_1 + i c_0) at mode kq and its conjugate at N - kq, c = the
+                        // line's U_x G row
+                        for (int q = 0; q < P.nkp; ++q) {
+                            const int kq = P.kp[q];
+                            const T hc = (T)(0.5 * P.ke[q]);
+                            const T dr = hc * DL[l * 4 + 2 * q + 1], di = hc * DL[l * 4 + 2 * q];
+#pragma unroll
+                            for (int k1 = 0; k1 < N1; ++k1) {
+                                const int k = k2 + N2 * k1;
+                                if (k == kq) { ar[k1] += dr; ai[k1] += di; }
+                                else if (k == N - kq) { ar[k1] += dr; ai[k1] -= di; }
+                            }
+                        }
+                    }
                     dft_n1<G, 1>(ar, ai, cb, sb, zr, zi);
 #pragma unroll
                     for (int n1 = 0; n1 < N1; ++n1) {
@@ -306,7 +360,14 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
                 dft_n1<G, -1>(ar, ai, cb, sb, fr, fi);
                 auto back = [&](int kind, T* dst) {
 #pragma unroll
-                    for (int k1 = 0; k1 < N1; ++k1) spec_mul(kind, p, k2 + N2 * k1, N, fr[k1], fi[k1], ar[k1], ai[k1]);
+                    for (int k1 = 0; k1 < N1; ++k1) {
+                        if (FILT && kind == 4) {
+                            const T m = fmul(k2 + N2 * k1);
+                            ar[k1] = m * fr[k1]; ai[k1] = m * fi[k1];
+                        } else {
+                            spec_mul(kind, p, k2 + N2 * k1, N, fr[k1], fi[k1], ar[k1], ai[k1]);
+                        }
+                    }
                     dft_n1<G, 1>(ar, ai, cb, sb, zr, zi);
 #pragma unroll
                     for (int n1 = 0; n1 < N1; ++n1) dst[n1 * N2] = fmat(zr[n1], tc[n1], -(zi[n1] * ts[n1]));
@@ -321,14 +382,16 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
     // store of output Ou from region Rr: the transform (+ hold: the addend and, for a two-source output, the first
     // source's result); the NaN flag from the stored values when the output checks them
     struct Line { bool ok; uint32_t e0, b; int mrow, jt; };
-    auto store = [&](const LOut& Ou, const T* Rr, const Line& L, const T (&hold)[PF]) {
+    // cst: the filter's line constant (its first value times the row sum of the output's operator)
+    auto store = [&](const LOut& Ou, const T* Rr, const Line& L, const T (&hold)[PF], T cst) {
         bool bad = false;
         auto st = [&](auto tag) {
             using S_ = decltype(tag);
             const Rs rs = rs_make(Ou.out, fbytes<S_>(P));
 #pragma unroll
             for (int r = 0; r < PF; ++r) {
-                const T v = Rr[L.mrow * S + min(L.jt + 16 * r, N - 1)] + hold[r];
+                const T y = Rr[L.mrow * S + min(L.jt + 16 * r, N - 1)];
+                const T v = FILT ? (y + cst) + hold[r] : y + hold[r];
                 const uint32_t ei = eidx(L.ok, L.jt, L.b, r);
                 bstr<S_>(rs, ei, (S_)v);
                 bad |= ei != Z_OOB && v != v;
@@ -396,6 +459,7 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
 #pragma unroll
         for (int t = 0; t < NS; ++t) {
             // the source's lines, relative to their first value
+            T cst = (T)0;
             {
                 const Line L = line_at(fresh((int)threadIdx.x));
                 R pa[PF], pref;
@@ -408,6 +472,11 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
                 };
                 if (Gr.src_dt[t] == DT_F64) load(double{});
                 else load(float{});
+                if constexpr (FILT) {
+                    int rsum = 0;
+                    for (int e = 0; e < O.nterms; ++e) rsum += O.term[e].rsum;
+                    cst = rsum ? (T)rsum * (T)pref : (T)0;
+                }
                 __syncthreads();                 // the previous unit is done with the line region
 #pragma unroll
                 for (int r = 0; r < PF; ++r) {
@@ -416,28 +485,37 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
                 }
                 __syncthreads();
             }
+            // the y-pass filter's cross term: U_x G of the tile's lines into DL (ends with a barrier); the x-pass's
+            // coefficients of absent unpaired modes are zero (stage B writes those of the present ones)
+            if (FILT && gadd) fold_lr_from_g<T, 1>(P, O, DL, GL, x0, z);
+            if (FILT && Og && P.nkp < 2)
+                for (int idx = (int)threadIdx.x; idx < LB * 4; idx += LP_THREADS)
+                    if ((idx & 3) >> 1 >= P.nkp) DL[idx] = (T)0;
             fwd_a();
             int kind = 1;
-            for (int e = 0; e < O.nterms; ++e)
-                if (O.term[e].tile == t) kind = O.term[e].sk;
+            for (int e = 0; e < O.nterms; ++e)            // (a filter output's rank-4 term is its cross term)
+                if (O.term[e].tile == t && !O.term[e].lr) kind = O.term[e].sk;
             fused_b(std::integral_constant<bool, NO == 2>{}, kind, NO == 2 ? Gr.out[1].term[0].sk : 1);
+            // the x-pass filter's cross term: this 16-line block's partials of G from the coefficients in DL
+            if (FILT && Og) fold_g_partials<T, 1>(P, *Og, DL, line0, nl);
             inv_a(Rg);
             const Line L = line_at(fresh((int)threadIdx.x));
             // (the barriers since the first source's store order it before this read of the same elements)
             load_addend(O, L, hold, t > 0);
-            store(O, Rg, L, hold);
+            store(O, Rg, L, hold, cst);
             if constexpr (NO == 2) {
                 const Line L2 = line_at(fresh((int)threadIdx.x));
                 load_addend(Gr.out[1], L2, hold, false);
                 inv_a(Rg + LB * S);
-                store(Gr.out[1], Rg + LB * S, L2, hold);
+                store(Gr.out[1], Rg + LB * S, L2, hold, (T)0);
             }
         }
     }
     };
     // (the host gives FP64 passes one output per group: their kernels leave out the two-output shape)
+    // (a filter group's second output is the cross term's partials, not a second transform)
     if (nsrc == 2) tiles(std::integral_constant<int, 2>{}, std::integral_constant<int, 1>{});
-    else if constexpr (sizeof(T) == 4) {
+    else if constexpr (sizeof(T) == 4 && !FILT) {
         if (Gr.nout == 2) tiles(std::integral_constant<int, 1>{}, std::integral_constant<int, 2>{});
         else tiles(std::integral_constant<int, 1>{}, std::integral_constant<int, 1>{});
     } else {
@@ -445,20 +523,25 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
     }
 }
 
-template <typename T, int DIR, int N1, int N2, typename R>
+template <typename T, int DIR, int N1, int N2, typename R, bool FILT>
 __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4))) void k_spec1_pass(
     const LPass* __restrict__ Pp, const Members M, const int xcd_order) {
-    spec1_body<T, DIR, N1, N2, R>(Pp, M, xcd_order, (int)blockIdx.x, (int)gridDim.x, (int)blockIdx.y, (int)gridDim.y);
+    spec1_body<T, DIR, N1, N2, R, FILT>(Pp, M, xcd_order, (int)blockIdx.x, (int)gridDim.x, (int)blockIdx.y,
+                                        (int)gridDim.y);
 }
 
 // The constraint's x- and y-pass (one source and one derivative output each, independent of each other) in one
-// launch: the first nb0 work-groups run the x-pass, the rest the y-pass, each exactly as its own launch would.
+// launch: the first nb0 work-groups run the x-pass, the y-pass's nb1 start at nb0 rounded up to a multiple of 8
+// (the work-groups between return at once), so both passes keep the XCD assignment (work-group i on XCD i mod 8)
+// their own launches would have.
 template <typename T, int N1, int N2>
 __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4))) void k_spec_pair(
-    const LPass* __restrict__ X, const LPass* __restrict__ Y, const int nb0, const int order0, const int order1) {
+    const LPass* __restrict__ X, const LPass* __restrict__ Y, const int nb0, const int nb1, const int order0,
+    const int order1) {
     const Members one{nullptr, 1};
-    if ((int)blockIdx.x < nb0) spec1_body<T, 0, N1, N2, T>(X, one, order0, (int)blockIdx.x, nb0, 0, 1);
-    else spec1_body<T, 1, N1, N2, T>(Y, one, order1, (int)blockIdx.x - nb0, (int)gridDim.x - nb0, 0, 1);
+    const int b = (int)blockIdx.x, nb0p = (nb0 + 7) & ~7;
+    if (b < nb0) spec1_body<T, 0, N1, N2, T>(X, one, order0, b, nb0, 0, 1);
+    else if (b >= nb0p) spec1_body<T, 1, N1, N2, T>(Y, one, order1, b - nb0p, nb1, 0, 1);
 }
 
 template <typename T, int N1, int N2>
@@ -473,7 +556,8 @@ hipError_t spec_pair_launch_one(const LPass* x, const LPass* y, int nb0, int nb1
             done = true;
         }
     }
-    hipLaunchKernelGGL((k_spec_pair<T, N1, N2>), dim3((unsigned)(nb0 + nb1)), dim3(LP_THREADS), lds, s, x, y, nb0, o0, o1);
+    hipLaunchKernelGGL((k_spec_pair<T, N1, N2>), dim3((unsigned)(((nb0 + 7) & ~7) + nb1)), dim3(LP_THREADS), lds, s, x, y, nb0,
+                       nb1, o0, o1);
     return hipGetLastError();
 }
 
@@ -494,9 +578,9 @@ hipError_t spec_pair_launch(const LPass* x, const LPass* y, int nb0, int nb1, in
     }
 }
 
-template <typename T, int DIR, int N1, int N2, typename R>
+template <typename T, int DIR, int N1, int N2, typename R, bool FILT>
 hipError_t spec1_launch_one(const LPass* d, Members M, dim3 grid, size_t lds, hipStream_t s, int order) {
-    const void* fn = reinterpret_cast<const void*>(&k_spec1_pass<T, DIR, N1, N2, R>);
+    const void* fn = reinterpret_cast<const void*>(&k_spec1_pass<T, DIR, N1, N2, R, FILT>);
     if (lds > 64 * 1024) {
         static bool done = false;
         if (!done) {
@@ -505,35 +589,36 @@ hipError_t spec1_launch_one(const LPass* d, Members M, dim3 grid, size_t lds, hi
             done = true;
         }
     }
-    hipLaunchKernelGGL((k_spec1_pass<T, DIR, N1, N2, R>), grid, dim3(LP_THREADS), lds, s, d, M, order);
+    hipLaunchKernelGGL((k_spec1_pass<T, DIR, N1, N2, R, FILT>), grid, dim3(LP_THREADS), lds, s, d, M, order);
     return hipGetLastError();
 }
 
-template <typename T, int DIR, typename R>
+template <typename T, int DIR, typename R, bool FILT>
 hipError_t spec1_launch_r(const LPass* d, Members M, dim3 grid, int fid, size_t lds, hipStream_t s, int order) {
     switch (fid) {
-        case 0: return spec1_launch_one<T, DIR, 7, 43, R>(d, M, grid, lds, s, order);
-        case 1: return spec1_launch_one<T, DIR, 4, 8, R>(d, M, grid, lds, s, order);
-        case 2: return spec1_launch_one<T, DIR, 8, 8, R>(d, M, grid, lds, s, order);
-        case 3: return spec1_launch_one<T, DIR, 8, 12, R>(d, M, grid, lds, s, order);
-        case 4: return spec1_launch_one<T, DIR, 8, 16, R>(d, M, grid, lds, s, order);
-        case 5: return spec1_launch_one<T, DIR, 8, 24, R>(d, M, grid, lds, s, order);
-        case 6: return spec1_launch_one<T, DIR, 8, 32, R>(d, M, grid, lds, s, order);
+        case 0: return spec1_launch_one<T, DIR, 7, 43, R, FILT>(d, M, grid, lds, s, order);
+        case 1: return spec1_launch_one<T, DIR, 4, 8, R, FILT>(d, M, grid, lds, s, order);
+        case 2: return spec1_launch_one<T, DIR, 8, 8, R, FILT>(d, M, grid, lds, s, order);
+        case 3: return spec1_launch_one<T, DIR, 8, 12, R, FILT>(d, M, grid, lds, s, order);
+        case 4: return spec1_launch_one<T, DIR, 8, 16, R, FILT>(d, M, grid, lds, s, order);
+        case 5: return spec1_launch_one<T, DIR, 8, 24, R, FILT>(d, M, grid, lds, s, order);
+        case 6: return spec1_launch_one<T, DIR, 8, 32, R, FILT>(d, M, grid, lds, s, order);
         default: return hipErrorInvalidValue;
     }
 }
 
-// Every single-region factored-DFT instance of one (precision, direction) family; explicitly instantiated by
-// the hqt_spec_*.hip translation units.  wide: a double source in an FP32 pass (prefetch registers in double).
-template <typename T, int DIR>
+// Every single-region factored-DFT instance of one (precision, direction, RK-stage or filter) family; explicitly
+// instantiated by the hqt_spec*.hip translation units.  wide: a double source in an FP32 pass (prefetch registers
+// in double).
+template <typename T, int DIR, bool FILT>
 hipError_t spec1_launch_td(const LPass* d, Members M, dim3 grid, int fid, size_t lds, hipStream_t s, int order,
                            int wide) {
     if constexpr (sizeof(T) == 8) {
         (void)wide;
-        return spec1_launch_r<T, DIR, double>(d, M, grid, fid, lds, s, order);
+        return spec1_launch_r<T, DIR, double, FILT>(d, M, grid, fid, lds, s, order);
     } else {
-        if (wide) return spec1_launch_r<T, DIR, double>(d, M, grid, fid, lds, s, order);
-        return spec1_launch_r<T, DIR, float>(d, M, grid, fid, lds, s, order);
+        if (wide) return spec1_launch_r<T, DIR, double, FILT>(d, M, grid, fid, lds, s, order);
+        return spec1_launch_r<T, DIR, float, FILT>(d, M, grid, fid, lds, s, order);
     }
 }
 

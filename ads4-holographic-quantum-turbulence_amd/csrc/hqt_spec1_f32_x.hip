@@ -2,5 +2,5 @@
 #include "hqt_spec1_core.h"
 
 namespace hqt {
-template hipError_t spec1_launch_td<float, 0>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
+template hipError_t spec1_launch_td<float, 0, false>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
 }  // namespace hqt

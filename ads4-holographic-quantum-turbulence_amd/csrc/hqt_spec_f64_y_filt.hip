@@ -1,6 +1,6 @@
-// Factored-DFT line passes: explicit instances of the double y-direction family, filter passes.
-#include "hqt_spec_core.h"
+// Factored-DFT line passes, single-region form: explicit instances of the double y-direction family, filter passes.
+#include "hqt_spec1_core.h"
 
 namespace hqt {
-template hipError_t spec_launch_td<double, 1, true>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
+template hipError_t spec1_launch_td<double, 1, true>(const LPass*, Members, dim3, int, size_t, hipStream_t, int, int);
 }  // namespace hqt

@@ -295,8 +295,9 @@ def mixed_accuracy_leg(checkpoints=(500, 5000), seed=3):
     """The mixed-precision contract on the vortex-lattice workload: nx = ny = 64, nz = 12, L = 50,
     mu = 6, dt = 0.02, 64 +-1 windings on an 8 x 8 jittered square lattice (jitter 0.5), winding
     removal off (tevol_lattice.m:168-170).  The FP64 and mixed forms step side by side; at each
-    checkpoint (t = 10 and t = 100) the per-field relative max-norm error of the mixed state against
-    the FP64 one, the condensate |O| error (max norm over the boundary plane, and of its spatial mean),
+    checkpoint (t = 10 and t = 100) the per-field relative max-norm and L2 errors of the mixed state
+    against the FP64 one, the condensate |O| error (max norm and L2 norm over the boundary plane, and of
+    its spatial mean),
     and the vortex counts and plaquette positions.  Beside
     them, ``fp64_perturbed``: the same FP64 evolution from an initial Psi perturbed by 1e-7 relative
     (seeded, the size of an FP32 rounding), which measures how far the dynamics carry a perturbation
@@ -331,7 +332,10 @@ def mixed_accuracy_leg(checkpoints=(500, 5000), seed=3):
             c64, cm = diagnostics.countN(O64), diagnostics.countN(Om)
             out["checkpoints"]["t=%g" % (n * 0.02)] = {
                 "steps": n, "err_vs_fp64": field_errors(states["mixed"], states["fp64"]),
+                "err_vs_fp64_l2": field_errors_l2(states["mixed"], states["fp64"]),
                 "O_abs_rel_err": float(np.max(np.abs(np.abs(Om) - np.abs(O64))) / np.max(np.abs(O64))),
+                "O_abs_rel_err_l2": float(np.linalg.norm((np.abs(Om) - np.abs(O64)).ravel())
+                                          / np.linalg.norm(np.abs(O64).ravel())),
                 "condensate_mean_rel_err": float(abs(np.abs(Om).mean() - np.abs(O64).mean()) / np.abs(O64).mean()),
                 "vortices_fp64": c64, "vortices_mixed": cm,
                 "positions_equal": diagnostics.vortex_positions(Om) == diagnostics.vortex_positions(O64),
@@ -369,6 +373,13 @@ def field_errors(a, b):
     for nm, u, v in zip(names, a, b):
         out[nm] = float(np.max(np.abs(u - v)) / max(np.max(np.abs(v)), 1e-300))
     return out
+
+
+def field_errors_l2(a, b):
+    """Per-field relative L2 error ||a - b||_2 / ||b||_2 over the whole volume."""
+    names = ("Psi", "A_t", "A_x", "A_y", "rho")
+    return {nm: float(np.linalg.norm((u - v).ravel()) / max(np.linalg.norm(v.ravel()), 1e-300))
+            for nm, u, v in zip(names, a, b)}
 
 
 def main():

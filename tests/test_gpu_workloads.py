@@ -6,9 +6,10 @@
 * The mixed-precision workload at full length: 64 x 64 x 12 on L = 50, an 8 x 8 jittered lattice (jitter 0.5)
   of +-1 windings, winding removal off (``scripts/simulation/tevol_lattice.m:101-135,168-170``), 5000 steps
   (t = 100) of the mixed form beside the FP64 form: vortex count and plaquette positions equal at t = 10 and
-  t = 100; the condensate |O| within 1e-5 relative (max norm) at t = 10 and within 5e-5 at t = 100 (measured
-  2.0e-5, the limit of FP32 state storage and FP32 derivative transforms carried by the dynamics), its spatial
-  mean within 1e-6; FP32 arithmetic in the filter, by contrast, leaves 6e-3 at t = 100.
+  t = 100; the condensate |O| within 1e-5 relative in the L2 norm over the boundary plane at both times, within
+  1e-5 in the max norm at t = 10 and within 5e-5 at t = 100 (measured 2.0e-5, the limit of FP32 state storage and
+  FP32 derivative transforms carried by the dynamics, at a few vortex cores), its spatial mean within 1e-6; FP32
+  arithmetic in the filter, by contrast, leaves 6e-3 at t = 100.
 * The fine-grid time-step limit: the oracle runs of ``tests/golden/make_instability.py`` (32^2 x 12
   at spacing 0.17 turns non-finite, at spacing 0.48 stays finite), repeated on the GPU.
 * The batched ensemble of 48 independent evolutions drawn from the ensemble workload's ranges
@@ -53,12 +54,15 @@ def test_mixed_lattice_5000_steps():
             done = n
             a64, am = np.abs(O["fp64"]), np.abs(O["mixed"])
             err = float(np.max(np.abs(am - a64)) / np.max(a64))
+            el2 = float(np.linalg.norm((am - a64).ravel()) / np.linalg.norm(a64.ravel()))
             emean = float(abs(am.mean() - a64.mean()) / a64.mean())
-            print(f"  t = {n * 0.02:g}: |O| max-norm rel err {err:.2e}, mean {emean:.2e}, vortices "
+            print(f"  t = {n * 0.02:g}: |O| max-norm rel err {err:.2e}, L2 {el2:.2e}, mean {emean:.2e}, vortices "
                   f"{diagnostics.countN(O['fp64'])} {diagnostics.countN(O['mixed'])}", flush=True)
             assert diagnostics.countN(O["mixed"]) == diagnostics.countN(O["fp64"])
             assert diagnostics.vortex_positions(O["mixed"]) == diagnostics.vortex_positions(O["fp64"])
-            assert err < tol and emean < 1e-6
+            # the condensate within 1e-5 relative in the L2 norm over the plane and in its spatial mean at t = 10
+            # and t = 100 (measured 4.1e-7 and 1.2e-6 in L2); the max norm, set by a few vortex cores, within tol
+            assert err < tol and el2 < 1e-5 and emean < 1e-6
     finally:
         for st in sts.values():
             st.close()

@@ -1,7 +1,8 @@
-// Factored-DFT line passes of the RK stages and the constraint, single-region form (gfx950).
+// Factored-DFT line passes of the RK stages, the constraint and the Fourier filter, single-region form (gfx950).
 //
-// Reference: tevol_driven_cluster.m:60-61 (func_Lx / func_Ly) and :129 (the constraint's d_x A_x,
-// d_y A_y), third_party/dmsuite/fourdif.m.  The transform is the two-stage real DFT of
+// Reference: tevol_driven_cluster.m:60-61 (func_Lx / func_Ly), :129 (the constraint's d_x A_x, d_y A_y) and
+// :142 with src/numerics/ff.m:7-11 (the filter), third_party/dmsuite/fourdif.m.  The transform is the two-stage
+// real DFT of
 // hqt_spec_core.h (N = N1 N2, stage A a real DFT of length N2 of each of the N1 sub-sequences,
 // stage B a complex DFT of length N1 per kept column); what differs is how a work-group uses the CU:
 //
@@ -25,7 +26,9 @@
 //     60 VGPRs of the y-pass to scratch: line_x_stage 74.1 -> 68.3 us, line_y_stage 96.8 -> 94.0 us at 301^2 x 20).
 //
 // The stage passes apply d/dx (i p s), d2/dx2 (-(p s)^2) and -d/dx (-i p s), s the signed wavenumber (0 at
-// the Nyquist mode of even N): exactly the eigenvalues of the fourdif matrix Lx = p Dx and of Lx Lx.
+// the Nyquist mode of even N): exactly the eigenvalues of the fourdif matrix Lx = p Dx and of Lx Lx.  The
+// filter's passes (FILT) apply the even part of its keep-mask as the multiplier and carry the mask's rank-4 odd
+// part through the cross-term slice matrices G (hqt_spec_core.h describes the decomposition).
 #pragma once
 #include "hqt_spec_core.h"
 

@@ -114,28 +114,6 @@ __device__ __forceinline__ void load_slots(const void* f, int64_t c, bool cok, i
     }
 }
 
-// RK4 bookkeeping for one value of one evolved field (:120-128).
-//   stage 1: acc = k;        next = z + k/2
-//   stage 2: acc = acc + 2k; next = z + k/2
-//   stage 3: acc = acc + 2k; next = z + k
-//   stage 4: z = z + (1/6)*(acc + k)
-template <typename TS>
-__device__ __forceinline__ void rk_update(int stage, double kv, void* acc, void* z, void* nxt, int64_t i) {
-    if (stage == 1) {
-        st<TS>(acc, i, kv);
-        st<TS>(nxt, i, ld<TS>(z, i) + kv * 0.5);
-    } else if (stage == 2) {
-        st<TS>(acc, i, ld<TS>(acc, i) + 2.0 * kv);
-        st<TS>(nxt, i, ld<TS>(z, i) + kv * 0.5);
-    } else if (stage == 3) {
-        st<TS>(acc, i, ld<TS>(acc, i) + 2.0 * kv);
-        st<TS>(nxt, i, ld<TS>(z, i) + kv);
-    } else {
-        const double fin = (1.0 / 6.0) * (ld<TS>(acc, i) + kv);
-        st<TS>(z, i, ld<TS>(z, i) + fin);
-    }
-}
-
 // ---- one RK stage for 16 columns per wave (F1, F3, F4, F2 of :72-76 + RK bookkeeping) ----
 //   E1 = 1i*At*dPsi + 0.5i*Psi*dAt + 0.5*f*d2Psi - 1.5*z^2*dPsi - 0.5*z*Psi
 //        - 0.5*(Ax^2 + Ay^2)*Psi + 0.5*(dxx+dyy)Psi - 0.5i*Psi*(dxAx+dyAy) - 1i*(Ax*dxPsi + Ay*dyPsi)
@@ -164,15 +142,6 @@ __device__ __forceinline__ RkRs rk_rs(int stage, void* acc, void* z, void* nxt, 
     const Rs ra = rs_make(acc, bytes), rz = rs_make(z, bytes), rn = rs_make(nxt, bytes), r0 = rs_make(acc, 0);
     return RkRs{stage == 1 ? r0 : ra, rz, stage == 4 ? rz : ra, stage == 4 ? r0 : rn};
 }
-template <typename TS>
-__device__ __forceinline__ void rk_apply(int stage, const RkRs& R, uint32_t e, double kv) {
-    const double a0 = bld<TS>(R.acc_ld, e), z0 = bld<TS>(R.z_ld, e);
-    const double v1 = stage == 4 ? z0 + (1.0 / 6.0) * (a0 + kv) : (stage == 1 ? kv : a0 + 2.0 * kv);
-    const double v2 = z0 + (stage == 3 ? kv : kv * 0.5);
-    bst<TS>(R.st1, e, v1);
-    bst<TS>(R.st2, e, v2);
-}
-
 template <int NZ> struct ZLds {
     static constexpr int KS = ZG<NZ>::KS, RT = ZG<NZ>::RT, OPN = RT * KS * 64, NZP = 4 * KS;
     double lz[OPN], l1[OPN];

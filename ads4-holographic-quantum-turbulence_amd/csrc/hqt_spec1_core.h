@@ -431,8 +431,8 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
     // the tile loop, compiled for the three group shapes: one source and one output, two sources summed into
     // one output (the first source's result stored to the output and read back as the second one's addend),
     // one source and two outputs (one forward transform, the second output's columns in the second LDS
-    // region).  An output's addend is loaded after stage B, so its latency overlaps the inverse transform and
-    // it occupies no registers during the forward transform and stage B.
+    // region).  An output's addend is loaded just before its store, so it occupies no registers during the
+    // transforms.
     auto tiles = [&](auto ns_, auto no_) {
     constexpr int NS = decltype(ns_)::value, NO = decltype(no_)::value;
     for (int it = 0; it < ntile; ++it) {

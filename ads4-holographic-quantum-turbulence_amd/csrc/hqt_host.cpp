@@ -400,6 +400,9 @@ int make_spec_pass(hqt_ctx* c, LPass& P, int dir) {
             LOut& O = P.g[g].out[o];
             if (O.nterms != P.g[g].ntiles || O.gmode)
                 return fail(HQT_ERR_ARG, "internal: factored-DFT output without one term per source");
+            // a two-source output holds its first source's result until the second source is in
+            if (P.g[g].ntiles == 2 && O.out == P.g[g].src[1])
+                return fail(HQT_ERR_ARG, "internal: factored-DFT two-source output aliases its second source");
             for (int e = 0; e < O.nterms; ++e)
                 if (O.term[e].tile != e || O.term[e].lr)
                     return fail(HQT_ERR_ARG, "internal: factored-DFT output without one term per source");

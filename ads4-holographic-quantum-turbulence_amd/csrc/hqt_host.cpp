@@ -1274,6 +1274,62 @@ int hqt_ff(int32_t nx, int32_t ny, int32_t nslices, const double* x_re, const do
     return rc;
 }
 
+int hqt_ff_mask(int32_t nx, int32_t ny, int32_t nslices, const double* x_re, const double* x_im, const uint8_t* mask,
+                int32_t mask_slices, double* w) {
+    if (!x_re || !mask || !w) return fail(HQT_ERR_ARG, "NULL argument");
+    if (nx < 1 || ny < 1 || nx > 4096 || ny > 4096 || nslices < 1) return fail(HQT_ERR_ARG, "bad shape");
+    if (mask_slices != 1 && mask_slices != nslices) return fail(HQT_ERR_ARG, "mask_slices must be 1 or nslices");
+    hqt_ctx tmp;   // scratch holder for allocations only
+    const size_t n = (size_t)nx * ny * nslices, nm = (size_t)nx * ny * mask_slices;
+    int rc = HQT_OK;
+    void *re = nullptr, *im = nullptr, *mk = nullptr, *twx = nullptr, *twy = nullptr, *a = nullptr, *b = nullptr,
+         *out = nullptr;
+    hipStream_t s = nullptr;
+    do {
+        if ((rc = dalloc(&tmp, n * 8, &re))) break;
+        if (x_im && (rc = dalloc(&tmp, n * 8, &im))) break;
+        if ((rc = dalloc(&tmp, nm, &mk))) break;
+        if ((rc = dalloc(&tmp, (size_t)nx * 16, &twx))) break;
+        if ((rc = dalloc(&tmp, (size_t)ny * 16, &twy))) break;
+        if ((rc = dalloc(&tmp, n * 16, &a))) break;
+        if ((rc = dalloc(&tmp, n * 16, &b))) break;
+        if ((rc = dalloc(&tmp, n * 8, &out))) break;
+        // cos, sin (2 pi m / N), m < N, in long double
+        std::vector<double> tx(2 * (size_t)nx), ty(2 * (size_t)ny);
+        for (int m = 0; m < nx; ++m) {
+            const long double t = 2.0L * (long double)M_PI * m / nx;
+            tx[2 * m] = (double)cosl(t); tx[2 * m + 1] = (double)sinl(t);
+        }
+        for (int m = 0; m < ny; ++m) {
+            const long double t = 2.0L * (long double)M_PI * m / ny;
+            ty[2 * m] = (double)cosl(t); ty[2 * m + 1] = (double)sinl(t);
+        }
+        std::vector<uint8_t> mb(nm);
+        for (size_t i = 0; i < nm; ++i) mb[i] = mask[i] ? 1 : 0;
+        if (hipMemcpy(re, x_re, n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            (x_im && hipMemcpy(im, x_im, n * 8, hipMemcpyHostToDevice) != hipSuccess) ||
+            hipMemcpy(mk, mb.data(), nm, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(twx, tx.data(), tx.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(twy, ty.data(), ty.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+            rc = fail(HQT_ERR_HIP, "hqt_ff_mask: hipMemcpy");
+            break;
+        }
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) { rc = fail(HQT_ERR_HIP, "stream"); break; }
+        hipError_t e = launch_ff_mask(static_cast<const double*>(re), static_cast<const double*>(im),
+                                      static_cast<const uint8_t*>(mk), mask_slices == 1 ? 0 : (int64_t)nx * ny,
+                                      static_cast<const double*>(twx), static_cast<const double*>(twy),
+                                      static_cast<double2*>(a), static_cast<double2*>(b), static_cast<double*>(out), nx,
+                                      ny, nslices, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) { rc = fail(HQT_ERR_HIP, std::string("hqt_ff_mask: ") + hipGetErrorString(e)); break; }
+        if (hipMemcpy(w, out, n * 8, hipMemcpyDeviceToHost) != hipSuccess) { rc = fail(HQT_ERR_HIP, "hipMemcpy w"); break; }
+    } while (0);
+    if (s) (void)hipStreamDestroy(s);
+    for (void* q : tmp.allocs) (void)hipFree(q);
+    tmp.allocs.clear();
+    return rc;
+}
+
 int hqt_fourdif(int32_t N, int32_t m, double* x, double* DM) {
     if (N < 1 || m < 0 || !x || !DM) return fail(HQT_ERR_ARG, "bad argument");
     fourdif_dense(N, m, x, DM);

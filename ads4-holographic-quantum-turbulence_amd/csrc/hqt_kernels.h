@@ -265,6 +265,12 @@ hipError_t launch_nancheck(const NanCheck* d, Members M, int blocks, int fp32, h
 bool zkernels_support_nz(int nz);
 // Plain streaming copy of `bytes` (multiple of 32) used to calibrate traffic counters.
 hipError_t launch_copy_calib(const void* src, void* dst, int64_t bytes, hipStream_t s);
+// The Fourier filter for any mask (hqt_ffmask.hip): re/im [nx, ny, nslices] (im NULL: real), mask 0/1 bytes
+// [nx, ny] per slice at stride mstride (0: one mask for every slice), twiddle tables twx, twy (cos, sin of
+// 2 pi m / N, m < N), work buffers a, b of nx ny nslices complex values, output w.
+hipError_t launch_ff_mask(const double* re, const double* im, const uint8_t* mask, int64_t mstride, const double* twx,
+                          const double* twy, double2* a, double2* b, double* w, int nx, int ny, int nslices,
+                          hipStream_t s);
 // Batched Newton (one work-group per problem); every job of one launch has the same nz <= 64.
 // njobs problems of radial size at most maxnz; lds: every Jacobian in LDS (each nz resident), else in
 // NewtonJob::work.

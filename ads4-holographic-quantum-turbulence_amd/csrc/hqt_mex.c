@@ -11,7 +11,8 @@
  *   hqt_mex('set', h, Z1, Z2, Z3, Z4, rhoi)
  *   [Z1,Z2,Z3,Z4,rhoi] = hqt_mex('step', h, nsteps)      % :120-142 iterated nsteps times
  *   [F1,F2,F3,F4] = hqt_mex('rhs', h, Z1, Z2, Z3, Z4)     % the closures of :72-76
- *   w = hqt_mex('ff', x, fx, fy)                          % src/numerics/ff.m with mask fx|fy
+ *   w = hqt_mex('ff', x, y)                               % src/numerics/ff.m, any logical mask y
+ *   w = hqt_mex('ff', x, fx, fy)                          % the same with the separable mask fx|fy
  *   hqt_mex('init', h, Z1, Z2, Vy1, rhoi, amp_noise, phase_noise, seed)
  *         % :81-102 on the device from the nz0-point profile (1e-2 amplitude noise as :101)
  *   hqt_mex('checkpoint', h, fileName)                    % save(fileName,'Z1',...,'rhoi') of :178-181,
@@ -70,15 +71,15 @@ static mxArray* field_out(int32_t nx, int32_t ny, int32_t nz, int cplx) {
 }
 
 /* fx / fy as 0/1 bytes from a logical or double vector (the reference builds them with false(...)). */
-static void mask_bytes(const mxArray* a, uint8_t* out, int32_t n) {
+static void mask_bytes(const mxArray* a, uint8_t* out, size_t n) {
     if (mxIsLogical(a)) {
         const mxLogical* v = mxGetLogicals(a);
-        for (int32_t i = 0; i < n; ++i) out[i] = v[i] ? 1 : 0;
+        for (size_t i = 0; i < n; ++i) out[i] = v[i] ? 1 : 0;
     } else if (mxIsDouble(a) && !mxIsComplex(a)) {
         const double* v = mxGetDoubles(a);
-        for (int32_t i = 0; i < n; ++i) out[i] = v[i] != 0.0 ? 1 : 0;
+        for (size_t i = 0; i < n; ++i) out[i] = v[i] != 0.0 ? 1 : 0;
     } else {
-        mexErrMsgIdAndTxt("hqt:arg", "mask factors must be logical or double vectors");
+        mexErrMsgIdAndTxt("hqt:arg", "masks must be logical or double arrays");
     }
 }
 
@@ -113,17 +114,28 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
         return;
     }
     if (!strcmp(cmd, "ff")) {
-        if (nrhs != 4) mexErrMsgIdAndTxt("hqt:arg", "ff needs x, fx, fy");
+        if (nrhs != 3 && nrhs != 4) mexErrMsgIdAndTxt("hqt:arg", "ff needs x and a mask y, or x, fx, fy");
         const mwSize* d = mxGetDimensions(prhs[1]);
         int32_t nx = (int32_t)d[0], ny = (int32_t)d[1];
         int32_t ns = mxGetNumberOfDimensions(prhs[1]) > 2 ? (int32_t)d[2] : 1;
         size_t n = (size_t)nx * ny * ns;
-        if (mxGetNumberOfElements(prhs[2]) != (size_t)nx || mxGetNumberOfElements(prhs[3]) != (size_t)ny)
-            mexErrMsgIdAndTxt("hqt:arg", "fx, fy must have nx, ny entries");
-        uint8_t* fx = (uint8_t*)mxMalloc(nx);
-        uint8_t* fy = (uint8_t*)mxMalloc(ny);
-        mask_bytes(prhs[2], fx, nx);
-        mask_bytes(prhs[3], fy, ny);
+        uint8_t *fx = NULL, *fy = NULL, *mk = NULL;
+        int32_t mslices = 1;
+        if (nrhs == 3) {
+            /* ff(x, y): any logical mask of nx*ny (every slice) or nx*ny*ns entries */
+            size_t nm = mxGetNumberOfElements(prhs[2]);
+            if (nm != (size_t)nx * ny && nm != n) mexErrMsgIdAndTxt("hqt:arg", "the mask must have nx*ny or numel(x) entries");
+            mslices = nm == (size_t)nx * ny ? 1 : ns;
+            mk = (uint8_t*)mxMalloc(nm);
+            mask_bytes(prhs[2], mk, nm);
+        } else {
+            if (mxGetNumberOfElements(prhs[2]) != (size_t)nx || mxGetNumberOfElements(prhs[3]) != (size_t)ny)
+                mexErrMsgIdAndTxt("hqt:arg", "fx, fy must have nx, ny entries");
+            fx = (uint8_t*)mxMalloc(nx);
+            fy = (uint8_t*)mxMalloc(ny);
+            mask_bytes(prhs[2], fx, nx);
+            mask_bytes(prhs[3], fy, ny);
+        }
         double* re = (double*)mxMalloc(n * sizeof(double));
         double* im = NULL;
         if (mxIsComplex(prhs[1])) {
@@ -134,8 +146,9 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
             memcpy(re, mxGetDoubles(prhs[1]), n * sizeof(double));
         }
         plhs[0] = mxCreateNumericArray(mxGetNumberOfDimensions(prhs[1]), d, mxDOUBLE_CLASS, mxREAL);
-        int rc = hqt_ff(nx, ny, ns, re, im, fx, fy, mxGetDoubles(plhs[0]));
-        mxFree(re); if (im) mxFree(im); mxFree(fx); mxFree(fy);
+        int rc = mk ? hqt_ff_mask(nx, ny, ns, re, im, mk, mslices, mxGetDoubles(plhs[0]))
+                    : hqt_ff(nx, ny, ns, re, im, fx, fy, mxGetDoubles(plhs[0]));
+        mxFree(re); if (im) mxFree(im); if (fx) mxFree(fx); if (fy) mxFree(fy); if (mk) mxFree(mk);
         if (rc) fail(rc);
         return;
     }

@@ -32,7 +32,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "csrc", "libhqt.so")
 
 EXPORTS = (
     "hqt_ctx_create", "hqt_ctx_destroy", "hqt_set_state", "hqt_get_state", "hqt_step_n", "hqt_sync",
-    "hqt_last_winding", "hqt_rhs", "hqt_order_parameter", "hqt_ff", "hqt_fourdif", "hqt_chebdif",
+    "hqt_last_winding", "hqt_rhs", "hqt_order_parameter", "hqt_ff", "hqt_ff_mask", "hqt_fourdif", "hqt_chebdif",
     "hqt_newton_vy", "hqt_newton_vy_batch", "hqt_newton_vy_batch_noalloc", "hqt_newton_vy_continue_batch", "hqt_vortex_census", "hqt_ensemble_create", "hqt_ensemble_step_n", "hqt_ensemble_sync",
     "hqt_ensemble_launches", "hqt_ensemble_destroy", "hqt_server_create", "hqt_server_destroy",
     "hqt_server_submit_evolution", "hqt_server_submit_newton", "hqt_server_submit_newton_continued", "hqt_server_wait", "hqt_server_evolution_result",
@@ -128,6 +128,7 @@ def lib():
     L.hqt_spline.argtypes = [i32, dp, dp, i32, dp, dp]
     L.hqt_vortex_census.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_int8), dp]
     L.hqt_ff.argtypes = [i32, i32, i32, dp, dp, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), dp]
+    L.hqt_ff_mask.argtypes = [i32, i32, i32, dp, dp, C.POINTER(C.c_uint8), i32, dp]
     L.hqt_fourdif.argtypes = [i32, i32, dp, dp]
     L.hqt_chebdif.argtypes = [i32, i32, dp, dp]
     L.hqt_newton_vy.argtypes = [i32, C.c_double, C.c_double, C.c_double, dp, dp, dp, C.c_double, dp, dp, dp,

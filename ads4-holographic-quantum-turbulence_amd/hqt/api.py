@@ -59,7 +59,10 @@ def mask_factors(mask: np.ndarray):
 def ff(x, y):
     """``w = ff(x, y)``: zero the masked 2-D Fourier modes of every slice and return the real
     part (``src/numerics/ff.m:7-11``).  ``x`` is ``[nx, ny]`` or ``[nx, ny, nslices]``, real or
-    complex; ``y`` is a logical mask of the form ``fx | fy``."""
+    complex; ``y`` is a logical mask ``[nx, ny]`` (every slice) or ``[nx, ny, nslices]``.  A mask of
+    the form ``fx | fy`` repeated over the slices (the stepper's, ``tevol_driven_cluster.m:27-31``)
+    runs through the separable line passes (``hqt_ff``), any other through direct 2-D DFTs
+    (``hqt_ff_mask``); both on the GPU."""
     x = np.asarray(x)
     shape = x.shape
     if x.ndim == 2:
@@ -67,16 +70,24 @@ def ff(x, y):
     if x.ndim != 3:
         raise HQTError(HQT_ERR_ARG, "ff: x must be 2-D or 3-D")
     nx, ny, ns = x.shape
-    fx, fy = mask_factors(y)
-    if fx.shape[0] != nx or fy.shape[0] != ny:
+    m = np.asarray(y, dtype=bool)
+    if m.ndim not in (2, 3) or m.shape[:2] != (nx, ny) or (m.ndim == 3 and m.shape[2] not in (1, ns)):
         raise HQTError(HQT_ERR_ARG, "ff: mask shape does not match x")
     xr = np.asfortranarray(np.real(x), dtype=np.float64)
     xi = np.asfortranarray(np.imag(x), dtype=np.float64) if np.iscomplexobj(x) else None
     w = np.empty((nx, ny, ns), order="F")
-    fxb = np.ascontiguousarray(fx, dtype=np.uint8)
-    fyb = np.ascontiguousarray(fy, dtype=np.uint8)
     u8 = C.POINTER(C.c_uint8)
-    check(lib().hqt_ff(nx, ny, ns, dptr(xr), dptr(xi), fxb.ctypes.data_as(u8), fyb.ctypes.data_as(u8), dptr(w)))
+    try:
+        fx, fy = mask_factors(m)
+    except HQTError:
+        fx = fy = None
+    if fx is not None and nx <= 512 and ny <= 512:
+        fxb = np.ascontiguousarray(fx, dtype=np.uint8)
+        fyb = np.ascontiguousarray(fy, dtype=np.uint8)
+        check(lib().hqt_ff(nx, ny, ns, dptr(xr), dptr(xi), fxb.ctypes.data_as(u8), fyb.ctypes.data_as(u8), dptr(w)))
+    else:
+        mb = np.asfortranarray(m if m.ndim == 3 else m[:, :, None], dtype=np.uint8)
+        check(lib().hqt_ff_mask(nx, ny, ns, dptr(xr), dptr(xi), mb.ctypes.data_as(u8), mb.shape[2], dptr(w)))
     return w.reshape(shape)
 
 

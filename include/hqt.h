@@ -247,6 +247,13 @@ int hqt_ctx_device_bytes(hqt_ctx* ctx, int64_t* bytes);
  * fx [nx], fy [ny] are 0/1 bytes; the 2-D mask is fx(i) | fy(j).  Computed on the GPU. */
 int hqt_ff(int32_t nx, int32_t ny, int32_t nslices, const double* x_re, const double* x_im,
            const uint8_t* fx, const uint8_t* fy, double* w);
+/* w = ff(x, y) for any logical mask y: src/numerics/ff.m:7-11 (b = fft2(x); b(y) = 0; w = real(ifft2(b))).
+ * x is [nx, ny, nslices], real (x_im == NULL) or complex; mask is [nx, ny] 0/1 bytes in the field's order
+ * (entry kx + nx ky masks the Fourier mode (kx, ky)), the same for every slice (mask_slices = 1) or one per slice
+ * (mask_slices = nslices).  Computed on the GPU by direct DFTs in FP64, O(nx ny (nx + ny)) per slice; hqt_ff is
+ * the fast form for the separable masks fx | fy the stepper uses. */
+int hqt_ff_mask(int32_t nx, int32_t ny, int32_t nslices, const double* x_re, const double* x_im,
+                const uint8_t* mask, int32_t mask_slices, double* w);
 
 /* [x, DM] = fourdif(N, m): third_party/dmsuite/fourdif.m:30-71.  DM column-major N*N. */
 int hqt_fourdif(int32_t N, int32_t m, double* x, double* DM);

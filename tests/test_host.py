@@ -162,6 +162,27 @@ def test_newton_continuation_rejects_bad_arguments(seed, target, dmu):
     assert "problem 1" in L.hqt_last_error().decode()
 
 
+@pytest.mark.parametrize("nx,ny,ns,ms", [(8, 8, 3, 2), (0, 8, 1, 1), (8, 8, 0, 1), (8, 8, 2, 0)])
+def test_ff_mask_rejects_bad_arguments(nx, ny, ns, ms):
+    """hqt_ff_mask checks its shape and that the mask covers one slice or every slice before touching the GPU."""
+    import ctypes as C
+    import hqt
+    L = hqt.lib()
+    x = np.zeros(max(nx * ny * ns, 1))
+    m = np.zeros(max(nx * ny * max(ms, 1), 1), dtype=np.uint8)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    rc = L.hqt_ff_mask(nx, ny, ns, dp(x), None, m.ctypes.data_as(C.POINTER(C.c_uint8)), ms, dp(x))
+    assert rc == hqt.HQT_ERR_ARG
+
+
+def test_ff_mask_shape_checked_in_python():
+    import hqt
+    with pytest.raises(hqt.HQTError):
+        hqt.ff(np.zeros((8, 6, 3)), np.zeros((8, 6, 2), dtype=bool))
+    with pytest.raises(hqt.HQTError):
+        hqt.ff(np.zeros((8, 6)), np.zeros((6, 8), dtype=bool))
+
+
 @pytest.mark.parametrize("kw", [dict(nx=1), dict(nz=3), dict(nz=33), dict(nx=600), dict(Lx=0.0), dict(Ly=-1.0)])
 def test_ctx_create_rejects_bad_arguments(kw):
     """Argument checks happen before any device call, so they run without a GPU."""

@@ -277,8 +277,7 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
     };
     // kinds: the multiplier of the region's output; TWO: a second output from the same forward transform,
     // its column written to the second region R2 (same offsets)
-    auto fused_b = [&](auto two, int kind0, int kind1) {
-        constexpr bool TWO = decltype(two)::value;
+    auto fused_b = [&](bool two, int kind0, int kind1) {
         const cptr<T> tb = phase_table<T>(P.st);
         T cb[N1], sb[N1];
 #pragma unroll
@@ -345,7 +344,7 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
                     }
                 };
                 back(kind0, Rg + off);
-                if constexpr (TWO) back(kind1, Rg + LB * S + off);
+                if (two) back(kind1, Rg + LB * S + off);
             } else {
                 const int k2 = c > HI ? H : 0;
                 const int off = l * S + k2;
@@ -376,7 +375,7 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
                     for (int n1 = 0; n1 < N1; ++n1) dst[n1 * N2] = fmat(zr[n1], tc[n1], -(zi[n1] * ts[n1]));
                 };
                 back(kind0, Rg + off);
-                if constexpr (TWO) back(kind1, Rg + LB * S + off);
+                if (two) back(kind1, Rg + LB * S + off);
             }
         }
         __syncthreads();
@@ -428,13 +427,15 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
         }
     };
 
-    // the tile loop, compiled for the three group shapes: one source and one output, two sources summed into
-    // one output (the first source's result stored to the output and read back as the second one's addend),
-    // one source and two outputs (one forward transform, the second output's columns in the second LDS
-    // region).  An output's addend is loaded just before its store, so it occupies no registers during the
-    // transforms.
-    auto tiles = [&](auto ns_, auto no_) {
-    constexpr int NS = decltype(ns_)::value, NO = decltype(no_)::value;
+    // The tile loop.  A unit is one source of a tile: it is staged, transformed forward, multiplied and
+    // transformed back, and each of its outputs stored.  A two-source output stores its first source's result and
+    // reads it back as the second source's addend, so both sources run the same code; a group with two outputs
+    // (one source) keeps the second output's columns in the second LDS region.  The loops over sources and
+    // outputs stay rolled: every transform phase is compiled once, and a kernel's code fits the instruction cache.
+    // An output's addend is loaded just before its store, so it occupies no registers during the transforms.
+    // (The host gives FP64 passes one output per group; a filter group's second output is the cross term's
+    // partials, not a second transform.)
+    const bool two = sizeof(T) == 4 && !FILT && Gr.nout == 2;
     for (int it = 0; it < ntile; ++it) {
         const int lt = tile0 + it;
         int line0, z, x0;
@@ -459,8 +460,8 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
         };
         // hold: the output's addend (for a two-source output's second source, the first source's stored result)
         T hold[PF];
-#pragma unroll
-        for (int t = 0; t < NS; ++t) {
+#pragma unroll 1
+        for (int t = 0; t < nsrc; ++t) {
             // the source's lines, relative to their first value
             T cst = (T)0;
             {
@@ -498,31 +499,20 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
             int kind = 1;
             for (int e = 0; e < O.nterms; ++e)            // (a filter output's rank-4 term is its cross term)
                 if (O.term[e].tile == t && !O.term[e].lr) kind = O.term[e].sk;
-            fused_b(std::integral_constant<bool, NO == 2>{}, kind, NO == 2 ? Gr.out[1].term[0].sk : 1);
+            fused_b(two, kind, two ? Gr.out[1].term[0].sk : 1);
             // the x-pass filter's cross term: this 16-line block's partials of G from the coefficients in DL
             if (FILT && Og) fold_g_partials<T, 1>(P, *Og, DL, line0, nl);
-            inv_a(Rg);
-            const Line L = line_at(fresh((int)threadIdx.x));
-            // (the barriers since the first source's store order it before this read of the same elements)
-            load_addend(O, L, hold, t > 0);
-            store(O, Rg, L, hold, cst);
-            if constexpr (NO == 2) {
-                const Line L2 = line_at(fresh((int)threadIdx.x));
-                load_addend(Gr.out[1], L2, hold, false);
-                inv_a(Rg + LB * S);
-                store(Gr.out[1], Rg + LB * S, L2, hold, (T)0);
+#pragma unroll 1
+            for (int o = 0; o < (two ? 2 : 1); ++o) {
+                const LOut& Ou = Gr.out[o];
+                T* const Rr = Rg + o * LB * S;
+                inv_a(Rr);
+                const Line L = line_at(fresh((int)threadIdx.x));
+                // (the barriers since the first source's store order it before this read of the same elements)
+                load_addend(Ou, L, hold, t > 0);
+                store(Ou, Rr, L, hold, o == 0 ? cst : (T)0);
             }
         }
-    }
-    };
-    // (the host gives FP64 passes one output per group: their kernels leave out the two-output shape)
-    // (a filter group's second output is the cross term's partials, not a second transform)
-    if (nsrc == 2) tiles(std::integral_constant<int, 2>{}, std::integral_constant<int, 1>{});
-    else if constexpr (sizeof(T) == 4 && !FILT) {
-        if (Gr.nout == 2) tiles(std::integral_constant<int, 1>{}, std::integral_constant<int, 2>{});
-        else tiles(std::integral_constant<int, 1>{}, std::integral_constant<int, 1>{});
-    } else {
-        tiles(std::integral_constant<int, 1>{}, std::integral_constant<int, 1>{});
     }
 }
 

@@ -13,7 +13,7 @@
 //   * stage B fused with the spectral multiply and its own inverse: a column task transforms its column
 //     forward, multiplies it by the output term's eigenvalues and transforms it back in registers, in
 //     place in the LDS line region.  No trig table in LDS: a work-group holds 16 lines (38.5 KB in FP64,
-//     half that in FP32), so three FP64 (four FP32) work-groups share a CU;
+//     half that in FP32), so four FP64 (more FP32) work-groups share a CU;
 //   * stage A accumulates in streaming form (acc[k] += v_j * cos(2 pi j k / N2) over the folded inputs
 //     v_j), with the cos / sin constants of the compile-time sizes read by scalar loads into SGPRs, so the
 //     phase holds one accumulator per output and no constant in vector registers; the inverse writes the
@@ -21,9 +21,10 @@
 //     added), so the output phase reads it like the input phase wrote it;
 //   * an output with two sources (d_y(d_y A_x - d_x A_y)) stores the first source's result to the output field
 //     and reads it back as the second source's addend; an output's addend is loaded just before the store.
-//     Nothing but the transform's own values is held in registers across the transforms: the FP64 passes fit
-//     168 VGPRs (three work-groups per CU) without spills (holding the addend from the source load on spilled
-//     60 VGPRs of the y-pass to scratch: line_x_stage 74.1 -> 68.3 us, line_y_stage 96.8 -> 94.0 us at 301^2 x 20).
+//     Nothing but the transform's own values is held in registers across the transforms, and every transform
+//     phase is compiled once (rolled loops over sources and outputs): the FP64 stage passes fit 128 VGPRs without
+//     spills, so four work-groups share a CU (4 x 38.5 KB of LDS; line_x_stage 68.5 -> 63.2 us, line_y_stage
+//     91.8 -> 83.2 us at 301^2 x 20 against three).
 //
 // The stage passes apply d/dx (i p s), d2/dx2 (-(p s)^2) and -d/dx (-i p s), s the signed wavenumber (0 at
 // the Nyquist mode of even N): exactly the eigenvalues of the fourdif matrix Lx = p Dx and of Lx Lx.  The
@@ -517,7 +518,7 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
 }
 
 template <typename T, int DIR, int N1, int N2, typename R, bool FILT>
-__global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4))) void k_spec1_pass(
+__global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_spec1_pass(
     const LPass* __restrict__ Pp, const Members M, const int xcd_order) {
     spec1_body<T, DIR, N1, N2, R, FILT>(Pp, M, xcd_order, (int)blockIdx.x, (int)gridDim.x, (int)blockIdx.y,
                                         (int)gridDim.y);
@@ -528,7 +529,7 @@ __global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(size
 // (the work-groups between return at once), so both passes keep the XCD assignment (work-group i on XCD i mod 8)
 // their own launches would have.
 template <typename T, int N1, int N2>
-__global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4))) void k_spec_pair(
+__global__ __launch_bounds__(LP_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_spec_pair(
     const LPass* __restrict__ X, const LPass* __restrict__ Y, const int nb0, const int nb1, const int order0,
     const int order1) {
     const Members one{nullptr, 1};

@@ -377,17 +377,14 @@ int make_spec_pass(hqt_ctx* c, LPass& P, int dir) {
                 else if (op == OP_ND1X || op == OP_ND1Y) tm.sk = 3;
                 else return fail(HQT_ERR_ARG, "internal: factored-DFT pass with a term that is not a derivative");
             }
-    // A group keeps two outputs only where they share one source and the pass can afford a second LDS line
-    // region: the mixed form's y-passes (two FP32 regions are 38.5 KB at N = 301, and the two outputs share one
-    // forward transform: line_y_stage 71 -> 67 us at 301^2 x 20, 19.0 -> 17.5 us at 128^2 x 12).  Elsewhere a
-    // group's second output becomes a group of its own on the same sources (the kernel transforms them again):
-    // the mixed x-passes lose with fewer, longer work-groups (49.5 -> 49.1 us at 301^2, 12.8 -> 14.6 us at
-    // 128^2), and two FP64 regions leave 2 work-groups per CU instead of 3 and need 221 VGPRs (FP64 with
-    // shared sources: line_x_stage 69.4 -> 81.0 us, line_y_stage 94.1 -> 117.7 us).
-    const bool share = c->fp32 != 0 && dir == 1;
+    // One output per group: a group's second output becomes a group of its own on the same sources (the kernel
+    // transforms them again), so a work-group needs one LDS line region.  Two outputs sharing one forward
+    // transform in a second region lost in both precisions once the kernels fit four work-groups per CU
+    // (mixed line_y_stage 64.6 -> 63.2 us with one output per group; FP64 needs 221 VGPRs for the second output:
+    // line_x_stage 69.4 -> 81.0 us, line_y_stage 94.1 -> 117.7 us).
     const int ng0 = P.ngroups;
     for (int g = 0; g < ng0; ++g)
-        while (P.g[g].nout > (share && P.g[g].ntiles == 1 ? 2 : 1)) {
+        while (P.g[g].nout > 1) {
             if (P.ngroups == LP_MAXG) return fail(HQT_ERR_ARG, "internal: factored-DFT pass with too many outputs");
             LGroup& h = P.g[P.ngroups++];
             h = P.g[g];
@@ -407,9 +404,6 @@ int make_spec_pass(hqt_ctx* c, LPass& P, int dir) {
                 if (O.term[e].tile != e || O.term[e].lr)
                     return fail(HQT_ERR_ARG, "internal: factored-DFT output without one term per source");
         }
-    // of two outputs, the one with an addend first
-    for (int g = 0; g < P.ngroups; ++g)
-        if (P.g[g].nout == 2 && !P.g[g].out[0].addend && P.g[g].out[1].addend) std::swap(P.g[g].out[0], P.g[g].out[1]);
     P.fold = 3;
     P.Mp = 64 * (fid + 1);
     P.S = N | 1;

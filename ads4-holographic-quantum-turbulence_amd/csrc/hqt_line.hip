@@ -40,13 +40,8 @@ size_t line_pass_lds_bytes(const LPass& P, int fp32) {
     if (P.fold == 3 || P.fold == 4) {
         const int fid = P.Mp / 64 - 1;
         const int es = fp32 ? (int)sizeof(float) : (int)sizeof(double);
-        // the RK-stage and constraint passes (hqt_spec1_core.h): one line region (lb x S), two when a group has
-        // two outputs
-        if (!spec_filter_pass(P)) {
-            int regions = 1;
-            for (int g = 0; g < P.ngroups; ++g) regions = P.g[g].nout > regions ? P.g[g].nout : regions;
-            return (size_t)es * (size_t)regions * (size_t)P.lb * P.S;
-        }
+        // the RK-stage and constraint passes (hqt_spec1_core.h): one line region (lb x S)
+        if (!spec_filter_pass(P)) return (size_t)es * (size_t)P.lb * P.S;
         // the filter: the line region and the cross-term area (G, its partials and the lines' coefficients)
         (void)fid;
         return (size_t)spec1_cross_offset(P.S, es) + (size_t)spec_cross_bytes(es);

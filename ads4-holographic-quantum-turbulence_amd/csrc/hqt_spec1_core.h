@@ -6,10 +6,10 @@
 // hqt_spec_core.h (N = N1 N2, stage A a real DFT of length N2 of each of the N1 sub-sequences,
 // stage B a complex DFT of length N1 per kept column); what differs is how a work-group uses the CU:
 //
-//   * one output per group, as a rule.  A pass's groups are split on the host so that a group has one output
-//     (a source with two outputs becomes two groups, each transforming the source again), so the spectrum
-//     never has to outlive its output.  Where a second LDS region is affordable (the mixed form's y-passes),
-//     a source keeps its two outputs and stage B writes the second one's columns there;
+//   * one output per group: a pass's groups are split on the host so that a group has one output (a source
+//     with two outputs becomes two groups, each transforming the source again), so the spectrum never has to
+//     outlive its output and a work-group needs one LDS line region (the filter's x-pass groups carry a second,
+//     non-transform output: the cross term's partials);
 //   * stage B fused with the spectral multiply and its own inverse: a column task transforms its column
 //     forward, multiplies it by the output term's eigenvalues and transforms it back in registers, in
 //     place in the LDS line region.  No trig table in LDS: a work-group holds 16 lines (38.5 KB in FP64,
@@ -103,8 +103,7 @@ __device__ __forceinline__ void inv_sin_stream(const T* row, T rn2, T (&acc)[G::
 }
 
 // The pass body for work-group (bx0, by0) of an nbx x nby grid.  A group has one output whose terms are one per
-// source (term e on source e), or one source and two outputs of one term each (the second LDS region then holds
-// the second output); every term a d/dx, d2/dx2 or -d/dx multiplier.
+// source (term e on source e); every term a d/dx, d2/dx2 or -d/dx multiplier.
 // FILT: the Fourier filter's passes (ff.m:7-11).  A group has one source and one output whose term is the filter's
 // real part (kind 4: the even part of the keep-mask, a spectral multiplier; plus the line's first value times the
 // operator's row sum at the output); the x-pass group's second output (gmode 1) takes the spectrum's coefficients at
@@ -276,9 +275,8 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
             yr = -kk * xi; yi = kk * xr;
         }
     };
-    // kinds: the multiplier of the region's output; TWO: a second output from the same forward transform,
-    // its column written to the second region R2 (same offsets)
-    auto fused_b = [&](bool two, int kind0, int kind1) {
+    // kind: the multiplier of the output
+    auto fused_b = [&](int kind0) {
         const cptr<T> tb = phase_table<T>(P.st);
         T cb[N1], sb[N1];
 #pragma unroll
@@ -345,7 +343,7 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
                     }
                 };
                 back(kind0, Rg + off);
-                if (two) back(kind1, Rg + LB * S + off);
+
             } else {
                 const int k2 = c > HI ? H : 0;
                 const int off = l * S + k2;
@@ -376,7 +374,7 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
                     for (int n1 = 0; n1 < N1; ++n1) dst[n1 * N2] = fmat(zr[n1], tc[n1], -(zi[n1] * ts[n1]));
                 };
                 back(kind0, Rg + off);
-                if (two) back(kind1, Rg + LB * S + off);
+
             }
         }
         __syncthreads();
@@ -428,15 +426,11 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
         }
     };
 
-    // The tile loop.  A unit is one source of a tile: it is staged, transformed forward, multiplied and
-    // transformed back, and each of its outputs stored.  A two-source output stores its first source's result and
-    // reads it back as the second source's addend, so both sources run the same code; a group with two outputs
-    // (one source) keeps the second output's columns in the second LDS region.  The loops over sources and
-    // outputs stay rolled: every transform phase is compiled once, and a kernel's code fits the instruction cache.
-    // An output's addend is loaded just before its store, so it occupies no registers during the transforms.
-    // (The host gives FP64 passes one output per group; a filter group's second output is the cross term's
-    // partials, not a second transform.)
-    const bool two = sizeof(T) == 4 && !FILT && Gr.nout == 2;
+    // The tile loop.  A unit is one source of a tile: it is staged, transformed forward, multiplied, transformed
+    // back and its output stored.  A two-source output stores its first source's result and reads it back as the
+    // second source's addend, so both sources run the same code; the loop over sources stays rolled, so every
+    // transform phase is compiled once and a kernel's code fits the instruction cache.  An output's addend is
+    // loaded just before its store, so it occupies no registers during the transforms.
     for (int it = 0; it < ntile; ++it) {
         const int lt = tile0 + it;
         int line0, z, x0;
@@ -500,19 +494,14 @@ __device__ __forceinline__ void spec1_body(const LPass* __restrict__ Pp, const M
             int kind = 1;
             for (int e = 0; e < O.nterms; ++e)            // (a filter output's rank-4 term is its cross term)
                 if (O.term[e].tile == t && !O.term[e].lr) kind = O.term[e].sk;
-            fused_b(two, kind, two ? Gr.out[1].term[0].sk : 1);
+            fused_b(kind);
             // the x-pass filter's cross term: this 16-line block's partials of G from the coefficients in DL
             if (FILT && Og) fold_g_partials<T, 1>(P, *Og, DL, line0, nl);
-#pragma unroll 1
-            for (int o = 0; o < (two ? 2 : 1); ++o) {
-                const LOut& Ou = Gr.out[o];
-                T* const Rr = Rg + o * LB * S;
-                inv_a(Rr);
-                const Line L = line_at(fresh((int)threadIdx.x));
-                // (the barriers since the first source's store order it before this read of the same elements)
-                load_addend(Ou, L, hold, t > 0);
-                store(Ou, Rr, L, hold, o == 0 ? cst : (T)0);
-            }
+            inv_a(Rg);
+            const Line L = line_at(fresh((int)threadIdx.x));
+            // (the barriers since the first source's store order it before this read of the same elements)
+            load_addend(O, L, hold, t > 0);
+            store(O, Rg, L, hold, cst);
         }
     }
 }

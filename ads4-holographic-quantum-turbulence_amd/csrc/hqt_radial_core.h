@@ -29,11 +29,15 @@ namespace hqt {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+// Field and operator accesses through global-address-space pointers: through generic pointers the compiler
+// issues flat loads, which count against the scalar-load wait counter too, so every wait for a kernel argument
+// also waited for the field loads in flight.
+template <typename TS> using gptr = __attribute__((address_space(1))) TS*;
 template <typename TS> __device__ __forceinline__ double ld(const void* p, int64_t i) {
-    return (double)static_cast<const TS*>(p)[i];
+    return (double)((const gptr<const TS>)p)[i];
 }
 template <typename TS> __device__ __forceinline__ void st(void* p, int64_t i, double v) {
-    static_cast<TS*>(p)[i] = (TS)v;
+    ((gptr<TS>)p)[i] = (TS)v;
 }
 
 __device__ __forceinline__ bool stopped(const int32_t* flags) {
@@ -54,7 +58,7 @@ __device__ __forceinline__ void load_op(const double* __restrict__ M, int ldm, i
 #pragma unroll
         for (int s = 0; s < ZG<NZ>::KS; ++s) {
             const int i = 16 * t + m, j = 4 * s + kk;
-            A[t][s] = (i < n && j < n) ? M[i * ldm + j] : 0.0;
+            A[t][s] = (i < n && j < n) ? ((gptr<const double>)M)[i * ldm + j] : 0.0;
         }
 }
 
@@ -402,7 +406,7 @@ __global__ __launch_bounds__(256) void k_zpost(const ZPost* __restrict__ Ap, con
     // rho and the boundary values are loaded with the fields: one memory round trip for the kernel
     double rh[KS];
     load_slots<NZ, double>(A.rho, c, cok, g, n, rh);
-    const double axb = cok ? A.axb[c] : 0.0, ayb = cok ? A.ayb[c] : 0.0;
+    const double axb = cok ? ld<double>(A.axb, c) : 0.0, ayb = cok ? ld<double>(A.ayb, c) : 0.0;
 #pragma unroll
     for (int s = 0; s < KS; ++s) a[s] += b[s];
     mv<NZ>(Lz, a, w);                // Lz(dxAx + dyAy)
@@ -425,7 +429,7 @@ __global__ __launch_bounds__(256) void k_zpost(const ZPost* __restrict__ Ap, con
     for (int s = 0; s < KS; ++s) {
         const int k = 4 * s + g;
         if (!cok || k >= NZ) continue;
-        A.At[c + n * k] = (k < NZ - 1) ? a[s] + A.mu : A.mu;
+        st<double>(A.At, c + n * k, (k < NZ - 1) ? a[s] + A.mu : A.mu);
         if (k == NZ - 1) {
             // boundary reset: Z1(:,:,nz)=0; Z3(:,:,nz)=ax_b; Z4(:,:,nz)=ay_b  (Z2 row set above)
             const int64_t pb = c + n * k;
@@ -441,7 +445,7 @@ __global__ __launch_bounds__(256) void k_zpost(const ZPost* __restrict__ Ap, con
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
         const int k = 4 * s + g;
-        if (cok && k == NZ - 1) { A.O[2 * c] = a[s]; A.O[2 * c + 1] = b[s]; }
+        if (cok && k == NZ - 1) { st<double>(A.O, 2 * c, a[s]); st<double>(A.O, 2 * c + 1, b[s]); }
     }
 }
 
